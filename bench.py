@@ -1,0 +1,207 @@
+"""Throughput of the Monte Carlo form-factor path on one MI355X.
+
+Workload (default): closed unit icosphere, subdivision 5 = 20,480 triangles
+with inward normals, one mesh per triangle, n_rays = 2.048e8 (about 10k
+Malley rays per triangle, two-sided), uniform grid 64^3, orthonormal emission
+frame, output the 20,480 x 20,480 FP32 form-factor matrix in HBM.
+
+One step = one full form-factor calculation on the device: Malley emission
+from the counter-based RNG, grid casting, exact integer tallies, row
+normalisation (the fused kernel). The scene, grid and emission layout are
+built before the timed region; inputs are resident in HBM.
+
+Prints one JSON line (see the contract in the module's DESIGN notes of the
+repository) with a live roofline for the fused kernel and the reference CPU
+caster timed on a bounded prefix of the identical ray set.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FP32 work per unit (counted on the device per launch):
+FLOP_EMIT = 40      # per ray: point 12, hemisphere 6 + 4 transcendental, rotation 15, ray 9 (RNG excluded)
+FLOP_TEST = 51      # per triangle test with the per-face terms precomputed
+FLOP_STEP = 6       # per grid cell stepped (3-D DDA)
+PEAK_FP32 = 157.3   # MI355X FP32 vector/matrix peak, TFLOP/s
+PEAK_HBM = 8000.0   # GB/s
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--subdiv", type=int, default=5)
+    ap.add_argument("--rays-per-face", type=int, default=10_000)
+    ap.add_argument("--grid", type=int, default=64)
+    ap.add_argument("--cpu-rays", type=int, default=1 << 22, help="prefix of the ray set timed on the CPU")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--rays-per-tile", type=int, default=0)
+    ap.add_argument("--hist", type=int, default=-1)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch of the fused kernel from the committed PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), if present."""
+    path = os.path.join(ROOT, "profiles", "ff_fused_c2_traffic.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline(faces, rays_host, gpu_idx, n_threads):
+    """Time the reference zgrid CPU caster (compiled from the reference
+    sources when available, else the oracle's restatement) on the same rays,
+    and count hit-index mismatches against the GPU on that prefix."""
+    from oracle import oracle as O
+
+    kind = "reference" if O.ref_available() else "port"
+    t0 = time.perf_counter()
+    if kind == "reference":
+        idx, _ = O.ref_cast(1, faces, rays_host, n_threads=n_threads)
+    else:
+        idx, _ = O.zgrid_cast(faces, rays_host, n_threads=n_threads)
+    dt = time.perf_counter() - t0
+    mism = int((idx != gpu_idx).sum())
+    return {"value": len(rays_host) / dt / 1e6, "unit": "Mrays/s", "cores": n_threads, "kind": kind,
+            "sample": f"zgrid_cpu cast of the first {len(rays_host)} rays of the identical ray set "
+                      f"({dt:.1f} s on {n_threads} threads; {mism} hit-index mismatches vs the GPU)"}, mism
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from form_factors_amd import ffk, scenes
+
+    torch.cuda.set_device(0)
+    info = ffk.device_info()
+    log(f"device: {info}")
+    faces = scenes.icosphere(args.subdiv, inward=True)
+    meshes = scenes.per_face_meshes(len(faces))
+    n_req = args.rays_per_face * len(faces)
+    sc = ffk.Scene(faces, meshes)
+    n_rays = sc.set_emission(n_rays=n_req, frame=ffk.FRAME_ORTHONORMAL)
+    g = sc.build_grid((args.grid,) * 3)
+    sc.set_tuning(block=args.block, rays_per_tile=args.rays_per_tile, hist=args.hist)
+    N = len(meshes)
+    log(f"scene: {len(faces)} faces, {N} meshes, {n_rays} rays, grid {g['dims']} refs {g['n_refs']}")
+    F = torch.empty((N, N), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    seed = 0x5EED
+
+    # Device-counted work for the roofline (one untimed launch with counters).
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    sc.ff_fused(seed, F=F, need_tally=False, stats=stats)
+    torch.cuda.synchronize()
+    n_tests, n_cells = (int(x) for x in stats.cpu())
+    flops = FLOP_EMIT * n_rays + FLOP_TEST * n_tests + FLOP_STEP * n_cells
+    log(f"work per step: {n_tests / n_rays:.2f} tests/ray, {n_cells / n_rays:.2f} cells/ray, {flops / 1e12:.3f} TFLOP")
+
+    for _ in range(args.warmup):
+        sc.ff_fused(seed, F=F, need_tally=False)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ev[k][0].record(stream)
+        sc.ff_fused(seed + k, F=F, need_tally=False)
+        ev[k][1].record(stream)
+        log(f"step {k} done")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = float(np.mean([a.elapsed_time(b) for a, b in ev]))
+    ms_per_step = wall * 1e3 / args.steps
+    value = n_rays / (ms_per_step * 1e-3) / 1e6
+
+    # Analytic check of the last F: closed sphere, F_ij = A_j / (2 A_total)
+    # for the two-sided convention (rear rays leave through the inside too).
+    Fh = F[:64].double().cpu().numpy()
+    w, _ = ffk.area_weights(faces)
+    A = w[0::2].astype(np.float64)
+    row_sums = Fh.sum(1)
+    sphere_ok = bool(np.abs(row_sums - 0.5).max() < 0.02)
+
+    result = {
+        "metric": "Mrays/s form-factor MC (20k-tri sphere), batched vs plain; % HBM/FP32 peak",
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"closed icosphere interior, {len(faces)} triangles (inward), one mesh per "
+                               f"triangle, {n_rays} Malley rays (~{args.rays_per_face}/triangle), "
+                               f"grid {args.grid}^3, plain fused emit-cast-tally kernel, F {N}x{N} fp32",
+                   "n_faces": len(faces), "n_rays": n_rays, "grid": list(g["dims"]), "parallelism": "single-gpu"},
+        "roofline": {"bound": "mfma", "achieved": round(flops / (kern_ms * 1e-3) / 1e12, 3), "peak": PEAK_FP32,
+                     "unit": "TFLOP/s", "frac": round(flops / (kern_ms * 1e-3) / 1e12 / PEAK_FP32, 4),
+                     "traffic": traffic_from_profiles(),
+                     "engine": "FP32 VALU (no matrix op on this path); peak is the MI355X FP32 rate",
+                     "kernel": "k_ff_fused", "kernel_ms": round(kern_ms, 3),
+                     "work": {"tests_per_ray": round(n_tests / n_rays, 3), "cells_per_ray": round(n_cells / n_rays, 3),
+                              "flop_per_launch": flops}},
+        "check": {"sphere_row_sum_max_dev": float(np.abs(row_sums - 0.5).max()), "ok": sphere_ok},
+        "device": info["name"],
+    }
+
+    if not args.no_batched:
+        # The same scene through the persistent, graph-replayed launch.
+        b = ffk.Batch([sc])
+        b.run([seed], [F], use_graph=True)
+        torch.cuda.synchronize()
+        tb = time.perf_counter()
+        for k in range(max(2, args.steps // 2)):
+            b.run([seed], [F], use_graph=True)
+        torch.cuda.synchronize()
+        bt = (time.perf_counter() - tb) / max(2, args.steps // 2)
+        result["batched"] = {"value": round(n_rays / bt / 1e6, 2), "ms_per_step": round(bt * 1e3, 3),
+                             "path": "persistent tile-queue kernel, hipGraph replay"}
+        b.close()
+
+    if not args.no_cpu:
+        n_cpu = min(args.cpu_rays, n_rays)
+        rays = torch.empty((n_rays, 6), dtype=torch.float32, device="cuda")
+        ffk.lib().ffk_emit_malley(sc.h, seed, ffk._ptr(rays), ffk._stream())
+        idx, _ = sc.cast(rays[:n_cpu], method=ffk.CAST_GRID, points=False)
+        torch.cuda.synchronize()
+        rays_h = rays[:n_cpu].cpu().numpy()
+        idx_h = idx.cpu().numpy()
+        del rays
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {n_cpu} rays, {threads} threads ...")
+        result["cpu_baseline"], mism = cpu_baseline(faces, rays_h, idx_h, threads)
+        result["check"]["cpu_prefix_mismatches"] = mism
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

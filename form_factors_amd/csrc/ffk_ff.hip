@@ -1,0 +1,732 @@
+// Emission, exact tallies, normalisation and the fused form-factor kernel.
+#include <algorithm>
+#include <cstring>
+
+#include "ffk_trace.hpp"
+
+namespace ffk {
+
+// ------------------------------------------------------------- emission --
+
+__global__ void __launch_bounds__(256) k_emit_malley(scene_desc S, uint32_t k0, uint32_t k1,
+                                                     float* __restrict__ rays) {
+  long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= S.n_rays) return;
+  int f = find_face(S.face_off, S.n_faces, g, 0);
+  int j = (int)(g - S.face_off[f]);
+  v3 o, e;
+  malley_ray(S, f, j, k0, k1, &o, &e);
+  float* r = rays + 6 * g;
+  r[0] = o.x; r[1] = o.y; r[2] = o.z;
+  r[3] = e.x; r[4] = e.y; r[5] = e.z;
+}
+
+// emission/parallel_rays_cpu.cpp:88-106: a point on a disk of the given
+// radius (rho = radius * sqrt(U), angle 2*pi*U'), rotated from z onto dir,
+// placed at -distance along it around center; the end point is 0.1 along dir.
+__global__ void __launch_bounds__(256) k_emit_parallel(v3 dir, v3 center, float distance, float radius,
+                                                       const float* __restrict__ rot, long long n,
+                                                       uint32_t k0, uint32_t k1, float* __restrict__ rays) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  u4 r = philox(u4{(uint32_t)i, 0u, 1u, (uint32_t)(i >> 32)}, k0, k1);
+  float rho = radius * sqrtf(u01(r.x));
+  float s, c;
+  sincos_turns(u01(r.y), &s, &c);
+  float x = rho * c, y = rho * s;
+  v3 o = mat_vec(rot, mk(x, y, -distance)) + center;
+  v3 e = o + dir * 0.1f;
+  float* q = rays + 6 * i;
+  q[0] = o.x; q[1] = o.y; q[2] = o.z;
+  q[3] = e.x; q[4] = e.y; q[5] = e.z;
+}
+
+// --------------------------------------------------------------- tally --
+
+__global__ void k_fill_u64(unsigned long long* __restrict__ p, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = 0ull;
+}
+
+__global__ void k_fill_i32(int* __restrict__ p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Mesh m whose tally range [mesh_off[m], mesh_off[m+1]) holds ray g.
+__device__ inline int find_mesh(const long long* __restrict__ off, int n, long long g) {
+  int lo = 0, hi = n - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (off[mid + 1] > g) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_tally(scene_desc S, const int* __restrict__ hit, long long n,
+                                               unsigned long long* __restrict__ tally) {
+  long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  if (g >= S.mesh_off[S.n_meshes]) return;  // rays past the mesh walk are not counted
+  int h = hit[g];
+  if (h < 0) return;
+  int dst = S.face2mesh[h];
+  if (dst < 0) return;
+  int src = find_mesh(S.mesh_off, S.n_meshes, g);
+  atomicAdd(&tally[(size_t)src * S.n_meshes + dst], 1ull);
+}
+
+// F[m, j] = (float)count * (1.f / (float)rays_of_m): the float count is
+// exact below 2^24 and then equals the reference's float accumulation.
+__device__ inline float normalise_one(unsigned long long cnt, float ratio) { return (float)cnt * ratio; }
+
+__global__ void __launch_bounds__(256) k_normalise_rows(const long long* __restrict__ mesh_off, int N,
+                                                        const int* __restrict__ rows, int n_rows,
+                                                        const unsigned long long* __restrict__ tally,
+                                                        float* __restrict__ F) {
+  const int r = blockIdx.x;
+  if (r >= n_rows) return;
+  const int m = rows ? rows[r] : r;
+  const float ratio = 1.f / (float)(int)(mesh_off[m + 1] - mesh_off[m]);
+  for (int j = threadIdx.x; j < N; j += blockDim.x)
+    F[(size_t)m * N + j] = normalise_one(tally[(size_t)m * N + j], ratio);
+}
+
+__global__ void k_zero_rows(int N, const int* __restrict__ rows, int n_rows,
+                            unsigned long long* __restrict__ tally) {
+  const int r = blockIdx.x;
+  if (r >= n_rows) return;
+  const int m = rows[r];
+  for (int j = threadIdx.x; j < N; j += blockDim.x) tally[(size_t)m * N + j] = 0ull;
+}
+
+// ------------------------------------------------------- fused kernel --
+// HIST: 0 = global uint64 atomics per hit; 1 = uint32 LDS histogram of the
+// tile's row; 2 = two uint16 counters per LDS word (tiles <= 65535 rays).
+// PERSIST: blocks pull tiles from a queue (batched launch) instead of one
+// tile per block (plain launch).
+struct fused_args {
+  const scene_desc* descs;
+  const tile* tiles;
+  int n_tiles;
+  int* queue;
+  const unsigned long long* seeds;
+  unsigned long long* const* tallies;
+  float* const* Fs;
+  unsigned long long* stats;
+};
+
+template <int HIST, bool PERSIST, bool STATS>
+__global__ void __launch_bounds__(1024) k_ff_fused(fused_args A) {
+  extern __shared__ uint32_t hist[];
+  __shared__ int s_tile;
+  uint32_t nt = 0, nc = 0;
+  int t_idx = blockIdx.x;
+  while (true) {
+    if (PERSIST) {
+      __syncthreads();
+      if (threadIdx.x == 0) s_tile = atomicAdd(A.queue, 1);
+      __syncthreads();
+      t_idx = s_tile;
+    }
+    if (t_idx >= A.n_tiles) break;
+    const tile T = A.tiles[t_idx];
+    const int sc = __builtin_amdgcn_readfirstlane(T.scene);
+    const scene_desc S = A.descs[sc];
+    const int N = S.n_meshes;
+    const int words = HIST == 1 ? N : (N + 1) >> 1;
+    if (HIST != 0) {
+      for (int w = threadIdx.x; w < words; w += blockDim.x) hist[w] = 0u;
+      __syncthreads();
+    }
+    const unsigned long long seed = A.seeds[sc];
+    const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+    unsigned long long* tally = A.tallies ? A.tallies[sc] : nullptr;
+    int hint = find_face(S.face_off, S.n_faces, T.g0, 0);
+    for (long long g = T.g0 + threadIdx.x; g < T.g1; g += blockDim.x) {
+      const int f = find_face(S.face_off, S.n_faces, g, hint);
+      hint = f;
+      v3 o, e;
+      malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+      hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
+      cast_grid<STATS>(S, o, e - o, h, nt, nc);
+      if (h.f >= 0) {
+        const int m = S.face2mesh[h.f];
+        if (m >= 0) {
+          if (HIST == 0) atomicAdd(&tally[(size_t)T.mesh * N + m], 1ull);
+          else if (HIST == 1) atomicAdd(&hist[m], 1u);
+          else atomicAdd(&hist[m >> 1], (m & 1) ? 65536u : 1u);
+        }
+      }
+    }
+    if (HIST != 0) {
+      __syncthreads();
+      if (T.direct) {
+        float* F = A.Fs[sc] + (size_t)T.mesh * N;
+        const float ratio = 1.f / (float)(int)(T.g1 - T.g0);
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+          uint32_t c = HIST == 1 ? hist[j] : ((hist[j >> 1] >> ((j & 1) * 16)) & 0xffffu);
+          F[j] = normalise_one(c, ratio);
+        }
+      } else {
+        unsigned long long* row = tally + (size_t)T.mesh * N;
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+          uint32_t c = HIST == 1 ? hist[j] : ((hist[j >> 1] >> ((j & 1) * 16)) & 0xffffu);
+          if (c) atomicAdd(&row[j], (unsigned long long)c);
+        }
+      }
+    }
+    if (!PERSIST) break;
+  }
+  if (STATS) {
+    unsigned long long t = nt, c = nc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      t += __shfl_xor(t, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&A.stats[0], t);
+      atomicAdd(&A.stats[1], c);
+    }
+  }
+}
+
+// ---------------------------------------------------------- planning --
+
+static int max_lds_bytes() {
+  static int cached = 0;
+  if (!cached) {
+    int dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+      cached = (int)std::max<size_t>(p.sharedMemPerBlock, 65536);
+    else
+      cached = 65536;
+  }
+  return cached;
+}
+
+// Histogram mode for a row of N meshes and tiles of at most max_tile rays.
+static int pick_hist(int N, long long max_tile, int requested) {
+  const int lds = max_lds_bytes();
+  if (requested == 0) return 0;
+  if (requested == 1 && 4LL * N <= lds) return 1;
+  if (requested == 2 && 2LL * ((N + 1) / 2) * 2 <= lds && max_tile <= 65535) return 2;
+  if (4LL * N <= 65536) return 1;
+  if (4LL * ((N + 1) / 2) <= std::min(lds, 98304) && max_tile <= 65535) return 2;
+  return 0;
+}
+
+static size_t hist_bytes(int mode, int N) {
+  if (mode == 1) return 4 * (size_t)N;
+  if (mode == 2) return 4 * (size_t)((N + 1) / 2);
+  return 0;
+}
+
+template <int H, bool P, bool S>
+static void set_lds_attr(size_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute((const void*)&k_ff_fused<H, P, S>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+template <bool P, bool S>
+static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStream_t st,
+                              const fused_args& A) {
+  switch (hist) {
+    case 1:
+      set_lds_attr<1, P, S>(lds);
+      k_ff_fused<1, P, S><<<grid, block, lds, st>>>(A);
+      break;
+    case 2:
+      set_lds_attr<2, P, S>(lds);
+      k_ff_fused<2, P, S><<<grid, block, lds, st>>>(A);
+      break;
+    default:
+      k_ff_fused<0, P, S><<<grid, block, 0, st>>>(A);
+      break;
+  }
+  return hipGetLastError();
+}
+
+static hipError_t launch_fused(int hist, bool persist, bool stats, dim3 grid, int block, size_t lds,
+                               hipStream_t st, const fused_args& A) {
+  if (persist)
+    return stats ? launch_mode<true, true>(hist, grid, block, lds, st, A)
+                 : launch_mode<true, false>(hist, grid, block, lds, st, A);
+  return stats ? launch_mode<false, true>(hist, grid, block, lds, st, A)
+               : launch_mode<false, false>(hist, grid, block, lds, st, A);
+}
+
+static long long auto_rays_per_tile(long long total) {
+  long long per = (total + 8LL * device_cu_count() - 1) / (8LL * device_cu_count());
+  return std::max(1024LL, std::min(65535LL, per));
+}
+
+// Tiles of one scene: each mesh range split into pieces of at most
+// rays_per_tile; a mesh covered by one tile writes its F row directly.
+static void scene_tiles(const ffk_scene* s, int scene_id, long long rpt, std::vector<tile>& out,
+                        std::vector<int>& indirect) {
+  for (int m = 0; m < s->n_meshes; ++m) {
+    long long a = s->h_mesh_off[m], b = s->h_mesh_off[m + 1];
+    if (b <= a) {  // no rays: 0 * (1/0) in the reference, produced by the tally path
+      indirect.push_back(m);
+      continue;
+    }
+    if (b - a <= rpt) {
+      out.push_back(tile{a, b, m, scene_id, 1, 0});
+      continue;
+    }
+    indirect.push_back(m);
+    for (long long g = a; g < b; g += rpt) out.push_back(tile{g, std::min(b, g + rpt), m, scene_id, 0, 0});
+  }
+}
+
+int build_tiles(ffk_scene* s, int rays_per_tile, hipStream_t st) {
+  dfree(s->d_tiles);
+  dfree(s->d_indirect_rows);
+  dfree(s->d_desc);
+  s->d_tiles = nullptr;
+  s->d_indirect_rows = nullptr;
+  s->d_desc = nullptr;
+  long long rpt = rays_per_tile > 0 ? rays_per_tile : auto_rays_per_tile(s->n_rays);
+  rpt = std::min(rpt, 65535LL);
+  std::vector<tile> tiles;
+  std::vector<int> indirect;
+  scene_tiles(s, 0, rpt, tiles, indirect);
+  s->n_tiles = (int)tiles.size();
+  s->n_indirect_rows = (int)indirect.size();
+  s->tiles_rays_per_tile = (int)rpt;
+  int r;
+  if ((r = dalloc(&s->d_tiles, std::max<size_t>(1, tiles.size()))) < 0) return r;
+  if ((r = dalloc(&s->d_indirect_rows, std::max<size_t>(1, indirect.size()))) < 0) return r;
+  if ((r = dalloc(&s->d_desc, 1)) < 0) return r;
+  if (!tiles.empty())
+    FFK_HIP(hipMemcpy(s->d_tiles, tiles.data(), tiles.size() * sizeof(tile), hipMemcpyHostToDevice));
+  if (!indirect.empty())
+    FFK_HIP(hipMemcpy(s->d_indirect_rows, indirect.data(), indirect.size() * 4, hipMemcpyHostToDevice));
+  (void)st;
+  return FFK_OK;
+}
+
+}  // namespace ffk
+
+using namespace ffk;
+
+// ------------------------------------------------------------ batches --
+
+struct ffk_batch {
+  std::vector<ffk_scene*> scenes;
+  tile* d_tiles = nullptr;
+  int n_tiles = 0;
+  scene_desc* d_descs = nullptr;
+  int2* d_rows = nullptr;  // (scene, mesh) rows tallied through the global tally
+  int n_rows = 0;
+  int max_rows_n = 0;
+  int* d_queue = nullptr;
+  unsigned long long* d_seeds = nullptr;
+  unsigned long long** d_tallies = nullptr;
+  float** d_Fs = nullptr;
+  int hist = 0;
+  int max_N = 0;
+  long long max_tile = 0;
+  long long total_rays = 0;
+  int block = 0, rpt = 0, n_blocks = 0;
+  // graph state
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<unsigned long long*> g_tallies;
+  std::vector<float*> g_Fs;
+  std::vector<unsigned long long> g_seeds;
+};
+
+namespace ffk {
+
+__global__ void k_zero_rows_batch(const int2* __restrict__ rows, int n_rows, const scene_desc* __restrict__ d,
+                                  unsigned long long* const* __restrict__ tallies) {
+  const int r = blockIdx.x;
+  if (r >= n_rows) return;
+  const int2 sm = rows[r];
+  const int N = d[sm.x].n_meshes;
+  unsigned long long* t = tallies[sm.x] + (size_t)sm.y * N;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) t[j] = 0ull;
+}
+
+__global__ void k_normalise_batch(const int2* __restrict__ rows, int n_rows, const scene_desc* __restrict__ d,
+                                  unsigned long long* const* __restrict__ tallies, float* const* __restrict__ Fs) {
+  const int r = blockIdx.x;
+  if (r >= n_rows) return;
+  const int2 sm = rows[r];
+  const scene_desc& S = d[sm.x];
+  const int N = S.n_meshes, m = sm.y;
+  const float ratio = 1.f / (float)(int)(S.mesh_off[m + 1] - S.mesh_off[m]);
+  const unsigned long long* t = tallies[sm.x] + (size_t)m * N;
+  float* F = Fs[sm.x] + (size_t)m * N;
+  for (int j = threadIdx.x; j < N; j += blockDim.x) F[j] = normalise_one(t[j], ratio);
+}
+
+static void free_graph(ffk_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+static int batch_plan(ffk_batch* b) {
+  free_graph(b);
+  dfree(b->d_tiles);
+  dfree(b->d_rows);
+  b->d_tiles = nullptr;
+  b->d_rows = nullptr;
+  long long total = 0;
+  int max_N = 0;
+  for (ffk_scene* s : b->scenes) {
+    total += s->n_rays;
+    max_N = std::max(max_N, s->n_meshes);
+  }
+  long long rpt = b->rpt > 0 ? b->rpt : auto_rays_per_tile(total);
+  rpt = std::min(rpt, 65535LL);
+  std::vector<tile> tiles;
+  std::vector<int2> rows;
+  long long max_tile = 0;
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    std::vector<int> ind;
+    scene_tiles(b->scenes[i], (int)i, rpt, tiles, ind);
+    for (int m : ind) rows.push_back(make_int2((int)i, m));
+  }
+  // Longest tiles first: a greedy order that keeps the persistent blocks
+  // busy to the end of the queue.
+  std::stable_sort(tiles.begin(), tiles.end(),
+                   [](const tile& a, const tile& c) { return (a.g1 - a.g0) > (c.g1 - c.g0); });
+  for (const tile& t : tiles) max_tile = std::max(max_tile, t.g1 - t.g0);
+  b->n_tiles = (int)tiles.size();
+  b->n_rows = (int)rows.size();
+  b->max_N = max_N;
+  b->max_tile = max_tile;
+  b->total_rays = total;
+  b->hist = pick_hist(max_N, max_tile, -1);
+  int r;
+  if ((r = dalloc(&b->d_tiles, std::max<size_t>(1, tiles.size()))) < 0) return r;
+  if ((r = dalloc(&b->d_rows, std::max<size_t>(1, rows.size()))) < 0) return r;
+  if (!tiles.empty())
+    FFK_HIP(hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(tile), hipMemcpyHostToDevice));
+  if (!rows.empty())
+    FFK_HIP(hipMemcpy(b->d_rows, rows.data(), rows.size() * sizeof(int2), hipMemcpyHostToDevice));
+  return FFK_OK;
+}
+
+static int default_block(int hist) { return hist == 0 ? 256 : 512; }
+
+}  // namespace ffk
+
+extern "C" {
+
+int ffk_emit_malley(ffk_scene* s, unsigned long long seed, float* d_rays, void* stream) {
+  FFK_REQUIRE(s && s->has_emission, FFK_ERR_ARG, "ffk_emit_malley: emission layout not set");
+  if (s->n_rays == 0) return FFK_OK;
+  FFK_REQUIRE(d_rays, FFK_ERR_ARG, "ffk_emit_malley: NULL ray buffer");
+  hipStream_t st = as_stream(stream);
+  k_emit_malley<<<(unsigned)((s->n_rays + 255) / 256), 256, 0, st>>>(s->desc(), (uint32_t)seed,
+                                                                     (uint32_t)(seed >> 32), d_rays);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
+int ffk_emit_parallel(const float* h_dir3, const float* h_center3, float distance, float radius,
+                      long long n_rays, unsigned long long seed, float* d_rays, void* stream) {
+  FFK_REQUIRE(h_dir3 && h_center3 && n_rays >= 0, FFK_ERR_ARG, "ffk_emit_parallel: bad args");
+  if (n_rays == 0) return FFK_OK;
+  FFK_REQUIRE(d_rays, FFK_ERR_ARG, "ffk_emit_parallel: NULL ray buffer");
+  hipStream_t st = as_stream(stream);
+  v3 dir = mk(h_dir3[0], h_dir3[1], h_dir3[2]);
+  v3 center = mk(h_center3[0], h_center3[1], h_center3[2]);
+  float rot[9];
+  rotate_towards(mk(0.f, 0.f, 1.f), dir, rot);
+  float* d_rot = nullptr;
+  int r;
+  if ((r = dalloc(&d_rot, 9)) < 0) return r;
+  FFK_HIP(hipMemcpyAsync(d_rot, rot, sizeof(rot), hipMemcpyHostToDevice, st));
+  k_emit_parallel<<<(unsigned)((n_rays + 255) / 256), 256, 0, st>>>(
+      dir, center, distance, radius, d_rot, n_rays, (uint32_t)seed, (uint32_t)(seed >> 32), d_rays);
+  hipError_t e = hipGetLastError();
+  hipError_t e2 = hipStreamSynchronize(st);
+  dfree(d_rot);
+  FFK_HIP(e);
+  FFK_HIP(e2);
+  return FFK_OK;
+}
+
+int ffk_tally_zero(const ffk_scene* s, unsigned long long* d_tally, void* stream) {
+  FFK_REQUIRE(s && s->has_meshes && d_tally, FFK_ERR_ARG, "ffk_tally_zero: bad args");
+  long long n = (long long)s->n_meshes * s->n_meshes;
+  int blocks = (int)std::min<long long>((n + 255) / 256, 65536);
+  k_fill_u64<<<blocks, 256, 0, as_stream(stream)>>>(d_tally, n);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
+int ffk_tally(const ffk_scene* s, const int* d_hit_idx, long long n_rays, unsigned long long* d_tally,
+              void* stream) {
+  FFK_REQUIRE(s && s->has_emission, FFK_ERR_ARG, "ffk_tally: emission layout not set");
+  FFK_REQUIRE(n_rays >= 0 && n_rays <= s->n_rays, FFK_ERR_ARG, "ffk_tally: %lld rays, layout has %lld",
+              n_rays, s->n_rays);
+  if (n_rays == 0) return FFK_OK;
+  FFK_REQUIRE(d_hit_idx && d_tally, FFK_ERR_ARG, "ffk_tally: NULL buffer");
+  k_tally<<<(unsigned)((n_rays + 255) / 256), 256, 0, as_stream(stream)>>>(s->desc(), d_hit_idx, n_rays,
+                                                                          d_tally);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
+int ffk_normalise(const ffk_scene* s, const unsigned long long* d_tally, float* d_F, void* stream) {
+  FFK_REQUIRE(s && s->has_emission && d_tally && d_F, FFK_ERR_ARG, "ffk_normalise: bad args");
+  const int N = s->n_meshes;
+  dim3 grid((unsigned)N);
+  k_normalise_rows<<<grid, 256, 0, as_stream(stream)>>>(s->d_mesh_off, N, nullptr, N, d_tally, d_F);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
+int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode) {
+  FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
+  FFK_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024),
+              FFK_ERR_ARG, "block size %d is not a multiple of 64 in [64, 1024]", block_size);
+  FFK_REQUIRE(rays_per_tile >= 0 && rays_per_tile <= 65535, FFK_ERR_LIMIT,
+              "rays per tile %d outside [0, 65535]", rays_per_tile);
+  FFK_REQUIRE(hist_mode >= -1 && hist_mode <= 2, FFK_ERR_ARG, "hist mode %d", hist_mode);
+  s->tune_block = block_size;
+  s->tune_hist = hist_mode;
+  if (rays_per_tile != s->tune_rays_per_tile) {
+    s->tune_rays_per_tile = rays_per_tile;
+    if (s->has_emission) return build_tiles(s, rays_per_tile, nullptr);
+  }
+  return FFK_OK;
+}
+
+int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned long long* d_tally, float* d_F,
+                 unsigned long long* d_stats, void* stream) {
+  FFK_REQUIRE(s && s->has_emission && s->has_grid, FFK_ERR_ARG,
+              "ffk_ff_fused: emission layout and grid must be set");
+  FFK_REQUIRE(method == FFK_CAST_GRID, FFK_ERR_ARG, "ffk_ff_fused: only the grid caster is fused");
+  FFK_REQUIRE(d_F, FFK_ERR_ARG, "ffk_ff_fused: NULL F");
+  hipStream_t st = as_stream(stream);
+  const int N = s->n_meshes;
+  long long max_tile = std::min<long long>(s->tiles_rays_per_tile, 65535);
+  int hist = pick_hist(N, max_tile, s->tune_hist);
+  const bool needs_tally = hist == 0 || s->n_indirect_rows > 0;
+  FFK_REQUIRE(!needs_tally || d_tally, FFK_ERR_ARG,
+              "ffk_ff_fused: this scene needs the N*N uint64 tally buffer");
+  // per-launch argument block in device memory
+  scene_desc d = s->desc();
+  FFK_HIP(hipMemcpyAsync(s->d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
+  static thread_local unsigned long long* d_misc = nullptr;  // seed, tally ptr, F ptr
+  if (!d_misc) {
+    int r;
+    if ((r = dalloc(&d_misc, 4)) < 0) return r;
+  }
+  unsigned long long misc[3] = {seed, (unsigned long long)(uintptr_t)d_tally,
+                                (unsigned long long)(uintptr_t)d_F};
+  FFK_HIP(hipMemcpyAsync(d_misc, misc, sizeof(misc), hipMemcpyHostToDevice, st));
+  if (hist == 0) {
+    long long n = (long long)N * N;
+    k_fill_u64<<<(int)std::min<long long>((n + 255) / 256, 65536), 256, 0, st>>>(d_tally, n);
+  } else if (s->n_indirect_rows > 0) {
+    dim3 grid((unsigned)s->n_indirect_rows);
+    k_zero_rows<<<grid, 256, 0, st>>>(N, s->d_indirect_rows, s->n_indirect_rows, d_tally);
+  }
+  FFK_HIP(hipGetLastError());
+  fused_args A;
+  A.descs = s->d_desc;
+  A.tiles = s->d_tiles;
+  A.n_tiles = s->n_tiles;
+  A.queue = nullptr;
+  A.seeds = d_misc;
+  A.tallies = reinterpret_cast<unsigned long long* const*>(d_misc + 1);
+  A.Fs = reinterpret_cast<float* const*>(d_misc + 2);
+  A.stats = d_stats;
+  const int block = s->tune_block > 0 ? s->tune_block : default_block(hist);
+  if (s->n_tiles > 0)
+    FFK_HIP(launch_fused(hist, false, d_stats != nullptr, dim3((unsigned)s->n_tiles), block,
+                         hist_bytes(hist, N), st, A));
+  if (hist == 0) {
+    dim3 grid((unsigned)N);
+    k_normalise_rows<<<grid, 256, 0, st>>>(s->d_mesh_off, N, nullptr, N, d_tally, d_F);
+  } else if (s->n_indirect_rows > 0) {
+    dim3 grid((unsigned)s->n_indirect_rows);
+    k_normalise_rows<<<grid, 256, 0, st>>>(s->d_mesh_off, N, s->d_indirect_rows, s->n_indirect_rows,
+                                           d_tally, d_F);
+  }
+  FFK_HIP(hipGetLastError());
+  // The argument block is reused by the next call on this thread.
+  FFK_HIP(hipStreamSynchronize(st));
+  return FFK_OK;
+}
+
+int ffk_batch_create(ffk_batch** out, ffk_scene* const* scenes, int n_scenes, void* stream) {
+  FFK_REQUIRE(out && scenes && n_scenes > 0, FFK_ERR_ARG, "ffk_batch_create: bad args");
+  for (int i = 0; i < n_scenes; ++i)
+    FFK_REQUIRE(scenes[i] && scenes[i]->has_emission && scenes[i]->has_grid, FFK_ERR_ARG,
+                "ffk_batch_create: scene %d lacks emission layout or grid", i);
+  ffk_batch* b = new ffk_batch();
+  b->scenes.assign(scenes, scenes + n_scenes);
+  int r;
+  if ((r = dalloc(&b->d_descs, (size_t)n_scenes)) < 0 || (r = dalloc(&b->d_queue, 1)) < 0 ||
+      (r = dalloc(&b->d_seeds, (size_t)n_scenes)) < 0 || (r = dalloc(&b->d_tallies, (size_t)n_scenes)) < 0 ||
+      (r = dalloc(&b->d_Fs, (size_t)n_scenes)) < 0) {
+    ffk_batch_destroy(b);
+    return r;
+  }
+  std::vector<scene_desc> descs(n_scenes);
+  for (int i = 0; i < n_scenes; ++i) descs[i] = scenes[i]->desc();
+  hipError_t e = hipMemcpy(b->d_descs, descs.data(), descs.size() * sizeof(scene_desc), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    set_error("hipMemcpy failed: %s", hipGetErrorString(e));
+    ffk_batch_destroy(b);
+    return FFK_ERR_HIP;
+  }
+  if ((r = batch_plan(b)) < 0) {
+    ffk_batch_destroy(b);
+    return r;
+  }
+  (void)stream;
+  *out = b;
+  return FFK_OK;
+}
+
+void ffk_batch_destroy(ffk_batch* b) {
+  if (!b) return;
+  (void)hipDeviceSynchronize();
+  free_graph(b);
+  dfree(b->d_tiles);
+  dfree(b->d_descs);
+  dfree(b->d_rows);
+  dfree(b->d_queue);
+  dfree(b->d_seeds);
+  dfree(b->d_tallies);
+  dfree(b->d_Fs);
+  delete b;
+}
+
+int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  FFK_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024),
+              FFK_ERR_ARG, "block size %d", block_size);
+  FFK_REQUIRE(rays_per_tile >= 0 && rays_per_tile <= 65535, FFK_ERR_LIMIT, "rays per tile %d", rays_per_tile);
+  FFK_REQUIRE(n_blocks >= 0, FFK_ERR_ARG, "n_blocks %d", n_blocks);
+  b->block = block_size;
+  b->n_blocks = n_blocks;
+  if (rays_per_tile != b->rpt) {
+    b->rpt = rays_per_tile;
+    return batch_plan(b);
+  }
+  free_graph(b);
+  return FFK_OK;
+}
+
+static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, hipStream_t st) {
+  const int block = b->block > 0 ? b->block : default_block(b->hist);
+  const size_t lds = hist_bytes(b->hist, b->max_N);
+  int n_blocks = b->n_blocks;
+  if (n_blocks <= 0) {
+    int per_cu = 0;
+    hipError_t e;
+    switch (b->hist) {
+      case 1:
+        set_lds_attr<1, true, false>(lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ff_fused<1, true, false>, block, lds);
+        break;
+      case 2:
+        set_lds_attr<2, true, false>(lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ff_fused<2, true, false>, block, lds);
+        break;
+      default:
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ff_fused<0, true, false>, block, 0);
+        break;
+    }
+    if (e != hipSuccess || per_cu < 1) per_cu = 1;
+    n_blocks = per_cu * device_cu_count();
+  }
+  n_blocks = std::max(1, std::min(n_blocks, b->n_tiles));
+  k_fill_i32<<<1, 64, 0, st>>>(b->d_queue, 1, 0);
+  if (b->hist == 0) {
+    for (size_t i = 0; i < b->scenes.size(); ++i) {
+      long long n = (long long)b->scenes[i]->n_meshes * b->scenes[i]->n_meshes;
+      k_fill_u64<<<(int)std::min<long long>((n + 255) / 256, 65536), 256, 0, st>>>(b->g_tallies[i], n);
+    }
+  } else if (b->n_rows > 0) {
+    dim3 grid((unsigned)b->n_rows);
+    k_zero_rows_batch<<<grid, 256, 0, st>>>(b->d_rows, b->n_rows, b->d_descs, b->d_tallies);
+  }
+  fused_args A;
+  A.descs = b->d_descs;
+  A.tiles = b->d_tiles;
+  A.n_tiles = b->n_tiles;
+  A.queue = b->d_queue;
+  A.seeds = b->d_seeds;
+  A.tallies = b->d_tallies;
+  A.Fs = b->d_Fs;
+  A.stats = d_stats;
+  FFK_HIP(launch_fused(b->hist, true, stats, dim3((unsigned)n_blocks), block, lds, st, A));
+  if (b->hist == 0) {
+    for (size_t i = 0; i < b->scenes.size(); ++i) {
+      const ffk_scene* s = b->scenes[i];
+      const int N = s->n_meshes;
+      dim3 grid((unsigned)N);
+      k_normalise_rows<<<grid, 256, 0, st>>>(s->d_mesh_off, N, nullptr, N, b->g_tallies[i], b->g_Fs[i]);
+    }
+  } else if (b->n_rows > 0) {
+    dim3 grid((unsigned)b->n_rows);
+    k_normalise_batch<<<grid, 256, 0, st>>>(b->d_rows, b->n_rows, b->d_descs, b->d_tallies, b->d_Fs);
+  }
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
+int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
+                  float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream) {
+  FFK_REQUIRE(b && seeds && d_Fs, FFK_ERR_ARG, "ffk_batch_run: bad args");
+  const size_t n = b->scenes.size();
+  bool need_tally = b->hist == 0 || b->n_rows > 0;
+  for (size_t i = 0; i < n; ++i) {
+    FFK_REQUIRE(d_Fs[i], FFK_ERR_ARG, "ffk_batch_run: NULL F for scene %zu", i);
+    FFK_REQUIRE(!need_tally || (d_tallies && d_tallies[i]), FFK_ERR_ARG,
+                "ffk_batch_run: scene %zu needs a tally buffer", i);
+  }
+  hipStream_t st = as_stream(stream);
+  std::vector<unsigned long long*> tal(n, nullptr);
+  std::vector<float*> fs(d_Fs, d_Fs + n);
+  if (d_tallies) tal.assign(d_tallies, d_tallies + n);
+  std::vector<unsigned long long> sd(seeds, seeds + n);
+  const bool same = b->exec && tal == b->g_tallies && fs == b->g_Fs && sd == b->g_seeds;
+  if (!same) {
+    free_graph(b);
+    FFK_HIP(hipMemcpyAsync(b->d_seeds, sd.data(), n * 8, hipMemcpyHostToDevice, st));
+    FFK_HIP(hipMemcpyAsync(b->d_tallies, tal.data(), n * 8, hipMemcpyHostToDevice, st));
+    FFK_HIP(hipMemcpyAsync(b->d_Fs, fs.data(), n * 8, hipMemcpyHostToDevice, st));
+    FFK_HIP(hipStreamSynchronize(st));
+    b->g_tallies = tal;
+    b->g_Fs = fs;
+    b->g_seeds = sd;
+  }
+  if (d_stats || !use_graph) return batch_enqueue(b, d_stats != nullptr, d_stats, st);
+  if (!b->exec) {
+    FFK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    int r = batch_enqueue(b, false, nullptr, st);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (r < 0) {
+      if (g) (void)hipGraphDestroy(g);
+      return r;
+    }
+    FFK_HIP(e);
+    b->graph = g;
+    FFK_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  }
+  FFK_HIP(hipGraphLaunch(b->exec, st));
+  return FFK_OK;
+}
+
+}  // extern "C"

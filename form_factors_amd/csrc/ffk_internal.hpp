@@ -1,0 +1,295 @@
+// Internal definitions shared by the ffk translation units (gfx950 only).
+//
+// Every file of the library is compiled with -ffp-contract=off and IEEE
+// division/sqrt, so the float expressions below round exactly like the
+// reference's SSE code (math/triangle.cpp, math/mat.cpp, emission/*.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "ffk.h"
+
+namespace ffk {
+
+// ---------------------------------------------------------------- errors --
+void set_error(const char* fmt, ...);
+
+#define FFK_HIP(call)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (call);                                                             \
+    if (e_ != hipSuccess) {                                                             \
+      ::ffk::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                       __LINE__);                                                       \
+      return FFK_ERR_HIP;                                                               \
+    }                                                                                   \
+  } while (0)
+
+#define FFK_REQUIRE(cond, code, ...) \
+  do {                               \
+    if (!(cond)) {                   \
+      ::ffk::set_error(__VA_ARGS__); \
+      return (code);                 \
+    }                                \
+  } while (0)
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Device allocation that records failures as FFK_ERR_NOMEM.
+int dmalloc(void** p, size_t bytes);
+template <class T>
+int dalloc(T** p, size_t n) {
+  return dmalloc(reinterpret_cast<void**>(p), n * sizeof(T));
+}
+void dfree(void* p);
+
+// ------------------------------------------------------------ constants --
+// (double)x < 1e-8  <=>  x <= 9.99999993922529e-09f for every float x: the
+// reference compares float values against the double literal _FLT_EPSILON
+// (math/operations.h:31, math/triangle.cpp:38,45).
+constexpr float kEps8 = 9.99999993922529e-09f;
+
+// ---------------------------------------------------------- vector math --
+struct v3 {
+  float x, y, z;
+};
+__host__ __device__ inline v3 mk(float x, float y, float z) { return v3{x, y, z}; }
+__host__ __device__ inline v3 operator+(v3 a, v3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__host__ __device__ inline v3 operator-(v3 a, v3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__host__ __device__ inline v3 operator*(v3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__host__ __device__ inline float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline v3 cross(v3 a, v3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+
+// Per-face intersection terms (64 B, four float4):
+//   q0 = {p0, uu}  q1 = {u, uv}  q2 = {v, vv}  q3 = {n, D}
+// u = p1-p0, v = p2-p0, n = u x v, uu/uv/vv = dots, D = uv*uv - uu*vv, all
+// with the reference's expressions (math/triangle.cpp:35-38,57-63). A
+// degenerate face ((double)dot(n,n) < 1e-8) is stored with n = 0, which the
+// parallel-plane test then rejects for every ray, as the reference rejects it.
+struct tri_terms {
+  float4 q0, q1, q2, q3;
+};
+
+__host__ __device__ inline tri_terms make_terms(const float* f) {
+  v3 p0 = mk(f[0], f[1], f[2]), p1 = mk(f[3], f[4], f[5]), p2 = mk(f[6], f[7], f[8]);
+  v3 u = p1 - p0, v = p2 - p0, n = cross(u, v);
+  if (dot(n, n) <= kEps8) n = mk(0.f, 0.f, 0.f);
+  float uu = dot(u, u), uv = dot(u, v), vv = dot(v, v);
+  float D = uv * uv - uu * vv;
+  tri_terms t;
+  t.q0 = make_float4(p0.x, p0.y, p0.z, uu);
+  t.q1 = make_float4(u.x, u.y, u.z, uv);
+  t.q2 = make_float4(v.x, v.y, v.z, vv);
+  t.q3 = make_float4(n.x, n.y, n.z, D);
+  return t;
+}
+
+// The reference test (math/triangle.cpp:28-79) on precomputed terms, for a
+// ray with origin o and direction dir = end - origin. Returns true for a
+// unique hit and writes the point and its squared distance to o.
+__host__ __device__ inline bool hit_terms(const float4 q0, const float4 q1, const float4 q2,
+                                          const float4 q3, v3 o, v3 dir, v3* point, float* d2) {
+  v3 n = mk(q3.x, q3.y, q3.z);
+  v3 p0 = mk(q0.x, q0.y, q0.z);
+  v3 w0 = o - p0;
+  float a = -dot(n, w0);
+  float b = dot(n, dir);
+  if (fabsf(b) <= kEps8) return false;
+  float r = a / b;
+  if (r < 0.0f) return false;
+  v3 p = o + dir * r;
+  v3 w = p - p0;
+  v3 u = mk(q1.x, q1.y, q1.z), v = mk(q2.x, q2.y, q2.z);
+  float uu = q0.w, uv = q1.w, vv = q2.w, D = q3.w;
+  float wu = dot(w, u), wv = dot(w, v);
+  float s = (uv * wv - vv * wu) / D;
+  if (s < 0.0f || s > 1.0f) return false;
+  float t = (uv * wu - uu * wv) / D;
+  if (t < 0.0f || (s + t) > 1.0f) return false;
+  v3 sd = p - o;
+  *point = p;
+  *d2 = dot(sd, sd);
+  return true;
+}
+
+// rotate_towards (math/mat.cpp:25-49) with the element placement of the
+// make_mat33-based helpers in math/mat.h:64-97; m[row*3+col] = p[row][col].
+__host__ __device__ inline void rotate_towards(v3 s, v3 to, float* m) {
+  v3 v = cross(s, to);
+  float s2 = dot(v, v);
+  float c = dot(s, to);
+  float r[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+  if (s2 > 1.17549435e-38f /* FLT_MIN */) {
+    const float a[9] = {0, v.z, -v.y, -v.z, 0, v.x, v.y, -v.x, 0};
+#define A(i, j) a[(i)*3 + (j)]
+    float q[9] = {A(0, 1) * A(1, 0) + A(0, 2) * A(2, 0), A(1, 2) * A(2, 0), A(2, 1) * A(1, 0),
+                  A(0, 1) * A(1, 1) + A(0, 2) * A(2, 1), A(1, 0) * A(0, 1) + A(1, 2) * A(2, 1),
+                  A(2, 0) * A(0, 1), A(0, 1) * A(1, 2), A(1, 0) * A(0, 2),
+                  A(2, 0) * A(0, 2) + A(2, 1) * A(1, 2)};
+#undef A
+    float k = (1 - c) / s2;
+    float qs[9];  // (q * k) stored transposed, as operator*(mat33, float) builds it
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) qs[i * 3 + j] = q[j * 3 + i] * k;
+    float s1[9];  // (I + a) transposed, as operator+ builds it
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) s1[i * 3 + j] = r[j * 3 + i] + a[j * 3 + i];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) r[i * 3 + j] = s1[j * 3 + i] + qs[j * 3 + i];
+  } else {
+    v3 f = s + to;
+    float l = dot(f, f);
+    if (l < 1.1920929e-07f /* FLT_EPSILON */) {
+      r[4] = -1;
+      r[8] = -1;
+    }
+  }
+  for (int i = 0; i < 9; ++i) m[i] = r[i];
+}
+
+// mat33 * vec3 with the reference's column access (math/mat.h:30-37).
+__host__ __device__ inline v3 mat_vec(const float* m, v3 v) {
+  return mk((m[0] * v.x) + (m[3] * v.y) + (m[6] * v.z), (m[1] * v.x) + (m[4] * v.y) + (m[7] * v.z),
+            (m[2] * v.x) + (m[5] * v.y) + (m[8] * v.z));
+}
+
+// --------------------------------------------------- counter-based RNG --
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), written from the paper.
+struct u4 {
+  uint32_t x, y, z, w;
+};
+__host__ __device__ inline u4 philox(u4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
+    uint64_t p1 = (uint64_t)0xCD9E8D57u * c.z;
+    u4 n;
+    n.x = (uint32_t)(p1 >> 32) ^ c.y ^ k0;
+    n.y = (uint32_t)p1;
+    n.z = (uint32_t)(p0 >> 32) ^ c.w ^ k1;
+    n.w = (uint32_t)p0;
+    c = n;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+__host__ __device__ inline float u01(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-08f; }
+
+// sin and cos of 2*pi*u, u in [0,1): exact octant reduction, then Taylor
+// polynomials on [0, pi/4] in Horner form (truncation error < 2e-9).
+__host__ __device__ inline void sincos_turns(float u, float* s_out, float* c_out) {
+  float u8 = u * 8.0f;
+  int o = (int)u8;
+  float f = u8 - (float)o;
+  if (o & 1) f = 1.0f - f;
+  float x = f * 0.785398163397448f;
+  float x2 = x * x;
+  float sp = 1.0f + x2 * (-1.66666667e-1f +
+                          x2 * (8.33333333e-3f + x2 * (-1.98412698e-4f + x2 * 2.75573192e-6f)));
+  float s = x * sp;
+  float c = 1.0f + x2 * (-0.5f + x2 * (4.16666667e-2f +
+                                       x2 * (-1.38888889e-3f +
+                                             x2 * (2.48015873e-5f + x2 * -2.75573192e-7f))));
+  int k = ((o + 1) >> 1) & 3;
+  float bs = (o & 1) ? -s : s;
+  float bc = c;
+  float rs, rc;
+  if (k == 0) {
+    rs = bs; rc = bc;
+  } else if (k == 1) {
+    rs = bc; rc = -bs;
+  } else if (k == 2) {
+    rs = -bs; rc = -bc;
+  } else {
+    rs = -bc; rc = bs;
+  }
+  *s_out = rs;
+  *c_out = rc;
+}
+
+// ---------------------------------------------------------- scene state --
+// Grid over the scene AABB: cell (ix,iy,iz) -> c = ix + nx*(iy + ny*iz);
+// refs[cell_start[c] .. cell_start[c+1]) are face indices in ascending order.
+struct grid_desc {
+  int n[3];
+  float lo[3];
+  float hi[3];
+  float h[3];      // cell size
+  float inv_h[3];  // 1 / cell size
+};
+
+// Everything a kernel needs to emit, cast and tally one scene (POD; lives in
+// device memory for the batched kernel).
+struct scene_desc {
+  const tri_terms* tri;
+  const float* rot;            // 9 per face, emission frame
+  const long long* face_off;   // n_faces+1: first ray of each face
+  const int* front;            // front rays per face
+  const int* face2mesh;        // -1 for faces outside every mesh
+  const long long* mesh_off;   // n_meshes+1: mesh-order tally ranges
+  const int* cell_start;
+  const int* refs;
+  grid_desc g;
+  int n_faces;
+  int n_meshes;
+  long long n_rays;
+};
+
+// One unit of fused work: rays [g0, g1) tallied for mesh `mesh` of scene
+// `scene`. `direct` = the tile covers the whole mesh range, so the row of F
+// is written from the tile's histogram without the global tally.
+struct tile {
+  long long g0, g1;
+  int mesh;
+  int scene;
+  int direct;
+  int pad;
+};
+
+}  // namespace ffk
+
+struct ffk_scene {
+  int n_faces = 0, n_meshes = 0;
+  long long n_rays = 0;
+  int frame = FFK_FRAME_REFERENCE;
+  bool has_faces = false, has_meshes = false, has_emission = false, has_grid = false;
+  // device buffers
+  float* d_faces = nullptr;
+  ffk::tri_terms* d_tri = nullptr;
+  float* d_rot = nullptr;
+  int* d_face2mesh = nullptr;
+  long long* d_face_off = nullptr;
+  int* d_front = nullptr;
+  long long* d_mesh_off = nullptr;
+  int* d_cell_start = nullptr;
+  int* d_refs = nullptr;
+  long long n_refs = 0;
+  ffk::grid_desc grid = {};
+  // fused-kernel work list
+  ffk::tile* d_tiles = nullptr;
+  int n_tiles = 0;
+  int n_direct_rows = 0;
+  int* d_indirect_rows = nullptr;  // meshes tallied through the global tally
+  int n_indirect_rows = 0;
+  ffk::scene_desc* d_desc = nullptr;
+  // host copies
+  std::vector<float> h_faces;
+  std::vector<int> h_meshes, h_front, h_rear;
+  std::vector<long long> h_face_off, h_mesh_off;
+  // tuning (0 = automatic)
+  int tune_block = 0, tune_rays_per_tile = 0, tune_hist = -1;
+  int tiles_rays_per_tile = 0;  // value the tile list was built with
+
+  ffk::scene_desc desc() const;
+};
+
+namespace ffk {
+int build_tiles(ffk_scene* s, int rays_per_tile, hipStream_t st);
+int device_cu_count();
+}  // namespace ffk

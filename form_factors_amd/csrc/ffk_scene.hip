@@ -1,0 +1,507 @@
+// Scene state on the device: per-face intersection terms, face->mesh map,
+// emission layout, fused-work tiles and the uniform grid built on the GPU.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "ffk_internal.hpp"
+
+namespace ffk {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+}
+
+int dmalloc(void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    (void)hipGetLastError();
+    set_error("hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return FFK_ERR_NOMEM;
+  }
+  return FFK_OK;
+}
+
+void dfree(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+int device_cu_count() {
+  static int cached = 0;
+  if (!cached) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      cached = prop.multiProcessorCount;
+    else
+      cached = 256;
+  }
+  return cached;
+}
+
+// ------------------------------------------------------------- kernels --
+
+__global__ void k_face_terms(const float* __restrict__ faces, int n, tri_terms* __restrict__ out) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  out[f] = make_terms(faces + 9 * (size_t)f);
+}
+
+// Emission frame per face: rotate_towards(z, n) with the unnormalised normal
+// (emission/malley_cpu.cpp:125-129) or with n / |n|.
+__global__ void k_face_rotation(const float* __restrict__ faces, int n, int frame,
+                                float* __restrict__ rot) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  const float* p = faces + 9 * (size_t)f;
+  v3 p0 = mk(p[0], p[1], p[2]), p1 = mk(p[3], p[4], p[5]), p2 = mk(p[6], p[7], p[8]);
+  v3 nn = cross(p1 - p0, p2 - p0);
+  if (frame == FFK_FRAME_ORTHONORMAL) {
+    float len = sqrtf(dot(nn, nn));
+    nn = mk(nn.x / len, nn.y / len, nn.z / len);
+  }
+  rotate_towards(mk(0.f, 0.f, 1.f), nn, rot + 9 * (size_t)f);
+}
+
+// Order-preserving float <-> uint map for atomic min/max of floats.
+__device__ inline uint32_t ord(float f) {
+  uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__host__ inline float unord(uint32_t u) {
+  uint32_t b = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+  float f;
+  memcpy(&f, &b, 4);
+  return f;
+}
+
+__global__ void k_bounds(const float* __restrict__ faces, int n, uint32_t* __restrict__ box) {
+  __shared__ uint32_t smin[3], smax[3];
+  if (threadIdx.x < 3) {
+    smin[threadIdx.x] = 0xffffffffu;
+    smax[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f < n) {
+    const float* p = faces + 9 * (size_t)f;
+    for (int a = 0; a < 3; ++a) {
+      float lo = fminf(fminf(p[a], p[3 + a]), p[6 + a]);
+      float hi = fmaxf(fmaxf(p[a], p[3 + a]), p[6 + a]);
+      atomicMin(&smin[a], ord(lo));
+      atomicMax(&smax[a], ord(hi));
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    atomicMin(&box[threadIdx.x], smin[threadIdx.x]);
+    atomicMax(&box[3 + threadIdx.x], smax[threadIdx.x]);
+  }
+}
+
+// Conservative cell range of a face: its AABB grown by `pad` on every side.
+__device__ inline void cell_range(const float* p, const grid_desc& g, float pad, int* c0, int* c1) {
+  for (int a = 0; a < 3; ++a) {
+    float lo = fminf(fminf(p[a], p[3 + a]), p[6 + a]) - pad;
+    float hi = fmaxf(fmaxf(p[a], p[3 + a]), p[6 + a]) + pad;
+    int i0 = (int)floorf((lo - g.lo[a]) * g.inv_h[a]);
+    int i1 = (int)floorf((hi - g.lo[a]) * g.inv_h[a]);
+    c0[a] = min(max(i0, 0), g.n[a] - 1);
+    c1[a] = min(max(i1, 0), g.n[a] - 1);
+  }
+}
+
+__global__ void k_ref_count(const float* __restrict__ faces, int n, grid_desc g, float pad,
+                            uint32_t* __restrict__ cnt) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int c0[3], c1[3];
+  cell_range(faces + 9 * (size_t)f, g, pad, c0, c1);
+  cnt[f] = (uint32_t)((c1[0] - c0[0] + 1) * (c1[1] - c0[1] + 1) * (c1[2] - c0[2] + 1));
+}
+
+__global__ void k_ref_emit(const float* __restrict__ faces, int n, grid_desc g, float pad,
+                           const uint32_t* __restrict__ off, uint32_t* __restrict__ keys,
+                           int* __restrict__ vals) {
+  int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= n) return;
+  int c0[3], c1[3];
+  cell_range(faces + 9 * (size_t)f, g, pad, c0, c1);
+  uint32_t o = off[f];
+  for (int z = c0[2]; z <= c1[2]; ++z)
+    for (int y = c0[1]; y <= c1[1]; ++y)
+      for (int x = c0[0]; x <= c1[0]; ++x) {
+        keys[o] = (uint32_t)(x + g.n[0] * (y + g.n[1] * z));
+        vals[o] = f;
+        ++o;
+      }
+}
+
+__global__ void k_cell_count(const uint32_t* __restrict__ keys, long long n, int* __restrict__ cnt) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(&cnt[keys[i]], 1);
+}
+
+}  // namespace ffk
+
+using namespace ffk;
+
+ffk::scene_desc ffk_scene::desc() const {
+  scene_desc d;
+  d.tri = d_tri;
+  d.rot = d_rot;
+  d.face_off = d_face_off;
+  d.front = d_front;
+  d.face2mesh = d_face2mesh;
+  d.mesh_off = d_mesh_off;
+  d.cell_start = d_cell_start;
+  d.refs = d_refs;
+  d.g = grid;
+  d.n_faces = n_faces;
+  d.n_meshes = n_meshes;
+  d.n_rays = n_rays;
+  return d;
+}
+
+static void free_grid(ffk_scene* s) {
+  dfree(s->d_cell_start);
+  dfree(s->d_refs);
+  s->d_cell_start = nullptr;
+  s->d_refs = nullptr;
+  s->n_refs = 0;
+  s->has_grid = false;
+}
+
+static void free_emission(ffk_scene* s) {
+  dfree(s->d_rot);
+  dfree(s->d_face_off);
+  dfree(s->d_front);
+  dfree(s->d_mesh_off);
+  dfree(s->d_tiles);
+  dfree(s->d_indirect_rows);
+  s->d_rot = nullptr;
+  s->d_face_off = nullptr;
+  s->d_front = nullptr;
+  s->d_mesh_off = nullptr;
+  s->d_tiles = nullptr;
+  s->d_indirect_rows = nullptr;
+  s->n_tiles = 0;
+  s->has_emission = false;
+}
+
+extern "C" {
+
+const char* ffk_last_error(void) { return g_err; }
+
+int ffk_device_info(int* n_cu, int* clock_khz, int* mem_clock_khz, int* bus_width_bits,
+                    int* l2_bytes, char* name, int name_len) {
+  int dev = 0;
+  FFK_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t p;
+  FFK_HIP(hipGetDeviceProperties(&p, dev));
+  if (n_cu) *n_cu = p.multiProcessorCount;
+  if (clock_khz) *clock_khz = p.clockRate;
+  if (mem_clock_khz) *mem_clock_khz = p.memoryClockRate;
+  if (bus_width_bits) *bus_width_bits = p.memoryBusWidth;
+  if (l2_bytes) *l2_bytes = p.l2CacheSize;
+  if (name && name_len > 0) {
+    snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+  }
+  return FFK_OK;
+}
+
+int ffk_scene_create(ffk_scene** out) {
+  FFK_REQUIRE(out, FFK_ERR_ARG, "ffk_scene_create: out is NULL");
+  *out = new ffk_scene();
+  return FFK_OK;
+}
+
+void ffk_scene_destroy(ffk_scene* s) {
+  if (!s) return;
+  (void)hipDeviceSynchronize();
+  free_grid(s);
+  free_emission(s);
+  dfree(s->d_faces);
+  dfree(s->d_tri);
+  dfree(s->d_face2mesh);
+  dfree(s->d_desc);
+  delete s;
+}
+
+int ffk_scene_set_faces(ffk_scene* s, const float* h_faces, int n_faces, void* stream) {
+  FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
+  FFK_REQUIRE(n_faces > 0 && h_faces, FFK_ERR_EMPTY, "ffk_scene_set_faces: empty scene");
+  hipStream_t st = as_stream(stream);
+  FFK_HIP(hipStreamSynchronize(st));
+  free_grid(s);
+  free_emission(s);
+  dfree(s->d_faces);
+  dfree(s->d_tri);
+  dfree(s->d_face2mesh);
+  s->d_faces = nullptr;
+  s->d_tri = nullptr;
+  s->d_face2mesh = nullptr;
+  s->has_meshes = false;
+  s->n_faces = n_faces;
+  s->h_faces.assign(h_faces, h_faces + 9 * (size_t)n_faces);
+  int r;
+  if ((r = dalloc(&s->d_faces, 9 * (size_t)n_faces)) < 0) return r;
+  if ((r = dalloc(&s->d_tri, (size_t)n_faces)) < 0) return r;
+  FFK_HIP(hipMemcpyAsync(s->d_faces, h_faces, 36 * (size_t)n_faces, hipMemcpyHostToDevice, st));
+  k_face_terms<<<(n_faces + 255) / 256, 256, 0, st>>>(s->d_faces, n_faces, s->d_tri);
+  FFK_HIP(hipGetLastError());
+  s->has_faces = true;
+  return FFK_OK;
+}
+
+int ffk_scene_set_meshes(ffk_scene* s, const int* h_meshes, int n_meshes, void* stream) {
+  FFK_REQUIRE(s && s->has_faces, FFK_ERR_ARG, "ffk_scene_set_meshes: faces not set");
+  FFK_REQUIRE(n_meshes > 0 && h_meshes, FFK_ERR_EMPTY, "ffk_scene_set_meshes: no meshes");
+  hipStream_t st = as_stream(stream);
+  std::vector<int> f2m(s->n_faces, -1);
+  for (int m = 0; m < n_meshes; ++m) {
+    int first = h_meshes[3 * m], cnt = h_meshes[3 * m + 1];
+    FFK_REQUIRE(first >= 0 && cnt >= 0 && (long long)first + cnt <= s->n_faces, FFK_ERR_ARG,
+                "mesh %d (first %d, n %d) outside %d faces", m, first, cnt, s->n_faces);
+    for (int f = 0; f < cnt; ++f) f2m[first + f] = m;
+  }
+  FFK_HIP(hipStreamSynchronize(st));
+  free_emission(s);
+  dfree(s->d_face2mesh);
+  s->d_face2mesh = nullptr;
+  int r;
+  if ((r = dalloc(&s->d_face2mesh, (size_t)s->n_faces)) < 0) return r;
+  FFK_HIP(hipMemcpy(s->d_face2mesh, f2m.data(), 4 * (size_t)s->n_faces, hipMemcpyHostToDevice));
+  s->h_meshes.assign(h_meshes, h_meshes + 3 * (size_t)n_meshes);
+  s->n_meshes = n_meshes;
+  s->has_meshes = true;
+  return FFK_OK;
+}
+
+int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, int frame,
+                           long long* n_rays_out, void* stream) {
+  FFK_REQUIRE(s && s->has_faces && s->has_meshes, FFK_ERR_ARG,
+              "ffk_scene_set_emission: faces and meshes must be set first");
+  FFK_REQUIRE(frame == FFK_FRAME_REFERENCE || frame == FFK_FRAME_ORTHONORMAL, FFK_ERR_ARG,
+              "unknown frame %d", frame);
+  hipStream_t st = as_stream(stream);
+  FFK_HIP(hipStreamSynchronize(st));
+  free_emission(s);
+  const int nf = s->n_faces, nm = s->n_meshes;
+  s->h_front.assign(h_front, h_front + nf);
+  s->h_rear.assign(h_rear, h_rear + nf);
+  // Face-order layout: face f owns rays [face_off[f], face_off[f+1]), front
+  // block first (emission/malley_cpu.cpp:150-186).
+  s->h_face_off.assign(nf + 1, 0);
+  for (int f = 0; f < nf; ++f) {
+    FFK_REQUIRE(h_front[f] >= 0 && h_rear[f] >= 0, FFK_ERR_ARG, "negative ray count at face %d", f);
+    s->h_face_off[f + 1] = s->h_face_off[f] + h_front[f] + h_rear[f];
+  }
+  const long long total = s->h_face_off[nf];
+  // Mesh-order walk of the tally (cpu_system.cpp:165-191): mesh m consumes
+  // the next sum of its faces' ray counts, capped at the total.
+  s->h_mesh_off.assign(nm + 1, 0);
+  for (int m = 0; m < nm; ++m) {
+    long long c = 0;
+    for (int f = 0; f < s->h_meshes[3 * m + 1]; ++f) {
+      int fi = s->h_meshes[3 * m] + f;
+      c += h_front[fi] + h_rear[fi];
+    }
+    s->h_mesh_off[m + 1] = std::min(total, s->h_mesh_off[m] + c);
+  }
+  s->n_rays = total;
+  s->frame = frame;
+  int r;
+  if ((r = dalloc(&s->d_rot, 9 * (size_t)nf)) < 0) return r;
+  if ((r = dalloc(&s->d_face_off, (size_t)nf + 1)) < 0) return r;
+  if ((r = dalloc(&s->d_front, (size_t)nf)) < 0) return r;
+  if ((r = dalloc(&s->d_mesh_off, (size_t)nm + 1)) < 0) return r;
+  FFK_HIP(hipMemcpy(s->d_face_off, s->h_face_off.data(), 8 * ((size_t)nf + 1), hipMemcpyHostToDevice));
+  FFK_HIP(hipMemcpy(s->d_front, h_front, 4 * (size_t)nf, hipMemcpyHostToDevice));
+  FFK_HIP(hipMemcpy(s->d_mesh_off, s->h_mesh_off.data(), 8 * ((size_t)nm + 1), hipMemcpyHostToDevice));
+  k_face_rotation<<<(nf + 255) / 256, 256, 0, st>>>(s->d_faces, nf, frame, s->d_rot);
+  FFK_HIP(hipGetLastError());
+  s->has_emission = true;
+  if (n_rays_out) *n_rays_out = total;
+  return ffk::build_tiles(s, s->tune_rays_per_tile, st);
+}
+
+int ffk_scene_counts(const ffk_scene* s, int* n_faces, int* n_meshes, long long* n_rays) {
+  FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
+  if (n_faces) *n_faces = s->n_faces;
+  if (n_meshes) *n_meshes = s->n_meshes;
+  if (n_rays) *n_rays = s->n_rays;
+  return FFK_OK;
+}
+
+int ffk_scene_build_grid(ffk_scene* s, int nx, int ny, int nz, void* stream) {
+  FFK_REQUIRE(s && s->has_faces, FFK_ERR_ARG, "ffk_scene_build_grid: faces not set");
+  FFK_REQUIRE(nx >= 0 && ny >= 0 && nz >= 0 && nx <= 1024 && ny <= 1024 && nz <= 1024, FFK_ERR_LIMIT,
+              "grid dims %dx%dx%d outside [0,1024]", nx, ny, nz);
+  hipStream_t st = as_stream(stream);
+  const int nf = s->n_faces;
+  free_grid(s);
+  int r;
+  // 1. bounds on the device
+  uint32_t* d_box = nullptr;
+  if ((r = dalloc(&d_box, 6)) < 0) return r;
+  uint32_t init[6] = {0xffffffffu, 0xffffffffu, 0xffffffffu, 0, 0, 0};
+  FFK_HIP(hipMemcpyAsync(d_box, init, sizeof(init), hipMemcpyHostToDevice, st));
+  k_bounds<<<(nf + 255) / 256, 256, 0, st>>>(s->d_faces, nf, d_box);
+  FFK_HIP(hipGetLastError());
+  uint32_t box[6];
+  FFK_HIP(hipMemcpyAsync(box, d_box, sizeof(box), hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipStreamSynchronize(st));
+  dfree(d_box);
+  float lo[3], hi[3], ext[3];
+  float maxext = 0.f;
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = unord(box[a]);
+    hi[a] = unord(box[3 + a]);
+    ext[a] = hi[a] - lo[a];
+    maxext = std::max(maxext, ext[a]);
+  }
+  FFK_REQUIRE(std::isfinite(maxext), FFK_ERR_ARG, "non-finite scene bounds");
+  if (maxext <= 0.f) maxext = 1.f;
+  // Grow the box so that every vertex and every hit point computed on a face
+  // lies strictly inside; flat scenes get a thickness.
+  const float grow = 1e-4f * maxext;
+  for (int a = 0; a < 3; ++a) {
+    lo[a] -= grow;
+    hi[a] += grow;
+    ext[a] = hi[a] - lo[a];
+  }
+  int dims[3] = {nx, ny, nz};
+  if (nx == 0 || ny == 0 || nz == 0) {
+    // About two cells per face, cells as close to cubes as the box allows.
+    double vol = (double)ext[0] * ext[1] * ext[2];
+    double target = 2.0 * nf;
+    double cell = std::cbrt(vol / target);
+    for (int a = 0; a < 3; ++a)
+      dims[a] = (int)std::min(512.0, std::max(1.0, std::ceil(ext[a] / cell)));
+  }
+  grid_desc g;
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = dims[a];
+    g.lo[a] = lo[a];
+    g.hi[a] = hi[a];
+    g.h[a] = ext[a] / (float)dims[a];
+    g.inv_h[a] = (float)dims[a] / ext[a];
+  }
+  const float pad = 1e-3f * std::min(g.h[0], std::min(g.h[1], g.h[2])) + 1e-6f * maxext;
+  const long long nc = (long long)dims[0] * dims[1] * dims[2];
+  FFK_REQUIRE(nc < (1LL << 31), FFK_ERR_LIMIT, "grid has %lld cells", nc);
+  // 2. per-face reference counts and their exclusive scan
+  uint32_t *d_cnt = nullptr, *d_off = nullptr;
+  if ((r = dalloc(&d_cnt, (size_t)nf + 1)) < 0) return r;
+  if ((r = dalloc(&d_off, (size_t)nf + 1)) < 0) return r;
+  FFK_HIP(hipMemsetAsync(d_cnt, 0, 4 * ((size_t)nf + 1), st));
+  k_ref_count<<<(nf + 255) / 256, 256, 0, st>>>(s->d_faces, nf, g, pad, d_cnt);
+  FFK_HIP(hipGetLastError());
+  size_t tmp_bytes = 0;
+  FFK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp_bytes, d_cnt, d_off, nf + 1, st));
+  void* d_tmp = nullptr;
+  if ((r = dmalloc(&d_tmp, tmp_bytes)) < 0) return r;
+  FFK_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp, tmp_bytes, d_cnt, d_off, nf + 1, st));
+  uint32_t n_refs = 0;
+  FFK_HIP(hipMemcpyAsync(&n_refs, d_off + nf, 4, hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipStreamSynchronize(st));
+  dfree(d_tmp);
+  // 3. (cell, face) pairs in face order, stably sorted by cell
+  uint32_t *d_keys = nullptr, *d_keys2 = nullptr;
+  int* d_vals = nullptr;
+  if ((r = dalloc(&d_keys, n_refs)) < 0) return r;
+  if ((r = dalloc(&d_keys2, n_refs)) < 0) return r;
+  if ((r = dalloc(&d_vals, n_refs)) < 0) return r;
+  if ((r = dalloc(&s->d_refs, n_refs)) < 0) return r;
+  k_ref_emit<<<(nf + 255) / 256, 256, 0, st>>>(s->d_faces, nf, g, pad, d_off, d_keys, d_vals);
+  FFK_HIP(hipGetLastError());
+  int end_bit = 1;
+  while ((1LL << end_bit) < nc) ++end_bit;
+  tmp_bytes = 0;
+  FFK_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, d_keys, d_keys2, d_vals, s->d_refs,
+                                             (int)n_refs, 0, end_bit, st));
+  if ((r = dmalloc(&d_tmp, tmp_bytes)) < 0) return r;
+  FFK_HIP(hipcub::DeviceRadixSort::SortPairs(d_tmp, tmp_bytes, d_keys, d_keys2, d_vals, s->d_refs,
+                                             (int)n_refs, 0, end_bit, st));
+  // 4. cell starts = exclusive scan of per-cell counts
+  int* d_ccnt = nullptr;
+  if ((r = dalloc(&d_ccnt, (size_t)nc + 1)) < 0) return r;
+  if ((r = dalloc(&s->d_cell_start, (size_t)nc + 1)) < 0) return r;
+  FFK_HIP(hipMemsetAsync(d_ccnt, 0, 4 * ((size_t)nc + 1), st));
+  if (n_refs) {
+    k_cell_count<<<(unsigned)((n_refs + 255) / 256), 256, 0, st>>>(d_keys2, n_refs, d_ccnt);
+    FFK_HIP(hipGetLastError());
+  }
+  size_t tmp2 = 0;
+  FFK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp2, d_ccnt, s->d_cell_start, nc + 1, st));
+  void* d_tmp2 = nullptr;
+  if ((r = dmalloc(&d_tmp2, tmp2)) < 0) return r;
+  FFK_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp2, tmp2, d_ccnt, s->d_cell_start, nc + 1, st));
+  FFK_HIP(hipStreamSynchronize(st));
+  dfree(d_tmp);
+  dfree(d_tmp2);
+  dfree(d_ccnt);
+  dfree(d_keys);
+  dfree(d_keys2);
+  dfree(d_vals);
+  dfree(d_cnt);
+  dfree(d_off);
+  s->grid = g;
+  s->n_refs = n_refs;
+  s->has_grid = true;
+  return FFK_OK;
+}
+
+int ffk_scene_grid_info(const ffk_scene* s, int* dims3, float* lo3, float* hi3, long long* n_refs) {
+  FFK_REQUIRE(s && s->has_grid, FFK_ERR_ARG, "grid not built");
+  for (int a = 0; a < 3; ++a) {
+    if (dims3) dims3[a] = s->grid.n[a];
+    if (lo3) lo3[a] = s->grid.lo[a];
+    if (hi3) hi3[a] = s->grid.hi[a];
+  }
+  if (n_refs) *n_refs = s->n_refs;
+  return FFK_OK;
+}
+
+int ffk_area_weights(const float* h_faces, int n_faces, float* h_weights, float* total) {
+  FFK_REQUIRE(h_faces && h_weights && total && n_faces >= 0, FFK_ERR_ARG, "ffk_area_weights: bad args");
+  float t = 0;
+  for (int i = 0; i < n_faces; ++i) {
+    const float* p = h_faces + 9 * (size_t)i;
+    v3 u = mk(p[3], p[4], p[5]) - mk(p[0], p[1], p[2]);
+    v3 w = mk(p[6], p[7], p[8]) - mk(p[0], p[1], p[2]);
+    v3 n = cross(u, w);
+    float a = sqrtf(dot(n, n)) / 2;  // math/triangle.cpp:86-92
+    t += 2.f * a;
+    h_weights[2 * i] = a;
+    h_weights[2 * i + 1] = a;
+  }
+  *total = t;
+  return FFK_OK;
+}
+
+int ffk_emitted_counts(const float* w, int n_faces, int n_rays, float total, int* front, int* rear) {
+  FFK_REQUIRE(w && front && rear && n_faces >= 0, FFK_ERR_ARG, "ffk_emitted_counts: bad args");
+  for (int f = 0; f < n_faces; ++f) {
+    float a = w[2 * f], b = w[2 * f + 1];
+    front[f] = a > 0 ? std::max<int>(1, (int)(n_rays * (a / total))) : 0;
+    rear[f] = b > 0 ? std::max<int>(1, (int)(n_rays * (b / total))) : 0;
+  }
+  return FFK_OK;
+}
+
+}  // extern "C"

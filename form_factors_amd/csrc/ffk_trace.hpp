@@ -1,0 +1,161 @@
+// Device functions for emission and nearest-hit casting (gfx950).
+#pragma once
+
+#include "ffk_internal.hpp"
+
+namespace ffk {
+
+struct hit_rec {
+  float d2;
+  int f;
+  v3 p;
+};
+
+// Nearest-hit update with the naive caster's order: strict "<" on the
+// squared distance, equal distances resolved to the lower face index
+// (ray_caster/naive_cpu.cpp:82-92 visits faces in ascending order).
+__device__ inline void consider(hit_rec& h, int f, float d2, v3 p) {
+  if (d2 < h.d2 || (d2 == h.d2 && f < h.f)) {
+    h.d2 = d2;
+    h.f = f;
+    h.p = p;
+  }
+}
+
+__device__ inline void test_face(const tri_terms* __restrict__ tri, int f, v3 o, v3 dir, hit_rec& h) {
+  const float4* q = reinterpret_cast<const float4*>(tri + f);
+  float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+  v3 p;
+  float d2;
+  if (hit_terms(q0, q1, q2, q3, o, dir, &p, &d2)) consider(h, f, d2, p);
+}
+
+// Walk the uniform grid front to back (3-D DDA). Every face whose
+// (padded) box overlaps a visited cell is tested with the reference test;
+// the walk stops once the best squared distance is below the squared
+// distance to the current cell's exit (with a 1e-4 relative margin, far
+// above the rounding of both), so the result equals testing every face.
+template <bool STATS>
+__device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, uint32_t& n_tests,
+                                 uint32_t& n_cells) {
+  const grid_desc& g = S.g;
+  // Clip the ray to the grid box (slabs); t is in units of dir.
+  float t0 = 0.f, t1 = 3.4e38f;
+  float ix = 0.f, iy = 0.f, iz = 0.f;
+  if (dir.x != 0.f) {
+    ix = 1.0f / dir.x;
+    float ta = (g.lo[0] - o.x) * ix, tb = (g.hi[0] - o.x) * ix;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.x < g.lo[0] || o.x > g.hi[0]) {
+    return;
+  }
+  if (dir.y != 0.f) {
+    iy = 1.0f / dir.y;
+    float ta = (g.lo[1] - o.y) * iy, tb = (g.hi[1] - o.y) * iy;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.y < g.lo[1] || o.y > g.hi[1]) {
+    return;
+  }
+  if (dir.z != 0.f) {
+    iz = 1.0f / dir.z;
+    float ta = (g.lo[2] - o.z) * iz, tb = (g.hi[2] - o.z) * iz;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.z < g.lo[2] || o.z > g.hi[2]) {
+    return;
+  }
+  if (!(t0 <= t1)) return;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  // Start cell, per-axis step, next boundary (tn) and boundary spacing (td).
+  int cx = min(max((int)floorf((o.x + dir.x * t0 - g.lo[0]) * g.inv_h[0]), 0), nx - 1);
+  int cy = min(max((int)floorf((o.y + dir.y * t0 - g.lo[1]) * g.inv_h[1]), 0), ny - 1);
+  int cz = min(max((int)floorf((o.z + dir.z * t0 - g.lo[2]) * g.inv_h[2]), 0), nz - 1);
+  const int sx = dir.x > 0.f ? 1 : (dir.x < 0.f ? -1 : 0);
+  const int sy = dir.y > 0.f ? 1 : (dir.y < 0.f ? -1 : 0);
+  const int sz = dir.z > 0.f ? 1 : (dir.z < 0.f ? -1 : 0);
+  float tnx = sx ? (g.lo[0] + (float)(cx + (sx > 0)) * g.h[0] - o.x) * ix : 3.4e38f;
+  float tny = sy ? (g.lo[1] + (float)(cy + (sy > 0)) * g.h[1] - o.y) * iy : 3.4e38f;
+  float tnz = sz ? (g.lo[2] + (float)(cz + (sz > 0)) * g.h[2] - o.z) * iz : 3.4e38f;
+  const float tdx = sx ? fabsf(g.h[0] * ix) : 0.f;
+  const float tdy = sy ? fabsf(g.h[1] * iy) : 0.f;
+  const float tdz = sz ? fabsf(g.h[2] * iz) : 0.f;
+  // Distances along the ray are compared as (t * |dir|)^2 so that neither
+  // factor overflows for very short or very long direction vectors.
+  const float dlen = sqrtf(dot(dir, dir));
+  const int nxy = nx * ny;
+  int cell = cx + nx * cy + nxy * cz;
+  while (true) {
+    const int k0 = S.cell_start[cell], k1 = S.cell_start[cell + 1];
+    if (STATS) {
+      n_cells += 1;
+      n_tests += (uint32_t)(k1 - k0);
+    }
+    for (int k = k0; k < k1; ++k) test_face(S.tri, S.refs[k], o, dir, h);
+    const bool ax = (tnx <= tny) && (tnx <= tnz);
+    const bool ay = !ax && (tny <= tnz);
+    const float t_exit = ax ? tnx : (ay ? tny : tnz);
+    const float de = t_exit * dlen;
+    if (h.f >= 0 && h.d2 < de * de * 0.9999f) break;
+    const int st = ax ? sx : (ay ? sy : sz);
+    if (st == 0) break;  // zero or NaN direction: the start cell was all
+    if (ax) {
+      cx += sx;
+      if (cx < 0 || cx >= nx) break;
+      cell += sx;
+      tnx += tdx;
+    } else if (ay) {
+      cy += sy;
+      if (cy < 0 || cy >= ny) break;
+      cell += sy * nx;
+      tny += tdy;
+    } else {
+      cz += sz;
+      if (cz < 0 || cz >= nz) break;
+      cell += sz * nxy;
+      tnz += tdz;
+    }
+  }
+}
+
+// First face f with face_off[f+1] > g, starting the search at `hint`.
+__device__ inline int find_face(const long long* __restrict__ off, int n_faces, long long g, int hint) {
+  if (off[hint] <= g && g < off[hint + 1]) return hint;
+  int lo = (off[hint] <= g) ? hint + 1 : 0;
+  int hi = (off[hint] <= g) ? n_faces - 1 : hint;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (off[mid + 1] > g) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
+// Malley ray j of face f (emission/malley_cpu.cpp:109-186): uniform point on
+// the face, cosine-weighted direction about the face frame, rear rays with
+// the z component negated, origin pushed 1e-4 along the direction. The four
+// uniforms come from one Philox4x32-10 block, counter (j, f, 0, 0).
+__device__ inline void malley_ray(const scene_desc& S, int f, int j, uint32_t k0, uint32_t k1,
+                                  v3* origin_out, v3* end_out) {
+  u4 r = philox(u4{(uint32_t)j, (uint32_t)f, 0u, 0u}, k0, k1);
+  float a = u01(r.x), b = u01(r.y), rr = u01(r.z);
+  float sphi, cphi;
+  sincos_turns(u01(r.w), &sphi, &cphi);
+  if (a + b > 1) {
+    a = 1.f - a;
+    b = 1.f - b;
+  }
+  const float4* q = reinterpret_cast<const float4*>(S.tri + f);
+  float4 q0 = q[0], q1 = q[1], q2 = q[2];
+  v3 p0 = mk(q0.x, q0.y, q0.z), v0 = mk(q1.x, q1.y, q1.z), v1 = mk(q2.x, q2.y, q2.z);
+  v3 origin = (p0 + v0 * a) + v1 * b;
+  float rad = sqrtf(rr);
+  v3 m = mk(rad * cphi, rad * sphi, sqrtf(1 - rr));
+  if (j >= S.front[f]) m.z = -m.z;
+  v3 d = mat_vec(S.rot + 9 * (size_t)f, m);
+  *origin_out = origin + d * 0.0001f;
+  *end_out = origin + d;
+}
+
+}  // namespace ffk

@@ -1,0 +1,314 @@
+"""ctypes bindings of libff_amd.so (include/ffk.h) and a thin host-side API.
+
+The shared library is the product: every compute call below goes to a HIP
+kernel on the GPU. There is no CPU fallback; if the library is missing or no
+GPU is present, the calls raise.
+
+Device buffers are torch tensors on the current HIP device (torch supplies
+memory and the stream; it does no compute here).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libff_amd.so")
+
+CAST_NAIVE, CAST_GRID = 0, 1
+FRAME_REFERENCE, FRAME_ORTHONORMAL = 0, 1
+F32, BF16, F16 = 0, 1, 2
+
+_vp = C.c_void_p
+_i = C.POINTER(C.c_int)
+_f = C.POINTER(C.c_float)
+_ll = C.c_longlong
+_ull = C.c_ulonglong
+
+# name -> (restype, argtypes); every function declared in include/ffk.h
+SIGNATURES = {
+    "ffk_last_error": (C.c_char_p, []),
+    "ffk_device_info": (C.c_int, [_i, _i, _i, _i, _i, C.c_char_p, C.c_int]),
+    "ffk_scene_create": (C.c_int, [C.POINTER(_vp)]),
+    "ffk_scene_destroy": (None, [_vp]),
+    "ffk_scene_set_faces": (C.c_int, [_vp, _f, C.c_int, _vp]),
+    "ffk_scene_set_meshes": (C.c_int, [_vp, _i, C.c_int, _vp]),
+    "ffk_scene_set_emission": (C.c_int, [_vp, _i, _i, C.c_int, C.POINTER(_ll), _vp]),
+    "ffk_scene_build_grid": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp]),
+    "ffk_scene_grid_info": (C.c_int, [_vp, _i, _f, _f, C.POINTER(_ll)]),
+    "ffk_scene_counts": (C.c_int, [_vp, _i, _i, C.POINTER(_ll)]),
+    "ffk_cast": (C.c_int, [_vp, C.c_int, _vp, _ll, _vp, _vp, _vp, _vp]),
+    "ffk_emitted_counts": (C.c_int, [_f, C.c_int, C.c_int, C.c_float, _i, _i]),
+    "ffk_area_weights": (C.c_int, [_f, C.c_int, _f, _f]),
+    "ffk_emit_malley": (C.c_int, [_vp, _ull, _vp, _vp]),
+    "ffk_emit_parallel": (C.c_int, [_f, _f, C.c_float, C.c_float, _ll, _ull, _vp, _vp]),
+    "ffk_tally_zero": (C.c_int, [_vp, _vp, _vp]),
+    "ffk_tally": (C.c_int, [_vp, _vp, _ll, _vp, _vp]),
+    "ffk_normalise": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "ffk_ff_fused": (C.c_int, [_vp, C.c_int, _ull, _vp, _vp, _vp, _vp]),
+    "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ffk_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp]),
+    "ffk_batch_destroy": (None, [_vp]),
+    "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
+    "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ffk_convert_f": (C.c_int, [_vp, _ll, C.c_int, _vp, _vp]),
+    "ffk_sbff": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_conduction": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_adams_step": (C.c_int, [C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+}
+
+_lib = None
+
+
+class FFKError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libff_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FFKError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what=""):
+    if rc < 0:
+        msg = lib().ffk_last_error().decode(errors="replace")
+        raise FFKError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+def _np(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+def _ptr(t):
+    """Device pointer of a torch tensor (or None)."""
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(stream=None):
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def require_gpu():
+    import torch
+
+    if not torch.cuda.is_available():
+        raise FFKError("no HIP device: the form-factor kernels run on the GPU only")
+
+
+# ------------------------------------------------------------ host helpers --
+
+def area_weights(faces):
+    """Reference two-sided area weights and float total (host arithmetic)."""
+    faces = _np(faces, np.float32).reshape(-1, 9)
+    w = np.zeros(2 * len(faces), np.float32)
+    total = C.c_float()
+    check(lib().ffk_area_weights(faces.ctypes.data_as(_f), len(faces), w.ctypes.data_as(_f),
+                                 C.byref(total)), "ffk_area_weights")
+    return w, np.float32(total.value)
+
+
+def emitted_counts(weights, n_rays, total):
+    w = _np(weights, np.float32)
+    nf = len(w) // 2
+    fr = np.zeros(nf, np.int32)
+    rr = np.zeros(nf, np.int32)
+    check(lib().ffk_emitted_counts(w.ctypes.data_as(_f), nf, int(n_rays), float(total),
+                                   fr.ctypes.data_as(_i), rr.ctypes.data_as(_i)), "ffk_emitted_counts")
+    return fr, rr
+
+
+def device_info():
+    n_cu, clk, mclk, bus, l2 = (C.c_int() for _ in range(5))
+    name = C.create_string_buffer(256)
+    check(lib().ffk_device_info(C.byref(n_cu), C.byref(clk), C.byref(mclk), C.byref(bus),
+                                C.byref(l2), name, 256), "ffk_device_info")
+    return {"n_cu": n_cu.value, "clock_khz": clk.value, "mem_clock_khz": mclk.value,
+            "bus_width_bits": bus.value, "l2_bytes": l2.value, "name": name.value.decode()}
+
+
+# ------------------------------------------------------------------ scene --
+
+class Scene:
+    """A device-resident scene (ffk_scene) with its emission layout and grid."""
+
+    def __init__(self, faces, meshes=None):
+        require_gpu()
+        self.faces = _np(faces, np.float32).reshape(-1, 9)
+        if meshes is None:  # one mesh per face
+            n = len(self.faces)
+            meshes = np.stack([np.arange(n), np.ones(n), np.zeros(n)], 1)
+        self.meshes = _np(meshes, np.int32).reshape(-1, 3)
+        h = C.c_void_p()
+        check(lib().ffk_scene_create(C.byref(h)), "ffk_scene_create")
+        self.h = h
+        check(lib().ffk_scene_set_faces(self.h, self.faces.ctypes.data_as(_f), len(self.faces),
+                                        _stream()), "ffk_scene_set_faces")
+        check(lib().ffk_scene_set_meshes(self.h, self.meshes.ctypes.data_as(_i), len(self.meshes),
+                                         _stream()), "ffk_scene_set_meshes")
+        self.n_rays = 0
+        self.front = self.rear = None
+
+    @property
+    def n_faces(self):
+        return len(self.faces)
+
+    @property
+    def n_meshes(self):
+        return len(self.meshes)
+
+    def set_emission(self, n_rays=None, front=None, rear=None, weights=None, total=None,
+                     frame=FRAME_REFERENCE):
+        """Emission layout from explicit counts, or from n_rays and weights
+        (default: the reference's two-sided area weights)."""
+        if front is None:
+            if weights is None:
+                weights, total = area_weights(self.faces)
+            front, rear = emitted_counts(weights, n_rays, total)
+        self.front, self.rear = _np(front, np.int32), _np(rear, np.int32)
+        n = C.c_longlong()
+        check(lib().ffk_scene_set_emission(self.h, self.front.ctypes.data_as(_i),
+                                           self.rear.ctypes.data_as(_i), frame, C.byref(n), _stream()),
+              "ffk_scene_set_emission")
+        self.n_rays = n.value
+        self.frame = frame
+        return self.n_rays
+
+    def build_grid(self, dims=(0, 0, 0)):
+        check(lib().ffk_scene_build_grid(self.h, int(dims[0]), int(dims[1]), int(dims[2]), _stream()),
+              "ffk_scene_build_grid")
+        return self.grid_info()
+
+    def grid_info(self):
+        dims = (C.c_int * 3)()
+        lo = (C.c_float * 3)()
+        hi = (C.c_float * 3)()
+        nr = C.c_longlong()
+        check(lib().ffk_scene_grid_info(self.h, dims, lo, hi, C.byref(nr)), "ffk_scene_grid_info")
+        return {"dims": tuple(dims), "lo": tuple(lo), "hi": tuple(hi), "n_refs": nr.value}
+
+    def set_tuning(self, block=0, rays_per_tile=0, hist=-1):
+        check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist), "ffk_set_tuning")
+
+    # -- device calls (torch tensors in, torch tensors out) --
+    def cast(self, rays, method=CAST_GRID, points=True, stats=None):
+        import torch
+
+        n = rays.shape[0]
+        idx = torch.empty(n, dtype=torch.int32, device=rays.device)
+        pts = torch.zeros((n, 3), dtype=torch.float32, device=rays.device) if points else None
+        check(lib().ffk_cast(self.h, method, _ptr(rays), n, _ptr(idx), _ptr(pts), _ptr(stats), _stream()),
+              "ffk_cast")
+        return idx, pts
+
+    def emit_malley(self, seed):
+        import torch
+
+        rays = torch.empty((self.n_rays, 6), dtype=torch.float32, device="cuda")
+        check(lib().ffk_emit_malley(self.h, seed, _ptr(rays), _stream()), "ffk_emit_malley")
+        return rays
+
+    def tally(self, hit_idx, tally=None):
+        import torch
+
+        N = self.n_meshes
+        if tally is None:
+            tally = torch.empty((N, N), dtype=torch.int64, device="cuda")
+            check(lib().ffk_tally_zero(self.h, _ptr(tally), _stream()), "ffk_tally_zero")
+        check(lib().ffk_tally(self.h, _ptr(hit_idx), hit_idx.shape[0], _ptr(tally), _stream()), "ffk_tally")
+        return tally
+
+    def normalise(self, tally, F=None):
+        import torch
+
+        N = self.n_meshes
+        if F is None:
+            F = torch.empty((N, N), dtype=torch.float32, device="cuda")
+        check(lib().ffk_normalise(self.h, _ptr(tally), _ptr(F), _stream()), "ffk_normalise")
+        return F
+
+    def ff_fused(self, seed, F=None, tally=None, stats=None, need_tally=None):
+        """Fused emit -> cast -> tally -> normalise. Allocates the uint64
+        tally only when some row spans several tiles."""
+        import torch
+
+        N = self.n_meshes
+        if F is None:
+            F = torch.empty((N, N), dtype=torch.float32, device="cuda")
+        if tally is None and (need_tally if need_tally is not None else True):
+            tally = torch.empty((N, N), dtype=torch.int64, device="cuda")
+        check(lib().ffk_ff_fused(self.h, CAST_GRID, seed, _ptr(tally), _ptr(F), _ptr(stats), _stream()),
+              "ffk_ff_fused")
+        return F
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ffk_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def emit_parallel(direction, center, distance, radius, n_rays, seed):
+    import torch
+
+    rays = torch.empty((n_rays, 6), dtype=torch.float32, device="cuda")
+    d = _np(direction, np.float32)
+    c = _np(center, np.float32)
+    check(lib().ffk_emit_parallel(d.ctypes.data_as(_f), c.ctypes.data_as(_f), float(distance),
+                                  float(radius), n_rays, seed, _ptr(rays), _stream()), "ffk_emit_parallel")
+    return rays
+
+
+class Batch:
+    """Many scenes through one persistent (optionally graph-replayed) launch."""
+
+    def __init__(self, scenes):
+        self.scenes = list(scenes)
+        arr = (C.c_void_p * len(self.scenes))(*[s.h.value for s in self.scenes])
+        h = C.c_void_p()
+        check(lib().ffk_batch_create(C.byref(h), arr, len(self.scenes), _stream()), "ffk_batch_create")
+        self.h = h
+
+    def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0):
+        check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks), "ffk_batch_set_tuning")
+
+    def run(self, seeds, Fs, tallies=None, use_graph=True, stats=None):
+        n = len(self.scenes)
+        sd = (C.c_ulonglong * n)(*[int(x) for x in seeds])
+        fp = (C.c_void_p * n)(*[F.data_ptr() for F in Fs])
+        tp = None
+        if tallies is not None:
+            tp = (C.c_void_p * n)(*[(t.data_ptr() if t is not None else 0) for t in tallies])
+        check(lib().ffk_batch_run(self.h, sd, tp, fp, 1 if use_graph else 0, _ptr(stats), _stream()),
+              "ffk_batch_run")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ffk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

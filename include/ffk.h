@@ -1,0 +1,185 @@
+/*
+ * ffk -- C ABI of the MI355X (gfx950) Monte Carlo form-factor kernels.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int return codes, and a
+ * hipStream_t passed as void*. "d_" pointers are device (HBM) pointers,
+ * "h_" pointers are host pointers. Every call returns FFK_OK (0) or a
+ * negative code; HIP failures never abort the process.
+ *
+ * The reference's plugin API (ray_caster / emission / form_factors /
+ * thermal_equation / thermal_solution system_t method tables) is implemented
+ * on top of these calls, see include/ff_plugin.h. Each entry below names the
+ * reference function whose work it performs.
+ *
+ * Layouts in HBM:
+ *   faces : float[9*n]  p0 p1 p2 (AoS, math/types.h:51-54)
+ *   rays  : float[6*n]  origin, END POINT (math/types.h:72-76; the direction
+ *                        is end - origin, math/triangle.cpp:41)
+ *   hit   : int32[n]    face index or -1;  points: float[3*n] (hit only)
+ *   tally : uint64[N*N] per (source mesh, hit mesh) exact ray counts
+ *   F     : float[N*N]  row-major form factors (two-sided convention)
+ */
+#ifndef FFK_H
+#define FFK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- return codes -------------------------------------------------------- */
+#define FFK_OK 0
+#define FFK_ERR_ARG (-1)      /* invalid argument or state                     */
+#define FFK_ERR_HIP (-2)      /* a HIP runtime call failed                     */
+#define FFK_ERR_NOMEM (-3)    /* device allocation failed                      */
+#define FFK_ERR_LIMIT (-4)    /* size beyond a documented limit                */
+#define FFK_ERR_EMPTY (-5)    /* empty scene (no faces / no meshes)            */
+
+/* ---- enums ------------------------------------------------------------------ */
+#define FFK_CAST_NAIVE 0      /* brute force, triangle SoA tiles staged in LDS  */
+#define FFK_CAST_GRID 1       /* uniform 3-D grid built on the GPU, DDA walk    */
+
+#define FFK_FRAME_REFERENCE 0   /* rotate_towards(z, unnormalised normal),     */
+                                /* as emission/malley_cpu.cpp:125-129           */
+#define FFK_FRAME_ORTHONORMAL 1 /* rotate_towards(z, unit normal)              */
+
+#define FFK_F32 0
+#define FFK_BF16 1
+#define FFK_F16 2
+
+typedef struct ffk_scene ffk_scene;
+typedef struct ffk_batch ffk_batch;
+
+/* Last error text for this thread (static storage, never NULL). */
+const char* ffk_last_error(void);
+
+/* Device properties used for launch sizing and roofline reporting. */
+int ffk_device_info(int* n_cu, int* clock_khz, int* mem_clock_khz, int* bus_width_bits,
+                    int* l2_bytes, char* name, int name_len);
+
+/* ---- scene ------------------------------------------------------------------ */
+/* A device-resident scene: faces, per-face intersection terms, face->mesh map,
+ * emission layout and the cast grid. */
+int ffk_scene_create(ffk_scene** out);
+void ffk_scene_destroy(ffk_scene* s);
+
+/* Upload n_faces faces (host AoS) and precompute per-face terms on the device.
+ * Replaces the faces half of ray_caster::system_set_scene + prepare
+ * (ray_caster/system.h:94-98; cuda_ray_caster/system.cu:93-108). */
+int ffk_scene_set_faces(ffk_scene* s, const float* h_faces, int n_faces, void* stream);
+
+/* Mesh table (first_idx, n_faces, pad) triples in scene order
+ * (form_factors/system.h:33-38); builds face->mesh on the device
+ * (cpuFactorsCalculator/cpu_system.cpp:108-118). */
+int ffk_scene_set_meshes(ffk_scene* s, const int* h_meshes, int n_meshes, void* stream);
+
+/* Emission layout: per-face front/rear ray counts (emitted_front/rear,
+ * emission/malley_emission.cpp:50-60, evaluated by the caller or by
+ * ffk_emitted_counts) and the rotation frame. Rays are packed face by face,
+ * front block then rear block (emission/system.h:48). Also records the
+ * mesh-order ray ranges that the tally walks (cpu_system.cpp:163-191).
+ * Returns the total ray count through *n_rays_out. */
+int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, int frame,
+                           long long* n_rays_out, void* stream);
+
+/* Build the uniform grid on the device (count, sort by cell, cell starts).
+ * nx*ny*nz == 0 picks a resolution from the triangle count. Replaces the
+ * host-built 2-D zgrid (ray_caster/zgrid_cpu.cpp:57-68). */
+int ffk_scene_build_grid(ffk_scene* s, int nx, int ny, int nz, void* stream);
+int ffk_scene_grid_info(const ffk_scene* s, int* dims3, float* lo3, float* hi3,
+                        long long* n_refs);
+int ffk_scene_counts(const ffk_scene* s, int* n_faces, int* n_meshes, long long* n_rays);
+
+/* ---- casting ---------------------------------------------------------------- */
+/* Nearest hit for n_rays device rays: naive_cpu::cast semantics
+ * (ray_caster/naive_cpu.cpp:65-97): the reference triangle test bit for bit,
+ * nearest by squared distance, ties to the lowest face index. d_hit_pts may
+ * be NULL. d_stats (uint64[2], may be NULL) accumulates triangle tests and
+ * grid cells visited. */
+int ffk_cast(ffk_scene* s, int method, const float* d_rays, long long n_rays, int* d_hit_idx,
+             float* d_hit_pts, unsigned long long* d_stats, void* stream);
+
+/* ---- emission --------------------------------------------------------------- */
+/* emitted_front/rear for every face: max(1, (int)(n_rays * (w / total))) in
+ * float for w > 0, else 0 (emission/malley_emission.cpp:50-60). Host arrays. */
+int ffk_emitted_counts(const float* h_weights, int n_faces, int n_rays, float total,
+                       int* h_front, int* h_rear);
+
+/* Two-sided area weights (w[2f] = w[2f+1] = area_f) and their float total,
+ * accumulated face by face as total += 2.f * area
+ * (cpuFactorsCalculator/cpu_system.cpp:120-129). Host arrays. */
+int ffk_area_weights(const float* h_faces, int n_faces, float* h_weights, float* total);
+
+/* Malley cosine-weighted rays for the whole emission layout
+ * (emission/malley_cpu.cpp:142-189) from Philox4x32-10 keyed by seed, counter
+ * (ray-in-face j, face, 0, 0). d_rays: float[6*n_rays]. */
+int ffk_emit_malley(ffk_scene* s, unsigned long long seed, float* d_rays, void* stream);
+
+/* Parallel rays from a disk of radius `radius` centred at center - distance*dir
+ * (emission/parallel_rays_cpu.cpp:88-106), counter (ray, 0, 1, 0). */
+int ffk_emit_parallel(const float* h_dir3, const float* h_center3, float distance, float radius,
+                      long long n_rays, unsigned long long seed, float* d_rays, void* stream);
+
+/* ---- tally and form factors ------------------------------------------------- */
+/* Zero the N*N uint64 tally (fill kernel; graph-capture safe). */
+int ffk_tally_zero(const ffk_scene* s, unsigned long long* d_tally, void* stream);
+/* Exact integer form of cpu_system.cpp:163-191: ray g is counted for the mesh
+ * whose mesh-order range holds g, against the mesh of the face it hit. */
+int ffk_tally(const ffk_scene* s, const int* d_hit_idx, long long n_rays,
+              unsigned long long* d_tally, void* stream);
+/* F[m,j] = (float)tally[m,j] * (1.f / (float)rays_of_mesh_m)
+ * (cpu_system.cpp:193-195). */
+int ffk_normalise(const ffk_scene* s, const unsigned long long* d_tally, float* d_F, void* stream);
+
+/* Fused emit -> cast -> tally -> normalise in one pass over the emission
+ * layout (form_factors::system_calculate, form_factors/system.h:110-116).
+ * d_tally: uint64[N*N] scratch owned by the caller. */
+int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned long long* d_tally,
+                 float* d_F, unsigned long long* d_stats, void* stream);
+
+/* Launch tuning of the fused kernel (0 = automatic). */
+int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode);
+
+/* ---- batched form factors ---------------------------------------------------- */
+/* Many scenes of uneven size through one persistent launch over a queue of
+ * (scene, mesh, ray-range) tiles. Scenes must have faces, meshes, emission
+ * and grid set. */
+int ffk_batch_create(ffk_batch** out, ffk_scene* const* scenes, int n_scenes, void* stream);
+void ffk_batch_destroy(ffk_batch* b);
+/* seeds[i] keys scene i; d_tallies[i] and d_Fs[i] are that scene's buffers.
+ * use_graph != 0 records the launch sequence into a hipGraph on first use and
+ * replays it afterwards (same buffers required). */
+int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
+                  float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream);
+int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
+
+/* ---- thermal ---------------------------------------------------------------- */
+/* Convert F (float32 N*N) to the storage type (FFK_BF16/FFK_F16/FFK_F32). */
+int ffk_convert_f(const float* d_F, long long n_elems, int dtype, void* d_out, void* stream);
+/* Radiative exchange of thermal_equation/form_factors.cpp:109-128:
+ *   E_m = (eps_f_m + eps_r_m) * sigma * T_m^4 * A_m        (emission[m] += E_m)
+ *   absorption[n] += sum_m E_m * F[m*N + n]
+ * F stored as dtype, accumulated in FP32, T/area/eps/outputs in FP64. */
+int ffk_sbff(int n, const void* d_F, int dtype, const double* d_T, const double* d_area,
+             const double* d_eps_front, const double* d_eps_rear, double* d_emission,
+             double* d_absorption, void* stream);
+/* Conduction (thermal_equation/conductive_cpu.cpp:83-134) as a CSR operator
+ * over mesh adjacency: flow_m = sum_k G_mk (T_k - T_m); positive flow is
+ * absorbed, negative is emitted. */
+int ffk_conduction(int n, const int* d_row_ptr, const int* d_cols, const double* d_G,
+                   const double* d_T, double* d_emission, double* d_absorption, void* stream);
+/* Adams-Bashforth step of thermal_solution/cpu_adams.cpp:107-182 in FP64:
+ *   E_n[m] = (absorption - emission) / C_m; C_m = A_m * thickness * rho * c
+ *   T_next = T_n + dt * sum_{l<=k} u[k][l] E_{n-l},  k = min(n_step, 4).
+ * d_E_hist is a ring of 5 rows of n doubles (row = step % 5); d_balance
+ * (double[2], may be NULL) receives sum(absorption - emission) and
+ * sum(C * (T_next - T_n)) / dt for the energy-balance check. */
+int ffk_adams_step(int n, int n_step, double dt, const double* d_C, const double* d_emission,
+                   const double* d_absorption, double* d_E_hist, const double* d_T_n,
+                   double* d_T_next, double* d_balance, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FFK_H */

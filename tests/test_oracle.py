@@ -1,0 +1,468 @@
+"""Pins the CPU oracle (oracle/ff_oracle.cpp).
+
+Three kinds of evidence, so that a dropped term, a wrong sign or a transposed
+operand in the restatement fails somewhere:
+  * golden vectors written by tests/golden/make_golden.py from the compiled
+    reference (bit-exact comparisons);
+  * the reference's own known-answer tests (test/ray_casting.cpp,
+    test/grid.cpp, test/math_matrix.cpp, test/math_ray.cpp,
+    test/form_factors.cpp, test/adams_solution.cpp), restated as data;
+  * closed forms and invariants (brute force, analytic form factors,
+    published Philox vectors).
+Where oracle/_ref (the compiled reference) is present, extra randomized
+cross-checks run against it live.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bits, golden
+
+
+# ----------------------------------------------------------------- math --
+
+def test_triangle_intersect_matches_reference_bits(oracle):
+    g = golden("intersect.npz")
+    n = len(g["codes"])
+    for i in range(n):
+        code, pt = oracle.triangle_intersect(g["rays"][i], g["tris"][i])
+        assert code == g["codes"][i], i
+        if code == 0:
+            assert np.array_equal(bits(pt), bits(g["points"][i])), i
+    # the fixture exercises every outcome
+    assert set(np.unique(g["codes"])) >= {0, -1, -2}
+
+
+def test_degenerate_threshold_is_the_double_literal(oracle):
+    # math/triangle.cpp:38: dot(n,n) < 1e-8 in double. A right triangle with
+    # legs s has dot(n,n) = s^4; s = 0.01 gives 9.99999905e-09 (< 1e-8).
+    ray = np.array([0.002, 0.002, 1, 0.002, 0.002, 0], np.float32)
+    for s, expect in [(0.01, -2), (0.0101, 0)]:
+        tri = np.array([0, 0, 0, s, 0, 0, 0, s, 0], np.float32)
+        assert oracle.triangle_intersect(ray, tri)[0] == expect
+
+
+def test_rotate_towards_matches_reference_bits(oracle):
+    g = golden("rotate.npz")
+    for i in range(len(g["to"])):
+        m = oracle.rotate_towards(g["frm"][i], g["to"][i])
+        assert np.array_equal(bits(m.reshape(9)), bits(g["mats"][i])), i
+        assert np.array_equal(bits(oracle.mat_vec(m, g["vec"][i])), bits(g["mat_vec"][i])), i
+
+
+@pytest.mark.parametrize("frm,to", [((0, 0, 1), (0, 0, 1)), ((0, 0, 1), (0, 0, -1)), ((0, 0, 1), (1, 0, 0)),
+                                    ((1, 0, 0), (10, -1, 3))])
+def test_rotate_towards_reference_kat(oracle, frm, to):
+    # test/math_matrix.cpp:31-65: rotation * from == to (near_enough, 1e-8 on the squared distance)
+    m = oracle.rotate_towards(frm, to)
+    r = oracle.mat_vec(m, np.array(frm, np.float32))
+    d = r - np.array(to, np.float32)
+    assert float(np.dot(d, d)) < 1e-8
+
+
+def test_rotate_towards_unnormalised_scales_the_frame(oracle):
+    # H1: with |n| = L != 1 the matrix maps z to n exactly, i.e. it is not a
+    # rotation; with the unit normal it is orthonormal.
+    n = np.array([3e-3, -1e-3, 2e-3], np.float32)
+    m = oracle.rotate_towards([0, 0, 1], n).astype(np.float64)
+    assert np.allclose(oracle.mat_vec(m.astype(np.float32), np.array([0, 0, 1], np.float32)), n, rtol=1e-4, atol=1e-9)
+    assert abs(np.linalg.det(m)) < 1e-3
+    u = (n / np.linalg.norm(n)).astype(np.float32)
+    mu = oracle.rotate_towards([0, 0, 1], u).astype(np.float64)
+    assert np.allclose(mu @ mu.T, np.eye(3), atol=1e-6)
+
+
+def test_axis_rotation_golden_is_a_rotation():
+    g = golden("ff.npz")
+    m = g["axis_rotation"].astype(np.float64)
+    assert np.allclose(m @ m.T, np.eye(3), atol=1e-6)
+
+
+def test_philox_published_vectors(oracle):
+    # Known-answer vectors of Philox4x32-10 (Salmon et al., SC'11, Random123).
+    assert [f"{x:08x}" for x in oracle.philox([0, 0, 0, 0], [0, 0])] == ["6627e8d5", "e169c58d", "bc57ac4c", "9b00dbd8"]
+    assert [f"{x:08x}" for x in oracle.philox([0xffffffff] * 4, [0xffffffff] * 2)] == \
+        ["408f276d", "41c83b0e", "a20bc7c6", "6d5451fd"]
+    assert [f"{x:08x}" for x in oracle.philox([0x243f6a88, 0x85a308d3, 0x13198a2e, 0x03707344],
+                                              [0xa4093822, 0x299f31d0])] == ["d16cfe09", "94fdcceb", "5001e420", "24126ea1"]
+
+
+def test_sincos_turns_against_libm(oracle):
+    us = np.concatenate([np.linspace(0, 1, 20001, endpoint=False), [0.125, 0.25, 0.375, 0.5, 0.75]])
+    for u in us.astype(np.float32):
+        s, c = oracle.sincos_turns(float(u))
+        a = 2 * math.pi * float(u)
+        assert abs(s - math.sin(a)) < 2.5e-7 and abs(c - math.cos(a)) < 2.5e-7, u
+    # exact quadrant points
+    assert oracle.sincos_turns(0.0) == (0.0, 1.0)
+    s, c = oracle.sincos_turns(0.25)
+    assert c == 0.0 or abs(c) < 1e-7
+
+
+# ---------------------------------------------------------------- grids --
+
+def test_grid_walks_match_reference(oracle):
+    import ctypes as C
+
+    g = golden("grid_kat.npz")
+    L = oracle.lib()
+    base, size = g["base"], g["size"]
+    for i, r in enumerate(g["rays"]):
+        buf = np.zeros(128, np.int32)
+        ray = np.ascontiguousarray(r[:6], np.float32)
+        n = L.or_grid_traverse(base.ctypes.data_as(oracle._f), size.ctypes.data_as(oracle._f), 3, 3,
+                               ray.ctypes.data_as(oracle._f), int(r[6]), buf.ctypes.data_as(oracle._i), 64)
+        assert n == g["counts"][i], i
+        assert np.array_equal(buf[:2 * n].reshape(-1, 2), g["cells"][i, :n]), i
+    for i, t in enumerate(g["tris"]):
+        buf = np.zeros(32, np.int32)
+        tri = np.ascontiguousarray(t, np.float32)
+        n = L.or_grid_rasterize(base.ctypes.data_as(oracle._f), size.ctypes.data_as(oracle._f), 3, 3,
+                                tri.ctypes.data_as(oracle._f), buf.ctypes.data_as(oracle._i), 16)
+        assert n == g["rcounts"][i], i
+        assert np.array_equal(buf[:2 * n].reshape(-1, 2), g["rcells"][i, :n]), i
+    for i in range(2):
+        s = C.c_float()
+        hit = L.or_ray_intersect_segment(np.ascontiguousarray(g["seg_p"][i]).ctypes.data_as(oracle._f),
+                                         np.ascontiguousarray(g["seg_q"][i]).ctypes.data_as(oracle._f), C.byref(s))
+        assert hit == g["seg_hit"][i]
+        if hit:
+            assert np.float32(s.value) == g["seg_s"][i]
+
+
+def _cells(oracle, ray, seg):
+    import numpy as np
+    base = np.zeros(3, np.float32)
+    size = np.array([1.5, 1.5, 0], np.float32)
+    buf = np.zeros(128, np.int32)
+    r = np.array(ray, np.float32)
+    n = oracle.lib().or_grid_traverse(base.ctypes.data_as(oracle._f), size.ctypes.data_as(oracle._f), 3, 3,
+                                      r.ctypes.data_as(oracle._f), seg, buf.ctypes.data_as(oracle._i), 64)
+    return {tuple(c) for c in buf[:2 * n].reshape(-1, 2)}
+
+
+def test_grid_reference_kat_sets(oracle):
+    # test/grid.cpp:79-160 expected cell sets on the 3x3 grid of side 0.5
+    assert _cells(oracle, (1.25, 1.25, 0, 1.5, 1.5, 0), 0) == {(2, 2)}
+    assert _cells(oracle, (0, .7, 0, 2, .8, 0), 0) == {(0, 1), (1, 1), (2, 1)}
+    assert _cells(oracle, (.7, .1, 0, .7, .2, 0), 0) == {(1, 0), (1, 1), (1, 2)}
+    assert _cells(oracle, (.3, .4, 0, .4, .5, 0), 0) == {(0, 0), (0, 1), (1, 1), (1, 2), (2, 2)}
+    assert _cells(oracle, (1.4, 1.49, 0, 1.3, 1.4, 0), 0) == {(0, 0), (0, 1), (1, 1), (1, 2), (2, 2)}
+    assert _cells(oracle, (1.1, 0, 0, 1, .1, 0), 0) == {(2, 0), (1, 0), (1, 1), (0, 1), (0, 2)}
+    for r in [(-.5, 2, 0, .25, 1.5, 0), (-.5, 2, 0, 0, 1.25, 0), (-.5, -.5, 0, 0, .25, 0), (-.5, -.5, 0, .25, 0, 0),
+              (2, 2, 0, 1.25, 1.5, 0), (2, 2, 0, 1.5, 1.25, 0), (2, -.5, 0, 1.5, .25, 0), (2, -.5, 0, 1.25, 0, 0)]:
+        assert len(_cells(oracle, r, 0)) == 3
+    assert _cells(oracle, (.75, 1.25, 0, .25, .25, 0), 1) == {(1, 2), (1, 1), (0, 1), (0, 0)}
+    assert _cells(oracle, (.55, .95, 0, .45, .55, 0), 1) == {(1, 1), (0, 1)}
+    assert _cells(oracle, (.55, .95, 0, .95, .55, 0), 1) == {(1, 1)}
+
+
+@pytest.mark.parametrize("name", ["plates", "sphere3", "confetti"])
+def test_zgrid_structure_matches_reference(oracle, name):
+    g = golden(f"zgrid_{name}.npz")
+    c = golden(f"cast_{name}.npz")
+    z = oracle.ZGrid(c["faces"])
+    nx, ny, base, side, size, nr = z.info()
+    assert (nx, ny) == (int(g["nx"]), int(g["ny"]))
+    assert np.array_equal(bits(base), bits(g["base"])) and np.array_equal(bits(size), bits(g["size"]))
+    assert np.array_equal(bits(side[:2]), bits(g["side"][:2]))
+    counts, offsets, refs = z.cells()
+    assert np.array_equal(counts, g["counts"]) and np.array_equal(offsets, g["offsets"])
+    assert np.array_equal(refs, g["refs"])
+
+
+# -------------------------------------------------------------- casters --
+
+@pytest.mark.parametrize("name", ["plates", "sphere3", "confetti"])
+def test_casters_match_reference(oracle, name):
+    g = golden(f"cast_{name}.npz")
+    idx, pts = oracle.naive_cast(g["faces"], g["rays"], n_threads=4)
+    assert np.array_equal(idx, g["naive_idx"])
+    hit = idx >= 0
+    assert np.array_equal(bits(pts[hit]), bits(g["naive_pts"][hit]))
+    zi, zp = oracle.zgrid_cast(g["faces"], g["rays"], n_threads=4)
+    assert np.array_equal(zi, g["zgrid_idx"])
+    want = g["naive_pts"] if bool(g["zgrid_pts_equal"]) else g["zgrid_pts"]
+    assert np.array_equal(bits(zp[zi >= 0]), bits(want[zi >= 0]))
+    assert hit.mean() > 0.05
+
+
+def test_naive_equals_brute_force_definition(oracle):
+    # nearest by squared distance, ties to the lowest index, from the single-face test
+    rng = np.random.default_rng(5)
+    faces = rng.uniform(0, 1, (40, 9)).astype(np.float32)
+    faces[7] = faces[3]  # an exact duplicate: ties must go to face 3
+    rays = np.concatenate([rng.uniform(-0.5, 1.5, (300, 3)), rng.uniform(-0.5, 1.5, (300, 3))], 1).astype(np.float32)
+    idx, pts = oracle.naive_cast(faces, rays)
+    for i, r in enumerate(rays):
+        best, bf = np.float32(np.finfo(np.float32).max), -1
+        for f in range(len(faces)):
+            code, p = oracle.triangle_intersect(r, faces[f])
+            if code == 0:
+                d = (p - r[:3]).astype(np.float32)
+                d2 = np.float32(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2])
+                if d2 < best:
+                    best, bf = d2, f
+        assert idx[i] == bf
+    assert 7 not in set(idx.tolist())
+
+
+def _floor_scene():
+    return np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0], [1, 0, 0, 0, 1, 0, 1, 1, 0]], np.float32)
+
+
+def _stack_scene():
+    return np.array([[0, 0, 0, 1, 0, 0, 0, 1, 0], [0, 0, -1, 1, 0, -1, 0, 1, -1]], np.float32)
+
+
+def _center(f):
+    return f.reshape(3, 3).mean(0).astype(np.float32)
+
+
+def _ray_to(origin, face):
+    c = (face.reshape(3, 3).sum(0) / np.float32(3)).astype(np.float32)
+    o = np.asarray(origin, np.float32)
+    return np.concatenate([o, o + (c - o) / np.float32(2)]).astype(np.float32)
+
+
+def reference_cast_kats():
+    """The single-ray scenarios of test/ray_casting.cpp:148-322 as
+    (faces, rays, expected hit point) data."""
+    out = []
+    fl = _floor_scene()
+    c0, c1 = _center(fl[0]), _center(fl[1])
+    out.append(("ProcessAllRays", fl, np.stack([_ray_to(c0 + [0, 0, 1], fl[0]), _ray_to(c1 + [0, 0, -1], fl[1])]),
+                np.stack([c0, c1])))
+    out.append(("JustWorks", fl, _ray_to(c0 + [0, 0, 1], fl[0])[None], c0[None]))
+    st = _stack_scene()
+    s0, s1 = _center(st[0]), _center(st[1])
+    out.append(("HandleRayIntersectingTriangle", st,
+                np.concatenate([s0 + [0, 0, .5], s0 - [0, 0, .5]]).astype(np.float32)[None], s0[None]))
+    out.append(("FindNearestTriangle", st, _ray_to(s0 + [0, 0, 1], st[0])[None], s0[None]))
+    out.append(("SkipTriangleNearButInOppositeDirection", st, _ray_to(s1 + [0, 0, .9], st[1])[None], s1[None]))
+    out.append(("IntersectTriganglesNotPlanes", st, _ray_to(s1 + [2, 2, 2], st[1])[None], s1[None]))
+    big = np.array([fl[0] + np.array([0, 0, .1 * i] * 3, np.float32) for i in range(-500, 500)], np.float32)
+    b0 = _center(big[0])
+    out.append(("ProcessLargeScene", big, _ray_to(b0 + [2, 2, -2], big[0])[None], b0[None]))
+    return out
+
+
+@pytest.mark.parametrize("case", reference_cast_kats(), ids=lambda c: c[0])
+def test_reference_cast_kats(oracle, case):
+    _, faces, rays, expect = case
+    for fn in (oracle.naive_cast, oracle.zgrid_cast):
+        idx, pts = fn(faces, rays)
+        assert (idx >= 0).all()
+        d = pts - expect
+        assert (np.einsum("ij,ij->i", d, d) < 1e-8).all()
+
+
+# ------------------------------------------------------------- emission --
+
+def test_emitted_counts_match_reference(oracle):
+    g = golden("emitted.npz")
+    for name in ("plates10", "sphere3", "confetti"):
+        w, total = g[f"{name}_weights"], g[f"{name}_total"]
+        for key in g.files:
+            if key.startswith(name + "_") and key.endswith("_front"):
+                n = int(key.split("_")[-2])
+                fr, rr = oracle.emitted_counts(w, n, total)
+                assert np.array_equal(fr, g[key]) and np.array_equal(rr, g[key.replace("front", "rear")])
+
+
+def test_emitted_counts_closed_form_c1(oracle):
+    # 400 equal triangles, n_rays = 4e6: exactly 5000 rays per side (10k per face)
+    from form_factors_amd import scenes
+
+    faces, _ = scenes.parallel_plates(10)
+    w, total = oracle.area_weights(faces)
+    fr, rr = oracle.emitted_counts(w, 4_000_000, total)
+    assert (fr == 5000).all() and (rr == 5000).all()
+    assert oracle.lib().or_malley_count(w.ctypes.data_as(oracle._f), 400, 4_000_000, float(total)) == 4_000_000
+
+
+def test_area_weights_match_reference_fixture(oracle):
+    g = golden("emitted.npz")
+    c = golden("cast_sphere3.npz")
+    w, total = oracle.area_weights(c["faces"])
+    assert np.array_equal(bits(w), bits(g["sphere3_weights"])) and np.float32(total) == g["sphere3_total"]
+
+
+@pytest.mark.parametrize("name", ["plates", "sphere3", "confetti"])
+def test_malley_rays_match_reference_in_mt19937_mode(oracle, name):
+    # The reference's own rays (mt19937 seeds 1..4, unnormalised frame: H1)
+    g = golden(f"cast_{name}.npz")
+    rays = oracle.malley_emit(g["faces"], g["weights"], g["total"], int(g["n_rays"]), rng=oracle.RNG_MT19937,
+                              frame=oracle.FRAME_REFERENCE)
+    assert np.array_equal(bits(rays), bits(g["rays"]))
+
+
+def test_malley_philox_rays_are_cosine_weighted(oracle):
+    # Orthonormal frame: E[cos] = 2/3 against the unit normal, front/rear split
+    from form_factors_amd import scenes
+
+    faces = scenes.icosphere(1)
+    w, total = oracle.area_weights(faces)
+    rays = oracle.malley_emit(faces, w, total, 400_000, rng=oracle.RNG_PHILOX, seed=7,
+                              frame=oracle.FRAME_ORTHONORMAL)
+    fr, rr = oracle.emitted_counts(w, 400_000, total)
+    p = faces.reshape(-1, 3, 3).astype(np.float64)
+    n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    face = np.repeat(np.arange(len(faces)), fr + rr)
+    j = np.concatenate([np.arange(a + b) for a, b in zip(fr, rr)])
+    front = j < fr[face]
+    d = rays[:, 3:].astype(np.float64) - rays[:, :3]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    cos = np.einsum("ij,ij->i", d, n[face])
+    assert abs(cos[front].mean() - 2 / 3) < 3e-3 and abs(-cos[~front].mean() - 2 / 3) < 3e-3
+    assert (cos[front] > -1e-6).all() and (cos[~front] < 1e-6).all()
+    # H1: the reference frame is far from cosine weighted on these faces
+    rays_ref = oracle.malley_emit(faces, w, total, 40_000, rng=oracle.RNG_PHILOX, seed=7, frame=oracle.FRAME_REFERENCE)
+    d = rays_ref[:, 3:].astype(np.float64) - rays_ref[:, :3]
+    fr2, rr2 = oracle.emitted_counts(w, 40_000, total)
+    face2 = np.repeat(np.arange(len(faces)), fr2 + rr2)
+    c2 = np.einsum("ij,ij->i", d / np.linalg.norm(d, axis=1, keepdims=True), n[face2])
+    assert abs(np.abs(c2).mean() - 2 / 3) > 0.1
+
+
+def test_parallel_rays_match_reference(oracle):
+    g = golden("parallel.npz")
+    rays = oracle.parallel_emit(g["direction"], g["center"], g["distance"], g["radius"], len(g["rays"]),
+                                rng=oracle.RNG_MT19937)
+    assert np.array_equal(bits(rays), bits(g["rays"]))
+
+
+# --------------------------------------------------------- form factors --
+
+def theor_parallel(a, b, c):
+    # test/form_factors.cpp:192-196 (already halved for two-sided emission)
+    x, y = a / c, b / c
+    xq, yq = x * x, y * y
+    r = 2 / math.pi / x / y * (math.log(math.sqrt((1 + xq) * (1 + yq) / (1 + xq + yq)))
+                               + x * math.sqrt(1 + yq) * math.atan(x / math.sqrt(1 + yq))
+                               + y * math.sqrt(1 + xq) * math.atan(y / math.sqrt(1 + xq)) - x * math.atan(x) - y * math.atan(y))
+    return r / 2
+
+
+def theor_perpendicular(H, W, L):
+    # test/form_factors.cpp:198-206
+    h, w = H / L, W / L
+    a = (1 + h * h) * (1 + w * w) / (1 + h * h + w * w)
+    b = w * w * (1 + h * h + w * w) / (1 + w * w) / (h * h + w * w)
+    c = h * h * (1 + h * h + w * w) / (1 + h * h) / (h * h + w * w)
+    return 0.5 / math.pi / w * (h * math.atan(1 / h) + w * math.atan(1 / w) - math.sqrt(h * h + w * w)
+                                * math.atan(1 / math.sqrt(h * h + w * w)) + 0.25 * math.log(a * b ** (w * w) * c ** (h * h)))
+
+
+@pytest.mark.parametrize("name", ["parallel_c1", "parallel_c1_rot", "perpendicular", "box", "plates5", "sphere2"])
+def test_ff_pipeline_matches_reference_bits(oracle, name):
+    g = golden("ff.npz")
+    r, F = oracle.ff_calculate(g[f"{name}_faces"], g[f"{name}_meshes"], int(g[f"{name}_n_rays"]),
+                               rng=oracle.RNG_MT19937, frame=oracle.FRAME_REFERENCE,
+                               caster=int(g[f"{name}_caster"]), n_threads=4)
+    assert r == 0
+    assert np.array_equal(bits(F), bits(g[f"{name}_F"]))
+
+
+def test_ff_reference_analytic_kats(oracle):
+    # test/form_factors.cpp:208-297, +-0.01 at 40000 rays (here Philox rays)
+    from form_factors_amd import scenes
+
+    for c in (0.5, 1, 2, 3, 10):
+        faces, meshes = scenes.ff_parallel_planes(c)
+        r, F = oracle.ff_calculate(faces, meshes, 40_000, seed=3)
+        t = theor_parallel(1, 1, c)
+        assert abs(F[0, 1] - t) < 0.01 and abs(F[1, 0] - t) < 0.01
+    faces, meshes = scenes.ff_perpendicular_planes()
+    _, F = oracle.ff_calculate(faces, meshes, 40_000, seed=3)
+    assert abs(F[1, 0] - theor_perpendicular(1, 1, 1)) < 0.01
+    faces, meshes = scenes.ff_box()
+    _, F = oracle.ff_calculate(faces, meshes, 40_000, seed=3)
+    assert abs(F[0, 0] - 0.5) < 0.01
+
+
+def test_ff_error_codes(oracle):
+    g = golden("ff.npz")
+    z = np.zeros((2, 9), np.float32)
+    r, _ = oracle.ff_calculate(z, np.array([[0, 2, 0]], np.int32), 1000)
+    assert r == int(g["code_zero_area"]) == -102
+    assert int(g["code_no_meshes"]) == -101
+
+
+# ------------------------------------------------------------- thermal --
+
+def test_adams_matches_reference_kat_trajectories(oracle):
+    # test/adams_solution.cpp:161-242: e=2, a=1 per step on a face of area
+    # ~1, so T drops by 1/(thickness*density*capacity) each step.
+    g = golden("adams.npz")
+    area = 1.41421 * 1.41421 / 2
+    for name in ("unit", "thick2", "rho2", "c2"):
+        rho, cap, thick = (float(x) for x in g[f"{name}_shell"])
+        C = area * thick * rho * cap
+        T = np.array([300.0])
+        E = np.zeros((5, 1))
+        for step in range(11):
+            E[step % 5, 0] = (1.0 - 2.0) / C
+            T = oracle.adams_combine_f64(step, 1.0, T, E)
+            assert abs(T[0] - g[f"{name}_T"][step, 0]) < 2e-3
+    # multi-mesh trajectory, different rates per mesh
+    faces, meshes = g["multi_faces"], g["multi_meshes"]
+    p = faces.reshape(-1, 3, 3).astype(np.float64)
+    fa = 0.5 * np.linalg.norm(np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0]), axis=1)
+    A = np.array([fa[m[0]:m[0] + m[1]].sum() for m in meshes])
+    rho, cap, thick = g["multi_shell"].T.astype(np.float64)
+    C = A * thick * rho * cap
+    E = np.zeros((5, 4))
+    T = g["multi_T0"].astype(np.float64)
+    for step in range(20):
+        E[step % 5] = (g["multi_a"].astype(np.float64) - g["multi_e"]) / C
+        T = oracle.adams_combine_f64(step, float(g["multi_dt"]), T, E)
+        assert np.allclose(T, g["multi_T"][step], rtol=2e-6)
+
+
+def test_sbff_f64_is_the_dense_product(oracle):
+    rng = np.random.default_rng(1)
+    n = 37
+    F = rng.uniform(0, 0.05, (n, n))
+    T = rng.uniform(250, 400, n)
+    A = rng.uniform(0.1, 1, n)
+    ef, er = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    em, ab = oracle.sbff_f64(F, T, A, ef, er)
+    E = (ef + er) * 5.6704e-8 * T ** 4 * A
+    assert np.allclose(em, E, rtol=1e-14)
+    assert np.allclose(ab, F.T @ E, rtol=1e-12)
+
+
+# ------------------------------------------- live checks vs compiled ref --
+
+needs_ref = pytest.mark.skipif("not __import__('oracle.oracle', fromlist=['x']).ref_available()",
+                               reason="oracle/_ref (compiled reference) not built here")
+
+
+@needs_ref
+def test_live_random_casts_vs_reference(oracle):
+    from form_factors_amd import scenes
+
+    rng = np.random.default_rng(11)
+    faces = np.concatenate([scenes.confetti(400, rng), scenes.cube_shell(4)]).astype(np.float32)
+    o = rng.uniform(-0.2, 1.2, (4000, 3))
+    rays = np.concatenate([o, o + rng.normal(0, 1, (4000, 3))], 1).astype(np.float32)
+    for kind, fn in ((0, oracle.naive_cast), (1, oracle.zgrid_cast)):
+        i1, p1 = fn(faces, rays)
+        i2, p2 = oracle.ref_cast(kind, faces, rays)
+        assert np.array_equal(i1, i2)
+        assert np.array_equal(bits(p1[i1 >= 0]), bits(p2[i2 >= 0]))
+
+
+@needs_ref
+def test_live_random_rotations_vs_reference(oracle):
+    R = oracle.ref()
+    rng = np.random.default_rng(12)
+    for _ in range(2000):
+        to = (rng.standard_normal(3) * 10 ** rng.uniform(-4, 2)).astype(np.float32)
+        frm = np.array([0, 0, 1], np.float32)
+        m2 = np.zeros(9, np.float32)
+        R.ffref_rotate_towards(frm.ctypes.data_as(oracle._f), to.ctypes.data_as(oracle._f), m2.ctypes.data_as(oracle._f))
+        assert np.array_equal(bits(oracle.rotate_towards(frm, to).reshape(9)), bits(m2))
