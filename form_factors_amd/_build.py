@@ -51,7 +51,7 @@ def _compile(src):
     if src.endswith(".hip"):
         cmd += DEVICE + ["-x", "hip"]
     else:
-        cmd += ["-x", "c++"]
+        cmd += ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I", "/opt/rocm/include"]
     cmd += ["-c", src, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r

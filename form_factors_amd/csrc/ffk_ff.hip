@@ -336,6 +336,7 @@ struct ffk_batch {
   long long total_rays = 0;
   int block = 0, rpt = 0, n_blocks = 0;
   // graph state
+  hipStream_t cap = nullptr;  // capture stream (default priority)
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<unsigned long long*> g_tallies;
@@ -600,6 +601,7 @@ void ffk_batch_destroy(ffk_batch* b) {
   if (!b) return;
   (void)hipDeviceSynchronize();
   free_graph(b);
+  if (b->cap) (void)hipStreamDestroy(b->cap);
   dfree(b->d_tiles);
   dfree(b->d_descs);
   dfree(b->d_rows);
@@ -713,10 +715,14 @@ int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long l
   }
   if (d_stats || !use_graph) return batch_enqueue(b, d_stats != nullptr, d_stats, st);
   if (!b->exec) {
-    FFK_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    int r = batch_enqueue(b, false, nullptr, st);
+    // Record on the batch's own stream (the caller's may be the legacy null
+    // stream, which cannot be captured); the graph is launched on the
+    // caller's stream below.
+    if (!b->cap) FFK_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+    FFK_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
+    int r = batch_enqueue(b, false, nullptr, b->cap);
     hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(st, &g);
+    hipError_t e = hipStreamEndCapture(b->cap, &g);
     if (r < 0) {
       if (g) (void)hipGraphDestroy(g);
       return r;

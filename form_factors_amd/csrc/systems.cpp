@@ -1,0 +1,1005 @@
+// HIP implementations of the reference's plugin systems (C-style method
+// tables) on top of the ffk C ABI, plus the flat C binding of ff_plugin.h.
+//
+// Each system is malloc-allocated with its C++ state placement-constructed
+// in init() and destroyed in shutdown(), because the reference frees
+// systems with free() after shutdown() (ray_caster/system.cpp:97-101).
+#include <hip/hip_runtime_api.h>
+
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ff_plugin.h"
+#include "ffk.h"
+#include "form_factors_amd/api.hpp"
+
+namespace {
+
+// A stream of default priority owned by each system.
+hipStream_t make_stream() {
+  hipStream_t s = nullptr;
+  if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
+  return s;
+}
+void drop_stream(hipStream_t s) {
+  if (s) (void)hipStreamDestroy(s);
+}
+
+template <class T>
+struct dbuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  bool reserve(size_t n) {
+    if (n <= cap) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+    if (hipMalloc(&p, (n ? n : 1) * sizeof(T)) != hipSuccess) {
+      (void)hipGetLastError();
+      return false;
+    }
+    cap = n;
+    return true;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+ray_caster::task_t* rc_task_create(int n) {  // ray_caster/system.cpp:44-52
+  ray_caster::task_t* t = (ray_caster::task_t*)malloc(sizeof(ray_caster::task_t));
+  t->n_tasks = n;
+  t->ray = (math::ray_t*)malloc((size_t)n * sizeof(math::ray_t));
+  t->hit_face = (ray_caster::face_t**)malloc((size_t)n * sizeof(ray_caster::face_t*));
+  t->hit_point = (math::vec3*)malloc((size_t)n * sizeof(math::vec3));
+  return t;
+}
+void rc_task_free(ray_caster::task_t* t) {  // ray_caster/system.cpp:54-63
+  if (!t) return;
+  free(t->ray);
+  free(t->hit_face);
+  free(t->hit_point);
+  free(t);
+}
+
+// Two-sided area weights and total, continuing from `total`
+// (cpuFactorsCalculator/cpu_system.cpp:120-129).
+float area_of(const math::triangle_t& f) {
+  const float ux = f.points[1].x - f.points[0].x, uy = f.points[1].y - f.points[0].y,
+              uz = f.points[1].z - f.points[0].z;
+  const float vx = f.points[2].x - f.points[0].x, vy = f.points[2].y - f.points[0].y,
+              vz = f.points[2].z - f.points[0].z;
+  const float nx = uy * vz - uz * vy, ny = uz * vx - ux * vz, nz = ux * vy - uy * vx;
+  return sqrtf(nx * nx + ny * ny + nz * nz) / 2;
+}
+
+}  // namespace
+
+// ============================================================ ray caster ==
+namespace hip_ray_caster {
+
+struct state {
+  int method = FFK_CAST_GRID;
+  ray_caster::scene_t* scene = nullptr;
+  ffk_scene* s = nullptr;
+  hipStream_t st = nullptr;
+  bool prepared = false;
+  dbuf<float> rays, pts;
+  dbuf<int> idx;
+  std::vector<int> h_idx;
+  std::vector<float> h_pts;
+};
+
+struct system : ray_caster::system_t {
+  state* st;
+  int method;
+};
+
+int init(system* sys) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -RAY_CASTER_ERROR;
+  sys->st->method = sys->method;
+  sys->st->st = make_stream();
+  return sys->st->st ? RAY_CASTER_OK : -RAY_CASTER_ERROR;
+}
+
+int shutdown(system* sys) {
+  state* s = sys->st;
+  if (!s) return RAY_CASTER_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  ffk_scene_destroy(s->s);
+  s->rays.release();
+  s->pts.release();
+  s->idx.release();
+  drop_stream(s->st);
+  delete s;
+  sys->st = nullptr;
+  return RAY_CASTER_OK;
+}
+
+int set_scene(system* sys, ray_caster::scene_t* scene) {
+  sys->st->scene = scene;
+  sys->st->prepared = false;
+  return RAY_CASTER_OK;
+}
+
+int prepare(system* sys) {
+  state* s = sys->st;
+  if (!s->scene || s->scene->n_faces == 0) return -RAY_CASTER_ERROR;  // ray_caster/naive_cpu.cpp:56-61
+  if (!s->s && ffk_scene_create(&s->s) < 0) return -RAY_CASTER_ERROR;
+  if (ffk_scene_set_faces(s->s, (const float*)s->scene->faces, s->scene->n_faces, s->st) < 0)
+    return -RAY_CASTER_ERROR;
+  if (s->method == FFK_CAST_GRID && ffk_scene_build_grid(s->s, 0, 0, 0, s->st) < 0) return -RAY_CASTER_ERROR;
+  if (hipStreamSynchronize(s->st) != hipSuccess) return -RAY_CASTER_ERROR;
+  s->prepared = true;
+  return RAY_CASTER_OK;
+}
+
+// Device-resident cast used by the HIP emitter when both sit on this caster.
+int cast_device(system* sys, const float* d_rays, long long n, int* d_idx, float* d_pts, hipStream_t st) {
+  state* s = sys->st;
+  if (!s->prepared) return -RAY_CASTER_ERROR;
+  return ffk_cast(s->s, s->method, d_rays, n, d_idx, d_pts, nullptr, st) < 0 ? -RAY_CASTER_ERROR : RAY_CASTER_OK;
+}
+
+// Fill the reference's output convention from device results.
+void scatter_hits(state* s, ray_caster::task_t* task, long long lo, long long n) {
+  for (long long i = 0; i < n; ++i) {
+    const int f = s->h_idx[i];
+    if (f >= 0) {
+      task->hit_face[lo + i] = &s->scene->faces[f];
+      task->hit_point[lo + i].x = s->h_pts[3 * i];
+      task->hit_point[lo + i].y = s->h_pts[3 * i + 1];
+      task->hit_point[lo + i].z = s->h_pts[3 * i + 2];
+    } else {
+      task->hit_face[lo + i] = 0;
+    }
+  }
+}
+
+int cast(system* sys, ray_caster::task_t* task) {
+  state* s = sys->st;
+  if (!s->prepared) return -RAY_CASTER_ERROR;
+  const long long total = task->n_tasks;
+  const long long chunk = 1LL << 24;
+  for (long long lo = 0; lo < total; lo += chunk) {
+    const long long n = total - lo < chunk ? total - lo : chunk;
+    if (!s->rays.reserve(6 * n) || !s->idx.reserve(n) || !s->pts.reserve(3 * n)) return -RAY_CASTER_ERROR;
+    if (hipMemcpyAsync(s->rays.p, task->ray + lo, 24 * n, hipMemcpyHostToDevice, s->st) != hipSuccess)
+      return -RAY_CASTER_ERROR;
+    if (ffk_cast(s->s, s->method, s->rays.p, n, s->idx.p, s->pts.p, nullptr, s->st) < 0) return -RAY_CASTER_ERROR;
+    s->h_idx.resize(n);
+    s->h_pts.resize(3 * n);
+    if (hipMemcpyAsync(s->h_idx.data(), s->idx.p, 4 * n, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+        hipMemcpyAsync(s->h_pts.data(), s->pts.p, 12 * n, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+        hipStreamSynchronize(s->st) != hipSuccess)
+      return -RAY_CASTER_ERROR;
+    scatter_hits(s, task, lo, n);
+  }
+  return RAY_CASTER_OK;
+}
+
+const ray_caster::system_methods_t methods = {
+    (int (*)(ray_caster::system_t*))&init,
+    (int (*)(ray_caster::system_t*))&shutdown,
+    (int (*)(ray_caster::system_t*, ray_caster::scene_t*))&set_scene,
+    (int (*)(ray_caster::system_t*))&prepare,
+    (int (*)(ray_caster::system_t*, ray_caster::task_t*))&cast,
+};
+
+static ray_caster::system_t* create(int method) {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  s->method = method;
+  return s;
+}
+
+ray_caster::system_t* system_create_naive() { return create(FFK_CAST_NAIVE); }
+ray_caster::system_t* system_create_grid() { return create(FFK_CAST_GRID); }
+
+bool is_hip(const ray_caster::system_t* s) { return s && s->methods == &methods; }
+ffk_scene* device_scene(ray_caster::system_t* s) { return ((system*)s)->st->s; }
+int method_of(ray_caster::system_t* s) { return ((system*)s)->st->method; }
+ray_caster::scene_t* scene_of(ray_caster::system_t* s) { return ((system*)s)->st->scene; }
+
+}  // namespace hip_ray_caster
+
+// ====================================================== Malley emission ==
+namespace hip_malley_emission {
+
+struct state {
+  ray_caster::system_t* caster = nullptr;
+  ray_caster::scene_t* scene = nullptr;
+  ffk_scene* s = nullptr;
+  hipStream_t st = nullptr;
+  unsigned long long seed = 0, calls = 0;
+  int frame = FFK_FRAME_REFERENCE;
+  std::vector<int> front, rear;
+  dbuf<float> rays, pts;
+  dbuf<int> idx;
+  std::vector<int> h_idx;
+  std::vector<float> h_pts;
+};
+
+struct system : emission::system_t {
+  state* st;
+};
+
+int init(system* sys, ray_caster::system_t* caster) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -EMISSION_ERROR;
+  sys->st->caster = caster;
+  sys->st->st = make_stream();
+  return sys->st->st ? EMISSION_OK : -EMISSION_ERROR;
+}
+
+int shutdown(system* sys) {
+  state* s = sys->st;
+  if (!s) return EMISSION_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  ffk_scene_destroy(s->s);
+  s->rays.release();
+  s->pts.release();
+  s->idx.release();
+  drop_stream(s->st);
+  delete s;
+  sys->st = nullptr;
+  return EMISSION_OK;
+}
+
+// emission/malley_cpu.cpp:77-92
+int set_scene(system* sys, ray_caster::scene_t* scene) {
+  state* s = sys->st;
+  if (scene == 0 || scene->n_faces == 0) return -EMISSION_ERROR;
+  s->scene = scene;
+  int r;
+  if ((r = s->caster->methods->set_scene(s->caster, scene)) < 0) return r;
+  if ((r = s->caster->methods->prepare(s->caster)) < 0) return r;
+  if (!s->s && ffk_scene_create(&s->s) < 0) return -EMISSION_ERROR;
+  if (ffk_scene_set_faces(s->s, (const float*)scene->faces, scene->n_faces, s->st) < 0) return -EMISSION_ERROR;
+  int whole[3] = {0, scene->n_faces, 0};  // the emitter itself needs no mesh split
+  if (ffk_scene_set_meshes(s->s, whole, 1, s->st) < 0) return -EMISSION_ERROR;
+  s->front.clear();
+  return EMISSION_OK;
+}
+
+// Key of the next calculate(): successive calls draw fresh rays, as the
+// reference's generators advance between calls.
+unsigned long long next_key(state* s) { return s->seed + 0x9E3779B97F4A7C15ull * (s->calls++); }
+
+// Per-face counts and the device layout for this task's weights.
+int layout(state* s, int n_rays, float total, const float* weights, long long* n_real) {
+  const int nf = s->scene->n_faces;
+  std::vector<int> fr(nf), rr(nf);
+  ffk_emitted_counts(weights, nf, n_rays, total, fr.data(), rr.data());
+  if (fr != s->front || rr != s->rear) {
+    if (ffk_scene_set_emission(s->s, fr.data(), rr.data(), s->frame, n_real, s->st) < 0) return -EMISSION_ERROR;
+    s->front.swap(fr);
+    s->rear.swap(rr);
+  } else {
+    ffk_scene_counts(s->s, nullptr, nullptr, n_real);
+  }
+  return EMISSION_OK;
+}
+
+// emission/malley_cpu.cpp:195-211: generate the rays, then cast them.
+int calculate(system* sys, emission::task_t* base) {
+  state* s = sys->st;
+  malley_emission::task_t* task = (malley_emission::task_t*)base;
+  if (task->rays) {
+    rc_task_free(task->rays);
+    task->rays = 0;
+  }
+  if (!s->scene) return -EMISSION_ERROR;
+  long long n = 0;
+  int r;
+  if ((r = layout(s, task->n_rays, task->total_weight, task->weights, &n)) < 0) return r;
+  if (n == 0) return EMISSION_OK;
+  if (n > 0x7fffffffLL) return -EMISSION_ERROR;  // task_t counts rays in an int
+  if (!s->rays.reserve(6 * n)) return -EMISSION_ERROR;
+  if (ffk_emit_malley(s->s, next_key(s), s->rays.p, s->st) < 0) return -EMISSION_ERROR;
+  ray_caster::task_t* rt = rc_task_create((int)n);
+  task->rays = rt;
+  if (hipMemcpyAsync(rt->ray, s->rays.p, 24 * n, hipMemcpyDeviceToHost, s->st) != hipSuccess) return -EMISSION_ERROR;
+  if (hip_ray_caster::is_hip(s->caster) && hip_ray_caster::scene_of(s->caster) == s->scene) {
+    // Both on the GPU: cast the device-resident rays, copy results back once.
+    if (!s->idx.reserve(n) || !s->pts.reserve(3 * n)) return -EMISSION_ERROR;
+    if (hip_ray_caster::cast_device((hip_ray_caster::system*)s->caster, s->rays.p, n, s->idx.p, s->pts.p, s->st) < 0)
+      return -EMISSION_ERROR;
+    s->h_idx.resize(n);
+    s->h_pts.resize(3 * n);
+    if (hipMemcpyAsync(s->h_idx.data(), s->idx.p, 4 * n, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+        hipMemcpyAsync(s->h_pts.data(), s->pts.p, 12 * n, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+        hipStreamSynchronize(s->st) != hipSuccess)
+      return -EMISSION_ERROR;
+    for (long long i = 0; i < n; ++i) {
+      const int f = s->h_idx[i];
+      if (f >= 0) {
+        rt->hit_face[i] = &s->scene->faces[f];
+        rt->hit_point[i].x = s->h_pts[3 * i];
+        rt->hit_point[i].y = s->h_pts[3 * i + 1];
+        rt->hit_point[i].z = s->h_pts[3 * i + 2];
+      } else {
+        rt->hit_face[i] = 0;
+      }
+    }
+    return EMISSION_OK;
+  }
+  if (hipStreamSynchronize(s->st) != hipSuccess) return -EMISSION_ERROR;
+  return s->caster->methods->cast(s->caster, rt);
+}
+
+const emission::system_methods_t methods = {
+    (int (*)(emission::system_t*, ray_caster::system_t*))&init,
+    (int (*)(emission::system_t*))&shutdown,
+    (int (*)(emission::system_t*, ray_caster::scene_t*))&set_scene,
+    (int (*)(emission::system_t*, emission::task_t*))&calculate,
+};
+
+emission::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  return s;
+}
+
+bool is_hip(const emission::system_t* s) { return s && s->methods == &methods; }
+state* state_of(emission::system_t* s) { return ((system*)s)->st; }
+
+}  // namespace hip_malley_emission
+
+// ================================================ parallel-rays emission ==
+namespace hip_parallel_rays_emission {
+
+struct state {
+  ray_caster::system_t* caster = nullptr;
+  ray_caster::scene_t* scene = nullptr;
+  unsigned long long seed = 0, calls = 0;
+  dbuf<float> rays;
+};
+struct system : emission::system_t {
+  state* st;
+};
+
+int init(system* sys, ray_caster::system_t* caster) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -EMISSION_ERROR;
+  sys->st->caster = caster;
+  return EMISSION_OK;
+}
+int shutdown(system* sys) {
+  if (sys->st) {
+    sys->st->rays.release();
+    delete sys->st;
+    sys->st = nullptr;
+  }
+  return EMISSION_OK;
+}
+// emission/parallel_rays_cpu.cpp:68-84
+int set_scene(system* sys, ray_caster::scene_t* scene) {
+  if (scene == 0 || scene->n_faces == 0) return -EMISSION_ERROR;
+  sys->st->scene = scene;
+  int r;
+  if ((r = sys->st->caster->methods->set_scene(sys->st->caster, scene)) < 0) return r;
+  if ((r = sys->st->caster->methods->prepare(sys->st->caster)) < 0) return r;
+  return EMISSION_OK;
+}
+// emission/parallel_rays_cpu.cpp:108-121
+int calculate(system* sys, emission::task_t* base) {
+  state* s = sys->st;
+  parallel_rays_emission::task_t* task = (parallel_rays_emission::task_t*)base;
+  if (task->rays) {
+    rc_task_free(task->rays);
+    task->rays = 0;
+  }
+  const long long n = task->n_rays;
+  if (n <= 0) return EMISSION_OK;
+  if (!s->rays.reserve(6 * n)) return -EMISSION_ERROR;
+  const unsigned long long key = s->seed + 0x9E3779B97F4A7C15ull * (s->calls++);
+  if (ffk_emit_parallel(&task->direction.x, &task->center.x, task->distance, task->radius, n, key, s->rays.p,
+                        nullptr) < 0)
+    return -EMISSION_ERROR;
+  ray_caster::task_t* rt = rc_task_create((int)n);
+  task->rays = rt;
+  if (hipMemcpy(rt->ray, s->rays.p, 24 * n, hipMemcpyDeviceToHost) != hipSuccess) return -EMISSION_ERROR;
+  return s->caster->methods->cast(s->caster, rt);
+}
+
+const emission::system_methods_t methods = {
+    (int (*)(emission::system_t*, ray_caster::system_t*))&init,
+    (int (*)(emission::system_t*))&shutdown,
+    (int (*)(emission::system_t*, ray_caster::scene_t*))&set_scene,
+    (int (*)(emission::system_t*, emission::task_t*))&calculate,
+};
+emission::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  return s;
+}
+
+}  // namespace hip_parallel_rays_emission
+
+// ========================================================= form factors ==
+namespace hip_form_factors {
+
+struct state {
+  emission::system_t* emitter = nullptr;
+  form_factors::scene_t* scene = nullptr;
+  ray_caster::scene_t rc_scene = {0, 0};
+  float total_area = 0;
+  std::vector<float> weights;
+  ffk_scene* s = nullptr;
+  hipStream_t st = nullptr;
+  bool prepared = false;
+  bool grid = false;
+  std::vector<int> front, rear;
+  int frame = -1;
+  dbuf<float> F;
+  dbuf<unsigned long long> tally;
+  dbuf<int> hits;
+  std::vector<int> h_hits;
+};
+struct system : form_factors::system_t {
+  state* st;
+};
+
+int init(system* sys, emission::system_t* emitter) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -FORM_FACTORS_ERROR;
+  sys->st->emitter = emitter;
+  sys->st->st = make_stream();
+  return sys->st->st ? FORM_FACTORS_OK : -FORM_FACTORS_ERROR;
+}
+
+int shutdown(system* sys) {
+  state* s = sys->st;
+  if (!s) return FORM_FACTORS_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  ffk_scene_destroy(s->s);
+  s->F.release();
+  s->tally.release();
+  s->hits.release();
+  drop_stream(s->st);
+  delete s;
+  sys->st = nullptr;
+  return FORM_FACTORS_OK;
+}
+
+// cpuFactorsCalculator/cpu_system.cpp:82-92
+int set_scene(system* sys, form_factors::scene_t* scene) {
+  state* s = sys->st;
+  s->scene = scene;
+  s->rc_scene.n_faces = scene->n_faces;
+  s->rc_scene.faces = scene->faces;
+  s->prepared = false;
+  return s->emitter->methods->set_scene(s->emitter, &s->rc_scene);
+}
+
+// cpuFactorsCalculator/cpu_system.cpp:95-136. The running total is not
+// reset between calls, as in the reference.
+int prepare(system* sys) {
+  state* s = sys->st;
+  if (s->scene == 0 || s->scene->n_faces == 0 || s->scene->n_meshes == 0) return -FORM_FACTORS_EMPTY_SCENE;
+  const int nf = s->scene->n_faces;
+  s->weights.assign(2 * (size_t)nf, 0.f);
+  for (int i = 0; i != nf; ++i) {
+    const float a = area_of(s->scene->faces[i]);
+    s->total_area += 2.f * a;
+    s->weights[2 * i] = a;
+    s->weights[2 * i + 1] = a;
+  }
+  if (s->total_area < FLT_EPSILON) return -FORM_FACTORS_SCENE_TOO_SMALL;
+  if (!s->s && ffk_scene_create(&s->s) < 0) return -FORM_FACTORS_ERROR;
+  if (ffk_scene_set_faces(s->s, (const float*)s->scene->faces, nf, s->st) < 0) return -FORM_FACTORS_ERROR;
+  if (ffk_scene_set_meshes(s->s, (const int*)s->scene->meshes, s->scene->n_meshes, s->st) < 0)
+    return -FORM_FACTORS_ERROR;
+  s->grid = false;
+  s->front.clear();
+  s->prepared = true;
+  return FORM_FACTORS_OK;
+}
+
+int ensure_layout(state* s, int n_rays, int frame) {
+  const int nf = s->scene->n_faces;
+  std::vector<int> fr(nf), rr(nf);
+  ffk_emitted_counts(s->weights.data(), nf, n_rays, s->total_area, fr.data(), rr.data());
+  if (fr != s->front || rr != s->rear || frame != s->frame) {
+    if (ffk_scene_set_emission(s->s, fr.data(), rr.data(), frame, nullptr, s->st) < 0) return -FORM_FACTORS_ERROR;
+    s->front.swap(fr);
+    s->rear.swap(rr);
+    s->frame = frame;
+  }
+  return FORM_FACTORS_OK;
+}
+
+// cpuFactorsCalculator/cpu_system.cpp:149-201 on the device.
+int calculate(system* sys, form_factors::task_t* task) {
+  state* s = sys->st;
+  if (!s->prepared) return -FORM_FACTORS_ERROR;
+  const int N = s->scene->n_meshes;
+  const size_t NN = (size_t)N * N;
+  if (!s->F.reserve(NN)) return -FORM_FACTORS_ERROR;
+  int r;
+  hip_malley_emission::state* em =
+      hip_malley_emission::is_hip(s->emitter) ? hip_malley_emission::state_of(s->emitter) : nullptr;
+  if (em && hip_ray_caster::is_hip(em->caster) && hip_ray_caster::method_of(em->caster) == FFK_CAST_GRID) {
+    // Fused: emission, grid cast, tally and normalisation in one device pass.
+    if ((r = ensure_layout(s, task->n_rays, em->frame)) < 0) return r;
+    if (!s->grid) {
+      if (ffk_scene_build_grid(s->s, 0, 0, 0, s->st) < 0) return -FORM_FACTORS_ERROR;
+      s->grid = true;
+    }
+    if (!s->tally.reserve(NN)) return -FORM_FACTORS_ERROR;
+    const unsigned long long key = hip_malley_emission::next_key(em);
+    if (ffk_ff_fused(s->s, FFK_CAST_GRID, key, s->tally.p, s->F.p, nullptr, s->st) < 0) return -FORM_FACTORS_ERROR;
+  } else {
+    // Any other emitter: it emits and casts; the tally runs on the device.
+    malley_emission::task_t et(task->n_rays, s->total_area, s->weights.data());
+    if ((r = s->emitter->methods->calculate(s->emitter, &et)) < 0) {
+      rc_task_free(et.rays);
+      return r;
+    }
+    if ((r = ensure_layout(s, task->n_rays, s->frame < 0 ? FFK_FRAME_REFERENCE : s->frame)) < 0) {
+      rc_task_free(et.rays);
+      return r;
+    }
+    const long long n = et.rays ? et.rays->n_tasks : 0;
+    s->h_hits.resize(n > 0 ? n : 1);
+    for (long long i = 0; i < n; ++i)
+      s->h_hits[i] = et.rays->hit_face[i] ? (int)(et.rays->hit_face[i] - s->scene->faces) : -1;
+    rc_task_free(et.rays);
+    if (!s->tally.reserve(NN) || !s->hits.reserve(n)) return -FORM_FACTORS_ERROR;
+    if (ffk_tally_zero(s->s, s->tally.p, s->st) < 0) return -FORM_FACTORS_ERROR;
+    if (n > 0) {
+      if (hipMemcpyAsync(s->hits.p, s->h_hits.data(), 4 * n, hipMemcpyHostToDevice, s->st) != hipSuccess)
+        return -FORM_FACTORS_ERROR;
+      if (ffk_tally(s->s, s->hits.p, n, s->tally.p, s->st) < 0) return -FORM_FACTORS_ERROR;
+    }
+    if (ffk_normalise(s->s, s->tally.p, s->F.p, s->st) < 0) return -FORM_FACTORS_ERROR;
+  }
+  if (hipMemcpyAsync(task->form_factors, s->F.p, 4 * NN, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+      hipStreamSynchronize(s->st) != hipSuccess)
+    return -FORM_FACTORS_ERROR;
+  return FORM_FACTORS_OK;
+}
+
+const form_factors::system_methods_t methods = {
+    (int (*)(form_factors::system_t*, emission::system_t*))&init,
+    (int (*)(form_factors::system_t*))&shutdown,
+    (int (*)(form_factors::system_t*, form_factors::scene_t*))&set_scene,
+    (int (*)(form_factors::system_t*))&prepare,
+    (int (*)(form_factors::system_t*, form_factors::task_t*))&calculate,
+};
+form_factors::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  return s;
+}
+
+}  // namespace hip_form_factors
+
+// ================================= SB_FF equation (radiative exchange) ==
+namespace hip_sb_ff {
+
+struct state {
+  params_t params = {nullptr, 0, FFK_F32};
+  subject::scene_t* scene = nullptr;
+  form_factors::scene_t ff_scene = {0, 0, 0, 0};
+  std::vector<float> F;
+  std::vector<double> T, area, ef, er, em, ab;
+  dbuf<float> dF32;
+  dbuf<unsigned short> dF16;
+  dbuf<double> dvec;  // T, area, ef, er, em, ab
+  hipStream_t st = nullptr;
+};
+struct system : thermal_equation::system_t {
+  state* st;
+};
+
+int init(system* sys, params_t* params) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -THERMAL_EQUATION_ERROR;
+  sys->st->params = *params;
+  sys->st->st = make_stream();
+  return sys->st->st ? THERMAL_EQUATION_OK : -THERMAL_EQUATION_ERROR;
+}
+int shutdown(system* sys) {
+  state* s = sys->st;
+  if (!s) return THERMAL_EQUATION_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  s->dF32.release();
+  s->dF16.release();
+  s->dvec.release();
+  drop_stream(s->st);
+  delete s;
+  sys->st = nullptr;
+  return THERMAL_EQUATION_OK;
+}
+
+// thermal_equation/form_factors.cpp:77-104: F is computed once here.
+int set_scene(system* sys, subject::scene_t* scene) {
+  state* s = sys->st;
+  s->scene = scene;
+  const int N = scene->n_meshes;
+  s->area.assign(N, 0.0);
+  s->ef.assign(N, 0.0);
+  s->er.assign(N, 0.0);
+  for (int m = 0; m < N; ++m) {
+    float a = 0;  // subject/system.cpp:48-56, float sum per mesh
+    for (int f = 0; f != scene->meshes[m].n_faces; ++f) a += area_of(scene->faces[scene->meshes[m].first_idx + f]);
+    s->area[m] = a;
+    const subject::material_t& mat = scene->materials[scene->meshes[m].material_idx];
+    s->ef[m] = mat.front.emissivity;
+    s->er[m] = mat.rear.emissivity;
+  }
+  s->ff_scene.n_faces = scene->n_faces;
+  s->ff_scene.faces = scene->faces;
+  s->ff_scene.n_meshes = N;
+  s->ff_scene.meshes = (form_factors::mesh_t*)scene->meshes;  // layout-compatible (form_factors.cpp:87)
+  form_factors::system_t* ff = s->params.form_factors_calculator;
+  int r;
+  if ((r = ff->methods->set_scene(ff, &s->ff_scene)) < 0) return r;
+  if ((r = ff->methods->prepare(ff)) < 0) return r;
+  s->F.assign((size_t)N * N, 0.f);
+  form_factors::task_t task = {s->params.n_rays, s->F.data()};
+  if ((r = ff->methods->calculate(ff, &task)) < 0) return r;
+  // F in the chosen storage type on the device
+  const size_t NN = (size_t)N * N;
+  if (!s->dF32.reserve(NN) || !s->dvec.reserve(6 * (size_t)N)) return -THERMAL_EQUATION_ERROR;
+  if (hipMemcpyAsync(s->dF32.p, s->F.data(), 4 * NN, hipMemcpyHostToDevice, s->st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  if (s->params.f_dtype != FFK_F32) {
+    if (!s->dF16.reserve(NN)) return -THERMAL_EQUATION_ERROR;
+    if (ffk_convert_f(s->dF32.p, (long long)NN, s->params.f_dtype, s->dF16.p, s->st) < 0)
+      return -THERMAL_EQUATION_ERROR;
+  }
+  double* d = s->dvec.p;
+  if (hipMemcpyAsync(d + N, s->area.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(d + 2 * N, s->ef.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(d + 3 * N, s->er.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipStreamSynchronize(s->st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  return THERMAL_EQUATION_OK;
+}
+
+// thermal_equation/form_factors.cpp:109-128; adds to the task's arrays.
+int calculate(system* sys, thermal_equation::task_t* task) {
+  state* s = sys->st;
+  const int N = s->scene->n_meshes;
+  s->T.assign(task->temperatures, task->temperatures + N);
+  double* d = s->dvec.p;
+  const void* F = s->params.f_dtype == FFK_F32 ? (const void*)s->dF32.p : (const void*)s->dF16.p;
+  if (hipMemcpyAsync(d, s->T.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemsetAsync(d + 4 * N, 0, 16 * (size_t)N, s->st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  if (ffk_sbff(N, F, s->params.f_dtype, d, d + N, d + 2 * N, d + 3 * N, d + 4 * N, d + 5 * N, s->st) < 0)
+    return -THERMAL_EQUATION_ERROR;
+  s->em.resize(N);
+  s->ab.resize(N);
+  if (hipMemcpyAsync(s->em.data(), d + 4 * N, 8 * N, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+      hipMemcpyAsync(s->ab.data(), d + 5 * N, 8 * N, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+      hipStreamSynchronize(s->st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  for (int m = 0; m < N; ++m) {
+    task->emission[m] += (float)s->em[m];
+    task->absorption[m] += (float)s->ab[m];
+  }
+  return THERMAL_EQUATION_OK;
+}
+
+const thermal_equation::system_methods_t methods = {
+    (int (*)(thermal_equation::system_t*, void*))&init,
+    (int (*)(thermal_equation::system_t*))&shutdown,
+    (int (*)(thermal_equation::system_t*, subject::scene_t*))&set_scene,
+    (int (*)(thermal_equation::system_t*, thermal_equation::task_t*))&calculate,
+};
+thermal_equation::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  return s;
+}
+
+}  // namespace hip_sb_ff
+
+// ==================================================== Adams solver (FP64) ==
+namespace hip_adams {
+
+struct state {
+  thermal_solution::params_t params = {0, nullptr};
+  subject::scene_t* scene = nullptr;
+  int n_step = 0;
+  std::vector<double> C, T;
+  std::vector<float> Tf, em, ab;
+  dbuf<double> d;  // C, em, ab, E ring (5N), T_n, T_next
+  hipStream_t st = nullptr;
+};
+struct system : thermal_solution::system_t {
+  state* st;
+};
+
+int init(system* sys, thermal_solution::params_t* params) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -THERMAL_SOLUTION_ERROR;
+  sys->st->params = *params;
+  sys->st->st = make_stream();
+  return sys->st->st ? THERMAL_SOLUTION_OK : -THERMAL_SOLUTION_ERROR;
+}
+int shutdown(system* sys) {
+  state* s = sys->st;
+  if (!s) return THERMAL_SOLUTION_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  s->d.release();
+  drop_stream(s->st);
+  delete s;
+  sys->st = nullptr;
+  return THERMAL_SOLUTION_OK;
+}
+
+// thermal_solution/cpu_adams.cpp:62-85
+int set_scene(system* sys, subject::scene_t* scene, float* temperatures) {
+  state* s = sys->st;
+  s->scene = scene;
+  int r;
+  for (int i = 0; i != s->params.n_equations; ++i) {
+    thermal_equation::system_t* eq = s->params.equations[i];
+    if ((r = eq->methods->set_scene(eq, scene)) < 0) return r;
+  }
+  const int N = scene->n_meshes;
+  s->n_step = 0;
+  s->C.assign(N, 0.0);
+  for (int m = 0; m < N; ++m) {
+    float a = 0;
+    for (int f = 0; f != scene->meshes[m].n_faces; ++f) a += area_of(scene->faces[scene->meshes[m].first_idx + f]);
+    const subject::shell_properties_t& sh = scene->materials[scene->meshes[m].material_idx].shell;
+    s->C[m] = (double)a * sh.thickness * sh.density * sh.heat_capacity;  // cpu_adams.cpp:127-129
+  }
+  s->T.assign(temperatures, temperatures + N);
+  if (!s->d.reserve(10 * (size_t)N)) return -THERMAL_SOLUTION_ERROR;
+  if (hipMemsetAsync(s->d.p, 0, 80 * (size_t)N, s->st) != hipSuccess ||
+      hipMemcpyAsync(s->d.p, s->C.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(s->d.p + 8 * (size_t)N, s->T.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipStreamSynchronize(s->st) != hipSuccess)
+    return -THERMAL_SOLUTION_ERROR;
+  return THERMAL_SOLUTION_OK;
+}
+
+// thermal_solution/cpu_adams.cpp:107-182 with the update in FP64.
+int calculate(system* sys, thermal_solution::task_t* task) {
+  state* s = sys->st;
+  const int N = s->scene->n_meshes;
+  s->Tf.resize(N);
+  for (int m = 0; m < N; ++m) s->Tf[m] = (float)s->T[m];
+  s->em.assign(N, 0.f);
+  s->ab.assign(N, 0.f);
+  thermal_equation::task_t et = {s->Tf.data(), s->em.data(), s->ab.data()};
+  int r;
+  for (int i = 0; i != s->params.n_equations; ++i) {
+    thermal_equation::system_t* eq = s->params.equations[i];
+    if ((r = eq->methods->calculate(eq, &et)) < 0) return r;
+  }
+  std::vector<double> emd(s->em.begin(), s->em.end()), abd(s->ab.begin(), s->ab.end());
+  double* d = s->d.p;
+  double *dC = d, *dem = d + N, *dab = d + 2 * (size_t)N, *dE = d + 3 * (size_t)N, *dT = d + 8 * (size_t)N,
+         *dTn = d + 9 * (size_t)N;
+  if (hipMemcpyAsync(dem, emd.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(dab, abd.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess)
+    return -THERMAL_SOLUTION_ERROR;
+  if (ffk_adams_step(N, s->n_step, task->time_delta, dC, dem, dab, dE, dT, dTn, nullptr, s->st) < 0)
+    return -THERMAL_SOLUTION_ERROR;
+  if (hipMemcpyAsync(dT, dTn, 8 * N, hipMemcpyDeviceToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(s->T.data(), dTn, 8 * N, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+      hipStreamSynchronize(s->st) != hipSuccess)
+    return -THERMAL_SOLUTION_ERROR;
+  for (int m = 0; m < N; ++m) task->temperatures[m] = (float)s->T[m];
+  task->n_step = ++s->n_step;
+  return THERMAL_SOLUTION_OK;
+}
+
+const thermal_solution::system_methods_t methods = {
+    (int (*)(thermal_solution::system_t*, thermal_solution::params_t*))&init,
+    (int (*)(thermal_solution::system_t*))&shutdown,
+    (int (*)(thermal_solution::system_t*, subject::scene_t*, float*))&set_scene,
+    (int (*)(thermal_solution::system_t*, thermal_solution::task_t*))&calculate,
+};
+thermal_solution::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  return s;
+}
+
+}  // namespace hip_adams
+
+// ============================================================ C binding ==
+extern "C" {
+
+void* ffp_ray_caster_create(int type) {
+  ray_caster::system_t* s = nullptr;
+  if (type == RAY_CASTER_NAIVE_HIP) s = hip_ray_caster::system_create_naive();
+  else if (type == RAY_CASTER_GRID_HIP) s = hip_ray_caster::system_create_grid();
+  if (!s) return nullptr;
+  if (s->methods->init(s) < 0) {
+    s->methods->shutdown(s);
+    free(s);
+    return nullptr;
+  }
+  return s;
+}
+int ffp_ray_caster_set_scene(void* s, ffp_rc_scene* scene) {
+  ray_caster::system_t* r = (ray_caster::system_t*)s;
+  return r->methods->set_scene(r, (ray_caster::scene_t*)scene);
+}
+int ffp_ray_caster_prepare(void* s) {
+  ray_caster::system_t* r = (ray_caster::system_t*)s;
+  return r->methods->prepare(r);
+}
+int ffp_ray_caster_cast(void* s, ffp_rc_task* task) {
+  ray_caster::system_t* r = (ray_caster::system_t*)s;
+  return r->methods->cast(r, (ray_caster::task_t*)task);
+}
+void ffp_ray_caster_free(void* s) {
+  if (!s) return;
+  ray_caster::system_t* r = (ray_caster::system_t*)s;
+  r->methods->shutdown(r);
+  free(r);
+}
+
+void* ffp_emission_create(int type, void* caster) {
+  emission::system_t* s = nullptr;
+  if (type == EMISSION_MALLEY_HIP) s = hip_malley_emission::system_create();
+  else if (type == EMISSION_PARALLEL_RAYS_HIP) s = hip_parallel_rays_emission::system_create();
+  if (!s) return nullptr;
+  if (s->methods->init(s, (ray_caster::system_t*)caster) < 0) {
+    s->methods->shutdown(s);
+    free(s);
+    return nullptr;
+  }
+  return s;
+}
+int ffp_emission_set_scene(void* s, ffp_rc_scene* scene) {
+  emission::system_t* e = (emission::system_t*)s;
+  return e->methods->set_scene(e, (ray_caster::scene_t*)scene);
+}
+int ffp_emission_calculate_malley(void* s, int n_rays, float total_weight, float* weights, ffp_rc_task** rays_out) {
+  emission::system_t* e = (emission::system_t*)s;
+  malley_emission::task_t task(n_rays, total_weight, weights);
+  int r = e->methods->calculate(e, &task);
+  *rays_out = (ffp_rc_task*)task.rays;
+  return r;
+}
+int ffp_emission_calculate_parallel(void* s, int n_rays, const float* dir3, const float* center3, float distance,
+                                    float radius, ffp_rc_task** rays_out) {
+  emission::system_t* e = (emission::system_t*)s;
+  parallel_rays_emission::task_t task(n_rays);
+  task.direction = {dir3[0], dir3[1], dir3[2]};
+  task.center = {center3[0], center3[1], center3[2]};
+  task.distance = distance;
+  task.radius = radius;
+  int r = e->methods->calculate(e, &task);
+  *rays_out = (ffp_rc_task*)task.rays;
+  return r;
+}
+int ffp_emission_set_seed(void* s, unsigned long long seed, int frame) {
+  emission::system_t* e = (emission::system_t*)s;
+  if (!hip_malley_emission::is_hip(e)) return -EMISSION_ERROR;
+  if (frame != FFK_FRAME_REFERENCE && frame != FFK_FRAME_ORTHONORMAL) return -EMISSION_ERROR;
+  hip_malley_emission::state* st = hip_malley_emission::state_of(e);
+  st->seed = seed;
+  st->calls = 0;
+  if (st->frame != frame) {
+    st->frame = frame;
+    st->front.clear();  // rebuild the layout with the new frame
+  }
+  return EMISSION_OK;
+}
+void ffp_emission_free(void* s) {
+  if (!s) return;
+  emission::system_t* e = (emission::system_t*)s;
+  e->methods->shutdown(e);
+  free(e);
+}
+void ffp_task_free(ffp_rc_task* t) { rc_task_free((ray_caster::task_t*)t); }
+
+void* ffp_form_factors_create(int type, void* emitter) {
+  if (type != FORM_FACTORS_HIP) return nullptr;
+  form_factors::system_t* s = hip_form_factors::system_create();
+  if (!s) return nullptr;
+  if (s->methods->init(s, (emission::system_t*)emitter) < 0) {
+    s->methods->shutdown(s);
+    free(s);
+    return nullptr;
+  }
+  return s;
+}
+int ffp_form_factors_set_scene(void* s, ffp_ff_scene* scene) {
+  form_factors::system_t* f = (form_factors::system_t*)s;
+  return f->methods->set_scene(f, (form_factors::scene_t*)scene);
+}
+int ffp_form_factors_prepare(void* s) {
+  form_factors::system_t* f = (form_factors::system_t*)s;
+  return f->methods->prepare(f);
+}
+int ffp_form_factors_calculate(void* s, ffp_ff_task* task) {
+  form_factors::system_t* f = (form_factors::system_t*)s;
+  return f->methods->calculate(f, (form_factors::task_t*)task);
+}
+void ffp_form_factors_free(void* s) {
+  if (!s) return;
+  form_factors::system_t* f = (form_factors::system_t*)s;
+  f->methods->shutdown(f);
+  free(f);
+}
+
+void* ffp_thermal_equation_create(int type, void* dep, int n_rays, int f_dtype) {
+  thermal_equation::system_t* s = nullptr;
+  if (type == THERMAL_EQUATION_SB_FF_HIP) {
+    s = hip_sb_ff::system_create();
+    if (!s) return nullptr;
+    hip_sb_ff::params_t p = {(form_factors::system_t*)dep, n_rays, f_dtype};
+    if (s->methods->init(s, &p) < 0) {
+      s->methods->shutdown(s);
+      free(s);
+      return nullptr;
+    }
+  }
+  return s;
+}
+int ffp_thermal_equation_set_scene(void* s, ffp_subject_scene* scene) {
+  thermal_equation::system_t* e = (thermal_equation::system_t*)s;
+  return e->methods->set_scene(e, (subject::scene_t*)scene);
+}
+int ffp_thermal_equation_calculate(void* s, ffp_te_task* task) {
+  thermal_equation::system_t* e = (thermal_equation::system_t*)s;
+  return e->methods->calculate(e, (thermal_equation::task_t*)task);
+}
+void ffp_thermal_equation_free(void* s) {
+  if (!s) return;
+  thermal_equation::system_t* e = (thermal_equation::system_t*)s;
+  e->methods->shutdown(e);
+  free(e);
+}
+
+void* ffp_thermal_solution_create(int type, int n_equations, void** equations) {
+  if (type != THERMAL_SOLUTION_HIP_ADAMS) return nullptr;
+  thermal_solution::system_t* s = hip_adams::system_create();
+  if (!s) return nullptr;
+  thermal_solution::params_t p = {n_equations, (thermal_equation::system_t**)equations};
+  if (s->methods->init(s, &p) < 0) {
+    s->methods->shutdown(s);
+    free(s);
+    return nullptr;
+  }
+  return s;
+}
+int ffp_thermal_solution_set_scene(void* s, ffp_subject_scene* scene, float* T) {
+  thermal_solution::system_t* t = (thermal_solution::system_t*)s;
+  return t->methods->set_scene(t, (subject::scene_t*)scene, T);
+}
+int ffp_thermal_solution_calculate(void* s, ffp_ts_task* task) {
+  thermal_solution::system_t* t = (thermal_solution::system_t*)s;
+  return t->methods->calculate(t, (thermal_solution::task_t*)task);
+}
+void ffp_thermal_solution_free(void* s) {
+  if (!s) return;
+  thermal_solution::system_t* t = (thermal_solution::system_t*)s;
+  t->methods->shutdown(t);
+  free(t);
+}
+
+}  // extern "C"

@@ -1,0 +1,69 @@
+"""Batched form factors: many scenes of uneven size through one persistent
+launch (optionally a replayed hipGraph) must give, scene by scene, the same
+F bits as the plain single-scene launch and as the oracle."""
+import numpy as np
+import pytest
+
+from conftest import bits
+
+pytestmark = pytest.mark.gpu
+
+
+def build_scenes(gpu, sizes, rays_per_tri, seed0=100):
+    from form_factors_amd import scenes
+
+    out = []
+    for k, n in enumerate(sizes):
+        faces, meshes = scenes.mixed_scene(n, seed0 + k)
+        sc = gpu.Scene(faces, meshes)
+        sc.set_emission(n_rays=rays_per_tri[k] * len(faces), frame=k % 2)
+        sc.build_grid((0, 0, 0))
+        out.append(sc)
+    return out
+
+
+def test_batched_equals_plain_and_oracle(gpu, oracle):
+    import torch
+
+    sizes = [40, 300, 120, 900, 60]
+    rpt = [400, 100, 300, 60, 1000]
+    scs = build_scenes(gpu, sizes, rpt)
+    seeds = [11 + 7 * k for k in range(len(scs))]
+    plain = [sc.ff_fused(s).cpu().numpy() for sc, s in zip(scs, seeds)]
+    for sc, s, F in zip(scs, seeds, plain):
+        r, want = oracle.ff_calculate(sc.faces, sc.meshes, int(rpt[scs.index(sc)] * len(sc.faces)), seed=s,
+                                      frame=scs.index(sc) % 2, n_threads=8)
+        assert np.array_equal(bits(F), bits(want))
+    Fs = [torch.full((sc.n_meshes, sc.n_meshes), -1.0, device="cuda") for sc in scs]
+    tallies = [torch.empty((sc.n_meshes, sc.n_meshes), dtype=torch.int64, device="cuda") for sc in scs]
+    b = gpu.Batch(scs)
+    for use_graph in (False, True):
+        for rep in range(3):
+            for F in Fs:
+                F.fill_(-1.0)
+            b.run(seeds, Fs, tallies=tallies, use_graph=use_graph)
+            torch.cuda.synchronize()
+            # eager work between graph replays
+            junk = torch.randn(1000, device="cuda").sum().item()
+            assert junk == junk
+            for F, want in zip(Fs, plain):
+                assert np.array_equal(bits(F.cpu().numpy()), bits(want)), (use_graph, rep)
+    b.close()
+
+
+def test_batched_tuning_does_not_change_bits(gpu):
+    import torch
+
+    scs = build_scenes(gpu, [200, 500], [200, 100], seed0=7)
+    seeds = [3, 4]
+    plain = [sc.ff_fused(s).cpu().numpy() for sc, s in zip(scs, seeds)]
+    b = gpu.Batch(scs)
+    for block, rays_per_tile, n_blocks in [(256, 1024, 0), (512, 4096, 64), (1024, 0, 7)]:
+        b.set_tuning(block, rays_per_tile, n_blocks)
+        Fs = [torch.empty((sc.n_meshes,) * 2, device="cuda") for sc in scs]
+        tallies = [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda") for sc in scs]
+        b.run(seeds, Fs, tallies=tallies, use_graph=True)
+        torch.cuda.synchronize()
+        for F, want in zip(Fs, plain):
+            assert np.array_equal(bits(F.cpu().numpy()), bits(want)), (block, rays_per_tile, n_blocks)
+    b.close()

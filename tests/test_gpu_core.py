@@ -77,7 +77,7 @@ def test_cast_edge_cases(gpu, oracle):
                       [0, 0, 1, 1, 0, 1, 0, 1, 1],
                       [0, 0, 1, 1, 0, 1, 0, 1, 1],            # duplicate of face 1
                       [0, 0, 2, 1, 0, 2, 1, 1, 2]], np.float32)
-    rays = np.array([[0.002, 0.002, 1.5, 0.002, 0.002, -1],   # through the degenerate face
+    rays = np.array([[0.002, 0.002, 0.5, 0.002, 0.002, -1],   # through the degenerate face only
                      [0.2, 0.2, 0, 0.2, 0.2, 0.5],            # hits the duplicate pair
                      [0.2, 0.2, 1, 0.7, 0.2, 1],              # in the plane of face 1
                      [0.2, 0.2, 3, 0.2, 0.2, 3],              # zero direction

@@ -1,0 +1,337 @@
+"""The drop-in path: the reference's own test scenarios driven through the
+plugin method tables of libff_amd.so (ff_plugin.h), for the HIP factory
+slots. Mirrors test/ray_casting.cpp, test/form_factors.cpp and
+test/adams_solution.cpp."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bits, golden
+
+pytestmark = pytest.mark.gpu
+
+CASTERS = [2, 4]  # RAY_CASTER_NAIVE_HIP, RAY_CASTER_GRID_HIP
+
+
+@pytest.fixture(scope="module")
+def P(gpu):
+    from form_factors_amd import plugin
+
+    plugin.lib()
+    return plugin
+
+
+def caster(P, t):
+    s = P.lib().ffp_ray_caster_create(t)
+    assert s
+    return s
+
+
+# ------------------------------------------------- test/ray_casting.cpp --
+
+@pytest.mark.parametrize("t", CASTERS)
+def test_ray_caster_lifecycle_and_errors(P, t):
+    L = P.lib()
+    s = caster(P, t)
+    # PrepareFailsForNoScene
+    assert L.ffp_ray_caster_prepare(s) == -P.RAY_CASTER_ERROR
+    # AcceptEmptyScene / PrepareFailsForEmptyScene
+    empty = P.RcScene(0, None)
+    assert L.ffp_ray_caster_set_scene(s, C.byref(empty)) == 0
+    assert L.ffp_ray_caster_prepare(s) == -P.RAY_CASTER_ERROR
+    # AcceptFloorScene / PreparePassForNotEmptyScene
+    from test_oracle import _floor_scene
+
+    fa = P.faces_array(_floor_scene())
+    sc = P.rc_scene(fa, 2)
+    assert L.ffp_ray_caster_set_scene(s, C.byref(sc)) == 0
+    assert L.ffp_ray_caster_prepare(s) == 0
+    L.ffp_ray_caster_free(s)
+    # unknown factory types are NULL, not a crash
+    assert not L.ffp_ray_caster_create(1) and not L.ffp_ray_caster_create(99)
+
+
+@pytest.mark.parametrize("t", CASTERS)
+def test_ray_caster_reference_kats(P, t):
+    from test_oracle import reference_cast_kats
+
+    L = P.lib()
+    for name, faces, rays, expect in reference_cast_kats():
+        s = caster(P, t)
+        fa = P.faces_array(faces)
+        sc = P.rc_scene(fa, len(faces))
+        assert L.ffp_ray_caster_set_scene(s, C.byref(sc)) == 0
+        assert L.ffp_ray_caster_prepare(s) == 0
+        ra = P.rays_array(rays)
+        task, hf, hp = P.rc_task(ra, len(rays))
+        assert L.ffp_ray_caster_cast(s, C.byref(task)) == 0
+        idx = P.hit_indices(task, fa)
+        assert (idx >= 0).all(), name
+        d = P.task_points(task) - expect
+        assert (np.einsum("ij,ij->i", d, d) < 1e-8).all(), name
+        L.ffp_ray_caster_free(s)
+
+
+@pytest.mark.parametrize("t", CASTERS)
+def test_ray_caster_golden_through_plugin(P, t):
+    L = P.lib()
+    g = golden("cast_confetti.npz")
+    s = caster(P, t)
+    fa = P.faces_array(g["faces"])
+    sc = P.rc_scene(fa, len(g["faces"]))
+    L.ffp_ray_caster_set_scene(s, C.byref(sc))
+    assert L.ffp_ray_caster_prepare(s) == 0
+    ra = P.rays_array(g["rays"])
+    task, hf, hp = P.rc_task(ra, len(g["rays"]))
+    assert L.ffp_ray_caster_cast(s, C.byref(task)) == 0
+    idx = P.hit_indices(task, fa)
+    assert np.array_equal(idx, g["naive_idx"])
+    pts = P.task_points(task)
+    assert np.array_equal(bits(pts[idx >= 0]), bits(g["naive_pts"][idx >= 0]))
+    L.ffp_ray_caster_free(s)
+
+
+# ---------------------------------------------------------- emission ----
+
+@pytest.mark.parametrize("t", CASTERS)
+def test_malley_emitter_rays_and_hits(P, oracle, t):
+    L = P.lib()
+    g = golden("cast_sphere3.npz")
+    faces = g["faces"]
+    rc = caster(P, t)
+    em = L.ffp_emission_create(P.EMISSION_MALLEY_HIP, rc)
+    assert L.ffp_emission_set_seed(em, 777, 0) == 0
+    fa = P.faces_array(faces)
+    sc = P.rc_scene(fa, len(faces))
+    assert L.ffp_emission_set_scene(em, C.byref(sc)) == 0
+    w, total = oracle.area_weights(faces)
+    out = C.POINTER(P.RcTask)()
+    wc = np.ascontiguousarray(w)
+    assert L.ffp_emission_calculate_malley(em, 20000, float(total), wc.ctypes.data_as(C.POINTER(C.c_float)),
+                                           C.byref(out)) == 0
+    task = out.contents
+    rays = P.task_rays(task)
+    want = oracle.malley_emit(faces, w, total, 20000, rng=oracle.RNG_PHILOX, seed=777, frame=0)
+    assert np.array_equal(bits(rays), bits(want))
+    idx = P.hit_indices(task, fa)
+    ref_i, _ = oracle.naive_cast(faces, want, n_threads=8)
+    assert np.array_equal(idx, ref_i)
+    L.ffp_task_free(out)
+    # a second call draws a different ray set (key advances)
+    assert L.ffp_emission_calculate_malley(em, 20000, float(total), wc.ctypes.data_as(C.POINTER(C.c_float)),
+                                           C.byref(out)) == 0
+    assert not np.array_equal(P.task_rays(out.contents), rays)
+    L.ffp_task_free(out)
+    # empty scene
+    empty = P.RcScene(0, None)
+    assert L.ffp_emission_set_scene(em, C.byref(empty)) == -P.EMISSION_ERROR
+    L.ffp_emission_free(em)
+    L.ffp_ray_caster_free(rc)
+
+
+# ------------------------------------------------- test/form_factors.cpp --
+
+class FF:
+    def __init__(self, P, caster_type=4, seed=0, frame=0):
+        self.P = P
+        L = P.lib()
+        self.rc = caster(P, caster_type)
+        self.em = L.ffp_emission_create(P.EMISSION_MALLEY_HIP, self.rc)
+        L.ffp_emission_set_seed(self.em, seed, frame)
+        self.ff = L.ffp_form_factors_create(P.FORM_FACTORS_HIP, self.em)
+        assert self.rc and self.em and self.ff
+
+    def run(self, faces, meshes, n_rays, prepare_code=0):
+        P, L = self.P, self.P.lib()
+        self.fa = P.faces_array(faces)
+        m = np.ascontiguousarray(meshes, np.int32).reshape(-1, 3)
+        self.ma = (P.Mesh * max(1, len(m)))()
+        C.memmove(self.ma, m.ctypes.data, m.nbytes)
+        self.scene = P.FfScene(len(faces), C.cast(self.fa, C.POINTER(P.Face)), len(m),
+                               C.cast(self.ma, C.POINTER(P.Mesh)))
+        L.ffp_form_factors_set_scene(self.ff, C.byref(self.scene))
+        r = L.ffp_form_factors_prepare(self.ff)
+        assert r == prepare_code
+        if r < 0:
+            return None
+        F = np.zeros(len(m) * len(m), np.float32)
+        task = P.FfTask(n_rays, F.ctypes.data_as(C.POINTER(C.c_float)))
+        assert L.ffp_form_factors_calculate(self.ff, C.byref(task)) == 0
+        return F.reshape(len(m), len(m))
+
+    def close(self):
+        L = self.P.lib()
+        L.ffp_form_factors_free(self.ff)
+        L.ffp_emission_free(self.em)
+        L.ffp_ray_caster_free(self.rc)
+
+
+def test_form_factors_reference_analytic(P):
+    from form_factors_amd import scenes
+    from test_oracle import theor_parallel, theor_perpendicular
+
+    g = golden("ff.npz")
+    axis = g["axis_rotation"]
+    for c in (0.5, 1, 2, 3, 10):
+        for rot in (None, axis):
+            ff = FF(P)
+            F = ff.run(*scenes.ff_parallel_planes(c, rot), 40_000)
+            t = theor_parallel(1, 1, c)
+            assert abs(F[0, 1] - t) < 0.01 and abs(F[1, 0] - t) < 0.01, (c, F)
+            ff.close()
+    for rot in (None, axis):
+        ff = FF(P)
+        F = ff.run(*scenes.ff_perpendicular_planes(rot), 40_000)
+        assert abs(F[1, 0] - theor_perpendicular(1, 1, 1)) < 0.01
+        ff.close()
+    ff = FF(P)
+    F = ff.run(*scenes.ff_box(), 40_000)
+    assert abs(F[0, 0] - 0.5) < 0.01
+    ff.close()
+
+
+@pytest.mark.parametrize("caster_type", CASTERS)
+def test_form_factors_plugin_bit_exact_vs_oracle(P, oracle, caster_type):
+    # grid caster: fused device path; naive caster: emitter casts, device tally
+    from form_factors_amd import scenes
+
+    faces, meshes = scenes.confetti_scene(120, cells=2, seed=2)
+    ff = FF(P, caster_type, seed=31)
+    F = ff.run(faces, meshes, 30_000)
+    r, want = oracle.ff_calculate(faces, meshes, 30_000, seed=31, n_threads=8)
+    assert np.array_equal(bits(F), bits(want))
+    ff.close()
+
+
+def test_form_factors_error_codes(P):
+    ff = FF(P)
+    z = np.zeros((2, 9), np.float32)
+    ff.run(z, np.array([[0, 2, 0]], np.int32), 1000, prepare_code=-P.FORM_FACTORS_SCENE_TOO_SMALL)
+    ff.close()
+    ff = FF(P)
+    from form_factors_amd import scenes
+
+    faces, _ = scenes.ff_box()
+    ff.run(faces, np.zeros((0, 3), np.int32), 1000, prepare_code=-P.FORM_FACTORS_EMPTY_SCENE)
+    ff.close()
+
+
+# --------------------------------------------- test/adams_solution.cpp --
+
+class PyEquation:
+    """A reference-shaped thermal equation whose calculate() adds constant
+    emission and absorption per mesh (the shape of the mock equation in
+    test/adams_solution.cpp:24-83)."""
+
+    def __init__(self, P, e, a):
+        self.e, self.a = e, a
+        self.n = 0
+        self.cbs = (P.EQ_INIT(lambda s, p: 0), P.EQ_SHUTDOWN(lambda s: 0), P.EQ_SET_SCENE(self._set),
+                    P.EQ_CALCULATE(self._calc))
+        self.methods = P.EqMethods(*self.cbs)
+        self.sys = P.EqSystem(C.pointer(self.methods))
+
+    def _set(self, s, scene):
+        self.n = scene.contents.n_meshes
+        return 0
+
+    def _calc(self, s, task):
+        t = task.contents
+        for m in range(self.n):
+            t.emission[m] += self.e[m]
+            t.absorption[m] += self.a[m]
+        return 0
+
+
+def subject_scene(P, faces, meshes, materials):
+    fa = P.faces_array(faces)
+    m = np.ascontiguousarray(meshes, np.int32).reshape(-1, 3)
+    ma = (P.SubjectMesh * len(m))()
+    C.memmove(ma, m.ctypes.data, m.nbytes)
+    mat = (P.Material * len(materials))(*materials)
+    sc = P.SubjectScene(len(faces), C.cast(fa, C.POINTER(P.Face)), len(m), C.cast(ma, C.POINTER(P.SubjectMesh)),
+                        len(materials), C.cast(mat, C.POINTER(P.Material)))
+    return sc, (fa, ma, mat)
+
+
+@pytest.mark.parametrize("name", ["unit", "thick2", "rho2", "c2"])
+def test_adams_reference_kats(P, name):
+    L = P.lib()
+    g = golden("adams.npz")
+    eq = PyEquation(P, [2.0], [1.0])
+    eqs = (C.c_void_p * 1)(C.addressof(eq.sys))
+    sol = L.ffp_thermal_solution_create(P.THERMAL_SOLUTION_HIP_ADAMS, 1, eqs)
+    mat = P.black_body()
+    rho, cap, thick = (float(x) for x in g[f"{name}_shell"])
+    mat.density, mat.heat_capacity, mat.thickness = rho, cap, thick
+    face = np.array([[0, 0, 0, 1.41421, 0, 0, 0, 1.41421, 0]], np.float32)
+    sc, keep = subject_scene(P, face, [[0, 1, 0]], [mat])
+    T0 = (C.c_float * 1)(300.0)
+    assert L.ffp_thermal_solution_set_scene(sol, C.byref(sc), T0) == 0
+    out = (C.c_float * 1)()
+    task = P.TsTask(0, 1.0, out)
+    for step in range(11):
+        assert L.ffp_thermal_solution_calculate(sol, C.byref(task)) == 0
+        assert task.n_step == step + 1
+        # FP64 update vs the reference's float trajectory
+        assert abs(out[0] - g[f"{name}_T"][step, 0]) < 2e-3
+        # test/adams_solution.cpp:161-242: 300 - step / (thickness*density*capacity), within 0.01
+        assert abs(out[0] - (300.0 - (step + 1) / (rho * cap * thick))) < 0.01
+    L.ffp_thermal_solution_free(sol)
+
+
+def test_sbff_equation_with_adams_two_plates(P, oracle):
+    """SB_FF (F in FP16/BF16 on the device) + FP64 Adams on two facing plates
+    against an all-FP64 recomputation from the same F."""
+    from form_factors_amd import scenes
+
+    L = P.lib()
+    faces, meshes = scenes.two_plates_thermal(cells=4, gap=0.5)
+    n = len(meshes)
+    mats = []
+    for k in range(2):
+        m = P.black_body()
+        m.front[4] = m.rear[4] = 0.8
+        m.density, m.heat_capacity, m.thickness = 2698.9, 903.0, 0.001
+        mats.append(m)
+    smeshes = meshes.copy()
+    smeshes[:, 2] = 0
+    sc, keep = subject_scene(P, faces, smeshes, mats)
+    results = {}
+    for dtype in (0, 1, 2):
+        ff = FF(P, 4, seed=5)
+        eq = L.ffp_thermal_equation_create(P.THERMAL_EQUATION_SB_FF_HIP, ff.ff, 200_000, dtype)
+        eqs = (C.c_void_p * 1)(eq)
+        sol = L.ffp_thermal_solution_create(P.THERMAL_SOLUTION_HIP_ADAMS, 1, eqs)
+        T0 = (C.c_float * n)(*([400.0] * (n // 2) + [300.0] * (n // 2)))
+        assert L.ffp_thermal_solution_set_scene(sol, C.byref(sc), T0) == 0
+        out = (C.c_float * n)()
+        task = P.TsTask(0, 1.0, out)
+        traj = []
+        for step in range(30):
+            assert L.ffp_thermal_solution_calculate(sol, C.byref(task)) == 0
+            traj.append(np.frombuffer(out, np.float32).copy())
+        results[dtype] = np.array(traj)
+        L.ffp_thermal_solution_free(sol)
+        L.ffp_thermal_equation_free(eq)
+        ff.close()
+    # FP64 oracle from the same F (same seed => same F as the FP32 run)
+    r, F = oracle.ff_calculate(faces, meshes, 200_000, seed=5, n_threads=8)
+    p = faces.reshape(-1, 3, 3).astype(np.float32)
+    fa = np.array([oracle.lib().or_triangle_area(np.ascontiguousarray(f).ctypes.data_as(oracle._f)) for f in faces])
+    A = np.array([np.float32(fa[a] + np.float32(fa[a + 1])) for a in range(0, len(faces), 2)], np.float64)
+    C_ = A * 0.001 * 2698.9 * 903.0
+    eps = np.full(n, 0.8)
+    T = np.array([400.0] * (n // 2) + [300.0] * (n // 2))
+    E = np.zeros((5, n))
+    for step in range(30):
+        em, ab = oracle.sbff_f64(F.astype(np.float64), T, A, eps, eps)
+        E[step % 5] = (ab - em) / C_
+        T = oracle.adams_combine_f64(step, 1.0, T, E)
+        rel32 = np.abs(results[0][step] - T).max() / T.max()
+        rel16 = np.abs(results[2][step] - T).max() / T.max()
+        relbf = np.abs(results[1][step] - T).max() / T.max()
+        assert rel32 < 2e-6 and rel16 < 1e-4 and relbf < 1e-4, (step, rel32, rel16, relbf)
+    assert results[0][-1][: n // 2].mean() < 400  # the hot plate cools
