@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--rays-per-tile", type=int, default=0)
     ap.add_argument("--hist", type=int, default=-1)
+    ap.add_argument("--refill", type=int, default=0)
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="run one untimed device copy of this size (HBM counter calibration under rocprofv3)")
     return ap.parse_args()
 
 
@@ -105,12 +108,18 @@ def main():
     sc = ffk.Scene(faces, meshes)
     n_rays = sc.set_emission(n_rays=n_req, frame=ffk.FRAME_ORTHONORMAL)
     g = sc.build_grid((args.grid,) * 3)
-    sc.set_tuning(block=args.block, rays_per_tile=args.rays_per_tile, hist=args.hist)
+    sc.set_tuning(block=args.block, rays_per_tile=args.rays_per_tile, hist=args.hist, refill=args.refill)
     N = len(meshes)
     log(f"scene: {len(faces)} faces, {N} meshes, {n_rays} rays, grid {g['dims']} refs {g['n_refs']}")
     F = torch.empty((N, N), dtype=torch.float32, device="cuda")
     stream = torch.cuda.current_stream()
     seed = 0x5EED
+    if args.calib_copy_mib:
+        src = torch.ones(args.calib_copy_mib << 18, dtype=torch.float32, device="cuda")
+        dst = src.clone()
+        torch.cuda.synchronize()
+        log(f"calibration copy: {args.calib_copy_mib} MiB read + {args.calib_copy_mib} MiB written")
+        del src, dst
 
     # Device-counted work for the roofline (one untimed launch with counters).
     stats = torch.zeros(2, dtype=torch.int64, device="cuda")

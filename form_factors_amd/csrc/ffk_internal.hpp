@@ -66,11 +66,12 @@ __host__ __device__ inline v3 cross(v3 a, v3 b) {
 }
 
 // Per-face intersection terms (64 B, four float4):
-//   q0 = {p0, uu}  q1 = {u, uv}  q2 = {v, vv}  q3 = {n, D}
-// u = p1-p0, v = p2-p0, n = u x v, uu/uv/vv = dots, D = uv*uv - uu*vv, all
-// with the reference's expressions (math/triangle.cpp:35-38,57-63). A
-// degenerate face ((double)dot(n,n) < 1e-8) is stored with n = 0, which the
-// parallel-plane test then rejects for every ray, as the reference rejects it.
+//   q0 = {p0, uu}  q1 = {u, uv}  q2 = {v, vv}  q3 = {1/D, D, degenerate, 0}
+// u = p1-p0, v = p2-p0, uu/uv/vv = dots, D = uv*uv - uu*vv, all with the
+// reference's expressions (math/triangle.cpp:35-38,57-63). The normal is
+// recomputed as u x v at test time (the reference's expression, so the same
+// bits). degenerate = 1 for (double)dot(n,n) < 1e-8: such a face is never
+// hit (math/triangle.cpp:38) and is never put in the grid.
 struct tri_terms {
   float4 q0, q1, q2, q3;
 };
@@ -78,40 +79,50 @@ struct tri_terms {
 __host__ __device__ inline tri_terms make_terms(const float* f) {
   v3 p0 = mk(f[0], f[1], f[2]), p1 = mk(f[3], f[4], f[5]), p2 = mk(f[6], f[7], f[8]);
   v3 u = p1 - p0, v = p2 - p0, n = cross(u, v);
-  if (dot(n, n) <= kEps8) n = mk(0.f, 0.f, 0.f);
+  const bool deg = !(dot(n, n) > kEps8);
   float uu = dot(u, u), uv = dot(u, v), vv = dot(v, v);
   float D = uv * uv - uu * vv;
   tri_terms t;
   t.q0 = make_float4(p0.x, p0.y, p0.z, uu);
   t.q1 = make_float4(u.x, u.y, u.z, uv);
   t.q2 = make_float4(v.x, v.y, v.z, vv);
-  t.q3 = make_float4(n.x, n.y, n.z, D);
+  t.q3 = make_float4(1.0f / D, D, deg ? 1.f : 0.f, 0.f);
   return t;
 }
 
 // The reference test (math/triangle.cpp:28-79) on precomputed terms, for a
 // ray with origin o and direction dir = end - origin. Returns true for a
 // unique hit and writes the point and its squared distance to o.
+//
+// Before the two divisions of the inside test, s' = sn * (1/D) and
+// t' = tn * (1/D) are compared with margins far above their rounding error
+// (<= 1.2e-7 relative): a face rejected there is also rejected by the exact
+// test, and every other face takes the exact test, so the result is the
+// reference's bit for bit.
 __host__ __device__ inline bool hit_terms(const float4 q0, const float4 q1, const float4 q2,
                                           const float4 q3, v3 o, v3 dir, v3* point, float* d2) {
-  v3 n = mk(q3.x, q3.y, q3.z);
-  v3 p0 = mk(q0.x, q0.y, q0.z);
-  v3 w0 = o - p0;
-  float a = -dot(n, w0);
-  float b = dot(n, dir);
+  if (q3.z != 0.f) return false;  // degenerate
+  const v3 p0 = mk(q0.x, q0.y, q0.z), u = mk(q1.x, q1.y, q1.z), v = mk(q2.x, q2.y, q2.z);
+  const v3 n = cross(u, v);
+  const v3 w0 = o - p0;
+  const float a = -dot(n, w0);
+  const float b = dot(n, dir);
   if (fabsf(b) <= kEps8) return false;
-  float r = a / b;
+  const float r = a / b;
   if (r < 0.0f) return false;
-  v3 p = o + dir * r;
-  v3 w = p - p0;
-  v3 u = mk(q1.x, q1.y, q1.z), v = mk(q2.x, q2.y, q2.z);
-  float uu = q0.w, uv = q1.w, vv = q2.w, D = q3.w;
-  float wu = dot(w, u), wv = dot(w, v);
-  float s = (uv * wv - vv * wu) / D;
+  const v3 p = o + dir * r;
+  const v3 w = p - p0;
+  const float uu = q0.w, uv = q1.w, vv = q2.w;
+  const float wu = dot(w, u), wv = dot(w, v);
+  const float sn = uv * wv - vv * wu;
+  const float tn = uv * wu - uu * wv;
+  const float s1 = sn * q3.x, t1 = tn * q3.x;
+  if (s1 < -1e-30f || s1 > 1.000001f || t1 < -1e-30f || s1 + t1 > 1.00001f) return false;
+  const float s = sn / q3.y;
   if (s < 0.0f || s > 1.0f) return false;
-  float t = (uv * wu - uu * wv) / D;
+  const float t = tn / q3.y;
   if (t < 0.0f || (s + t) > 1.0f) return false;
-  v3 sd = p - o;
+  const v3 sd = p - o;
   *point = p;
   *d2 = dot(sd, sd);
   return true;
@@ -234,7 +245,9 @@ struct scene_desc {
   const int* face2mesh;        // -1 for faces outside every mesh
   const long long* mesh_off;   // n_meshes+1: mesh-order tally ranges
   const int* cell_start;
-  const int* refs;
+  const int* refs;             // face of each reference, in cell order
+  const int2* cell_info;       // (first ref, count) or, for an empty cell, (r, 0):
+                               // every cell within Chebyshev distance r-1 is empty
   grid_desc g;
   int n_faces;
   int n_meshes;
@@ -269,6 +282,7 @@ struct ffk_scene {
   long long* d_mesh_off = nullptr;
   int* d_cell_start = nullptr;
   int* d_refs = nullptr;
+  int2* d_cell_info = nullptr;
   long long n_refs = 0;
   ffk::grid_desc grid = {};
   // fused-kernel work list
@@ -283,7 +297,7 @@ struct ffk_scene {
   std::vector<int> h_meshes, h_front, h_rear;
   std::vector<long long> h_face_off, h_mesh_off;
   // tuning (0 = automatic)
-  int tune_block = 0, tune_rays_per_tile = 0, tune_hist = -1;
+  int tune_block = 0, tune_rays_per_tile = 0, tune_hist = -1, tune_refill = 0;
   int tiles_rays_per_tile = 0;  // value the tile list was built with
 
   ffk::scene_desc desc() const;

@@ -123,13 +123,79 @@ __device__ inline void cell_range(const float* p, const grid_desc& g, float pad,
   }
 }
 
+// Separating-axis test of a triangle against an axis-aligned box (centre c,
+// half size h), in double: the 3 box axes, the triangle normal and the 9
+// edge x axis products (Akenine-Moller, "Fast 3D triangle-box overlap
+// testing", JGT 2001). Returns true when no axis separates them.
+__device__ inline bool tri_box_overlap(const float* p, const double c[3], const double h[3]) {
+  double v[3][3];
+  for (int k = 0; k < 3; ++k)
+    for (int a = 0; a < 3; ++a) v[k][a] = (double)p[3 * k + a] - c[a];
+  for (int a = 0; a < 3; ++a) {  // box axes
+    double lo = fmin(fmin(v[0][a], v[1][a]), v[2][a]), hi = fmax(fmax(v[0][a], v[1][a]), v[2][a]);
+    if (lo > h[a] || hi < -h[a]) return false;
+  }
+  double e[3][3];
+  for (int a = 0; a < 3; ++a) {
+    e[0][a] = v[1][a] - v[0][a];
+    e[1][a] = v[2][a] - v[1][a];
+    e[2][a] = v[0][a] - v[2][a];
+  }
+  double n[3] = {e[0][1] * e[1][2] - e[0][2] * e[1][1], e[0][2] * e[1][0] - e[0][0] * e[1][2],
+                 e[0][0] * e[1][1] - e[0][1] * e[1][0]};
+  {  // triangle plane
+    double d = n[0] * v[0][0] + n[1] * v[0][1] + n[2] * v[0][2];
+    double r = h[0] * fabs(n[0]) + h[1] * fabs(n[1]) + h[2] * fabs(n[2]);
+    if (fabs(d) > r) return false;
+  }
+  for (int i = 0; i < 3; ++i)      // edge
+    for (int a = 0; a < 3; ++a) {  // box axis
+      // axis = e_i x unit_a
+      double ax[3] = {0, 0, 0};
+      const int b = (a + 1) % 3, c2 = (a + 2) % 3;
+      ax[b] = e[i][c2];
+      ax[c2] = -e[i][b];
+      double p0 = ax[0] * v[0][0] + ax[1] * v[0][1] + ax[2] * v[0][2];
+      double p1 = ax[0] * v[1][0] + ax[1] * v[1][1] + ax[2] * v[1][2];
+      double p2 = ax[0] * v[2][0] + ax[1] * v[2][1] + ax[2] * v[2][2];
+      double lo = fmin(fmin(p0, p1), p2), hi = fmax(fmax(p0, p1), p2);
+      double r = h[0] * fabs(ax[0]) + h[1] * fabs(ax[1]) + h[2] * fabs(ax[2]);
+      if (lo > r || hi < -r) return false;
+    }
+  return true;
+}
+
+// Degenerate faces ((double)dot(n,n) < 1e-8, math/triangle.cpp:38) are never
+// hit, so they are not referenced.
+__device__ inline bool degenerate(const float* p) {
+  v3 p0 = mk(p[0], p[1], p[2]), p1 = mk(p[3], p[4], p[5]), p2 = mk(p[6], p[7], p[8]);
+  v3 n = cross(p1 - p0, p2 - p0);
+  return !(dot(n, n) > kEps8);
+}
+
+// Visit every cell whose box grown by `pad` the face overlaps.
+template <class F>
+__device__ inline void for_face_cells(const float* p, const grid_desc& g, float pad, F fn) {
+  if (degenerate(p)) return;
+  int c0[3], c1[3];
+  cell_range(p, g, pad, c0, c1);
+  const double h[3] = {0.5 * g.h[0] + pad, 0.5 * g.h[1] + pad, 0.5 * g.h[2] + pad};
+  for (int z = c0[2]; z <= c1[2]; ++z)
+    for (int y = c0[1]; y <= c1[1]; ++y)
+      for (int x = c0[0]; x <= c1[0]; ++x) {
+        const double c[3] = {g.lo[0] + (x + 0.5) * (double)g.h[0], g.lo[1] + (y + 0.5) * (double)g.h[1],
+                             g.lo[2] + (z + 0.5) * (double)g.h[2]};
+        if (tri_box_overlap(p, c, h)) fn((uint32_t)(x + g.n[0] * (y + g.n[1] * z)));
+      }
+}
+
 __global__ void k_ref_count(const float* __restrict__ faces, int n, grid_desc g, float pad,
                             uint32_t* __restrict__ cnt) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n) return;
-  int c0[3], c1[3];
-  cell_range(faces + 9 * (size_t)f, g, pad, c0, c1);
-  cnt[f] = (uint32_t)((c1[0] - c0[0] + 1) * (c1[1] - c0[1] + 1) * (c1[2] - c0[2] + 1));
+  uint32_t k = 0;
+  for_face_cells(faces + 9 * (size_t)f, g, pad, [&](uint32_t) { ++k; });
+  cnt[f] = k;
 }
 
 __global__ void k_ref_emit(const float* __restrict__ faces, int n, grid_desc g, float pad,
@@ -137,22 +203,55 @@ __global__ void k_ref_emit(const float* __restrict__ faces, int n, grid_desc g, 
                            int* __restrict__ vals) {
   int f = blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= n) return;
-  int c0[3], c1[3];
-  cell_range(faces + 9 * (size_t)f, g, pad, c0, c1);
   uint32_t o = off[f];
-  for (int z = c0[2]; z <= c1[2]; ++z)
-    for (int y = c0[1]; y <= c1[1]; ++y)
-      for (int x = c0[0]; x <= c1[0]; ++x) {
-        keys[o] = (uint32_t)(x + g.n[0] * (y + g.n[1] * z));
-        vals[o] = f;
-        ++o;
-      }
+  const uint32_t end = off[f + 1];
+  for_face_cells(faces + 9 * (size_t)f, g, pad, [&](uint32_t cell) {
+    if (o < end) {  // the count pass visited the same cells
+      keys[o] = cell;
+      vals[o] = f;
+      ++o;
+    }
+  });
 }
 
 __global__ void k_cell_count(const uint32_t* __restrict__ keys, long long n, int* __restrict__ cnt) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   atomicAdd(&cnt[keys[i]], 1);
+}
+
+// Chessboard distance (in cells) from each cell to the nearest non-empty
+// cell: 0 for non-empty cells, relaxed over the 26 neighbours one ring per
+// pass, capped at 255. Cells outside the grid count as empty.
+__global__ void k_empty_init(const int* __restrict__ cs, long long nc, uint8_t* __restrict__ d) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nc) d[c] = cs[c + 1] > cs[c] ? 0 : 255;
+}
+
+__global__ void k_empty_relax(grid_desc g, const uint8_t* __restrict__ in, uint8_t* __restrict__ out) {
+  const long long nc = (long long)g.n[0] * g.n[1] * g.n[2];
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int x = (int)(c % g.n[0]);
+  const int y = (int)((c / g.n[0]) % g.n[1]);
+  const int z = (int)(c / ((long long)g.n[0] * g.n[1]));
+  int best = in[c];
+  for (int dz = -1; dz <= 1; ++dz)
+    for (int dy = -1; dy <= 1; ++dy)
+      for (int dx = -1; dx <= 1; ++dx) {
+        const int X = x + dx, Y = y + dy, Z = z + dz;
+        if (X < 0 || Y < 0 || Z < 0 || X >= g.n[0] || Y >= g.n[1] || Z >= g.n[2]) continue;
+        best = min(best, (int)in[X + (long long)g.n[0] * (Y + (long long)g.n[1] * Z)] + 1);
+      }
+  out[c] = (uint8_t)min(best, 255);
+}
+
+__global__ void k_cell_info(const int* __restrict__ cs, const uint8_t* __restrict__ d, long long nc,
+                            int2* __restrict__ info) {
+  long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const int a = cs[c], b = cs[c + 1];
+  info[c] = b > a ? make_int2(a, b - a) : make_int2((int)d[c], 0);
 }
 
 }  // namespace ffk
@@ -169,6 +268,7 @@ ffk::scene_desc ffk_scene::desc() const {
   d.mesh_off = d_mesh_off;
   d.cell_start = d_cell_start;
   d.refs = d_refs;
+  d.cell_info = d_cell_info;
   d.g = grid;
   d.n_faces = n_faces;
   d.n_meshes = n_meshes;
@@ -179,8 +279,10 @@ ffk::scene_desc ffk_scene::desc() const {
 static void free_grid(ffk_scene* s) {
   dfree(s->d_cell_start);
   dfree(s->d_refs);
+  dfree(s->d_cell_info);
   s->d_cell_start = nullptr;
   s->d_refs = nullptr;
+  s->d_cell_info = nullptr;
   s->n_refs = 0;
   s->has_grid = false;
 }
@@ -451,7 +553,23 @@ int ffk_scene_build_grid(ffk_scene* s, int nx, int ny, int nz, void* stream) {
   void* d_tmp2 = nullptr;
   if ((r = dmalloc(&d_tmp2, tmp2)) < 0) return r;
   FFK_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp2, tmp2, d_ccnt, s->d_cell_start, nc + 1, st));
+  // 5. the per-cell (range | empty radius) words
+  if ((r = dalloc(&s->d_cell_info, (size_t)nc)) < 0) return r;
+  uint8_t *d_e0 = nullptr, *d_e1 = nullptr;
+  if ((r = dalloc(&d_e0, (size_t)nc)) < 0) return r;
+  if ((r = dalloc(&d_e1, (size_t)nc)) < 0) return r;
+  const unsigned cblocks = (unsigned)((nc + 255) / 256);
+  k_empty_init<<<cblocks, 256, 0, st>>>(s->d_cell_start, nc, d_e0);
+  const int passes = std::min(255, std::max(dims[0], std::max(dims[1], dims[2])));
+  for (int p = 0; p < passes; ++p) {
+    k_empty_relax<<<cblocks, 256, 0, st>>>(g, d_e0, d_e1);
+    std::swap(d_e0, d_e1);
+  }
+  k_cell_info<<<cblocks, 256, 0, st>>>(s->d_cell_start, d_e0, nc, s->d_cell_info);
+  FFK_HIP(hipGetLastError());
   FFK_HIP(hipStreamSynchronize(st));
+  dfree(d_e0);
+  dfree(d_e1);
   dfree(d_tmp);
   dfree(d_tmp2);
   dfree(d_ccnt);

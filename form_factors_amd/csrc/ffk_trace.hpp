@@ -30,16 +30,45 @@ __device__ inline void test_face(const tri_terms* __restrict__ tri, int f, v3 o,
   if (hit_terms(q0, q1, q2, q3, o, dir, &p, &d2)) consider(h, f, d2, p);
 }
 
-// Walk the uniform grid front to back (3-D DDA). Every face whose
-// (padded) box overlaps a visited cell is tested with the reference test;
-// the walk stops once the best squared distance is below the squared
-// distance to the current cell's exit (with a 1e-4 relative margin, far
-// above the rounding of both), so the result equals testing every face.
+// Faces tested recently by this ray: a face met again in a later cell gives
+// the same result, so it is not tested twice.
+struct mailbox {
+  int f0, f1, f2, f3;
+  __device__ void clear() { f0 = f1 = f2 = f3 = -1; }
+  __device__ bool seen(int f) const { return f == f0 || f == f1 || f == f2 || f == f3; }
+  __device__ void put(int f) {
+    f3 = f2;
+    f2 = f1;
+    f1 = f0;
+    f0 = f;
+  }
+};
+
+// Reference k of the grid (a face index, in cell order): the reference test
+// on that face's terms, unless the ray met the face in an earlier cell.
+// Returns 1 if a test was run.
+__device__ inline uint32_t test_ref(const scene_desc& S, int k, v3 o, v3 dir, hit_rec& h, mailbox& mb) {
+  const int f = S.refs[k];
+  if (mb.seen(f)) return 0;
+  mb.put(f);
+  test_face(S.tri, f, o, dir, h);
+  return 1;
+}
+
+// Walk the uniform grid front to back (3-D DDA). Every face whose padded
+// footprint overlaps a visited cell is tested with the reference test; the
+// walk stops once the best squared distance is below the squared distance
+// to the current cell's exit (with a 1e-4 relative margin, far above the
+// rounding of both), so the result equals testing every face. Empty cells
+// carry their Chebyshev distance r to the nearest occupied cell, and the
+// walk leaves the empty box of radius r-1 around them in one step.
+// (All walk state is kept in plain locals: selecting among struct members
+// here makes the compiler address the struct in scratch memory.)
 template <bool STATS>
 __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, uint32_t& n_tests,
                                  uint32_t& n_cells) {
   const grid_desc& g = S.g;
-  // Clip the ray to the grid box (slabs); t is in units of dir.
+  // Clip to the grid box (slabs); t is in units of dir.
   float t0 = 0.f, t1 = 3.4e38f;
   float ix = 0.f, iy = 0.f, iz = 0.f;
   if (dir.x != 0.f) {
@@ -68,31 +97,67 @@ __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, 
   }
   if (!(t0 <= t1)) return;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-  // Start cell, per-axis step, next boundary (tn) and boundary spacing (td).
   int cx = min(max((int)floorf((o.x + dir.x * t0 - g.lo[0]) * g.inv_h[0]), 0), nx - 1);
   int cy = min(max((int)floorf((o.y + dir.y * t0 - g.lo[1]) * g.inv_h[1]), 0), ny - 1);
   int cz = min(max((int)floorf((o.z + dir.z * t0 - g.lo[2]) * g.inv_h[2]), 0), nz - 1);
   const int sx = dir.x > 0.f ? 1 : (dir.x < 0.f ? -1 : 0);
   const int sy = dir.y > 0.f ? 1 : (dir.y < 0.f ? -1 : 0);
   const int sz = dir.z > 0.f ? 1 : (dir.z < 0.f ? -1 : 0);
-  float tnx = sx ? (g.lo[0] + (float)(cx + (sx > 0)) * g.h[0] - o.x) * ix : 3.4e38f;
-  float tny = sy ? (g.lo[1] + (float)(cy + (sy > 0)) * g.h[1] - o.y) * iy : 3.4e38f;
-  float tnz = sz ? (g.lo[2] + (float)(cz + (sz > 0)) * g.h[2] - o.z) * iz : 3.4e38f;
-  const float tdx = sx ? fabsf(g.h[0] * ix) : 0.f;
-  const float tdy = sy ? fabsf(g.h[1] * iy) : 0.f;
-  const float tdz = sz ? fabsf(g.h[2] * iz) : 0.f;
+  // Ray parameter of the next cell boundary along each axis.
+  auto bx_ = [&](int c) { return sx ? (g.lo[0] + (float)(c + (sx > 0)) * g.h[0] - o.x) * ix : 3.4e38f; };
+  auto by_ = [&](int c) { return sy ? (g.lo[1] + (float)(c + (sy > 0)) * g.h[1] - o.y) * iy : 3.4e38f; };
+  auto bz_ = [&](int c) { return sz ? (g.lo[2] + (float)(c + (sz > 0)) * g.h[2] - o.z) * iz : 3.4e38f; };
+  float tnx = bx_(cx), tny = by_(cy), tnz = bz_(cz);
   // Distances along the ray are compared as (t * |dir|)^2 so that neither
   // factor overflows for very short or very long direction vectors.
   const float dlen = sqrtf(dot(dir, dir));
   const int nxy = nx * ny;
   int cell = cx + nx * cy + nxy * cz;
+  mailbox mb;
+  mb.clear();
   while (true) {
-    const int k0 = S.cell_start[cell], k1 = S.cell_start[cell + 1];
-    if (STATS) {
-      n_cells += 1;
-      n_tests += (uint32_t)(k1 - k0);
+    const int2 ci = S.cell_info[cell];
+    if (STATS) n_cells += 1;
+    for (int k = ci.x; k < ci.x + ci.y; ++k) {
+      const uint32_t t = test_ref(S, k, o, dir, h, mb);
+      if (STATS) n_tests += t;
     }
-    for (int k = k0; k < k1; ++k) test_face(S.tri, S.refs[k], o, dir, h);
+    const int R = ci.y > 0 ? 0 : ci.x - 1;
+    if (R >= 1) {
+      // Every cell within R of this one is empty: leave the box
+      // [c-R, c+R] (clipped to the grid) in one step.
+      const int hx = sx > 0 ? min(cx + R, nx - 1) : max(cx - R, 0);
+      const int hy = sy > 0 ? min(cy + R, ny - 1) : max(cy - R, 0);
+      const int hz = sz > 0 ? min(cz + R, nz - 1) : max(cz - R, 0);
+      const float bx = bx_(hx), by = by_(hy), bz = bz_(hz);
+      const bool ex = (bx <= by) && (bx <= bz);
+      const bool ey = !ex && (by <= bz);
+      const float tb = ex ? bx : (ey ? by : bz);
+      if (!(tb < 3.4e38f)) break;
+      const float de = tb * dlen;
+      if (h.f >= 0 && h.d2 < de * de * 0.9999f) break;
+      // Cell just past the exit face; the other two coordinates come from
+      // the exit point, kept inside the box and never moved backwards.
+      const float px = o.x + dir.x * tb, py = o.y + dir.y * tb, pz = o.z + dir.z * tb;
+      int nxc = min(max((int)floorf((px - g.lo[0]) * g.inv_h[0]), max(cx - R, 0)), min(cx + R, nx - 1));
+      int nyc = min(max((int)floorf((py - g.lo[1]) * g.inv_h[1]), max(cy - R, 0)), min(cy + R, ny - 1));
+      int nzc = min(max((int)floorf((pz - g.lo[2]) * g.inv_h[2]), max(cz - R, 0)), min(cz + R, nz - 1));
+      nxc = sx > 0 ? max(nxc, cx) : (sx < 0 ? min(nxc, cx) : cx);
+      nyc = sy > 0 ? max(nyc, cy) : (sy < 0 ? min(nyc, cy) : cy);
+      nzc = sz > 0 ? max(nzc, cz) : (sz < 0 ? min(nzc, cz) : cz);
+      if (ex) nxc = hx + sx;
+      else if (ey) nyc = hy + sy;
+      else nzc = hz + sz;
+      if (nxc < 0 || nxc >= nx || nyc < 0 || nyc >= ny || nzc < 0 || nzc >= nz) break;
+      cx = nxc;
+      cy = nyc;
+      cz = nzc;
+      tnx = bx_(cx);
+      tny = by_(cy);
+      tnz = bz_(cz);
+      cell = cx + nx * cy + nxy * cz;
+      continue;
+    }
     const bool ax = (tnx <= tny) && (tnx <= tnz);
     const bool ay = !ax && (tny <= tnz);
     const float t_exit = ax ? tnx : (ay ? tny : tnz);
@@ -104,17 +169,17 @@ __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, 
       cx += sx;
       if (cx < 0 || cx >= nx) break;
       cell += sx;
-      tnx += tdx;
+      tnx = bx_(cx);
     } else if (ay) {
       cy += sy;
       if (cy < 0 || cy >= ny) break;
       cell += sy * nx;
-      tny += tdy;
+      tny = by_(cy);
     } else {
       cz += sz;
       if (cz < 0 || cz >= nz) break;
       cell += sz * nxy;
-      tnz += tdz;
+      tnz = bz_(cz);
     }
   }
 }

@@ -48,11 +48,11 @@ SIGNATURES = {
     "ffk_tally": (C.c_int, [_vp, _vp, _ll, _vp, _vp]),
     "ffk_normalise": (C.c_int, [_vp, _vp, _vp, _vp]),
     "ffk_ff_fused": (C.c_int, [_vp, C.c_int, _ull, _vp, _vp, _vp, _vp]),
-    "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "ffk_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp]),
     "ffk_batch_destroy": (None, [_vp]),
     "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
-    "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
     "ffk_convert_f": (C.c_int, [_vp, _ll, C.c_int, _vp, _vp]),
     "ffk_sbff": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -202,8 +202,8 @@ class Scene:
         check(lib().ffk_scene_grid_info(self.h, dims, lo, hi, C.byref(nr)), "ffk_scene_grid_info")
         return {"dims": tuple(dims), "lo": tuple(lo), "hi": tuple(hi), "n_refs": nr.value}
 
-    def set_tuning(self, block=0, rays_per_tile=0, hist=-1):
-        check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist), "ffk_set_tuning")
+    def set_tuning(self, block=0, rays_per_tile=0, hist=-1, refill=0):
+        check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist, refill), "ffk_set_tuning")
 
     # -- device calls (torch tensors in, torch tensors out) --
     def cast(self, rays, method=CAST_GRID, points=True, stats=None):
@@ -289,8 +289,8 @@ class Batch:
         check(lib().ffk_batch_create(C.byref(h), arr, len(self.scenes), _stream()), "ffk_batch_create")
         self.h = h
 
-    def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0):
-        check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks), "ffk_batch_set_tuning")
+    def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0, refill=0):
+        check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks, refill), "ffk_batch_set_tuning")
 
     def run(self, seeds, Fs, tallies=None, use_graph=True, stats=None):
         n = len(self.scenes)

@@ -138,8 +138,11 @@ int ffk_normalise(const ffk_scene* s, const unsigned long long* d_tally, float* 
 int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned long long* d_tally,
                  float* d_F, unsigned long long* d_stats, void* stream);
 
-/* Launch tuning of the fused kernel (0 = automatic). */
-int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode);
+/* Launch tuning of the fused kernel (0 = automatic; hist_mode -1 = automatic):
+ * threads per block, rays per tile (<= 65535), row histogram (0 global
+ * atomics, 1 uint32 LDS, 2 uint16 LDS) and the number of waiting lanes
+ * (1..64) at which a wavefront fetches new rays. */
+int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode, int refill);
 
 /* ---- batched form factors ---------------------------------------------------- */
 /* Many scenes of uneven size through one persistent launch over a queue of
@@ -152,7 +155,7 @@ void ffk_batch_destroy(ffk_batch* b);
  * replays it afterwards (same buffers required). */
 int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream);
-int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
+int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks, int refill);
 
 /* ---- thermal ---------------------------------------------------------------- */
 /* Convert F (float32 N*N) to the storage type (FFK_BF16/FFK_F16/FFK_F32). */
