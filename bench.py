@@ -10,9 +10,8 @@ from the counter-based RNG, grid casting, exact integer tallies, row
 normalisation (the fused kernel). The scene, grid and emission layout are
 built before the timed region; inputs are resident in HBM.
 
-Prints one JSON line (see the contract in the module's DESIGN notes of the
-repository) with a live roofline for the fused kernel and the reference CPU
-caster timed on a bounded prefix of the identical ray set.
+Prints one JSON line with a live roofline for the fused kernel and the
+reference CPU caster timed on a bounded prefix of the identical ray set.
 """
 from __future__ import annotations
 
@@ -50,7 +49,6 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--rays-per-tile", type=int, default=0)
     ap.add_argument("--hist", type=int, default=-1)
-    ap.add_argument("--refill", type=int, default=0)
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one untimed device copy of this size (HBM counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -108,7 +106,7 @@ def main():
     sc = ffk.Scene(faces, meshes)
     n_rays = sc.set_emission(n_rays=n_req, frame=ffk.FRAME_ORTHONORMAL)
     g = sc.build_grid((args.grid,) * 3)
-    sc.set_tuning(block=args.block, rays_per_tile=args.rays_per_tile, hist=args.hist, refill=args.refill)
+    sc.set_tuning(block=args.block, rays_per_tile=args.rays_per_tile, hist=args.hist)
     N = len(meshes)
     log(f"scene: {len(faces)} faces, {N} meshes, {n_rays} rays, grid {g['dims']} refs {g['n_refs']}")
     F = torch.empty((N, N), dtype=torch.float32, device="cuda")

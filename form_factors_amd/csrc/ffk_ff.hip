@@ -111,7 +111,7 @@ struct fused_args {
   const scene_desc* descs;
   const tile* tiles;
   int n_tiles;
-  int refill_min;
+
   int* queue;
   const unsigned long long* seeds;
   unsigned long long* const* tallies;
@@ -121,7 +121,7 @@ struct fused_args {
 
 template <int HIST, bool PERSIST, bool STATS>
 __global__ void __launch_bounds__(1024)
-    k_ff_fused(const scene_desc* __restrict__ descs, const tile* __restrict__ tiles, int n_tiles, int refill_min,
+    k_ff_fused(const scene_desc* __restrict__ descs, const tile* __restrict__ tiles, int n_tiles,
                int* __restrict__ queue, const unsigned long long* __restrict__ seeds,
                unsigned long long* const* __restrict__ tallies, float* const* __restrict__ Fs,
                unsigned long long* __restrict__ stats) {
@@ -151,7 +151,6 @@ __global__ void __launch_bounds__(1024)
     const unsigned long long seed = seeds[sc];
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     unsigned long long* tally = tallies ? tallies[sc] : nullptr;
-    (void)refill_min;
     int hint = find_face(S.face_off, S.n_faces, T.g0, 0);
     for (long long g = T.g0 + threadIdx.x; g < T.g1; g += blockDim.x) {
       const int f = find_face(S.face_off, S.n_faces, g, hint);
@@ -247,14 +246,14 @@ static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStr
   switch (hist) {
     case 1:
       set_lds_attr<1, P, S>(lds);
-      k_ff_fused<1, P, S><<<grid, block, lds, st>>>(A.descs, A.tiles, A.n_tiles, A.refill_min, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
+      k_ff_fused<1, P, S><<<grid, block, lds, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
       break;
     case 2:
       set_lds_attr<2, P, S>(lds);
-      k_ff_fused<2, P, S><<<grid, block, lds, st>>>(A.descs, A.tiles, A.n_tiles, A.refill_min, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
+      k_ff_fused<2, P, S><<<grid, block, lds, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
       break;
     default:
-      k_ff_fused<0, P, S><<<grid, block, 0, st>>>(A.descs, A.tiles, A.n_tiles, A.refill_min, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
+      k_ff_fused<0, P, S><<<grid, block, 0, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
       break;
   }
   return hipGetLastError();
@@ -342,7 +341,7 @@ struct ffk_batch {
   int max_N = 0;
   long long max_tile = 0;
   long long total_rays = 0;
-  int block = 0, rpt = 0, n_blocks = 0, refill = 0;
+  int block = 0, rpt = 0, n_blocks = 0;
   // graph state
   hipStream_t cap = nullptr;  // capture stream (default priority)
   hipGraph_t graph = nullptr;
@@ -428,8 +427,6 @@ static int batch_plan(ffk_batch* b) {
 }
 
 static int default_block(int hist) { return hist == 0 ? 256 : 512; }
-// Lanes that must be waiting before a wavefront fetches new rays.
-constexpr int kDefaultRefill = 16;
 
 }  // namespace ffk
 
@@ -501,17 +498,15 @@ int ffk_normalise(const ffk_scene* s, const unsigned long long* d_tally, float* 
   return FFK_OK;
 }
 
-int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode, int refill) {
+int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode) {
   FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
   FFK_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024),
               FFK_ERR_ARG, "block size %d is not a multiple of 64 in [64, 1024]", block_size);
   FFK_REQUIRE(rays_per_tile >= 0 && rays_per_tile <= 65535, FFK_ERR_LIMIT,
               "rays per tile %d outside [0, 65535]", rays_per_tile);
   FFK_REQUIRE(hist_mode >= -1 && hist_mode <= 2, FFK_ERR_ARG, "hist mode %d", hist_mode);
-  FFK_REQUIRE(refill >= 0 && refill <= 64, FFK_ERR_ARG, "refill %d outside [0, 64]", refill);
   s->tune_block = block_size;
   s->tune_hist = hist_mode;
-  s->tune_refill = refill;
   if (rays_per_tile != s->tune_rays_per_tile) {
     s->tune_rays_per_tile = rays_per_tile;
     if (s->has_emission) return build_tiles(s, rays_per_tile, nullptr);
@@ -555,7 +550,6 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
   A.descs = s->d_desc;
   A.tiles = s->d_tiles;
   A.n_tiles = s->n_tiles;
-  A.refill_min = s->tune_refill > 0 ? s->tune_refill : kDefaultRefill;
   A.queue = nullptr;
   A.seeds = d_misc;
   A.tallies = reinterpret_cast<unsigned long long* const*>(d_misc + 1);
@@ -625,16 +619,14 @@ void ffk_batch_destroy(ffk_batch* b) {
   delete b;
 }
 
-int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks, int refill) {
+int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks) {
   FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
   FFK_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024),
               FFK_ERR_ARG, "block size %d", block_size);
   FFK_REQUIRE(rays_per_tile >= 0 && rays_per_tile <= 65535, FFK_ERR_LIMIT, "rays per tile %d", rays_per_tile);
   FFK_REQUIRE(n_blocks >= 0, FFK_ERR_ARG, "n_blocks %d", n_blocks);
-  FFK_REQUIRE(refill >= 0 && refill <= 64, FFK_ERR_ARG, "refill %d outside [0, 64]", refill);
   b->block = block_size;
   b->n_blocks = n_blocks;
-  b->refill = refill;
   if (rays_per_tile != b->rpt) {
     b->rpt = rays_per_tile;
     return batch_plan(b);
@@ -681,7 +673,6 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
   A.descs = b->d_descs;
   A.tiles = b->d_tiles;
   A.n_tiles = b->n_tiles;
-  A.refill_min = b->refill > 0 ? b->refill : kDefaultRefill;
   A.queue = b->d_queue;
   A.seeds = b->d_seeds;
   A.tallies = b->d_tallies;

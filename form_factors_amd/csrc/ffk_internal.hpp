@@ -297,7 +297,7 @@ struct ffk_scene {
   std::vector<int> h_meshes, h_front, h_rear;
   std::vector<long long> h_face_off, h_mesh_off;
   // tuning (0 = automatic)
-  int tune_block = 0, tune_rays_per_tile = 0, tune_hist = -1, tune_refill = 0;
+  int tune_block = 0, tune_rays_per_tile = 0, tune_hist = -1;
   int tiles_rays_per_tile = 0;  // value the tile list was built with
 
   ffk::scene_desc desc() const;

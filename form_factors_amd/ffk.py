@@ -48,15 +48,23 @@ SIGNATURES = {
     "ffk_tally": (C.c_int, [_vp, _vp, _ll, _vp, _vp]),
     "ffk_normalise": (C.c_int, [_vp, _vp, _vp, _vp]),
     "ffk_ff_fused": (C.c_int, [_vp, C.c_int, _ull, _vp, _vp, _vp, _vp]),
-    "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ffk_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp]),
     "ffk_batch_destroy": (None, [_vp]),
     "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
-    "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ffk_convert_f": (C.c_int, [_vp, _ll, C.c_int, _vp, _vp]),
     "ffk_sbff": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_adams_step": (C.c_int, [C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_conduction_graph": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, C.POINTER(C.c_double), C.c_int,
+                                       _i]),
+    "ffk_thermal_create": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, _vp, _vp, _vp]),
+    "ffk_thermal_destroy": (None, [_vp]),
+    "ffk_thermal_set_exchange": (C.c_int, [_vp, _vp, C.c_int]),
+    "ffk_thermal_set_conduction": (C.c_int, [_vp, _i, _i, C.POINTER(C.c_double)]),
+    "ffk_thermal_set_sources": (C.c_int, [_vp, C.c_int, _i, C.POINTER(C.c_double)]),
+    "ffk_thermal_run": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _vp, _vp, C.c_int, _vp]),
 }
 
 _lib = None
@@ -202,8 +210,8 @@ class Scene:
         check(lib().ffk_scene_grid_info(self.h, dims, lo, hi, C.byref(nr)), "ffk_scene_grid_info")
         return {"dims": tuple(dims), "lo": tuple(lo), "hi": tuple(hi), "n_refs": nr.value}
 
-    def set_tuning(self, block=0, rays_per_tile=0, hist=-1, refill=0):
-        check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist, refill), "ffk_set_tuning")
+    def set_tuning(self, block=0, rays_per_tile=0, hist=-1):
+        check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist), "ffk_set_tuning")
 
     # -- device calls (torch tensors in, torch tensors out) --
     def cast(self, rays, method=CAST_GRID, points=True, stats=None):
@@ -289,8 +297,8 @@ class Batch:
         check(lib().ffk_batch_create(C.byref(h), arr, len(self.scenes), _stream()), "ffk_batch_create")
         self.h = h
 
-    def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0, refill=0):
-        check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks, refill), "ffk_batch_set_tuning")
+    def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0):
+        check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks), "ffk_batch_set_tuning")
 
     def run(self, seeds, Fs, tallies=None, use_graph=True, stats=None):
         n = len(self.scenes)
@@ -305,6 +313,92 @@ class Batch:
     def close(self):
         if getattr(self, "h", None):
             lib().ffk_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------- thermal --
+
+def conduction_graph(faces, meshes, conductivity, thickness):
+    """Symmetric CSR (row_ptr, cols, G) of mesh-to-mesh conductances
+    (host arithmetic of the reference's conduction formula)."""
+    faces = _np(faces, np.float32).reshape(-1, 9)
+    meshes = _np(meshes, np.int32).reshape(-1, 3)
+    k = _np(conductivity, np.float32)
+    t = _np(thickness, np.float32)
+    nnz = C.c_int()
+    L = lib()
+    check(L.ffk_conduction_graph(faces.ctypes.data_as(_f), len(faces), meshes.ctypes.data_as(_i), len(meshes),
+                                 k.ctypes.data_as(_f), t.ctypes.data_as(_f), None, None, None, 0, C.byref(nnz)),
+          "ffk_conduction_graph")
+    row = np.zeros(len(meshes) + 1, np.int32)
+    cols = np.zeros(max(1, nnz.value), np.int32)
+    G = np.zeros(max(1, nnz.value), np.float64)
+    check(L.ffk_conduction_graph(faces.ctypes.data_as(_f), len(faces), meshes.ctypes.data_as(_i), len(meshes),
+                                 k.ctypes.data_as(_f), t.ctypes.data_as(_f), row.ctypes.data_as(_i),
+                                 cols.ctypes.data_as(_i), G.ctypes.data_as(C.POINTER(C.c_double)), len(G),
+                                 C.byref(nnz)), "ffk_conduction_graph")
+    return row, cols[:nnz.value], G[:nnz.value]
+
+
+def convert_f(F, dtype):
+    """F (float32 device tensor) in the storage type (F32, BF16 or F16)."""
+    import torch
+
+    out = torch.empty(F.shape, dtype={F32: torch.float32, BF16: torch.bfloat16, F16: torch.float16}[dtype],
+                      device=F.device)
+    check(lib().ffk_convert_f(_ptr(F), F.numel(), dtype, _ptr(out), _stream()), "ffk_convert_f")
+    return out
+
+
+class Thermal:
+    """Device-resident transient run: SB_FF exchange + conduction + sources
+    with the FP64 Adams update."""
+
+    def __init__(self, C_, area, eps_front, eps_rear):
+        require_gpu()
+        self.n = len(C_)
+        self._keep = [np.ascontiguousarray(x, np.float64) for x in (C_, area, eps_front, eps_rear)]
+        h = C.c_void_p()
+        check(lib().ffk_thermal_create(C.byref(h), self.n, *[x.ctypes.data_as(_vp) for x in self._keep]),
+              "ffk_thermal_create")
+        self.h = h
+        self.F = None
+
+    def set_exchange(self, F_dev, dtype):
+        self.F = F_dev  # keep the device buffer alive
+        check(lib().ffk_thermal_set_exchange(self.h, _ptr(F_dev), dtype), "ffk_thermal_set_exchange")
+
+    def set_conduction(self, row, cols, G):
+        self._csr = (np.ascontiguousarray(row, np.int32), np.ascontiguousarray(cols, np.int32),
+                     np.ascontiguousarray(G, np.float64))
+        r, c, g = self._csr
+        check(lib().ffk_thermal_set_conduction(self.h, r.ctypes.data_as(_i), c.ctypes.data_as(_i),
+                                               g.ctypes.data_as(C.POINTER(C.c_double))), "ffk_thermal_set_conduction")
+
+    def set_sources(self, mesh, power):
+        m = np.ascontiguousarray(mesh, np.int32)
+        p = np.ascontiguousarray(power, np.float64)
+        check(lib().ffk_thermal_set_sources(self.h, len(m), m.ctypes.data_as(_i),
+                                            p.ctypes.data_as(C.POINTER(C.c_double))), "ffk_thermal_set_sources")
+
+    def run(self, n_steps, dt, T0=None, balance=False, use_graph=True):
+        T_out = np.zeros(self.n, np.float64)
+        bal = np.zeros(2 * max(1, n_steps), np.float64) if balance else None
+        t0 = None if T0 is None else np.ascontiguousarray(T0, np.float64)
+        check(lib().ffk_thermal_run(self.h, None if t0 is None else t0.ctypes.data_as(_vp), n_steps, dt,
+                                    T_out.ctypes.data_as(_vp), None if bal is None else bal.ctypes.data_as(_vp),
+                                    1 if use_graph else 0, _stream()), "ffk_thermal_run")
+        return (T_out, bal.reshape(-1, 2)[:n_steps]) if balance else T_out
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().ffk_thermal_destroy(self.h)
             self.h = None
 
     def __del__(self):

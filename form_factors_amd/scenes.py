@@ -193,6 +193,47 @@ def two_plates_thermal(cells=100, gap=0.5, size=1.0):
     return faces, np.concatenate([ma, mb]).astype(np.int32)
 
 
+def thermal_plates(cells=100, scale=10.0, gap=0.5, emissivity=0.8, thickness=0.001, power=1000.0,
+                   patch=10):
+    """The mixed-precision thermal workload: a heated plate facing a parallel
+    plate, cells x cells cells of two triangles each (one mesh per cell),
+    aluminium shell (subject/objects.cpp:167-197: density 2698.9 kg/m^3,
+    heat capacity 903 J/(kg K), conductivity 237 W/(m K)), emissivity 0.8 on
+    both sides, `power` watts spread over a patch x patch block of cells at
+    the heated plate's centre.
+
+    The plates are unit plates scaled by `scale` (10 m, gap 5 m by default):
+    form factors are scale invariant, and at 1 m a 100 x 100 grid's triangles
+    fall under the reference's 1e-8 degeneracy threshold (they would never be
+    hit), while a 1 cm aluminium cell makes the explicit Adams step unstable
+    at dt = 0.1 s.
+
+    Returns a dict of faces, meshes, per-mesh area / heat capacity C /
+    emissivities / conductivity / thickness, and the source meshes and powers.
+    """
+    from . import ffk
+
+    faces, meshes = two_plates_thermal(cells=cells, gap=gap * scale, size=scale)
+    n = len(meshes)
+    w, _ = ffk.area_weights(faces)
+    fa = w[0::2]
+    area = np.zeros(n, np.float32)
+    for m in range(n):  # subject/system.cpp:48-56: float sum per mesh
+        a = np.float32(0)
+        for f in range(meshes[m, 0], meshes[m, 0] + meshes[m, 1]):
+            a = np.float32(a + fa[f])
+        area[m] = a
+    rho, cap, k = 2.6989e3, 903.0, 237.0
+    C = area.astype(np.float64) * thickness * rho * cap
+    patch = min(patch, cells)
+    lo = (cells - patch) // 2
+    src = np.array([i * cells + j for i in range(lo, lo + patch) for j in range(lo, lo + patch)], np.int32)
+    return {"faces": faces, "meshes": meshes, "area": area.astype(np.float64), "C": C,
+            "eps_front": np.full(n, emissivity), "eps_rear": np.full(n, emissivity),
+            "conductivity": np.full(n, k, np.float32), "thickness": np.full(n, thickness, np.float32),
+            "src_mesh": src, "src_power": np.full(len(src), power / len(src))}
+
+
 # ------------------------------------------------ reference test scenes --
 # test/form_factors.cpp:41-187 and test/ray_casting.cpp:55-107.
 

@@ -142,7 +142,7 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
  * threads per block, rays per tile (<= 65535), row histogram (0 global
  * atomics, 1 uint32 LDS, 2 uint16 LDS) and the number of waiting lanes
  * (1..64) at which a wavefront fetches new rays. */
-int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode, int refill);
+int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode);
 
 /* ---- batched form factors ---------------------------------------------------- */
 /* Many scenes of uneven size through one persistent launch over a queue of
@@ -155,7 +155,7 @@ void ffk_batch_destroy(ffk_batch* b);
  * replays it afterwards (same buffers required). */
 int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream);
-int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks, int refill);
+int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
 
 /* ---- thermal ---------------------------------------------------------------- */
 /* Convert F (float32 N*N) to the storage type (FFK_BF16/FFK_F16/FFK_F32). */
@@ -181,6 +181,36 @@ int ffk_conduction(int n, const int* d_row_ptr, const int* d_cols, const double*
 int ffk_adams_step(int n, int n_step, double dt, const double* d_C, const double* d_emission,
                    const double* d_absorption, double* d_E_hist, const double* d_T_n,
                    double* d_T_next, double* d_balance, void* stream);
+
+/* ---- mesh adjacency for conduction (host, set-up time) ----------------------
+ * thermal_equation/conductive_cpu.cpp:83-141 over the face graph of
+ * subject/mesh_graph.cpp:116-279: every pair of faces of different meshes
+ * that share an edge adds G = 1/R to the symmetric mesh operator, with
+ * R = (d_l/k_l + d_r/k_r) / (min(thickness) * edge length) evaluated in float
+ * as the reference does. h_meshes are (first_idx, n_faces, pad) triples;
+ * conductivity and thickness are per mesh. Call with h_row_ptr == NULL to get
+ * *nnz_out, then with buffers of that capacity (CSR sorted by row, column). */
+int ffk_conduction_graph(const float* h_faces, int n_faces, const int* h_meshes, int n_meshes,
+                         const float* h_conductivity, const float* h_thickness, int* h_row_ptr,
+                         int* h_cols, double* h_G, int capacity, int* nnz_out);
+
+/* ---- device-resident transient run -------------------------------------------
+ * thermal_solution::calculate (cpu_adams.cpp:147-182) over the SB_FF equation
+ * (F on the device in FFK_F32/BF16/F16, FP32 exchange accumulation), CSR
+ * conduction and fixed heat sources, all on the device with the temperature
+ * update in FP64. Steps are replayed from a hipGraph of 10 steps when
+ * use_graph != 0. h_net_stored (double[2*n_steps], may be NULL) receives per
+ * step the net power sum(absorption - emission) and the stored power
+ * sum(C (T_next - T_n) / dt). h_T0 == NULL continues the previous run. */
+typedef struct ffk_thermal ffk_thermal;
+int ffk_thermal_create(ffk_thermal** out, int n_meshes, const double* h_C, const double* h_area,
+                       const double* h_eps_front, const double* h_eps_rear);
+void ffk_thermal_destroy(ffk_thermal* t);
+int ffk_thermal_set_exchange(ffk_thermal* t, const void* d_F, int dtype);
+int ffk_thermal_set_conduction(ffk_thermal* t, const int* h_row_ptr, const int* h_cols, const double* h_G);
+int ffk_thermal_set_sources(ffk_thermal* t, int n_sources, const int* h_mesh, const double* h_power);
+int ffk_thermal_run(ffk_thermal* t, const double* h_T0, int n_steps, double dt, double* h_T_out,
+                    double* h_net_stored, int use_graph, void* stream);
 
 #ifdef __cplusplus
 }

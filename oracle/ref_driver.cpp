@@ -24,6 +24,7 @@
 #include "ray_caster/system.h"
 #include "ray_caster/zgrid_cpu.h"
 #include "subject/system.h"
+#include "thermal_equation/conductive_cpu.h"
 #include "thermal_equation/radiance_cpu.h"
 #include "thermal_equation/system.h"
 #include "thermal_solution/cpu_adams.h"
@@ -365,6 +366,31 @@ EXPORT int ffref_adams(const float* faces, int n_faces, const int* meshes, int n
   }
   thermal_solution::task_free(task);
   thermal_solution::system_free(sol);
+  return r;
+}
+
+/* conductive_equation (thermal_equation/conductive_cpu.cpp): one calculate()
+ * with per-mesh conductivity and thickness, emission/absorption out. */
+EXPORT int ffref_conduction(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* k,
+                            const float* thickness, const float* T, float* emission, float* absorption) {
+  std::vector<subject::material_t> mats(n_meshes);
+  std::vector<subject::mesh_t> ms(n_meshes);
+  for (int m = 0; m < n_meshes; ++m) {
+    mats[m] = subject::black_body();
+    mats[m].shell.thermal_conductivity = k[m];
+    mats[m].shell.thickness = thickness[m];
+    ms[m] = subject::make_mesh(meshes[3 * m], meshes[3 * m + 1], m);
+  }
+  subject::scene_t scene = {n_faces, (subject::face_t*)faces, n_meshes, ms.data(), n_meshes, mats.data()};
+  thermal_equation::system_t* eq = conductive_equation::system_create();
+  conductive_equation::params_t params;
+  thermal_equation::system_init(eq, &params);
+  int r = thermal_equation::system_set_scene(eq, &scene);
+  thermal_equation::task_t task = {T, emission, absorption};
+  memset(emission, 0, n_meshes * sizeof(float));
+  memset(absorption, 0, n_meshes * sizeof(float));
+  if (r == 0) r = thermal_equation::system_calculate(eq, &task);
+  thermal_equation::system_free(eq);
   return r;
 }
 

@@ -295,6 +295,36 @@ def gen_parallel(rng):
     save("parallel.npz", direction=d, center=c, distance=np.float32(2.5), radius=np.float32(1.3), rays=rays)
 
 
+def gen_conduction(rng):
+    """conductive_equation (thermal_equation/conductive_cpu.cpp) on scenes
+    whose meshes share edges: the reference's emission/absorption arrays."""
+    R = O.ref()
+    R.ffref_conduction.restype = C.c_int
+    R.ffref_conduction.argtypes = [_f, C.c_int, _i, C.c_int, _f, _f, _f, _f, _f]
+    out = {}
+    box, _ = scenes.ff_box()
+    plate = scenes.plane_grid_faces(10, 10, 10, 10)  # cells of 1 unit, fuzzy vertex matches
+    two = scenes.two_plates_thermal(cells=6, gap=5.0, size=10.0)[0]
+    cases = [("plate4", scenes.plane_grid_faces(1, 1, 4, 4), scenes.plane_grid_meshes(4, 4)),
+             ("plate10", plate, scenes.plane_grid_meshes(10, 10)),
+             ("box12", box, scenes.per_face_meshes(12)),
+             ("two_plates6", two, scenes.two_plates_thermal(cells=6, gap=5.0, size=10.0)[1])]
+    for name, faces, meshes in cases:
+        faces = np.ascontiguousarray(faces, np.float32)
+        meshes = np.ascontiguousarray(meshes, np.int32)
+        n = len(meshes)
+        k = rng.uniform(50, 300, n).astype(np.float32)
+        th = rng.uniform(0.001, 0.01, n).astype(np.float32)
+        T = rng.uniform(250, 400, n).astype(np.float32)
+        em = np.zeros(n, np.float32)
+        ab = np.zeros(n, np.float32)
+        rc = R.ffref_conduction(P(faces), len(faces), P(meshes, _i), n, P(k), P(th), P(T), P(em), P(ab))
+        assert rc == 0, (name, rc)
+        out.update({f"{name}_faces": faces, f"{name}_meshes": meshes, f"{name}_k": k, f"{name}_thickness": th,
+                    f"{name}_T": T, f"{name}_emission": em, f"{name}_absorption": ab})
+    save("conduction.npz", **out)
+
+
 def main():
     if not O.ref_available():
         O.build()
@@ -308,6 +338,7 @@ def main():
     gen_adams(rng)
     gen_radiance(rng)
     gen_parallel(rng)
+    gen_conduction(rng)
 
 
 if __name__ == "__main__":
