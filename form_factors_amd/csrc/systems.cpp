@@ -601,8 +601,8 @@ struct state {
   std::vector<float> F;
   std::vector<double> T, area, ef, er, em, ab;
   dbuf<float> dF32;
-  dbuf<unsigned short> dF16;
-  dbuf<double> dvec;  // T, area, ef, er, em, ab
+  dbuf<unsigned short> dS;  // packed exchange storage (2 or 4 bytes per cell)
+  dbuf<double> dvec;  // T, area, ef, er, em, ab, row scale
   hipStream_t st = nullptr;
 };
 struct system : thermal_equation::system_t {
@@ -621,7 +621,7 @@ int shutdown(system* sys) {
   if (!s) return THERMAL_EQUATION_OK;
   if (s->st) (void)hipStreamSynchronize(s->st);
   s->dF32.release();
-  s->dF16.release();
+  s->dS.release();
   s->dvec.release();
   drop_stream(s->st);
   delete s;
@@ -658,15 +658,15 @@ int set_scene(system* sys, subject::scene_t* scene) {
   if ((r = ff->methods->calculate(ff, &task)) < 0) return r;
   // F in the chosen storage type on the device
   const size_t NN = (size_t)N * N;
-  if (!s->dF32.reserve(NN) || !s->dvec.reserve(6 * (size_t)N)) return -THERMAL_EQUATION_ERROR;
+  if (!s->dF32.reserve(NN) || !s->dvec.reserve(7 * (size_t)N)) return -THERMAL_EQUATION_ERROR;
   if (hipMemcpyAsync(s->dF32.p, s->F.data(), 4 * NN, hipMemcpyHostToDevice, s->st) != hipSuccess)
     return -THERMAL_EQUATION_ERROR;
-  if (s->params.f_dtype != FFK_F32) {
-    if (!s->dF16.reserve(NN)) return -THERMAL_EQUATION_ERROR;
-    if (ffk_convert_f(s->dF32.p, (long long)NN, s->params.f_dtype, s->dF16.p, s->st) < 0)
-      return -THERMAL_EQUATION_ERROR;
-  }
   double* d = s->dvec.p;
+  // packed (transposed) storage; count-valued rows are stored as exact counts
+  if (!s->dS.reserve(NN * (s->params.f_dtype == FFK_F32 ? 2 : 1))) return -THERMAL_EQUATION_ERROR;
+  if (ffk_pack_exchange(s->dF32.p, N, s->params.f_dtype, nullptr, s->dS.p, d + 6 * N, s->st) < 0)
+    return -THERMAL_EQUATION_ERROR;
+  s->dF32.release();
   if (hipMemcpyAsync(d + N, s->area.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
       hipMemcpyAsync(d + 2 * N, s->ef.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
       hipMemcpyAsync(d + 3 * N, s->er.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
@@ -681,11 +681,12 @@ int calculate(system* sys, thermal_equation::task_t* task) {
   const int N = s->scene->n_meshes;
   s->T.assign(task->temperatures, task->temperatures + N);
   double* d = s->dvec.p;
-  const void* F = s->params.f_dtype == FFK_F32 ? (const void*)s->dF32.p : (const void*)s->dF16.p;
+  const void* F = s->dS.p;
   if (hipMemcpyAsync(d, s->T.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
       hipMemsetAsync(d + 4 * N, 0, 16 * (size_t)N, s->st) != hipSuccess)
     return -THERMAL_EQUATION_ERROR;
-  if (ffk_sbff(N, F, s->params.f_dtype, d, d + N, d + 2 * N, d + 3 * N, d + 4 * N, d + 5 * N, s->st) < 0)
+  const double* scale = d + 6 * N;
+  if (ffk_sbff(N, F, s->params.f_dtype, scale, d, d + N, d + 2 * N, d + 3 * N, d + 4 * N, d + 5 * N, s->st) < 0)
     return -THERMAL_EQUATION_ERROR;
   s->em.resize(N);
   s->ab.resize(N);

@@ -54,14 +54,16 @@ SIGNATURES = {
     "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
     "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ffk_convert_f": (C.c_int, [_vp, _ll, C.c_int, _vp, _vp]),
-    "ffk_sbff": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_pack_exchange": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "ffk_sbff": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_adams_step": (C.c_int, [C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction_graph": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, C.POINTER(C.c_double), C.c_int,
                                        _i]),
     "ffk_thermal_create": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, _vp, _vp, _vp]),
     "ffk_thermal_destroy": (None, [_vp]),
-    "ffk_thermal_set_exchange": (C.c_int, [_vp, _vp, C.c_int]),
+    "ffk_thermal_set_exchange": (C.c_int, [_vp, _vp, C.c_int, _vp]),
+    "ffk_thermal_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ffk_thermal_set_conduction": (C.c_int, [_vp, _i, _i, C.POINTER(C.c_double)]),
     "ffk_thermal_set_sources": (C.c_int, [_vp, C.c_int, _i, C.POINTER(C.c_double)]),
     "ffk_thermal_run": (C.c_int, [_vp, _vp, C.c_int, C.c_double, _vp, _vp, C.c_int, _vp]),
@@ -196,6 +198,14 @@ class Scene:
         self.n_rays = n.value
         self.frame = frame
         return self.n_rays
+
+    def rays_per_mesh(self):
+        """Rays emitted by each mesh (front + rear over its faces): the row
+        divisor of F (cpuFactorsCalculator/cpu_system.cpp:193-195)."""
+        per_face = self.front.astype(np.int64) + self.rear.astype(np.int64)
+        first, count = self.meshes[:, 0], self.meshes[:, 1]
+        csum = np.concatenate([[0], np.cumsum(per_face)])
+        return csum[first + count] - csum[first]
 
     def build_grid(self, dims=(0, 0, 0)):
         check(lib().ffk_scene_build_grid(self.h, int(dims[0]), int(dims[1]), int(dims[2]), _stream()),
@@ -356,6 +366,24 @@ def convert_f(F, dtype):
     return out
 
 
+def pack_exchange(F, dtype, row_mul=None):
+    """Exchange storage of F (N x N float32 device tensor): returns
+    (St, row_scale) with St[j, m] = dtype(F[m, j] * mul_m) (transposed) and
+    row_scale = 1 / mul (see ffk_pack_exchange; row_mul None = detect
+    count-valued rows)."""
+    import torch
+
+    n = F.shape[0]
+    assert F.shape == (n, n) and F.dtype == torch.float32 and F.is_contiguous()
+    out = torch.empty(F.shape, dtype={F32: torch.float32, BF16: torch.bfloat16, F16: torch.float16}[dtype],
+                      device=F.device)
+    scale = torch.empty(n, dtype=torch.float64, device=F.device)
+    mul = None if row_mul is None else torch.as_tensor(np.asarray(row_mul, np.float64), device=F.device)
+    check(lib().ffk_pack_exchange(_ptr(F), n, dtype, _ptr(mul), _ptr(out), _ptr(scale), _stream()),
+          "ffk_pack_exchange")
+    return out, scale
+
+
 class Thermal:
     """Device-resident transient run: SB_FF exchange + conduction + sources
     with the FP64 Adams update."""
@@ -370,9 +398,13 @@ class Thermal:
         self.h = h
         self.F = None
 
-    def set_exchange(self, F_dev, dtype):
+    def set_exchange(self, F_dev, dtype, row_scale=None):
         self.F = F_dev  # keep the device buffer alive
-        check(lib().ffk_thermal_set_exchange(self.h, _ptr(F_dev), dtype), "ffk_thermal_set_exchange")
+        check(lib().ffk_thermal_set_exchange(self.h, _ptr(F_dev), dtype, _ptr(row_scale)),
+              "ffk_thermal_set_exchange")
+
+    def set_tuning(self, unroll=0, max_blocks=0):
+        check(lib().ffk_thermal_set_tuning(self.h, unroll, max_blocks), "ffk_thermal_set_tuning")
 
     def set_conduction(self, row, cols, G):
         self._csr = (np.ascontiguousarray(row, np.int32), np.ascontiguousarray(cols, np.int32),

@@ -158,15 +158,30 @@ int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long l
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
 
 /* ---- thermal ---------------------------------------------------------------- */
-/* Convert F (float32 N*N) to the storage type (FFK_BF16/FFK_F16/FFK_F32). */
+/* Elementwise conversion of F (float32) to the storage type
+ * (FFK_BF16/FFK_F16/FFK_F32), round to nearest even. */
 int ffk_convert_f(const float* d_F, long long n_elems, int dtype, void* d_out, void* stream);
+/* Exchange storage of an N x N form-factor matrix: S[m, :] = dtype(F[m, :] * mul_m)
+ * and d_row_scale[m] = 1 / mul_m (device, double[N]); the exchange then uses
+ * q_j = sum_m S[m, j] * (E_m * scale_m), which equals sum_m F[m, j] E_m.
+ * d_out is TRANSPOSED: d_out[j*N + m] = S[m, j], one contiguous row per
+ * target mesh (the layout ffk_sbff and ffk_thermal_set_exchange read).
+ * mul_m = d_row_mul[m] when given (e.g. the rays emitted by mesh m, which makes
+ * S the integer hit counts: exact in BF16 up to 256 and in FP16 up to 2048).
+ * With d_row_mul == NULL a count-valued row (every entry an integer multiple
+ * of its smallest nonzero entry 1/n) gets mul_m = n; other rows mul_m = 1.
+ * FP16 rows whose scaled maximum would reach 32768 use instead the power of
+ * two that puts the row maximum in [16384, 32768). FP32 keeps mul_m = 1. */
+int ffk_pack_exchange(const float* d_F, int n, int dtype, const double* d_row_mul, void* d_out,
+                      double* d_row_scale, void* stream);
 /* Radiative exchange of thermal_equation/form_factors.cpp:109-128:
  *   E_m = (eps_f_m + eps_r_m) * sigma * T_m^4 * A_m        (emission[m] += E_m)
  *   absorption[n] += sum_m E_m * F[m*N + n]
- * F stored as dtype, accumulated in FP32, T/area/eps/outputs in FP64. */
-int ffk_sbff(int n, const void* d_F, int dtype, const double* d_T, const double* d_area,
-             const double* d_eps_front, const double* d_eps_rear, double* d_emission,
-             double* d_absorption, void* stream);
+ * d_S is the packed storage of ffk_pack_exchange (d_row_scale from the same
+ * call), accumulated in FP32; T/area/eps/outputs in FP64. */
+int ffk_sbff(int n, const void* d_S, int dtype, const double* d_row_scale, const double* d_T,
+             const double* d_area, const double* d_eps_front, const double* d_eps_rear,
+             double* d_emission, double* d_absorption, void* stream);
 /* Conduction (thermal_equation/conductive_cpu.cpp:83-134) as a CSR operator
  * over mesh adjacency: flow_m = sum_k G_mk (T_k - T_m); positive flow is
  * absorbed, negative is emitted. */
@@ -198,15 +213,24 @@ int ffk_conduction_graph(const float* h_faces, int n_faces, const int* h_meshes,
  * thermal_solution::calculate (cpu_adams.cpp:147-182) over the SB_FF equation
  * (F on the device in FFK_F32/BF16/F16, FP32 exchange accumulation), CSR
  * conduction and fixed heat sources, all on the device with the temperature
- * update in FP64. Steps are replayed from a hipGraph of 10 steps when
- * use_graph != 0. h_net_stored (double[2*n_steps], may be NULL) receives per
+ * update in FP64. One step is one kernel: a wavefront per target mesh
+ * streams its row of the packed (transposed) exchange storage against E in
+ * LDS, then applies emission, conduction, sources and the Adams update into a
+ * second temperature buffer. Steps are replayed from a hipGraph of 10 steps
+ * when use_graph != 0. h_net_stored (double[2*n_steps], may be NULL) receives per
  * step the net power sum(absorption - emission) and the stored power
  * sum(C (T_next - T_n) / dt). h_T0 == NULL continues the previous run. */
 typedef struct ffk_thermal ffk_thermal;
 int ffk_thermal_create(ffk_thermal** out, int n_meshes, const double* h_C, const double* h_area,
                        const double* h_eps_front, const double* h_eps_rear);
 void ffk_thermal_destroy(ffk_thermal* t);
-int ffk_thermal_set_exchange(ffk_thermal* t, const void* d_F, int dtype);
+/* d_S: packed exchange storage from ffk_pack_exchange (device, N*N of dtype,
+ * stays owned by the caller; NULL = no radiative exchange); d_row_scale
+ * (device, double[N], may be NULL = 1) is copied. */
+int ffk_thermal_set_exchange(ffk_thermal* t, const void* d_S, int dtype, const double* d_row_scale);
+/* Step-kernel launch shape: 16-byte loads in flight per lane (2, 4 or 8;
+ * 0 = 4) and an upper bound on the grid (0 = every block that fits). */
+int ffk_thermal_set_tuning(ffk_thermal* t, int unroll, int max_blocks);
 int ffk_thermal_set_conduction(ffk_thermal* t, const int* h_row_ptr, const int* h_cols, const double* h_G);
 int ffk_thermal_set_sources(ffk_thermal* t, int n_sources, const int* h_mesh, const double* h_power);
 int ffk_thermal_run(ffk_thermal* t, const double* h_T0, int n_steps, double dt, double* h_T_out,

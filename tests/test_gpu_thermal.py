@@ -51,12 +51,14 @@ def setup(gpu, cells, n_rays_per_tri, seed=3):
     F = sc.ff_fused(seed)
     torch.cuda.synchronize()
     row, cols, G = gpu.conduction_graph(prob["faces"], prob["meshes"], prob["conductivity"], prob["thickness"])
+    prob["rays_per_mesh"] = sc.rays_per_mesh().astype(np.float64)
     return prob, F, (row, cols, G), n
 
 
 def make_run(gpu, prob, F, csr, dtype):
     th = gpu.Thermal(prob["C"], prob["area"], prob["eps_front"], prob["eps_rear"])
-    th.set_exchange(gpu.convert_f(F, dtype), dtype)
+    S, scale = gpu.pack_exchange(F, dtype, row_mul=prob["rays_per_mesh"])
+    th.set_exchange(S, dtype, scale)
     th.set_conduction(*csr)
     th.set_sources(prob["src_mesh"], prob["src_power"])
     return th
@@ -84,8 +86,85 @@ def test_thermal_reduced_10k_steps_vs_fp64(gpu):
             assert abs(stored[s] - expect) <= 1e-9 * (np.abs(net).max() + 1e-300), (dtype, s)
         th.close()
     print("max relative T error vs FP64:", errs)
+    # count-valued rows are exact in the packed BF16/FP16 storage
     assert errs[gpu.F32] < 1e-6
-    assert errs[gpu.BF16] <= 1e-4 and errs[gpu.F16] <= 1e-4
+    assert errs[gpu.BF16] <= 1e-6 and errs[gpu.F16] <= 1e-6
+
+
+def test_pack_exchange_rows(gpu):
+    """Count rows become exact counts with scale 1/n; other rows keep mul 1
+    (BF16) or a power of two putting the row maximum in [16384, 32768) (FP16);
+    the scaled product equals F^T E."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    n = 96
+    n_rays = rng.integers(50, 5000, n).astype(np.float32)
+    counts = rng.poisson(1.5, (n, n)).astype(np.float32)
+    counts[np.arange(n), rng.integers(0, n, n)] = 1.0  # every row has a count-1 cell
+    counts[:, 0] = np.minimum(counts[:, 0], 200)
+    F = (counts * (np.float32(1.0) / n_rays[:, None])).astype(np.float32)  # the reference's normalisation
+    F[7] = rng.random(n).astype(np.float32) * 1e-3  # a non-count row
+    F[9] = 0.0  # an empty row
+    Fd = torch.from_numpy(F).cuda()
+    S, scale = gpu.pack_exchange(Fd, gpu.BF16)
+    S = S.float().cpu().numpy().T  # stored transposed: S[j, m]
+    scale = scale.cpu().numpy()
+    rows = [m for m in range(n) if m not in (7, 9)]
+    assert np.array_equal(S[rows], counts[rows])
+    assert np.allclose(scale[rows], 1.0 / n_rays[rows], rtol=1e-6)
+    assert scale[7] == 1.0 and scale[9] == 1.0
+    S16, scale16 = gpu.pack_exchange(Fd, gpu.F16)
+    S16 = S16.float().cpu().numpy().T
+    scale16 = scale16.cpu().numpy()
+    assert np.array_equal(S16[rows], counts[rows])
+    mul7 = 1.0 / scale16[7]
+    assert mul7 == 2.0 ** np.round(np.log2(mul7)) and 16384 <= F[7].max() * mul7 < 32768 + 1e-3
+    # with explicit multipliers
+    S2, sc2 = gpu.pack_exchange(Fd, gpu.BF16, row_mul=n_rays)
+    assert np.array_equal(S2.float().cpu().numpy().T[rows], counts[rows])
+    nz = [m for m in range(n) if m != 9]  # an all-zero row keeps scale 1
+    assert np.allclose(sc2.cpu().numpy()[nz], 1.0 / n_rays[nz]) and sc2.cpu().numpy()[9] == 1.0
+    # the product through ffk_sbff
+    T = torch.from_numpy(rng.uniform(250, 350, n)).cuda()
+    A = torch.from_numpy(rng.uniform(0.5, 2.0, n)).cuda()
+    ef = torch.full((n,), 0.8, dtype=torch.float64, device="cuda")
+    er = torch.full((n,), 0.3, dtype=torch.float64, device="cuda")
+    Sd, scale_d = gpu.pack_exchange(Fd, gpu.BF16)
+    em = torch.zeros(n, dtype=torch.float64, device="cuda")
+    ab = torch.zeros(n, dtype=torch.float64, device="cuda")
+    p = gpu._ptr
+    gpu.check(gpu.lib().ffk_sbff(n, p(Sd), gpu.BF16, p(scale_d), p(T), p(A), p(ef), p(er), p(em), p(ab),
+                                 gpu._stream()), "ffk_sbff")
+    torch.cuda.synchronize()
+    E = (0.8 + 0.3) * 5.670400e-8 * T.cpu().numpy() ** 4 * A.cpu().numpy()
+    ref = F.astype(np.float64).T @ E
+    F_rt = F.copy()
+    F_rt[7] = S[7].astype(np.float64)  # BF16-rounded values of the non-count row
+    ref_rt = F_rt.astype(np.float64).T @ E
+    got = ab.cpu().numpy()
+    assert np.allclose(em.cpu().numpy(), E, rtol=1e-12)
+    assert np.abs(got - ref_rt).max() <= 2e-6 * np.abs(ref).max()
+
+
+def test_plain_bf16_rounding_is_coherent_and_packing_removes_it(gpu):
+    """Rounding count-valued F entries to BF16 gives every cell of equal count
+    the same relative error; the packed storage has none."""
+    import torch
+
+    n, n_rays = 64, 1000
+    rng = np.random.default_rng(2)
+    counts = rng.poisson(2.0, (n, n)).astype(np.float32)
+    counts[:, 3] = 1.0
+    F = (counts * (np.float32(1.0) / np.float32(n_rays))).astype(np.float32)
+    Fd = torch.from_numpy(F).cuda()
+    plain = gpu.convert_f(Fd, gpu.BF16).float().cpu().numpy().astype(np.float64)
+    rel = (plain - F) / np.where(F > 0, F, 1)
+    ones = counts == 1
+    assert np.ptp(rel[ones]) == 0 and abs(rel[ones][0]) > 1e-4  # one common error for all count-1 cells
+    S, scale = gpu.pack_exchange(Fd, gpu.BF16)
+    back = S.float().cpu().numpy().T.astype(np.float64) * scale.cpu().numpy()[:, None]
+    assert np.abs(back - F).max() <= 3e-7 * F.max()
 
 
 def test_thermal_graph_replay_equals_eager(gpu):
@@ -100,6 +179,19 @@ def test_thermal_graph_replay_equals_eager(gpu):
     Ta2 = a.run(45, 0.1, use_graph=True)
     Tc = b.run(45, 0.1, use_graph=False)
     assert np.array_equal(Ta2, Tc)
+    a.close()
+    b.close()
+
+
+@pytest.mark.parametrize("unroll,max_blocks", [(2, 0), (8, 0), (4, 3)])
+def test_thermal_launch_shapes_agree(gpu, unroll, max_blocks):
+    """Every launch shape computes each row in the same order: identical T."""
+    prob, F, csr, n = setup(gpu, cells=8, n_rays_per_tri=500)
+    T0 = np.linspace(280, 320, n)
+    a = make_run(gpu, prob, F, csr, gpu.F16)
+    b = make_run(gpu, prob, F, csr, gpu.F16)
+    b.set_tuning(unroll, max_blocks)
+    assert np.array_equal(a.run(30, 0.1, T0=T0), b.run(30, 0.1, T0=T0))
     a.close()
     b.close()
 
