@@ -292,6 +292,12 @@ struct ffk_scene {
   int* d_indirect_rows = nullptr;  // meshes tallied through the global tally
   int n_indirect_rows = 0;
   ffk::scene_desc* d_desc = nullptr;
+  // radiance work buffers, grown on demand
+  long long* d_walk_start = nullptr;
+  int* d_walk_face = nullptr;
+  unsigned long long* d_walk_q = nullptr;
+  unsigned long long* d_rad_acc = nullptr;
+  int rad_cap_walk = 0, rad_cap_meshes = 0;
   // host copies
   std::vector<float> h_faces;
   std::vector<int> h_meshes, h_front, h_rear;

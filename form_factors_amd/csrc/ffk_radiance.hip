@@ -1,0 +1,221 @@
+// radiance_equation::calculate on the device (thermal_equation/radiance_cpu.cpp:132-190).
+//
+// The reference emits Malley rays with weights T * eps * A per face side,
+// casts them, and walks the rays mesh by mesh, face by face (front rays, then
+// rear rays), depositing front_emission / front_rays (or the rear analogue)
+// on the mesh of the hit face, in float. Here one kernel emits, casts and
+// deposits each ray; the walk position of ray g (the reference walks rays in
+// emission order) is found in a table of walk entries (mesh-order faces with
+// their first walk position), and the powers are summed per mesh in 64-bit
+// fixed point: integer additions, so the result does not depend on the order
+// of the atomics and carries no float accumulation error.
+#include <cmath>
+
+#include "ffk_trace.hpp"
+
+namespace ffk {
+
+__global__ void k_fill_acc(unsigned long long* __restrict__ p, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+template <bool STATS>
+__global__ void __launch_bounds__(256) k_radiance(scene_desc S, const long long* __restrict__ walk_start,
+                                                  const int* __restrict__ walk_face, int n_walk,
+                                                  const unsigned long long* __restrict__ q, long long n_rays,
+                                                  uint32_t k0, uint32_t k1, unsigned long long* __restrict__ acc,
+                                                  unsigned long long* __restrict__ stats) {
+  uint32_t nt = 0, nc = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  int hint = 0, whint = 0;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n_rays; g += stride) {
+    const int f = find_face(S.face_off, S.n_faces, g, hint);
+    hint = f;
+    v3 o, e;
+    malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+    hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
+    cast_grid<STATS>(S, o, e - o, h, nt, nc);
+    if (h.f < 0) continue;
+    const int m = S.face2mesh[h.f];
+    if (m < 0) continue;
+    const int k = find_face(walk_start, n_walk, g, whint);
+    whint = k;
+    const int wf = walk_face[k];
+    const unsigned long long v = q[2 * k + ((g - walk_start[k]) < S.front[wf] ? 0 : 1)];
+    if (v) atomicAdd(&acc[m], v);
+  }
+  if (STATS) {
+    unsigned long long t = nt, c = nc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      t += __shfl_xor(t, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&stats[0], t);
+      atomicAdd(&stats[1], c);
+    }
+  }
+}
+
+}  // namespace ffk
+
+using namespace ffk;
+
+namespace {
+
+// math::triangle_area (math/triangle.cpp:86-92) in float.
+float face_area(const float* p) {
+  v3 u = mk(p[3], p[4], p[5]) - mk(p[0], p[1], p[2]);
+  v3 w = mk(p[6], p[7], p[8]) - mk(p[0], p[1], p[2]);
+  v3 n = cross(u, w);
+  return sqrtf(dot(n, n)) / 2;
+}
+
+bool meshes_valid(const int* h_meshes, int n_meshes, int n_faces) {
+  for (int m = 0; m < n_meshes; ++m) {
+    const long long f0 = h_meshes[3 * m], n = h_meshes[3 * m + 1];
+    if (f0 < 0 || n < 0 || f0 + n > n_faces) return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffk_radiance_weights(const float* h_faces, int n_faces, const int* h_meshes, int n_meshes,
+                         const float* h_eps2, const float* h_T, float* h_weights, float* total) {
+  FFK_REQUIRE(h_faces && h_meshes && h_eps2 && h_T && h_weights && total && n_faces >= 0 && n_meshes >= 0,
+              FFK_ERR_ARG, "ffk_radiance_weights: bad args");
+  FFK_REQUIRE(meshes_valid(h_meshes, n_meshes, n_faces), FFK_ERR_ARG, "ffk_radiance_weights: mesh out of range");
+  for (int i = 0; i < 2 * n_faces; ++i) h_weights[i] = 0.f;  // faces outside every mesh
+  float t = 0;
+  for (int m = 0; m < n_meshes; ++m) {
+    const float density = h_T[m];
+    const float front_density = density * h_eps2[2 * m];
+    const float rear_density = density * h_eps2[2 * m + 1];
+    for (int f = h_meshes[3 * m]; f < h_meshes[3 * m] + h_meshes[3 * m + 1]; ++f) {
+      const float a = face_area(h_faces + 9 * (size_t)f);
+      const float front = front_density * a;
+      const float rear = rear_density * a;
+      t += front + rear;
+      h_weights[2 * f] = front;
+      h_weights[2 * f + 1] = rear;
+    }
+  }
+  *total = t;
+  return FFK_OK;
+}
+
+int ffk_radiance_powers(const float* h_faces, int n_faces, const int* h_meshes, int n_meshes,
+                        const float* h_eps2, const float* h_T, const int* h_front, const int* h_rear,
+                        float* h_ray_power, float* h_emission) {
+  FFK_REQUIRE(h_faces && h_meshes && h_eps2 && h_T && h_front && h_rear && h_ray_power && h_emission &&
+                  n_faces >= 0 && n_meshes >= 0,
+              FFK_ERR_ARG, "ffk_radiance_powers: bad args");
+  FFK_REQUIRE(meshes_valid(h_meshes, n_meshes, n_faces), FFK_ERR_ARG, "ffk_radiance_powers: mesh out of range");
+  const float sigma = 5.670400e-8f;  // thermal_equation/radiance_cpu.cpp:29
+  for (int i = 0; i < 2 * n_faces; ++i) h_ray_power[i] = 0.f;
+  for (int m = 0; m < n_meshes; ++m) {
+    const float T = h_T[m];
+    const float power_density = sigma * (T * T * T * T);
+    const float front_density = power_density * h_eps2[2 * m];
+    const float rear_density = power_density * h_eps2[2 * m + 1];
+    for (int f = h_meshes[3 * m]; f < h_meshes[3 * m] + h_meshes[3 * m + 1]; ++f) {
+      const float a = face_area(h_faces + 9 * (size_t)f);
+      const float front_emission = front_density * a;
+      const float rear_emission = rear_density * a;
+      h_emission[m] += front_emission + rear_emission;
+      h_ray_power[2 * f] = h_front[f] > 0 ? front_emission / h_front[f] : 0.f;
+      h_ray_power[2 * f + 1] = h_rear[f] > 0 ? rear_emission / h_rear[f] : 0.f;
+    }
+  }
+  return FFK_OK;
+}
+
+int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power, double* h_absorption,
+                 unsigned long long* d_stats, void* stream) {
+  FFK_REQUIRE(s && h_ray_power && h_absorption, FFK_ERR_ARG, "ffk_radiance: bad args");
+  FFK_REQUIRE(s->has_emission && s->has_grid, FFK_ERR_ARG, "ffk_radiance: emission layout and grid must be set");
+  hipStream_t st = as_stream(stream);
+  const int nf = s->n_faces, nm = s->n_meshes;
+  const long long total = s->n_rays;
+  // Walk entries: faces in mesh order with their first walk position; the
+  // walk stops at the last emitted ray (radiance_cpu.cpp:170).
+  std::vector<long long> start;
+  std::vector<int> face;
+  std::vector<double> pw;
+  long long pos = 0;
+  double upper = 0.0;
+  for (int m = 0; m < nm && pos < total; ++m) {
+    const int f0 = s->h_meshes[3 * m], n = s->h_meshes[3 * m + 1];
+    for (int f = f0; f < f0 + n && pos < total; ++f) {
+      const float pf = h_ray_power[2 * f], pr = h_ray_power[2 * f + 1];
+      FFK_REQUIRE(pf >= 0.f && pr >= 0.f && std::isfinite(pf) && std::isfinite(pr), FFK_ERR_ARG,
+                  "ffk_radiance: ray power of face %d is negative or not finite", f);
+      start.push_back(pos);
+      face.push_back(f);
+      pw.push_back(pf);
+      pw.push_back(pr);
+      upper += (double)s->h_front[f] * pf + (double)s->h_rear[f] * pr;
+      pos += (long long)s->h_front[f] + s->h_rear[f];
+    }
+  }
+  const long long walk_total = std::min(pos, total);
+  for (int m = 0; m < nm; ++m) h_absorption[m] = 0.0;
+  if (start.empty() || upper <= 0.0) return FFK_OK;
+  start.push_back(walk_total);
+  // Fixed point: the largest possible sum stays below 2^62.
+  int e2 = 0;
+  std::frexp(upper, &e2);  // upper < 2^e2
+  const double scale = std::ldexp(1.0, 62 - e2);
+  std::vector<unsigned long long> q(pw.size());
+  for (size_t i = 0; i < pw.size(); ++i) q[i] = (unsigned long long)std::llrint(pw[i] * scale);
+  const int n_walk = (int)face.size();
+  int r;
+  if (n_walk + 1 > s->rad_cap_walk) {  // the three walk arrays share one capacity
+    FFK_HIP(hipStreamSynchronize(st));
+    dfree(s->d_walk_start);
+    dfree(s->d_walk_face);
+    dfree(s->d_walk_q);
+    s->d_walk_start = nullptr;
+    s->d_walk_face = nullptr;
+    s->d_walk_q = nullptr;
+    s->rad_cap_walk = 0;
+    if ((r = dalloc(&s->d_walk_start, (size_t)n_walk + 1)) < 0 || (r = dalloc(&s->d_walk_face, (size_t)n_walk + 1)) < 0 ||
+        (r = dalloc(&s->d_walk_q, 2 * ((size_t)n_walk + 1))) < 0)
+      return r;
+    s->rad_cap_walk = n_walk + 1;
+  }
+  if (nm > s->rad_cap_meshes) {
+    FFK_HIP(hipStreamSynchronize(st));
+    dfree(s->d_rad_acc);
+    s->d_rad_acc = nullptr;
+    s->rad_cap_meshes = 0;
+    if ((r = dalloc(&s->d_rad_acc, (size_t)nm)) < 0) return r;
+    s->rad_cap_meshes = nm;
+  }
+  FFK_HIP(hipMemcpyAsync(s->d_walk_start, start.data(), 8 * start.size(), hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(s->d_walk_face, face.data(), 4 * face.size(), hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(s->d_walk_q, q.data(), 8 * q.size(), hipMemcpyHostToDevice, st));
+  k_fill_acc<<<(nm + 255) / 256, 256, 0, st>>>(s->d_rad_acc, nm);
+  const long long blocks_needed = (walk_total + 255) / 256;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  if (d_stats)
+    k_radiance<true><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q,
+                                             walk_total, k0, k1, s->d_rad_acc, d_stats);
+  else
+    k_radiance<false><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q,
+                                              walk_total, k0, k1, s->d_rad_acc, nullptr);
+  FFK_HIP(hipGetLastError());
+  std::vector<unsigned long long> acc(nm);
+  FFK_HIP(hipMemcpyAsync(acc.data(), s->d_rad_acc, 8 * (size_t)nm, hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipStreamSynchronize(st));
+  for (int m = 0; m < nm; ++m) h_absorption[m] = (double)acc[m] / scale;
+  return FFK_OK;
+}
+
+}  // extern "C"

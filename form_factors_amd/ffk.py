@@ -53,6 +53,9 @@ SIGNATURES = {
     "ffk_batch_destroy": (None, [_vp]),
     "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
     "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ffk_radiance": (C.c_int, [_vp, _ull, _f, C.POINTER(C.c_double), _vp, _vp]),
+    "ffk_radiance_weights": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _f, _f]),
+    "ffk_radiance_powers": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, _f, _f]),
     "ffk_convert_f": (C.c_int, [_vp, _ll, C.c_int, _vp, _vp]),
     "ffk_pack_exchange": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "ffk_sbff": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -143,6 +146,31 @@ def emitted_counts(weights, n_rays, total):
     return fr, rr
 
 
+def radiance_weights(faces, meshes, eps2, T):
+    """Reference radiance weights (2 per face) and float total (host)."""
+    faces = _np(faces, np.float32).reshape(-1, 9)
+    meshes = _np(meshes, np.int32).reshape(-1, 3)
+    w = np.zeros(2 * len(faces), np.float32)
+    total = C.c_float()
+    check(lib().ffk_radiance_weights(faces.ctypes.data_as(_f), len(faces), meshes.ctypes.data_as(_i), len(meshes),
+                                     _np(eps2, np.float32).ctypes.data_as(_f), _np(T, np.float32).ctypes.data_as(_f),
+                                     w.ctypes.data_as(_f), C.byref(total)), "ffk_radiance_weights")
+    return w, np.float32(total.value)
+
+
+def radiance_powers(faces, meshes, eps2, T, front, rear):
+    """Per-ray powers (2 per face) and per-mesh emission (host, float)."""
+    faces = _np(faces, np.float32).reshape(-1, 9)
+    meshes = _np(meshes, np.int32).reshape(-1, 3)
+    p = np.zeros(2 * len(faces), np.float32)
+    em = np.zeros(len(meshes), np.float32)
+    check(lib().ffk_radiance_powers(faces.ctypes.data_as(_f), len(faces), meshes.ctypes.data_as(_i), len(meshes),
+                                    _np(eps2, np.float32).ctypes.data_as(_f), _np(T, np.float32).ctypes.data_as(_f),
+                                    _np(front, np.int32).ctypes.data_as(_i), _np(rear, np.int32).ctypes.data_as(_i),
+                                    p.ctypes.data_as(_f), em.ctypes.data_as(_f)), "ffk_radiance_powers")
+    return p, em
+
+
 def device_info():
     n_cu, clk, mclk, bus, l2 = (C.c_int() for _ in range(5))
     name = C.create_string_buffer(256)
@@ -198,6 +226,25 @@ class Scene:
         self.n_rays = n.value
         self.frame = frame
         return self.n_rays
+
+    def radiance_step(self, eps2, T, n_rays, seed, frame=FRAME_REFERENCE, stats=None):
+        """One radiance_equation::calculate: layout from the radiance weights,
+        then the device pass. Returns (emission float32, absorption float64)."""
+        w, total = radiance_weights(self.faces, self.meshes, eps2, T)
+        front, rear = emitted_counts(w, n_rays, total)
+        self.set_emission(front=front, rear=rear, frame=frame)
+        p, em = radiance_powers(self.faces, self.meshes, eps2, T, front, rear)
+        return em, self.radiance(seed, p, stats=stats)
+
+    def radiance(self, seed, ray_power, stats=None):
+        """Absorbed power per mesh (float64) for the current emission layout;
+        ray_power: float32[2 * n_faces], power of one front / rear ray."""
+        rp = _np(ray_power, np.float32)
+        assert len(rp) == 2 * self.n_faces
+        out = np.zeros(self.n_meshes, np.float64)
+        check(lib().ffk_radiance(self.h, seed, rp.ctypes.data_as(_f), out.ctypes.data_as(C.POINTER(C.c_double)),
+                                 _ptr(stats), _stream()), "ffk_radiance")
+        return out
 
     def rays_per_mesh(self):
         """Rays emitted by each mesh (front + rear over its faces): the row

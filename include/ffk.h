@@ -157,6 +157,35 @@ int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long l
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream);
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
 
+/* ---- radiance equation --------------------------------------------------------
+ * radiance_equation::calculate (thermal_equation/radiance_cpu.cpp:132-190) on
+ * the device, over the scene's current emission layout and grid. The caller
+ * sets the layout from the radiance weights T * eps * A per face side
+ * (radiance_cpu.cpp:96-122, e.g. with ffk_emitted_counts and
+ * ffk_scene_set_emission) and passes h_ray_power[2f], h_ray_power[2f+1] = the
+ * power of one front / rear ray of face f (front_emission / front_rays).
+ * Malley rays come from Philox keyed by seed (the stream of ffk_emit_malley);
+ * ray g is attributed to the face and side of the reference's mesh-order walk
+ * at position g, and its power is added to the mesh of the face it hits.
+ * h_absorption (double[n_meshes], overwritten) is the exact sum of the ray
+ * powers, accumulated in 64-bit fixed point (order-independent). d_stats as
+ * in ffk_cast. */
+int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power, double* h_absorption,
+                 unsigned long long* d_stats, void* stream);
+/* Host helpers with the reference's float arithmetic. Weights
+ * (radiance_cpu.cpp:96-122): w[2f] = T_m * eps_front_m * area_f,
+ * w[2f+1] = T_m * eps_rear_m * area_f for f in mesh m (0 outside every mesh),
+ * total accumulated in mesh order. Ray powers and emission
+ * (radiance_cpu.cpp:151-178): power[2f] = front_emission / front_rays (0 when
+ * the face emits no front rays), emission[m] += front_emission +
+ * rear_emission (added to the caller's values). h_eps2 holds the front and
+ * rear emissivity of each mesh. */
+int ffk_radiance_weights(const float* h_faces, int n_faces, const int* h_meshes, int n_meshes,
+                         const float* h_eps2, const float* h_T, float* h_weights, float* total);
+int ffk_radiance_powers(const float* h_faces, int n_faces, const int* h_meshes, int n_meshes,
+                        const float* h_eps2, const float* h_T, const int* h_front, const int* h_rear,
+                        float* h_ray_power, float* h_emission);
+
 /* ---- thermal ---------------------------------------------------------------- */
 /* Elementwise conversion of F (float32) to the storage type
  * (FFK_BF16/FFK_F16/FFK_F32), round to nearest even. */

@@ -682,6 +682,92 @@ int or_ff_calculate(const float* faces, int n_faces, const int* meshes, int n_me
 
 /* ---------------- thermal ----------------------------------------------- */
 
+// thermal_equation/radiance_cpu.cpp:96-122: emission weights T * eps * A per
+// face side, in float, accumulated in mesh order; faces outside every mesh
+// keep weight 0.
+float or_radiance_weights(const float* faces, int n_faces, const int* meshes, int n_meshes,
+                          const float* eps2, const float* T, float* weights) {
+  std::memset(weights, 0, sizeof(float) * 2 * (size_t)n_faces);
+  float total = 0;
+  for (int m = 0; m != n_meshes; ++m) {
+    const float density = T[m];
+    const float front_density = density * eps2[2 * m];
+    const float rear_density = density * eps2[2 * m + 1];
+    for (int f = meshes[3 * m]; f != meshes[3 * m] + meshes[3 * m + 1]; ++f) {
+      const float a = or_triangle_area(faces + 9 * (size_t)f);
+      const float front = front_density * a;
+      const float rear = rear_density * a;
+      total += front + rear;
+      weights[2 * f] = front;
+      weights[2 * f + 1] = rear;
+    }
+  }
+  return total;
+}
+
+// thermal_equation/radiance_cpu.cpp:132-190 over given hit indices of the
+// Malley rays emitted with those weights (emission order = face order). The
+// walk goes mesh by mesh, face by face, consuming front then rear rays of
+// each face; a ray that hits deposits front_emission / front_rays (or the
+// rear analogue) on the hit face's mesh. absorption is accumulated in float
+// in ray order, as the reference does; absorption64 (may be NULL) holds the
+// same sum in double. Hits on faces outside every mesh are dropped.
+void or_radiance_walk(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* eps2,
+                      const float* T, const int* front, const int* rear, long n_total, const int* hit_idx,
+                      float* emission, float* absorption, double* absorption64) {
+  const float sigma = 5.670400e-8f;
+  std::vector<int> f2m(n_faces, -1);
+  for (int m = 0; m != n_meshes; ++m)
+    for (int f = 0; f != meshes[3 * m + 1]; ++f) f2m[meshes[3 * m] + f] = m;
+  long n_ray = 0;
+  for (int m = 0; m != n_meshes; ++m) {
+    const float t = T[m];
+    const float power_density = sigma * (t * t * t * t);
+    const float front_density = power_density * eps2[2 * m];
+    const float rear_density = power_density * eps2[2 * m + 1];
+    for (int f = meshes[3 * m]; f != meshes[3 * m] + meshes[3 * m + 1]; ++f) {
+      const float a = or_triangle_area(faces + 9 * (size_t)f);
+      const float front_emission = front_density * a;
+      const float rear_emission = rear_density * a;
+      emission[m] += front_emission + rear_emission;
+      const int face_rays = front[f] + rear[f];
+      for (int j = 0; j != face_rays && n_ray < n_total; ++j, ++n_ray) {
+        const int h = hit_idx[n_ray];
+        if (h < 0 || f2m[h] < 0) continue;
+        const float ray_power = j < front[f] ? (front_emission / front[f]) : (rear_emission / rear[f]);
+        absorption[f2m[h]] += ray_power;
+        if (absorption64) absorption64[f2m[h]] += (double)ray_power;
+      }
+    }
+  }
+}
+
+// The whole radiance_equation::calculate: weights, emitted counts, Malley
+// rays (rng/seed/frame as or_malley_emit), cast (0 naive, 1 zgrid), walk.
+int or_radiance(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* eps2,
+                const float* T, int n_rays, int rng, uint64_t seed, int frame, int caster, int n_threads,
+                float* emission, float* absorption, double* absorption64) {
+  std::vector<float> w(2 * (size_t)n_faces);
+  const float total = or_radiance_weights(faces, n_faces, meshes, n_meshes, eps2, T, w.data());
+  std::vector<int> fr(n_faces), rr(n_faces);
+  or_emitted_counts(w.data(), n_faces, n_rays, total, fr.data(), rr.data());
+  const long n = or_malley_count(w.data(), n_faces, n_rays, total);
+  std::vector<float> rays(6 * (size_t)std::max(n, 1L));
+  std::vector<float> pts(3 * (size_t)std::max(n, 1L));
+  std::vector<int> idx(std::max(n, 1L));
+  or_malley_emit(faces, n_faces, w.data(), total, n_rays, rng, seed, frame, rays.data());
+  if (caster == 0) {
+    or_naive_cast(faces, n_faces, rays.data(), n, idx.data(), pts.data(), n_threads);
+  } else {
+    or_zgrid* z = or_zgrid_build(faces, n_faces);
+    or_zgrid_cast(z, rays.data(), n, idx.data(), pts.data(), n_threads);
+    or_zgrid_free(z);
+  }
+  or_radiance_walk(faces, n_faces, meshes, n_meshes, eps2, T, fr.data(), rr.data(), n, idx.data(), emission,
+                   absorption, absorption64);
+  return 0;
+}
+
 // thermal_equation/form_factors.cpp:109-128, in double precision.
 void or_sbff_f64(int n, const double* F, const double* T, const double* area,
                  const double* ef, const double* er, double* emission, double* absorption) {

@@ -96,6 +96,14 @@ int or_ff_calculate(const float* faces, int n_faces, const int* meshes, int n_me
 
 /* ---- thermal (thermal_equation/*.cpp, thermal_solution/cpu_adams.cpp) -- */
 /* q_n += sum_m E_m F[m*N+n] with E_m = (eps_f+eps_r) sigma T^4 A_m, in double. */
+float or_radiance_weights(const float* faces, int n_faces, const int* meshes, int n_meshes,
+                          const float* eps2, const float* T, float* weights);
+void or_radiance_walk(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* eps2,
+                      const float* T, const int* front, const int* rear, long n_total, const int* hit_idx,
+                      float* emission, float* absorption, double* absorption64);
+int or_radiance(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* eps2,
+                const float* T, int n_rays, int rng, uint64_t seed, int frame, int caster, int n_threads,
+                float* emission, float* absorption, double* absorption64);
 void or_sbff_f64(int n, const double* F, const double* T, const double* area,
                  const double* eps_front, const double* eps_rear,
                  double* emission, double* absorption);

@@ -81,6 +81,12 @@ def lib():
         _sig(L, "or_ff_tally", C.c_int, [_i, C.c_int, C.c_int, _i, _i, C.c_long, _i, _f])
         _sig(L, "or_ff_calculate", C.c_int,
              [_f, C.c_int, _i, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int, _f])
+        _sig(L, "or_radiance_weights", C.c_float, [_f, C.c_int, _i, C.c_int, _f, _f, _f])
+        _sig(L, "or_radiance_walk", None,
+             [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, C.c_long, _i, _f, _f, _d])
+        _sig(L, "or_radiance", C.c_int,
+             [_f, C.c_int, _i, C.c_int, _f, _f, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int,
+              _f, _f, _d])
         _sig(L, "or_sbff_f64", None, [C.c_int, _d, _d, _d, _d, _d, _d, _d])
         _sig(L, "or_adams_combine_f64", None, [C.c_int, C.c_int, C.c_double, _d, _d, _d])
         _lib = L
@@ -270,6 +276,43 @@ def sbff_f64(F, T, area, eps_front, eps_rear):
     ab = np.zeros(n)
     lib().or_sbff_f64(n, _p(F, _d), *[_p(a, _d) for a in args], _p(em, _d), _p(ab, _d))
     return em, ab
+
+
+def radiance_weights(faces, meshes, eps2, T):
+    """radiance_cpu.cpp:96-122 weights (2 per face) and their float total."""
+    faces, meshes = f32(faces).reshape(-1, 9), i32(meshes).reshape(-1, 3)
+    w = np.zeros(2 * len(faces), np.float32)
+    total = lib().or_radiance_weights(_p(faces, _f), len(faces), _p(meshes, _i), len(meshes),
+                                      _p(f32(eps2), _f), _p(f32(T), _f), _p(w, _f))
+    return w, np.float32(total)
+
+
+def radiance_walk(faces, meshes, eps2, T, front, rear, hit_idx):
+    """radiance_cpu.cpp:132-190 over given hits: (emission, absorption in
+    float ray order, absorption in double)."""
+    faces, meshes = f32(faces).reshape(-1, 9), i32(meshes).reshape(-1, 3)
+    nm = len(meshes)
+    em = np.zeros(nm, np.float32)
+    ab = np.zeros(nm, np.float32)
+    ab64 = np.zeros(nm, np.float64)
+    hit = i32(hit_idx)
+    lib().or_radiance_walk(_p(faces, _f), len(faces), _p(meshes, _i), nm, _p(f32(eps2), _f), _p(f32(T), _f),
+                           _p(i32(front), _i), _p(i32(rear), _i), len(hit), _p(hit, _i), _p(em, _f), _p(ab, _f),
+                           _p(ab64, _d))
+    return em, ab, ab64
+
+
+def radiance(faces, meshes, eps2, T, n_rays, rng=RNG_PHILOX, seed=0, frame=FRAME_REFERENCE, caster=0,
+             n_threads=1):
+    """The whole radiance_equation::calculate (radiance_cpu.cpp:132-190)."""
+    faces, meshes = f32(faces).reshape(-1, 9), i32(meshes).reshape(-1, 3)
+    nm = len(meshes)
+    em = np.zeros(nm, np.float32)
+    ab = np.zeros(nm, np.float32)
+    ab64 = np.zeros(nm, np.float64)
+    lib().or_radiance(_p(faces, _f), len(faces), _p(meshes, _i), nm, _p(f32(eps2), _f), _p(f32(T), _f),
+                      int(n_rays), rng, seed, frame, caster, n_threads, _p(em, _f), _p(ab, _f), _p(ab64, _d))
+    return em, ab, ab64
 
 
 def adams_combine_f64(n_step, dt, T_n, E_hist):

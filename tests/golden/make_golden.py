@@ -283,6 +283,23 @@ def gen_radiance(rng):
     assert rc == 0
     save("radiance.npz", faces=faces, meshes=meshes, eps=eps, T=T, n_rays=np.int32(20_000), emission=em,
          absorption=ab)
+    # multi-face meshes listed out of face order (the walk attribution), on an
+    # 80-triangle icosphere with inward normals
+    faces = scenes.icosphere(1, inward=True)
+    counts = [3, 1, 2, 4, 2, 3, 1, 4] * 4  # 80 faces
+    firsts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    r2 = np.random.default_rng(7)  # own stream: the fixtures after this one keep their inputs
+    order = r2.permutation(len(counts))
+    meshes = np.stack([firsts[order], np.array(counts)[order], np.zeros(len(counts), int)], 1).astype(np.int32)
+    nm = len(meshes)
+    eps = r2.uniform(0.2, 1.0, (nm, 2)).astype(np.float32)
+    T = r2.uniform(250, 400, nm).astype(np.float32)
+    em = np.zeros(nm, np.float32)
+    ab = np.zeros(nm, np.float32)
+    rc = R.ffref_radiance(P(faces), len(faces), P(meshes, _i), nm, P(eps), P(T), 30_000, P(em), P(ab))
+    assert rc == 0
+    save("radiance_multi.npz", faces=faces, meshes=meshes, eps=eps, T=T, n_rays=np.int32(30_000), emission=em,
+         absorption=ab)
 
 
 def gen_parallel(rng):

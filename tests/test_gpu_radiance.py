@@ -1,0 +1,91 @@
+"""radiance_equation::calculate on the GPU (ffk_radiance) against the oracle's
+restatement of thermal_equation/radiance_cpu.cpp on the identical Philox rays:
+emission bit for bit, absorption equal to the double-precision sum of the same
+float ray powers (the GPU sums them exactly in 64-bit fixed point)."""
+import numpy as np
+import pytest
+
+from tests.conftest import bits, golden
+
+pytestmark = pytest.mark.gpu
+
+
+def _partial_meshes():
+    from form_factors_amd import scenes
+
+    faces = scenes.icosphere(1, inward=True)
+    # faces 40..79 belong to no mesh: they emit nothing and absorb nothing
+    meshes = np.array([[i, 2, 0] for i in range(0, 40, 2)], np.int32)[::-1].copy()
+    return faces, meshes
+
+
+def _cases():
+    from form_factors_amd import scenes
+
+    rng = np.random.default_rng(11)
+    g = golden("radiance_multi.npz")
+    out = [("multi_out_of_order", g["faces"], g["meshes"], g["eps"], g["T"], 60_000)]
+    faces, _ = scenes.ff_box()
+    m = scenes.per_face_meshes(12)
+    out.append(("box", faces, m, rng.uniform(0.1, 1, (12, 2)).astype(np.float32),
+                rng.uniform(250, 400, 12).astype(np.float32), 100_000))
+    faces, meshes = _partial_meshes()
+    out.append(("partial_meshes", faces, meshes, rng.uniform(0.1, 1, (len(meshes), 2)).astype(np.float32),
+                rng.uniform(250, 400, len(meshes)).astype(np.float32), 50_000))
+    faces, meshes = scenes.confetti_scene(300, seed=5)
+    out.append(("confetti", faces, meshes, rng.uniform(0.1, 1, (len(meshes), 2)).astype(np.float32),
+                rng.uniform(250, 400, len(meshes)).astype(np.float32), 200_000))
+    return out
+
+
+@pytest.mark.parametrize("frame", [0, 1])
+@pytest.mark.parametrize("case", range(4))
+def test_radiance_matches_oracle(gpu, oracle, case, frame):
+    name, faces, meshes, eps, T, n_rays = _cases()[case]
+    print(f"radiance case {name} frame {frame}", flush=True)
+    sc = gpu.Scene(faces, meshes)
+    sc.build_grid()
+    seed = 0xA11CE + case
+    em, ab = sc.radiance_step(eps, T, n_rays, seed, frame=frame)
+    em_o, ab_o, ab64 = oracle.radiance(faces, meshes, eps, T, n_rays, rng=oracle.RNG_PHILOX, seed=seed,
+                                       frame=frame, caster=0, n_threads=8)
+    assert np.array_equal(bits(em), bits(em_o))
+    assert ab64.sum() > 0
+    assert np.abs(ab - ab64).max() <= 1e-12 * ab64.max()
+    # the reference's float accumulation in ray order differs only by rounding
+    assert np.allclose(ab, ab_o, rtol=2e-4, atol=1e-6 * ab64.max())
+
+
+def test_radiance_homogeneous_and_repeatable(gpu):
+    from form_factors_amd import scenes
+
+    faces = scenes.icosphere(2, inward=True)
+    meshes = scenes.per_face_meshes(len(faces))
+    eps = np.full((len(meshes), 2), 0.6, np.float32)
+    T = np.full(len(meshes), 300.0, np.float32)
+    sc = gpu.Scene(faces, meshes)
+    sc.build_grid()
+    em1, ab1 = sc.radiance_step(eps, T, 300_000, 7)
+    em1b, ab1b = sc.radiance_step(eps, T, 300_000, 7)
+    em2, ab2 = sc.radiance_step(eps, 2 * T, 300_000, 7)
+    assert np.array_equal(ab1, ab1b)  # deterministic
+    assert np.array_equal(ab2, 16 * ab1) and np.array_equal(em2, 16 * em1)
+    # closed sphere with inward normals: every front ray is absorbed; the rear
+    # rays leave through the outside, so half of the emission comes back
+    assert abs(ab1.sum() / em1.astype(np.float64).sum() - 0.5) < 1e-3
+
+
+def test_radiance_errors(gpu):
+    from form_factors_amd import ffk, scenes
+
+    faces, meshes = scenes.ff_parallel_planes(1.0)
+    sc = gpu.Scene(faces, meshes)
+    with pytest.raises(ffk.FFKError):  # no emission layout / grid yet
+        sc.radiance(1, np.zeros(2 * len(faces), np.float32))
+    sc.build_grid()
+    sc.set_emission(n_rays=1000)
+    bad = np.full(2 * len(faces), -1.0, np.float32)
+    with pytest.raises(ffk.FFKError):
+        sc.radiance(1, bad)
+    zero = sc.radiance(1, np.zeros(2 * len(faces), np.float32))
+    assert (zero == 0).all()

@@ -466,3 +466,56 @@ def test_live_random_rotations_vs_reference(oracle):
         m2 = np.zeros(9, np.float32)
         R.ffref_rotate_towards(frm.ctypes.data_as(oracle._f), to.ctypes.data_as(oracle._f), m2.ctypes.data_as(oracle._f))
         assert np.array_equal(bits(oracle.rotate_towards(frm, to).reshape(9)), bits(m2))
+
+
+# ---------------------------------------------------------------- radiance --
+
+@pytest.mark.parametrize("name", ["radiance", "radiance_multi"])
+def test_radiance_matches_reference_golden(oracle, name):
+    """radiance_equation::calculate (thermal_equation/radiance_cpu.cpp:132-190)
+    with the reference's MALLEY_CPU emission and naive caster, bit for bit;
+    radiance_multi has multi-face meshes listed out of face order."""
+    g = golden(name + ".npz")
+    em, ab, ab64 = oracle.radiance(g["faces"], g["meshes"], g["eps"], g["T"], int(g["n_rays"]),
+                                   rng=oracle.RNG_MT19937, frame=oracle.FRAME_REFERENCE, caster=0)
+    assert np.array_equal(bits(em), bits(g["emission"]))
+    assert np.array_equal(bits(ab), bits(g["absorption"]))
+    # the double sum of the same ray powers differs only by float accumulation
+    assert np.allclose(ab64, ab, rtol=1e-4)
+
+
+def test_radiance_is_homogeneous_in_temperature(oracle):
+    """Doubling every T leaves the weight ratios (hence the rays) unchanged and
+    scales every ray power by exactly 16: absorption and emission x16 bit for bit."""
+    from form_factors_amd import scenes
+
+    faces = scenes.icosphere(1, inward=True)
+    meshes = scenes.per_face_meshes(len(faces))
+    rng = np.random.default_rng(3)
+    eps = rng.uniform(0.2, 1.0, (len(meshes), 2)).astype(np.float32)
+    T = np.full(len(meshes), 300.0, np.float32)
+    a = oracle.radiance(faces, meshes, eps, T, 40_000)
+    b = oracle.radiance(faces, meshes, eps, 2 * T, 40_000)
+    assert np.array_equal(b[0], 16 * a[0]) and np.array_equal(b[1], 16 * a[1])
+
+
+def test_radiance_closed_box_absorbs_what_enters(oracle):
+    """Closed box: the rays of inward-facing sides all hit the box (up to
+    leakage along shared edges), so the absorbed total equals the emission of
+    those sides."""
+    from form_factors_amd import scenes
+
+    faces, _ = scenes.ff_box()
+    meshes = scenes.per_face_meshes(12)
+    f = faces.reshape(-1, 3, 3).astype(np.float64)
+    n = np.cross(f[:, 1] - f[:, 0], f[:, 2] - f[:, 0])
+    c = f.mean(1)
+    inward = (n * (c.mean(0) - c)).sum(1) > 0
+    eps = np.tile([0.7, 0.4], (12, 1)).astype(np.float32)
+    T = np.linspace(280, 330, 12).astype(np.float32)
+    em, ab, ab64 = oracle.radiance(faces, meshes, eps, T, 100_000)
+    area = 0.5 * np.linalg.norm(n, axis=1)
+    P = 5.670400e-8 * T.astype(np.float64) ** 4 * area
+    expected = np.where(inward, 0.7 * P, 0.4 * P).sum()
+    assert abs(ab64.sum() / expected - 1) < 1e-3
+    assert np.allclose(em, 1.1 * P, rtol=1e-5)
