@@ -717,6 +717,159 @@ thermal_equation::system_t* system_create() {
 
 }  // namespace hip_sb_ff
 
+// ======================================== radiance equation (Monte Carlo) ==
+namespace hip_radiance {
+
+struct state {
+  radiance_equation::params_t params = {nullptr, 0};
+  subject::scene_t* scene = nullptr;
+  ray_caster::scene_t emission_scene = {0, 0};
+  ffk_scene* s = nullptr;
+  hipStream_t st = nullptr;
+  bool grid = false;
+  int frame = -1;
+  std::vector<int> face2mesh, front, rear;
+  std::vector<float> eps2, weights, power;
+  std::vector<double> absorption;
+};
+struct system : thermal_equation::system_t {
+  state* st;
+};
+
+int init(system* sys, radiance_equation::params_t* params) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -THERMAL_EQUATION_ERROR;
+  sys->st->params = *params;
+  sys->st->st = make_stream();
+  return sys->st->st ? THERMAL_EQUATION_OK : -THERMAL_EQUATION_ERROR;
+}
+
+int shutdown(system* sys) {
+  state* s = sys->st;
+  if (!s) return THERMAL_EQUATION_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  ffk_scene_destroy(s->s);
+  drop_stream(s->st);
+  delete s;
+  sys->st = nullptr;
+  return THERMAL_EQUATION_OK;
+}
+
+// thermal_equation/radiance_cpu.cpp:78-94: the emitter gets the faces of the
+// subject scene (a layout-compatible prefix); the device scene is built here.
+int set_scene(system* sys, subject::scene_t* scene) {
+  state* s = sys->st;
+  s->scene = scene;
+  s->emission_scene.n_faces = scene->n_faces;
+  s->emission_scene.faces = scene->faces;
+  int r;
+  emission::system_t* em = s->params.emitter;
+  if ((r = em->methods->set_scene(em, &s->emission_scene)) < 0) return r;
+  const int nf = scene->n_faces, nm = scene->n_meshes;
+  s->face2mesh.assign(nf, -1);  // subject/system.cpp:98-112; faces outside every mesh: -1
+  s->eps2.assign(2 * (size_t)nm, 0.f);
+  for (int m = 0; m < nm; ++m) {
+    const subject::mesh_t& mesh = scene->meshes[m];
+    if (mesh.first_idx < 0 || mesh.n_faces < 0 || mesh.first_idx + mesh.n_faces > nf) return -THERMAL_EQUATION_ERROR;
+    for (int f = 0; f < mesh.n_faces; ++f) s->face2mesh[mesh.first_idx + f] = m;
+    const subject::material_t& mat = scene->materials[mesh.material_idx];
+    s->eps2[2 * m] = mat.front.emissivity;
+    s->eps2[2 * m + 1] = mat.rear.emissivity;
+  }
+  s->grid = false;
+  s->front.clear();
+  if (nf == 0 || nm == 0) return THERMAL_EQUATION_OK;
+  if (!s->s && ffk_scene_create(&s->s) < 0) return -THERMAL_EQUATION_ERROR;
+  if (ffk_scene_set_faces(s->s, (const float*)scene->faces, nf, s->st) < 0) return -THERMAL_EQUATION_ERROR;
+  if (ffk_scene_set_meshes(s->s, (const int*)scene->meshes, nm, s->st) < 0) return -THERMAL_EQUATION_ERROR;
+  return THERMAL_EQUATION_OK;
+}
+
+// thermal_equation/radiance_cpu.cpp:132-190; adds to the task's arrays. The
+// emission is added face by face with the reference's float expressions; the
+// absorbed power of each mesh is the exact sum of its ray powers, added once.
+int calculate(system* sys, thermal_equation::task_t* task) {
+  state* s = sys->st;
+  const subject::scene_t* scene = s->scene;
+  if (!scene) return -THERMAL_EQUATION_ERROR;
+  const int nf = scene->n_faces, nm = scene->n_meshes;
+  if (nf == 0 || nm == 0) return THERMAL_EQUATION_OK;
+  const int n_rays = s->params.n_rays;
+  float total = 0;
+  s->weights.assign(2 * (size_t)nf, 0.f);
+  if (ffk_radiance_weights((const float*)scene->faces, nf, (const int*)scene->meshes, nm, s->eps2.data(),
+                           task->temperatures, s->weights.data(), &total) < 0)
+    return -THERMAL_EQUATION_ERROR;
+  std::vector<int> fr(nf), rr(nf);
+  ffk_emitted_counts(s->weights.data(), nf, n_rays, total, fr.data(), rr.data());
+  s->power.assign(2 * (size_t)nf, 0.f);
+  if (ffk_radiance_powers((const float*)scene->faces, nf, (const int*)scene->meshes, nm, s->eps2.data(),
+                          task->temperatures, fr.data(), rr.data(), s->power.data(), task->emission) < 0)
+    return -THERMAL_EQUATION_ERROR;
+  s->absorption.assign(nm, 0.0);
+  emission::system_t* emitter = s->params.emitter;
+  hip_malley_emission::state* em =
+      hip_malley_emission::is_hip(emitter) ? hip_malley_emission::state_of(emitter) : nullptr;
+  int r;
+  if (em && hip_ray_caster::is_hip(em->caster) && hip_ray_caster::method_of(em->caster) == FFK_CAST_GRID) {
+    // Fused on the device: emission, grid cast and deposit in one pass.
+    if (fr != s->front || rr != s->rear || em->frame != s->frame) {
+      if (ffk_scene_set_emission(s->s, fr.data(), rr.data(), em->frame, nullptr, s->st) < 0)
+        return -THERMAL_EQUATION_ERROR;
+      s->front = fr;
+      s->rear = rr;
+      s->frame = em->frame;
+    }
+    if (!s->grid) {
+      if (ffk_scene_build_grid(s->s, 0, 0, 0, s->st) < 0) return -THERMAL_EQUATION_ERROR;
+      s->grid = true;
+    }
+    const unsigned long long key = hip_malley_emission::next_key(em);
+    if (ffk_radiance(s->s, key, s->power.data(), s->absorption.data(), nullptr, s->st) < 0)
+      return -THERMAL_EQUATION_ERROR;
+  } else {
+    // Any other emitter emits and casts; the walk runs here, in double.
+    malley_emission::task_t et(n_rays, total, s->weights.data());
+    if ((r = emitter->methods->calculate(emitter, &et)) < 0) {
+      rc_task_free(et.rays);
+      return r;
+    }
+    const long long n_total = et.rays ? et.rays->n_tasks : 0;
+    long long n_ray = 0;
+    for (int m = 0; m < nm; ++m) {
+      const subject::mesh_t& mesh = scene->meshes[m];
+      for (int f = mesh.first_idx; f < mesh.first_idx + mesh.n_faces; ++f) {
+        const int face_rays = fr[f] + rr[f];
+        for (int j = 0; j < face_rays && n_ray < n_total; ++j, ++n_ray) {
+          const math::triangle_t* h = et.rays->hit_face[n_ray];
+          if (!h) continue;
+          const int hm = s->face2mesh[h - scene->faces];
+          if (hm >= 0) s->absorption[hm] += (double)s->power[2 * (size_t)f + (j < fr[f] ? 0 : 1)];
+        }
+      }
+    }
+    rc_task_free(et.rays);
+  }
+  for (int m = 0; m < nm; ++m) task->absorption[m] += (float)s->absorption[m];
+  return THERMAL_EQUATION_OK;
+}
+
+const thermal_equation::system_methods_t methods = {
+    (int (*)(thermal_equation::system_t*, void*))&init,
+    (int (*)(thermal_equation::system_t*))&shutdown,
+    (int (*)(thermal_equation::system_t*, subject::scene_t*))&set_scene,
+    (int (*)(thermal_equation::system_t*, thermal_equation::task_t*))&calculate,
+};
+thermal_equation::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  return s;
+}
+
+}  // namespace hip_radiance
+
 // ==================================================== Adams solver (FP64) ==
 namespace hip_adams {
 
@@ -953,6 +1106,16 @@ void* ffp_thermal_equation_create(int type, void* dep, int n_rays, int f_dtype) 
     s = hip_sb_ff::system_create();
     if (!s) return nullptr;
     hip_sb_ff::params_t p = {(form_factors::system_t*)dep, n_rays, f_dtype};
+    if (s->methods->init(s, &p) < 0) {
+      s->methods->shutdown(s);
+      free(s);
+      return nullptr;
+    }
+  } else if (type == THERMAL_EQUATION_RADIANCE_HIP) {
+    if (!dep) return nullptr;
+    s = hip_radiance::system_create();
+    if (!s) return nullptr;
+    radiance_equation::params_t p = {(emission::system_t*)dep, n_rays};
     if (s->methods->init(s, &p) < 0) {
       s->methods->shutdown(s);
       free(s);

@@ -181,6 +181,22 @@ def mixed_scene(n_tris, seed):
     return faces, per_face_meshes(len(faces))
 
 
+def batch_scene_set(n_scenes=64, seed=0, tri_range=(200, 50_000), ray_range=(1_000, 20_000)):
+    """Independent scenes of uneven size: triangle counts log-uniform in
+    tri_range, each a random mix of a sphere, a cube shell and confetti from
+    its own seed, with a per-scene Malley ray budget uniform in ray_range rays
+    per triangle. Returns [(faces, meshes, rays_per_triangle, scene_seed)]."""
+    rng = np.random.default_rng(seed)
+    n_tris = np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]), n_scenes)).astype(int)
+    rays = rng.integers(ray_range[0], ray_range[1] + 1, n_scenes)
+    out = []
+    for k in range(n_scenes):
+        scene_seed = int(seed) * 1000 + k
+        faces, meshes = mixed_scene(int(n_tris[k]), scene_seed)
+        out.append((faces, meshes, int(rays[k]), scene_seed))
+    return out
+
+
 def two_plates_thermal(cells=100, gap=0.5, size=1.0):
     """Two size x size plates, cells x cells x 2 triangles each, one mesh per
     cell (2 faces), the second plate `gap` above the first."""

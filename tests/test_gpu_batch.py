@@ -67,3 +67,31 @@ def test_batched_tuning_does_not_change_bits(gpu):
         for F, want in zip(Fs, plain):
             assert np.array_equal(bits(F.cpu().numpy()), bits(want)), (block, rays_per_tile, n_blocks)
     b.close()
+
+
+def test_batch_scene_set_batched_equals_plain(gpu):
+    """The batched workload shape (uneven scenes: log-uniform triangle counts,
+    per-scene ray budgets), reduced in size: one persistent graph-replayed
+    launch gives every scene the bits of its own plain launch."""
+    import torch
+
+    from form_factors_amd import scenes
+
+    specs = scenes.batch_scene_set(12, seed=4, tri_range=(200, 4000), ray_range=(50, 400))
+    scs = []
+    for k, (faces, meshes, rpt, _) in enumerate(specs):
+        sc = gpu.Scene(faces, meshes)
+        sc.set_emission(n_rays=rpt * len(faces), frame=gpu.FRAME_ORTHONORMAL)
+        sc.build_grid()
+        scs.append(sc)
+    seeds = [s for (_, _, _, s) in specs]
+    plain = [sc.ff_fused(sd).cpu().numpy() for sc, sd in zip(scs, seeds)]
+    b = gpu.Batch(scs)
+    Fs = [torch.full((sc.n_meshes,) * 2, -1.0, device="cuda") for sc in scs]
+    tallies = [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda") for sc in scs]
+    for rep in range(2):
+        b.run(seeds, Fs, tallies=tallies, use_graph=True)
+        torch.cuda.synchronize()
+        for k, (F, want) in enumerate(zip(Fs, plain)):
+            assert np.array_equal(bits(F.cpu().numpy()), bits(want)), (rep, k)
+    b.close()

@@ -335,3 +335,48 @@ def test_sbff_equation_with_adams_two_plates(P, oracle):
         relbf = np.abs(results[1][step] - T).max() / T.max()
         assert rel32 < 2e-6 and rel16 < 1e-4 and relbf < 1e-4, (step, rel32, rel16, relbf)
     assert results[0][-1][: n // 2].mean() < 400  # the hot plate cools
+
+
+# ------------------------------- thermal_equation/radiance_cpu.cpp (HIP) --
+
+@pytest.mark.parametrize("caster_type", CASTERS)
+def test_radiance_equation_plugin(P, oracle, caster_type):
+    """Radiance HIP equation through the plugin table: grid caster = fused
+    device pass, naive caster = the emitter emits and casts, the walk runs on
+    the host. Emission bit for bit; absorption = the exact sum of the same
+    rays' float powers (FP64 oracle), added once as float. A second call uses
+    the emitter's next key."""
+    L = P.lib()
+    g = golden("radiance_multi.npz")
+    faces, meshes, eps, T = g["faces"], g["meshes"], g["eps"], g["T"]
+    nm = len(meshes)
+    mats = []
+    for m in range(nm):
+        mat = P.black_body()
+        mat.front[4], mat.rear[4] = float(eps[m, 0]), float(eps[m, 1])
+        mats.append(mat)
+    smeshes = meshes.copy()
+    smeshes[:, 2] = np.arange(nm)
+    sc, keep = subject_scene(P, faces, smeshes, mats)
+    rc = caster(P, caster_type)
+    em = L.ffp_emission_create(P.EMISSION_MALLEY_HIP, rc)
+    seed = 77
+    assert L.ffp_emission_set_seed(em, seed, 0) == 0
+    eq = L.ffp_thermal_equation_create(P.THERMAL_EQUATION_RADIANCE_HIP, em, 60_000, 0)
+    assert eq
+    assert L.ffp_thermal_equation_set_scene(eq, C.byref(sc)) == 0
+    for call in range(2):
+        Tc = (C.c_float * nm)(*[float(x) for x in T])
+        emi = (C.c_float * nm)()
+        ab = (C.c_float * nm)()
+        task = P.TeTask(Tc, emi, ab)
+        assert L.ffp_thermal_equation_calculate(eq, C.byref(task)) == 0
+        key = (seed + 0x9E3779B97F4A7C15 * call) % (1 << 64)
+        em_o, ab_o, ab64 = oracle.radiance(faces, meshes, eps, T, 60_000, seed=key, frame=0, n_threads=8)
+        assert np.array_equal(bits(np.frombuffer(emi, np.float32)), bits(em_o))
+        got = np.frombuffer(ab, np.float32).astype(np.float64)
+        assert np.abs(got - ab64).max() <= 1e-6 * ab64.max()
+    L.ffp_thermal_equation_free(eq)
+    L.ffp_emission_free(em)
+    L.ffp_ray_caster_free(rc)
+    assert not L.ffp_thermal_equation_create(P.THERMAL_EQUATION_RADIANCE_HIP, None, 1000, 0)
