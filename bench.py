@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-thermal", action="store_true")
+    ap.add_argument("--no-batch-set", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--thermal-steps", type=int, default=10_000)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--rays-per-tile", type=int, default=0)
@@ -90,6 +92,77 @@ def cpu_baseline(faces, rays_host, gpu_idx, n_threads):
     return {"value": len(rays_host) / dt / 1e6, "unit": "Mrays/s", "cores": n_threads, "kind": kind,
             "sample": f"zgrid_cpu cast of the first {len(rays_host)} rays of the identical ray set "
                       f"({dt:.1f} s on {n_threads} threads; {mism} hit-index mismatches vs the GPU)"}, mism
+
+
+def batch_set_leg(ffk, scenes, n_scenes):
+    """Uneven scene set: n_scenes scenes, triangle counts log-uniform in
+    [200, 50000], a per-scene mix of sphere / cube shell / confetti and
+    1k-20k Malley rays per triangle (scene set seed 0). Plain = one fused
+    launch per scene, back to back; batched = one graph-replayed persistent
+    launch per LDS class for all scenes. Both passes must give every scene the
+    same F bits (compared through integer checksums of the bit patterns)."""
+    import torch
+
+    t0 = time.perf_counter()
+    specs = scenes.batch_scene_set(n_scenes, seed=0)
+    scs, seeds = [], []
+    for faces, meshes, rpt, sd in specs:
+        sc = ffk.Scene(faces, meshes)
+        sc.set_emission(n_rays=rpt * len(faces), frame=ffk.FRAME_ORTHONORMAL)
+        sc.build_grid()
+        scs.append(sc)
+        seeds.append(sd)
+    n_rays = sum(sc.n_rays for sc in scs)
+    Fs = [torch.empty((sc.n_meshes,) * 2, dtype=torch.float32, device="cuda") for sc in scs]
+    plain_tallies = [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda")
+                     if ffk.lib().ffk_ff_needs_tally(sc.h) else None for sc in scs]
+    torch.cuda.synchronize()
+    log(f"scene set: {len(scs)} scenes, {sum(len(sc.faces) for sc in scs)} triangles, {n_rays} rays, "
+        f"F {sum(sc.n_meshes ** 2 for sc in scs) * 4 / 1e9:.1f} GB, set-up {time.perf_counter() - t0:.1f} s")
+
+    def checksums():
+        return [int(F.view(torch.int32).sum(dtype=torch.int64)) for F in Fs]
+
+    def plain_pass():
+        for sc, sd, F, T in zip(scs, seeds, Fs, plain_tallies):
+            sc.ff_fused(sd, F=F, tally=T, need_tally=T is not None)
+
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t, e0.elapsed_time(e1) / 1e3
+
+    plain_pass()  # warm-up
+    wall_p, ev_p = timed(plain_pass)
+    sums_plain = checksums()
+    log(f"scene set plain: {n_rays / wall_p / 1e6:.1f} Mrays/s ({wall_p:.3f} s)")
+    b = ffk.Batch(scs)
+    tallies = b.make_tallies()
+    for F in Fs:
+        F.fill_(-1.0)
+    b.run(seeds, Fs, tallies=tallies, use_graph=True)  # warm-up: plan and capture
+    wall_b, ev_b = timed(lambda: b.run(seeds, Fs, tallies=tallies, use_graph=True))
+    same = checksums() == sums_plain
+    log(f"scene set batched: {n_rays / wall_b / 1e6:.1f} Mrays/s ({wall_b:.3f} s), F identical: {same}")
+    b.close()
+    del Fs, tallies, plain_tallies
+    torch.cuda.empty_cache()
+    return {"workload": f"{len(scs)} scenes, triangles log-uniform in [200, 50000] ({sum(len(sc.faces) for sc in scs)} "
+                        f"total), sphere/cube/confetti mixes, 1k-20k rays per triangle, {n_rays} rays",
+            "n_rays": n_rays,
+            "plain": {"value": round(n_rays / wall_p / 1e6, 2), "seconds": round(wall_p, 4),
+                      "events_seconds": round(ev_p, 4), "path": "one fused launch per scene"},
+            "batched": {"value": round(n_rays / wall_b / 1e6, 2), "seconds": round(wall_b, 4),
+                        "events_seconds": round(ev_b, 4),
+                        "path": "persistent tile-queue launches per LDS class, one hipGraph"},
+            "speedup": round(wall_p / wall_b, 4), "F_bits_identical": same, "unit": "Mrays/s"}
 
 
 def thermal_leg(ffk, scenes, n_steps):
@@ -279,9 +352,11 @@ def main():
         except Exception as e:  # reported, never silently replaced
             result["batched"] = {"error": str(e)}
 
+    del F
+    torch.cuda.empty_cache()
+    if not args.no_batch_set:
+        result["batch_set"] = batch_set_leg(ffk, scenes, args.batch_scenes)
     if not args.no_thermal:
-        del F
-        torch.cuda.empty_cache()
         result["thermal"] = thermal_leg(ffk, scenes, args.thermal_steps)
 
     if not args.no_cpu:

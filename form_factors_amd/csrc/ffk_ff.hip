@@ -223,7 +223,7 @@ static int pick_hist(int N, long long max_tile, int requested) {
   if (requested == 1 && 4LL * N <= lds) return 1;
   if (requested == 2 && 2LL * ((N + 1) / 2) * 2 <= lds && max_tile <= 65535) return 2;
   if (4LL * N <= 65536) return 1;
-  if (4LL * ((N + 1) / 2) <= std::min(lds, 98304) && max_tile <= 65535) return 2;
+  if (4LL * ((N + 1) / 2) <= lds && max_tile <= 65535) return 2;
   return 0;
 }
 
@@ -332,14 +332,21 @@ struct ffk_batch {
   scene_desc* d_descs = nullptr;
   int2* d_rows = nullptr;  // (scene, mesh) rows tallied through the global tally
   int n_rows = 0;
-  int max_rows_n = 0;
-  int* d_queue = nullptr;
+  // Scenes grouped by the LDS their row histogram needs: one persistent
+  // launch per class, so small scenes are not run at the occupancy of the
+  // largest one. Tiles of class c are d_tiles[first, first + n).
+  struct klass {
+    int hist;
+    size_t lds;
+    int first, n;
+  };
+  std::vector<klass> classes;
+  std::vector<char> scene_global;  // scene tallied entirely through the global tally (hist 0)
+  std::vector<char> scene_tally;   // scene needs a tally buffer (global, or rows over several tiles)
+  int* d_queue = nullptr;          // one counter per class
   unsigned long long* d_seeds = nullptr;
   unsigned long long** d_tallies = nullptr;
   float** d_Fs = nullptr;
-  int hist = 0;
-  int max_N = 0;
-  long long max_tile = 0;
   long long total_rays = 0;
   int block = 0, rpt = 0, n_blocks = 0;
   // graph state
@@ -383,42 +390,70 @@ static void free_graph(ffk_batch* b) {
   b->graph = nullptr;
 }
 
+// Histogram class of a scene with N meshes (tiles of at most 65535 rays):
+// 0..4 = packed 16-bit LDS histogram of at most 8, 24, 48, 96 or 160 KB,
+// 5 = global uint64 tally.
+static int hist_class(int N) {
+  static const size_t limits[5] = {8192, 24576, 49152, 98304, 163840};
+  const size_t need = hist_bytes(2, N);
+  for (int c = 0; c < 5; ++c)
+    if (need <= limits[c] && (int)need <= max_lds_bytes()) return c;
+  return 5;
+}
+
 static int batch_plan(ffk_batch* b) {
   free_graph(b);
   dfree(b->d_tiles);
   dfree(b->d_rows);
+  dfree(b->d_queue);
   b->d_tiles = nullptr;
   b->d_rows = nullptr;
+  b->d_queue = nullptr;
   long long total = 0;
-  int max_N = 0;
-  for (ffk_scene* s : b->scenes) {
-    total += s->n_rays;
-    max_N = std::max(max_N, s->n_meshes);
-  }
+  for (ffk_scene* s : b->scenes) total += s->n_rays;
   long long rpt = b->rpt > 0 ? b->rpt : auto_rays_per_tile(total);
   rpt = std::min(rpt, 65535LL);
-  std::vector<tile> tiles;
+  std::vector<std::vector<tile>> per_class(6);
+  std::vector<size_t> class_lds(6, 0);
   std::vector<int2> rows;
-  long long max_tile = 0;
+  b->scene_global.assign(b->scenes.size(), 0);
+  b->scene_tally.assign(b->scenes.size(), 0);
   for (size_t i = 0; i < b->scenes.size(); ++i) {
+    const int c = hist_class(b->scenes[i]->n_meshes);
     std::vector<int> ind;
-    scene_tiles(b->scenes[i], (int)i, rpt, tiles, ind);
-    for (int m : ind) rows.push_back(make_int2((int)i, m));
+    scene_tiles(b->scenes[i], (int)i, rpt, per_class[c], ind);
+    b->scene_tally[i] = c == 5 || !ind.empty();
+    if (c == 5) {
+      b->scene_global[i] = 1;  // whole tally zeroed and normalised
+    } else {
+      for (int m : ind) rows.push_back(make_int2((int)i, m));
+      class_lds[c] = std::max(class_lds[c], hist_bytes(2, b->scenes[i]->n_meshes));
+    }
   }
-  // Longest tiles first: a greedy order that keeps the persistent blocks
-  // busy to the end of the queue.
-  std::stable_sort(tiles.begin(), tiles.end(),
-                   [](const tile& a, const tile& c) { return (a.g1 - a.g0) > (c.g1 - c.g0); });
-  for (const tile& t : tiles) max_tile = std::max(max_tile, t.g1 - t.g0);
+  // Classes with the largest histograms first; inside a class the longest
+  // tiles first, a greedy order that keeps the persistent blocks busy to the
+  // end of the queue.
+  std::vector<tile> tiles;
+  b->classes.clear();
+  for (int c = 5; c >= 0; --c) {
+    if (per_class[c].empty()) continue;
+    std::stable_sort(per_class[c].begin(), per_class[c].end(),
+                     [](const tile& x, const tile& y) { return (x.g1 - x.g0) > (y.g1 - y.g0); });
+    ffk_batch::klass k;
+    k.hist = c == 5 ? 0 : 2;
+    k.lds = c == 5 ? 0 : class_lds[c];
+    k.first = (int)tiles.size();
+    k.n = (int)per_class[c].size();
+    tiles.insert(tiles.end(), per_class[c].begin(), per_class[c].end());
+    b->classes.push_back(k);
+  }
   b->n_tiles = (int)tiles.size();
   b->n_rows = (int)rows.size();
-  b->max_N = max_N;
-  b->max_tile = max_tile;
   b->total_rays = total;
-  b->hist = pick_hist(max_N, max_tile, -1);
   int r;
   if ((r = dalloc(&b->d_tiles, std::max<size_t>(1, tiles.size()))) < 0) return r;
   if ((r = dalloc(&b->d_rows, std::max<size_t>(1, rows.size()))) < 0) return r;
+  if ((r = dalloc(&b->d_queue, std::max<size_t>(1, b->classes.size()))) < 0) return r;
   if (!tiles.empty())
     FFK_HIP(hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(tile), hipMemcpyHostToDevice));
   if (!rows.empty())
@@ -426,7 +461,9 @@ static int batch_plan(ffk_batch* b) {
   return FFK_OK;
 }
 
-static int default_block(int hist) { return hist == 0 ? 256 : 512; }
+// Threads per block: a histogram above 80 KB leaves room for one block per
+// CU, which then runs 16 wavefronts.
+static int default_block(int hist, size_t lds) { return hist == 0 ? 256 : (lds > 81920 ? 1024 : 512); }
 
 }  // namespace ffk
 
@@ -514,6 +551,19 @@ int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mod
   return FFK_OK;
 }
 
+int ffk_ff_needs_tally(const ffk_scene* s) {
+  if (!s || !s->has_emission) return 0;
+  const long long max_tile = std::min<long long>(s->tiles_rays_per_tile, 65535);
+  const int hist = pick_hist(s->n_meshes, max_tile, s->tune_hist);
+  return (hist == 0 || s->n_indirect_rows > 0) ? 1 : 0;
+}
+
+int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags) {
+  FFK_REQUIRE(b && h_flags, FFK_ERR_ARG, "ffk_batch_tally_flags: bad args");
+  for (size_t i = 0; i < b->scenes.size(); ++i) h_flags[i] = b->scene_tally[i] ? 1 : 0;
+  return FFK_OK;
+}
+
 int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned long long* d_tally, float* d_F,
                  unsigned long long* d_stats, void* stream) {
   FFK_REQUIRE(s && s->has_emission && s->has_grid, FFK_ERR_ARG,
@@ -555,7 +605,7 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
   A.tallies = reinterpret_cast<unsigned long long* const*>(d_misc + 1);
   A.Fs = reinterpret_cast<float* const*>(d_misc + 2);
   A.stats = d_stats;
-  const int block = s->tune_block > 0 ? s->tune_block : default_block(hist);
+  const int block = s->tune_block > 0 ? s->tune_block : default_block(hist, hist_bytes(hist, N));
   if (s->n_tiles > 0)
     FFK_HIP(launch_fused(hist, false, d_stats != nullptr, dim3((unsigned)s->n_tiles), block,
                          hist_bytes(hist, N), st, A));
@@ -581,7 +631,7 @@ int ffk_batch_create(ffk_batch** out, ffk_scene* const* scenes, int n_scenes, vo
   ffk_batch* b = new ffk_batch();
   b->scenes.assign(scenes, scenes + n_scenes);
   int r;
-  if ((r = dalloc(&b->d_descs, (size_t)n_scenes)) < 0 || (r = dalloc(&b->d_queue, 1)) < 0 ||
+  if ((r = dalloc(&b->d_descs, (size_t)n_scenes)) < 0 ||
       (r = dalloc(&b->d_seeds, (size_t)n_scenes)) < 0 || (r = dalloc(&b->d_tallies, (size_t)n_scenes)) < 0 ||
       (r = dalloc(&b->d_Fs, (size_t)n_scenes)) < 0) {
     ffk_batch_destroy(b);
@@ -635,14 +685,12 @@ int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_
   return FFK_OK;
 }
 
-static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, hipStream_t st) {
-  const int block = b->block > 0 ? b->block : default_block(b->hist);
-  const size_t lds = hist_bytes(b->hist, b->max_N);
+static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int n_tiles) {
   int n_blocks = b->n_blocks;
   if (n_blocks <= 0) {
     int per_cu = 0;
     hipError_t e;
-    switch (b->hist) {
+    switch (hist) {
       case 1:
         set_lds_attr<1, true, false>(lds);
         e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ff_fused<1, true, false>, block, lds);
@@ -658,35 +706,44 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     n_blocks = per_cu * device_cu_count();
   }
-  n_blocks = std::max(1, std::min(n_blocks, b->n_tiles));
-  k_fill_i32<<<1, 64, 0, st>>>(b->d_queue, 1, 0);
-  if (b->hist == 0) {
-    for (size_t i = 0; i < b->scenes.size(); ++i) {
-      long long n = (long long)b->scenes[i]->n_meshes * b->scenes[i]->n_meshes;
-      k_fill_u64<<<(int)std::min<long long>((n + 255) / 256, 65536), 256, 0, st>>>(b->g_tallies[i], n);
-    }
-  } else if (b->n_rows > 0) {
+  return std::max(1, std::min(n_blocks, n_tiles));
+}
+
+static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, hipStream_t st) {
+  const int n_classes = (int)b->classes.size();
+  k_fill_i32<<<1, 64, 0, st>>>(b->d_queue, std::max(1, n_classes), 0);
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    if (!b->scene_global[i]) continue;
+    long long n = (long long)b->scenes[i]->n_meshes * b->scenes[i]->n_meshes;
+    k_fill_u64<<<(int)std::min<long long>((n + 255) / 256, 65536), 256, 0, st>>>(b->g_tallies[i], n);
+  }
+  if (b->n_rows > 0) {
     dim3 grid((unsigned)b->n_rows);
     k_zero_rows_batch<<<grid, 256, 0, st>>>(b->d_rows, b->n_rows, b->d_descs, b->d_tallies);
   }
-  fused_args A;
-  A.descs = b->d_descs;
-  A.tiles = b->d_tiles;
-  A.n_tiles = b->n_tiles;
-  A.queue = b->d_queue;
-  A.seeds = b->d_seeds;
-  A.tallies = b->d_tallies;
-  A.Fs = b->d_Fs;
-  A.stats = d_stats;
-  FFK_HIP(launch_fused(b->hist, true, stats, dim3((unsigned)n_blocks), block, lds, st, A));
-  if (b->hist == 0) {
-    for (size_t i = 0; i < b->scenes.size(); ++i) {
-      const ffk_scene* s = b->scenes[i];
-      const int N = s->n_meshes;
-      dim3 grid((unsigned)N);
-      k_normalise_rows<<<grid, 256, 0, st>>>(s->d_mesh_off, N, nullptr, N, b->g_tallies[i], b->g_Fs[i]);
-    }
-  } else if (b->n_rows > 0) {
+  for (int c = 0; c < n_classes; ++c) {
+    const ffk_batch::klass& k = b->classes[c];
+    const int block = b->block > 0 ? b->block : default_block(k.hist, k.lds);
+    fused_args A;
+    A.descs = b->d_descs;
+    A.tiles = b->d_tiles + k.first;
+    A.n_tiles = k.n;
+    A.queue = b->d_queue + c;
+    A.seeds = b->d_seeds;
+    A.tallies = b->d_tallies;
+    A.Fs = b->d_Fs;
+    A.stats = d_stats;
+    const int n_blocks = batch_blocks(b, k.hist, k.lds, block, k.n);
+    FFK_HIP(launch_fused(k.hist, true, stats, dim3((unsigned)n_blocks), block, k.lds, st, A));
+  }
+  for (size_t i = 0; i < b->scenes.size(); ++i) {
+    if (!b->scene_global[i]) continue;
+    const ffk_scene* s = b->scenes[i];
+    const int N = s->n_meshes;
+    dim3 grid((unsigned)N);
+    k_normalise_rows<<<grid, 256, 0, st>>>(s->d_mesh_off, N, nullptr, N, b->g_tallies[i], b->g_Fs[i]);
+  }
+  if (b->n_rows > 0) {
     dim3 grid((unsigned)b->n_rows);
     k_normalise_batch<<<grid, 256, 0, st>>>(b->d_rows, b->n_rows, b->d_descs, b->d_tallies, b->d_Fs);
   }
@@ -698,10 +755,9 @@ int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long l
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream) {
   FFK_REQUIRE(b && seeds && d_Fs, FFK_ERR_ARG, "ffk_batch_run: bad args");
   const size_t n = b->scenes.size();
-  bool need_tally = b->hist == 0 || b->n_rows > 0;
   for (size_t i = 0; i < n; ++i) {
     FFK_REQUIRE(d_Fs[i], FFK_ERR_ARG, "ffk_batch_run: NULL F for scene %zu", i);
-    FFK_REQUIRE(!need_tally || (d_tallies && d_tallies[i]), FFK_ERR_ARG,
+    FFK_REQUIRE(!b->scene_tally[i] || (d_tallies && d_tallies[i]), FFK_ERR_ARG,
                 "ffk_batch_run: scene %zu needs a tally buffer", i);
   }
   hipStream_t st = as_stream(stream);

@@ -49,6 +49,8 @@ SIGNATURES = {
     "ffk_normalise": (C.c_int, [_vp, _vp, _vp, _vp]),
     "ffk_ff_fused": (C.c_int, [_vp, C.c_int, _ull, _vp, _vp, _vp, _vp]),
     "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ffk_ff_needs_tally": (C.c_int, [_vp]),
+    "ffk_batch_tally_flags": (C.c_int, [_vp, _i]),
     "ffk_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp]),
     "ffk_batch_destroy": (None, [_vp]),
     "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
@@ -315,7 +317,9 @@ class Scene:
         N = self.n_meshes
         if F is None:
             F = torch.empty((N, N), dtype=torch.float32, device="cuda")
-        if tally is None and (need_tally if need_tally is not None else True):
+        if need_tally is None:
+            need_tally = bool(lib().ffk_ff_needs_tally(self.h))
+        if tally is None and need_tally:
             tally = torch.empty((N, N), dtype=torch.int64, device="cuda")
         check(lib().ffk_ff_fused(self.h, CAST_GRID, seed, _ptr(tally), _ptr(F), _ptr(stats), _stream()),
               "ffk_ff_fused")
@@ -356,6 +360,19 @@ class Batch:
 
     def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0):
         check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks), "ffk_batch_set_tuning")
+
+    def tally_flags(self):
+        """Scenes that need a uint64 tally buffer in run()."""
+        flags = np.zeros(len(self.scenes), np.int32)
+        check(lib().ffk_batch_tally_flags(self.h, flags.ctypes.data_as(_i)), "ffk_batch_tally_flags")
+        return flags.astype(bool)
+
+    def make_tallies(self):
+        """Tally buffers for the scenes that need one (None elsewhere)."""
+        import torch
+
+        return [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda") if need else None
+                for sc, need in zip(self.scenes, self.tally_flags())]
 
     def run(self, seeds, Fs, tallies=None, use_graph=True, stats=None):
         n = len(self.scenes)

@@ -138,6 +138,10 @@ int ffk_normalise(const ffk_scene* s, const unsigned long long* d_tally, float* 
 int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned long long* d_tally,
                  float* d_F, unsigned long long* d_stats, void* stream);
 
+/* 1 when ffk_ff_fused needs d_tally for this scene and tuning (the rays of a
+ * mesh span several tiles, or the row histogram does not fit in LDS), else 0. */
+int ffk_ff_needs_tally(const ffk_scene* s);
+
 /* Launch tuning of the fused kernel (0 = automatic; hist_mode -1 = automatic):
  * threads per block, rays per tile (<= 65535), row histogram (0 global
  * atomics, 1 uint32 LDS, 2 uint16 LDS) and the number of waiting lanes
@@ -156,6 +160,8 @@ void ffk_batch_destroy(ffk_batch* b);
 int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream);
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
+/* h_flags[i] = 1 when scene i needs a tally buffer in ffk_batch_run, else 0. */
+int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags);
 
 /* ---- radiance equation --------------------------------------------------------
  * radiance_equation::calculate (thermal_equation/radiance_cpu.cpp:132-190) on
