@@ -164,14 +164,23 @@ __device__ inline double conduction_flow(int m, const int* __restrict__ row_ptr,
   return flow;
 }
 
+// The reference splits each face pair's flow (thermal_equation/conductive_cpu.cpp:123-132):
+// the mesh it flows into absorbs it, the other emits it. Per neighbour k of
+// mesh m: f = G_mk (T_k - T_m) > 0 is absorbed by m, else -f is emitted.
 __global__ void k_conduction(int n, const int* __restrict__ row_ptr, const int* __restrict__ cols,
                              const double* __restrict__ G, const double* __restrict__ T,
                              double* __restrict__ emission, double* __restrict__ absorption) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= n) return;
-  const double flow = conduction_flow(m, row_ptr, cols, G, T);
-  if (flow > 0) absorption[m] += flow;
-  else emission[m] -= flow;
+  const double tm = T[m];
+  double ab = 0.0, em = 0.0;
+  for (int k = row_ptr[m]; k < row_ptr[m + 1]; ++k) {
+    const double f = G[k] * (T[cols[k]] - tm);
+    if (f > 0) ab += f;
+    else em -= f;
+  }
+  absorption[m] += ab;
+  emission[m] += em;
 }
 
 // thermal_solution/cpu_adams.cpp:136-142 coefficients (AB1..AB5).

@@ -404,7 +404,7 @@ int calculate(system* sys, emission::task_t* base) {
   const long long n = task->n_rays;
   if (n <= 0) return EMISSION_OK;
   if (!s->rays.reserve(6 * n)) return -EMISSION_ERROR;
-  const unsigned long long key = s->seed + 0x9E3779B97F4A7C15ull * (s->calls++);
+  const unsigned long long key = s->seed + 0x9E3779B97F4A7C15ull * (s->calls++);  // as next_key
   if (ffk_emit_parallel(&task->direction.x, &task->center.x, task->distance, task->radius, n, key, s->rays.p,
                         nullptr) < 0)
     return -EMISSION_ERROR;
@@ -427,6 +427,10 @@ emission::system_t* system_create() {
   s->st = nullptr;
   return s;
 }
+
+bool is_hip(const emission::system_t* s) { return s && s->methods == &methods; }
+state* state_of(emission::system_t* s) { return ((system*)s)->st; }
+unsigned long long next_key(state* s) { return s->seed + 0x9E3779B97F4A7C15ull * (s->calls++); }
 
 }  // namespace hip_parallel_rays_emission
 
@@ -870,6 +874,313 @@ thermal_equation::system_t* system_create() {
 
 }  // namespace hip_radiance
 
+// ================================================ conduction (device) ==
+namespace hip_conduction {
+
+struct state {
+  subject::scene_t* scene = nullptr;
+  hipStream_t st = nullptr;
+  int n = 0;
+  dbuf<int> row, col;
+  dbuf<double> G, vec;  // vec: T, emission, absorption
+  std::vector<double> T, em, ab;
+};
+struct system : thermal_equation::system_t {
+  state* st;
+};
+
+int init(system* sys, void*) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -THERMAL_EQUATION_ERROR;
+  sys->st->st = make_stream();
+  return sys->st->st ? THERMAL_EQUATION_OK : -THERMAL_EQUATION_ERROR;
+}
+int shutdown(system* sys) {
+  state* s = sys->st;
+  if (!s) return THERMAL_EQUATION_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  s->row.release();
+  s->col.release();
+  s->G.release();
+  s->vec.release();
+  drop_stream(s->st);
+  delete s;
+  sys->st = nullptr;
+  return THERMAL_EQUATION_OK;
+}
+
+// thermal_equation/conductive_cpu.cpp:50-64: the face graph, here reduced
+// once to the mesh conductance operator (shell conductivity and thickness of
+// each mesh's material).
+int set_scene(system* sys, subject::scene_t* scene) {
+  state* s = sys->st;
+  s->scene = scene;
+  const int nm = scene->n_meshes;
+  s->n = nm;
+  if (nm == 0) return THERMAL_EQUATION_OK;
+  std::vector<float> k(nm), t(nm);
+  std::vector<int> meshes(3 * (size_t)nm);
+  for (int m = 0; m < nm; ++m) {
+    const subject::material_t& mat = scene->materials[scene->meshes[m].material_idx];
+    k[m] = mat.shell.thermal_conductivity;
+    t[m] = mat.shell.thickness;
+    meshes[3 * m] = scene->meshes[m].first_idx;
+    meshes[3 * m + 1] = scene->meshes[m].n_faces;
+    meshes[3 * m + 2] = 0;
+  }
+  int nnz = 0;
+  const float* faces = (const float*)scene->faces;
+  if (ffk_conduction_graph(faces, scene->n_faces, meshes.data(), nm, k.data(), t.data(), nullptr, nullptr, nullptr,
+                           0, &nnz) < 0)
+    return -THERMAL_EQUATION_ERROR;
+  std::vector<int> row(nm + 1), col(std::max(1, nnz));
+  std::vector<double> G(std::max(1, nnz));
+  if (ffk_conduction_graph(faces, scene->n_faces, meshes.data(), nm, k.data(), t.data(), row.data(), col.data(),
+                           G.data(), (int)G.size(), &nnz) < 0)
+    return -THERMAL_EQUATION_ERROR;
+  if (!s->row.reserve(nm + 1) || !s->col.reserve(col.size()) || !s->G.reserve(G.size()) ||
+      !s->vec.reserve(3 * (size_t)nm))
+    return -THERMAL_EQUATION_ERROR;
+  if (hipMemcpyAsync(s->row.p, row.data(), 4 * row.size(), hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(s->col.p, col.data(), 4 * col.size(), hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(s->G.p, G.data(), 8 * G.size(), hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipStreamSynchronize(s->st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  return THERMAL_EQUATION_OK;
+}
+
+// thermal_equation/conductive_cpu.cpp:83-141; adds to the task's arrays.
+int calculate(system* sys, thermal_equation::task_t* task) {
+  state* s = sys->st;
+  const int n = s->n;
+  if (n == 0) return THERMAL_EQUATION_OK;
+  s->T.assign(task->temperatures, task->temperatures + n);
+  s->em.assign(n, 0.0);
+  s->ab.assign(n, 0.0);
+  double* d = s->vec.p;
+  if (hipMemcpyAsync(d, s->T.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(d + n, s->em.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s->st) != hipSuccess ||
+      hipMemcpyAsync(d + 2 * n, s->ab.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s->st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  if (ffk_conduction(n, s->row.p, s->col.p, s->G.p, d, d + n, d + 2 * n, s->st) < 0) return -THERMAL_EQUATION_ERROR;
+  if (hipMemcpyAsync(s->em.data(), d + n, 8 * (size_t)n, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+      hipMemcpyAsync(s->ab.data(), d + 2 * n, 8 * (size_t)n, hipMemcpyDeviceToHost, s->st) != hipSuccess ||
+      hipStreamSynchronize(s->st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  for (int m = 0; m < n; ++m) {
+    task->emission[m] += (float)s->em[m];
+    task->absorption[m] += (float)s->ab[m];
+  }
+  return THERMAL_EQUATION_OK;
+}
+
+const thermal_equation::system_methods_t methods = {
+    (int (*)(thermal_equation::system_t*, void*))&init,
+    (int (*)(thermal_equation::system_t*))&shutdown,
+    (int (*)(thermal_equation::system_t*, subject::scene_t*))&set_scene,
+    (int (*)(thermal_equation::system_t*, thermal_equation::task_t*))&calculate,
+};
+thermal_equation::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  return s;
+}
+
+}  // namespace hip_conduction
+
+// ======================================================== heat sources ==
+// thermal_equation/heat_source_cpu.cpp:48-61: a handful of additions per
+// step, kept on the host (the device-resident run applies the same sources
+// inside its step kernel).
+namespace hip_heat_source {
+
+struct system : thermal_equation::system_t {
+  heat_source_equation::params_t* params;
+};
+
+int init(system* sys, heat_source_equation::params_t* params) {
+  sys->params = new (std::nothrow) heat_source_equation::params_t(*params);
+  return sys->params ? THERMAL_EQUATION_OK : -THERMAL_EQUATION_ERROR;
+}
+int shutdown(system* sys) {
+  delete sys->params;
+  sys->params = nullptr;
+  return THERMAL_EQUATION_OK;
+}
+int set_scene(system*, subject::scene_t*) { return THERMAL_EQUATION_OK; }
+int calculate(system* sys, thermal_equation::task_t* task) {
+  const heat_source_equation::params_t& p = *sys->params;
+  for (int i = 0; i != p.n_sources; ++i) {
+    const heat_source_equation::heat_source_t& src = p.sources[i];
+    if (src.power > 0) task->absorption[src.mesh_idx] += src.power;
+    else task->emission[src.mesh_idx] += -src.power;
+  }
+  return THERMAL_EQUATION_OK;
+}
+
+const thermal_equation::system_methods_t methods = {
+    (int (*)(thermal_equation::system_t*, void*))&init,
+    (int (*)(thermal_equation::system_t*))&shutdown,
+    (int (*)(thermal_equation::system_t*, subject::scene_t*))&set_scene,
+    (int (*)(thermal_equation::system_t*, thermal_equation::task_t*))&calculate,
+};
+thermal_equation::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->params = nullptr;
+  return s;
+}
+
+}  // namespace hip_heat_source
+
+// ============================================ distant source (parallel rays) ==
+namespace hip_distant_source {
+
+struct state {
+  parallel_rays_cpu::params_t params = {0, nullptr, nullptr, nullptr};
+  subject::scene_t* scene = nullptr;
+  ray_caster::scene_t emission_scene = {0, 0};
+  ffk_scene* s = nullptr;
+  hipStream_t st = nullptr;
+  bool grid = false;
+  float center[3] = {0, 0, 0}, bs_radius = 0;
+  std::vector<int> face2mesh;
+  std::vector<float> absorb2;
+  std::vector<unsigned long long> counts;
+  std::vector<double> absorption;
+};
+struct system : thermal_equation::system_t {
+  state* st;
+};
+
+int init(system* sys, parallel_rays_cpu::params_t* params) {
+  sys->st = new (std::nothrow) state();
+  if (!sys->st) return -THERMAL_EQUATION_ERROR;
+  sys->st->params = *params;
+  sys->st->st = make_stream();
+  return sys->st->st ? THERMAL_EQUATION_OK : -THERMAL_EQUATION_ERROR;
+}
+int shutdown(system* sys) {
+  state* s = sys->st;
+  if (!s) return THERMAL_EQUATION_OK;
+  if (s->st) (void)hipStreamSynchronize(s->st);
+  ffk_scene_destroy(s->s);
+  drop_stream(s->st);
+  delete s;
+  sys->st = nullptr;
+  return THERMAL_EQUATION_OK;
+}
+
+// thermal_equation/parallel_rays_cpu.cpp:68-83
+int set_scene(system* sys, subject::scene_t* scene) {
+  state* s = sys->st;
+  s->scene = scene;
+  s->emission_scene.n_faces = scene->n_faces;
+  s->emission_scene.faces = scene->faces;
+  int r;
+  emission::system_t* em = s->params.emitter;
+  if ((r = em->methods->set_scene(em, &s->emission_scene)) < 0) return r;
+  const int nf = scene->n_faces, nm = scene->n_meshes;
+  s->face2mesh.assign(nf, -1);
+  s->absorb2.assign(2 * (size_t)nm, 0.f);
+  for (int m = 0; m < nm; ++m) {
+    const subject::mesh_t& mesh = scene->meshes[m];
+    if (mesh.first_idx < 0 || mesh.n_faces < 0 || mesh.first_idx + mesh.n_faces > nf) return -THERMAL_EQUATION_ERROR;
+    for (int f = 0; f < mesh.n_faces; ++f) s->face2mesh[mesh.first_idx + f] = m;
+    const subject::material_t& mat = scene->materials[mesh.material_idx];
+    s->absorb2[2 * m] = mat.front.absorbance;
+    s->absorb2[2 * m + 1] = mat.rear.absorbance;
+  }
+  if (nf > 0 && ffk_bsphere((const float*)scene->faces, nf, s->center, &s->bs_radius) < 0)
+    return -THERMAL_EQUATION_ERROR;
+  s->grid = false;
+  if (nf == 0 || nm == 0) return THERMAL_EQUATION_OK;
+  if (!s->s && ffk_scene_create(&s->s) < 0) return -THERMAL_EQUATION_ERROR;
+  if (ffk_scene_set_faces(s->s, (const float*)scene->faces, nf, s->st) < 0) return -THERMAL_EQUATION_ERROR;
+  if (ffk_scene_set_meshes(s->s, (const int*)scene->meshes, nm, s->st) < 0) return -THERMAL_EQUATION_ERROR;
+  return THERMAL_EQUATION_OK;
+}
+
+// thermal_equation/parallel_rays_cpu.cpp:92-131; adds to task->absorption.
+int calculate(system* sys, thermal_equation::task_t* task) {
+  state* s = sys->st;
+  if (s->params.source == 0) return THERMAL_EQUATION_OK;
+  const subject::scene_t* scene = s->scene;
+  if (!scene || scene->n_faces == 0 || scene->n_meshes == 0) return THERMAL_EQUATION_OK;
+  const parallel_rays_cpu::source_t source = s->params.source(s->params.source_param);
+  const float radius = s->bs_radius * 1.1f;
+  const float distance = 1.1f + radius;
+  const int n_rays = s->params.n_rays;
+  const float ray_power = source.power * (float(M_PI) * radius * radius) / n_rays;
+  const int nm = scene->n_meshes;
+  s->absorption.assign(nm, 0.0);
+  emission::system_t* emitter = s->params.emitter;
+  hip_parallel_rays_emission::state* em =
+      hip_parallel_rays_emission::is_hip(emitter) ? hip_parallel_rays_emission::state_of(emitter) : nullptr;
+  int r;
+  if (em && hip_ray_caster::is_hip(em->caster) && hip_ray_caster::method_of(em->caster) == FFK_CAST_GRID) {
+    // Fused on the device: emission, grid cast and per-side hit counts.
+    if (!s->grid) {
+      if (ffk_scene_build_grid(s->s, 0, 0, 0, s->st) < 0) return -THERMAL_EQUATION_ERROR;
+      s->grid = true;
+    }
+    s->counts.assign(2 * (size_t)nm, 0ull);
+    const unsigned long long key = hip_parallel_rays_emission::next_key(em);
+    if (ffk_sun(s->s, &source.direction.x, s->center, distance, radius, 0, n_rays, key, s->counts.data(), nullptr,
+                s->st) < 0)
+      return -THERMAL_EQUATION_ERROR;
+    if (ffk_sun_deposit(nm, s->counts.data(), s->absorb2.data(), ray_power, s->absorption.data()) < 0)
+      return -THERMAL_EQUATION_ERROR;
+  } else {
+    parallel_rays_emission::task_t et(n_rays);
+    et.rays = nullptr;
+    et.center = math::vec3{s->center[0], s->center[1], s->center[2]};
+    et.distance = distance;
+    et.direction = source.direction;
+    et.radius = radius;
+    if ((r = emitter->methods->calculate(emitter, &et)) < 0) {
+      rc_task_free(et.rays);
+      return r;
+    }
+    const ray_caster::task_t* rt = et.rays;
+    for (int i = 0; rt && i != rt->n_tasks; ++i) {
+      const math::triangle_t* h = rt->hit_face[i];
+      if (!h) continue;
+      const int m = s->face2mesh[h - scene->faces];
+      if (m < 0) continue;
+      const math::vec3 u = {h->points[1].x - h->points[0].x, h->points[1].y - h->points[0].y,
+                            h->points[1].z - h->points[0].z};
+      const math::vec3 v = {h->points[2].x - h->points[0].x, h->points[2].y - h->points[0].y,
+                            h->points[2].z - h->points[0].z};
+      const float nx = u.y * v.z - u.z * v.y, ny = u.z * v.x - u.x * v.z, nz = u.x * v.y - u.y * v.x;
+      const float side = nx * source.direction.x + ny * source.direction.y + nz * source.direction.z;
+      s->absorption[m] += (double)((side < 0 ? s->absorb2[2 * m] : s->absorb2[2 * m + 1]) * ray_power);
+    }
+    rc_task_free(et.rays);
+  }
+  for (int m = 0; m < nm; ++m) task->absorption[m] += (float)s->absorption[m];
+  return THERMAL_EQUATION_OK;
+}
+
+const thermal_equation::system_methods_t methods = {
+    (int (*)(thermal_equation::system_t*, void*))&init,
+    (int (*)(thermal_equation::system_t*))&shutdown,
+    (int (*)(thermal_equation::system_t*, subject::scene_t*))&set_scene,
+    (int (*)(thermal_equation::system_t*, thermal_equation::task_t*))&calculate,
+};
+thermal_equation::system_t* system_create() {
+  system* s = (system*)malloc(sizeof(system));
+  if (!s) return nullptr;
+  s->methods = &methods;
+  s->st = nullptr;
+  return s;
+}
+
+}  // namespace hip_distant_source
+
 // ==================================================== Adams solver (FP64) ==
 namespace hip_adams {
 
@@ -1051,6 +1362,12 @@ int ffp_emission_calculate_parallel(void* s, int n_rays, const float* dir3, cons
 }
 int ffp_emission_set_seed(void* s, unsigned long long seed, int frame) {
   emission::system_t* e = (emission::system_t*)s;
+  if (hip_parallel_rays_emission::is_hip(e)) {  // no emission frame
+    hip_parallel_rays_emission::state* st = hip_parallel_rays_emission::state_of(e);
+    st->seed = seed;
+    st->calls = 0;
+    return EMISSION_OK;
+  }
   if (!hip_malley_emission::is_hip(e)) return -EMISSION_ERROR;
   if (frame != FFK_FRAME_REFERENCE && frame != FFK_FRAME_ORTHONORMAL) return -EMISSION_ERROR;
   hip_malley_emission::state* st = hip_malley_emission::state_of(e);
@@ -1111,6 +1428,14 @@ void* ffp_thermal_equation_create(int type, void* dep, int n_rays, int f_dtype) 
       free(s);
       return nullptr;
     }
+  } else if (type == THERMAL_EQUATION_CONDUCTIVE_HIP) {
+    s = hip_conduction::system_create();
+    if (!s) return nullptr;
+    if (s->methods->init(s, nullptr) < 0) {
+      s->methods->shutdown(s);
+      free(s);
+      return nullptr;
+    }
   } else if (type == THERMAL_EQUATION_RADIANCE_HIP) {
     if (!dep) return nullptr;
     s = hip_radiance::system_create();
@@ -1124,6 +1449,36 @@ void* ffp_thermal_equation_create(int type, void* dep, int n_rays, int f_dtype) 
   }
   return s;
 }
+void* ffp_heat_source_create(int n_sources, ffp_heat_source* sources) {
+  if (n_sources < 0 || (n_sources > 0 && !sources)) return nullptr;
+  thermal_equation::system_t* s = hip_heat_source::system_create();
+  if (!s) return nullptr;
+  heat_source_equation::params_t p = {n_sources, (heat_source_equation::heat_source_t*)sources};
+  if (s->methods->init(s, &p) < 0) {
+    s->methods->shutdown(s);
+    free(s);
+    return nullptr;
+  }
+  return s;
+}
+
+namespace {
+parallel_rays_cpu::source_t read_source(void* p) { return *(parallel_rays_cpu::source_t*)p; }
+}  // namespace
+
+void* ffp_distant_source_create(void* emitter, int n_rays, ffp_source* source) {
+  if (!emitter || n_rays <= 0 || !source) return nullptr;
+  thermal_equation::system_t* s = hip_distant_source::system_create();
+  if (!s) return nullptr;
+  parallel_rays_cpu::params_t p = {n_rays, (emission::system_t*)emitter, source, &read_source};
+  if (s->methods->init(s, &p) < 0) {
+    s->methods->shutdown(s);
+    free(s);
+    return nullptr;
+  }
+  return s;
+}
+
 int ffp_thermal_equation_set_scene(void* s, ffp_subject_scene* scene) {
   thermal_equation::system_t* e = (thermal_equation::system_t*)s;
   return e->methods->set_scene(e, (subject::scene_t*)scene);

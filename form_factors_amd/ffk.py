@@ -58,6 +58,9 @@ SIGNATURES = {
     "ffk_radiance": (C.c_int, [_vp, _ull, _f, C.POINTER(C.c_double), _vp, _vp]),
     "ffk_radiance_weights": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _f, _f]),
     "ffk_radiance_powers": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, _f, _f]),
+    "ffk_bsphere": (C.c_int, [_f, C.c_int, _f, _f]),
+    "ffk_sun": (C.c_int, [_vp, _f, _f, C.c_float, C.c_float, C.c_int, _ll, _ull, C.POINTER(_ull), _vp, _vp]),
+    "ffk_sun_deposit": (C.c_int, [C.c_int, C.POINTER(_ull), _f, C.c_float, C.POINTER(C.c_double)]),
     "ffk_convert_f": (C.c_int, [_vp, _ll, C.c_int, _vp, _vp]),
     "ffk_pack_exchange": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "ffk_sbff": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -148,6 +151,15 @@ def emitted_counts(weights, n_rays, total):
     return fr, rr
 
 
+def bsphere(faces):
+    """math::triangles_bsphere (host): (centre float32[3], radius float32)."""
+    faces = _np(faces, np.float32).reshape(-1, 9)
+    c = np.zeros(3, np.float32)
+    r = C.c_float()
+    check(lib().ffk_bsphere(faces.ctypes.data_as(_f), len(faces), c.ctypes.data_as(_f), C.byref(r)), "ffk_bsphere")
+    return c, np.float32(r.value)
+
+
 def radiance_weights(faces, meshes, eps2, T):
     """Reference radiance weights (2 per face) and float total (host)."""
     faces = _np(faces, np.float32).reshape(-1, 9)
@@ -228,6 +240,29 @@ class Scene:
         self.n_rays = n.value
         self.frame = frame
         return self.n_rays
+
+    def sun_counts(self, direction, center, distance, radius, n_rays, seed, aperture=0, stats=None):
+        """Hits per mesh side of n_rays parallel rays: uint64[n_meshes, 2] (front, rear)."""
+        counts = np.zeros(2 * self.n_meshes, np.uint64)
+        check(lib().ffk_sun(self.h, _np(direction, np.float32).ctypes.data_as(_f),
+                            _np(center, np.float32).ctypes.data_as(_f), float(distance), float(radius), int(aperture),
+                            int(n_rays), seed, counts.ctypes.data_as(C.POINTER(_ull)), _ptr(stats), _stream()),
+              "ffk_sun")
+        return counts.reshape(-1, 2)
+
+    def sun(self, absorb2, power, direction, n_rays, seed, aperture=0, stats=None):
+        """One parallel_rays_cpu::calculate with the reference's aperture
+        placement: absorbed power per mesh (float64, exact sum)."""
+        c, r0 = bsphere(self.faces)
+        radius = np.float32(r0 * np.float32(1.1))
+        distance = np.float32(np.float32(1.1) + radius)
+        counts = self.sun_counts(direction, c, distance, radius, n_rays, seed, aperture, stats)
+        ray_power = np.float32(np.float32(power) * (np.float32(np.pi) * radius * radius) / np.float32(n_rays))
+        out = np.zeros(self.n_meshes, np.float64)
+        check(lib().ffk_sun_deposit(self.n_meshes, np.ascontiguousarray(counts.reshape(-1)).ctypes.data_as(
+            C.POINTER(_ull)), _np(absorb2, np.float32).ctypes.data_as(_f), float(ray_power),
+            out.ctypes.data_as(C.POINTER(C.c_double))), "ffk_sun_deposit")
+        return out
 
     def radiance_step(self, eps2, T, n_rays, seed, frame=FRAME_REFERENCE, stats=None):
         """One radiance_equation::calculate: layout from the radiance weights,

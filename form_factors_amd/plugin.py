@@ -15,6 +15,7 @@ RAY_CASTER_NAIVE_HIP, RAY_CASTER_GRID_HIP = 2, 4
 EMISSION_MALLEY_HIP, EMISSION_PARALLEL_RAYS_HIP = 3, 4
 FORM_FACTORS_HIP = 2
 THERMAL_EQUATION_SB_FF_HIP, THERMAL_EQUATION_RADIANCE_HIP = 6, 7
+THERMAL_EQUATION_CONDUCTIVE_HIP, THERMAL_EQUATION_HEAT_SOURCE_HIP, THERMAL_EQUATION_DISTANT_SOURCE_HIP = 8, 9, 10
 THERMAL_SOLUTION_HIP_ADAMS = 2
 
 RAY_CASTER_ERROR = 1
@@ -76,6 +77,14 @@ class TeTask(C.Structure):
                 ("absorption", C.POINTER(C.c_float))]
 
 
+class HeatSource(C.Structure):
+    _fields_ = [("mesh_idx", C.c_int), ("power", C.c_float)]
+
+
+class Source(C.Structure):
+    _fields_ = [("power", C.c_float), ("direction", Vec3)]
+
+
 class TsTask(C.Structure):
     _fields_ = [("n_step", C.c_int), ("time_delta", C.c_float), ("temperatures", C.POINTER(C.c_float))]
 
@@ -118,6 +127,8 @@ SIGNATURES = {
     "ffp_form_factors_calculate": (C.c_int, [_P, C.POINTER(FfTask)]),
     "ffp_form_factors_free": (None, [_P]),
     "ffp_thermal_equation_create": (_P, [C.c_int, _P, C.c_int, C.c_int]),
+    "ffp_heat_source_create": (_P, [C.c_int, _P]),
+    "ffp_distant_source_create": (_P, [_P, C.c_int, _P]),
     "ffp_thermal_equation_set_scene": (C.c_int, [_P, C.POINTER(SubjectScene)]),
     "ffp_thermal_equation_calculate": (C.c_int, [_P, C.POINTER(TeTask)]),
     "ffp_thermal_equation_free": (None, [_P]),

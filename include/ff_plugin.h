@@ -74,7 +74,8 @@ int ffp_emission_calculate_parallel(void* system, int n_rays, const float* direc
                                     ffp_rc_task** rays_out);
 /* Counter-RNG key and emission frame of a HIP emitter (default 0 and the
  * reference's unnormalised frame). Call k of calculate() uses the key
- * seed + k * 0x9E3779B97F4A7C15. type-3 emitters only; others: -EMISSION_ERROR. */
+ * seed + k * 0x9E3779B97F4A7C15. Types 3 and 4 (the frame is ignored for
+ * type 4); others: -EMISSION_ERROR. */
 int ffp_emission_set_seed(void* system, unsigned long long seed, int frame);
 void ffp_emission_free(void* system);
 void ffp_task_free(ffp_rc_task* task);
@@ -90,8 +91,25 @@ void ffp_form_factors_free(void* system);
 
 /* ---- thermal equations (thermal_equation/system.h:56-92).
  * type 6 = SB_FF HIP: params = {form_factors system, n_rays, f_dtype}
- * type 7 = radiance HIP: params = {emitter, n_rays} */
+ * type 7 = radiance HIP: params = {emitter, n_rays}
+ * type 8 = conductive HIP: no parameters (dep, n_rays, f_dtype unused) */
 void* ffp_thermal_equation_create(int type, void* form_factors_or_emitter, int n_rays, int f_dtype);
+/* type 9 = heat sources (thermal_equation/heat_source_cpu.h:28-41); the
+ * caller keeps `sources` alive, as with the reference's params. */
+typedef struct ffp_heat_source {
+  int mesh_idx;
+  float power; /* negative: a heat sink */
+} ffp_heat_source;
+void* ffp_heat_source_create(int n_sources, ffp_heat_source* sources);
+/* type 10 = distant source (thermal_equation/parallel_rays_cpu.h:26-39) over
+ * a parallel-rays emitter (fused on the device for the HIP emitter over the
+ * grid caster); the caller keeps `source` alive and may change it between
+ * calls (the reference reads it through its source callback each step). */
+typedef struct ffp_source {
+  float power; /* W/m^2 */
+  ffp_vec3 direction;
+} ffp_source;
+void* ffp_distant_source_create(void* emitter, int n_rays, ffp_source* source);
 int ffp_thermal_equation_set_scene(void* system, ffp_subject_scene* scene);
 int ffp_thermal_equation_calculate(void* system, ffp_te_task* task);
 void ffp_thermal_equation_free(void* system);

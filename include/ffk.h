@@ -192,6 +192,26 @@ int ffk_radiance_powers(const float* h_faces, int n_faces, const int* h_meshes, 
                         const float* h_eps2, const float* h_T, const int* h_front, const int* h_rear,
                         float* h_ray_power, float* h_emission);
 
+/* ---- distant source (sun) absorption ------------------------------------------
+ * parallel_rays_cpu::calculate (thermal_equation/parallel_rays_cpu.cpp:92-131).
+ * ffk_bsphere is math::triangles_bsphere (math/triangle.cpp:361-383, whose
+ * radius loop skips triangle 0). The caller places the aperture as the
+ * reference does: radius = 1.1 * bsphere radius, distance = 1.1 + radius,
+ * centre = bsphere centre. ffk_sun casts n_rays parallel rays (aperture 0 =
+ * disk of `radius`, the reference; 1 = square of half side `radius`; Philox
+ * counter (i, 0, 1, i >> 32) keyed by seed, as ffk_emit_parallel) on the
+ * scene's grid and counts hits per mesh side: h_counts[2m] front
+ * (dot(normal, dir) < 0), h_counts[2m+1] rear. ffk_sun_deposit adds
+ * counts * (absorbance * ray_power) to h_absorption (double), the exact sum
+ * of the reference's per-ray float values; ray_power = power * (pi r^2) /
+ * n_rays in float. */
+int ffk_bsphere(const float* h_faces, int n_faces, float* center3, float* radius);
+int ffk_sun(ffk_scene* s, const float* h_dir3, const float* h_center3, float distance, float radius,
+            int aperture, long long n_rays, unsigned long long seed, unsigned long long* h_counts,
+            unsigned long long* d_stats, void* stream);
+int ffk_sun_deposit(int n_meshes, const unsigned long long* h_counts, const float* h_absorb2, float ray_power,
+                    double* h_absorption);
+
 /* ---- thermal ---------------------------------------------------------------- */
 /* Elementwise conversion of F (float32) to the storage type
  * (FFK_BF16/FFK_F16/FFK_F32), round to nearest even. */
@@ -218,8 +238,9 @@ int ffk_sbff(int n, const void* d_S, int dtype, const double* d_row_scale, const
              const double* d_area, const double* d_eps_front, const double* d_eps_rear,
              double* d_emission, double* d_absorption, void* stream);
 /* Conduction (thermal_equation/conductive_cpu.cpp:83-134) as a CSR operator
- * over mesh adjacency: flow_m = sum_k G_mk (T_k - T_m); positive flow is
- * absorbed, negative is emitted. */
+ * over mesh adjacency. For each neighbour k of mesh m, f = G_mk (T_k - T_m):
+ * f > 0 is added to absorption[m], else -f to emission[m] (the reference's
+ * per-pair split). */
 int ffk_conduction(int n, const int* d_row_ptr, const int* d_cols, const double* d_G,
                    const double* d_T, double* d_emission, double* d_absorption, void* stream);
 /* Adams-Bashforth step of thermal_solution/cpu_adams.cpp:107-182 in FP64:

@@ -142,6 +142,18 @@ struct system_methods_t {                                                 // :56
 #define THERMAL_EQUATION_DISNTANT_SOURCE_CPU 5
 #define THERMAL_EQUATION_SB_FF_HIP 6
 #define THERMAL_EQUATION_RADIANCE_HIP 7
+#define THERMAL_EQUATION_CONDUCTIVE_HIP 8
+#define THERMAL_EQUATION_HEAT_SOURCE_HIP 9
+#define THERMAL_EQUATION_DISTANT_SOURCE_HIP 10
+
+namespace heat_source_equation {                                          // thermal_equation/heat_source_cpu.h:28-41
+struct heat_source_t { int mesh_idx; float power; };
+struct params_t { int n_sources; heat_source_t* sources; };
+}
+namespace parallel_rays_cpu {                                             // thermal_equation/parallel_rays_cpu.h:26-39
+struct source_t { float power; math::vec3 direction; };
+struct params_t { int n_rays; emission::system_t* emitter; void* source_param; source_t (*source)(void* param); };
+}
 
 namespace form_factors_equation {
 struct params_t { form_factors::system_t* form_factors_calculator; int n_rays; };  // thermal_equation/form_factors.h
@@ -182,4 +194,7 @@ namespace hip_parallel_rays_emission { emission::system_t* system_create(); }
 namespace hip_form_factors { form_factors::system_t* system_create(); }
 namespace hip_sb_ff { thermal_equation::system_t* system_create(); }
 namespace hip_radiance { thermal_equation::system_t* system_create(); }
+namespace hip_conduction { thermal_equation::system_t* system_create(); }     // params: none
+namespace hip_heat_source { thermal_equation::system_t* system_create(); }    // heat_source_equation::params_t
+namespace hip_distant_source { thermal_equation::system_t* system_create(); } // parallel_rays_cpu::params_t
 namespace hip_adams { thermal_solution::system_t* system_create(); }

@@ -598,8 +598,19 @@ long or_malley_emit(const float* faces, int n_faces, const float* weights, float
 
 // emission/parallel_rays_cpu.cpp:88-106 (disk aperture of the given radius
 // at -distance along dir around center). PHILOX mode: counter (r, 0, 1, 0).
+void or_parallel_emit2(const float* dir3, const float* center3, float distance, float radius,
+                       int aperture, int n_rays, int rng, uint64_t seed, float* out);
+
 void or_parallel_emit(const float* dir3, const float* center3, float distance, float radius,
                       int n_rays, int rng, uint64_t seed, float* out) {
+  or_parallel_emit2(dir3, center3, distance, radius, 0, n_rays, rng, seed, out);
+}
+
+// aperture 0: disk of the given radius (emission/parallel_rays_cpu.cpp:88-106);
+// aperture 1: square of half side `radius` (Philox only), x = radius * (2u - 1),
+// y = radius * (2v - 1) in the frame rotated towards dir.
+void or_parallel_emit2(const float* dir3, const float* center3, float distance, float radius,
+                       int aperture, int n_rays, int rng, uint64_t seed, float* out) {
   v3 dir = mk(dir3[0], dir3[1], dir3[2]);
   v3 center = mk(center3[0], center3[1], center3[2]);
   m33 R = rotate_towards(mk(0, 0, 1), dir);
@@ -617,8 +628,14 @@ void or_parallel_emit(const float* dir3, const float* center3, float distance, f
     } else {
       uint32_t ctr[4] = {(uint32_t)r, 0u, 1u, 0u}, o[4];
       philox(ctr, key, o);
-      rad = radius * std::sqrt(u01(o[0]));
-      sincos_turns(u01(o[1]), &s, &c);
+      if (aperture == 1) {
+        rad = 1.f;
+        c = radius * (2.f * u01(o[0]) - 1.f);
+        s = radius * (2.f * u01(o[1]) - 1.f);
+      } else {
+        rad = radius * std::sqrt(u01(o[0]));
+        sincos_turns(u01(o[1]), &s, &c);
+      }
     }
     float x = rad * c, y = rad * s;
     v3 origin = add(mat_vec(R, mk(x, y, -distance)), center);
@@ -681,6 +698,76 @@ int or_ff_calculate(const float* faces, int n_faces, const int* meshes, int n_me
 }
 
 /* ---------------- thermal ----------------------------------------------- */
+
+// math::triangles_bsphere (math/triangle.cpp:361-383): the centre is the mean
+// of the triangle centres ((p0 + p1 + p2) / 3, summed in order, times 1/n);
+// the radius is the largest vertex distance over triangles 1..n-1 (the
+// reference's loop starts at 1, so triangle 0's vertices are not measured).
+void or_bsphere(const float* faces, int n_faces, float* center3, float* radius) {
+  v3 gc = mk(0, 0, 0);
+  for (int i = 0; i != n_faces; ++i) {
+    const float* t = faces + 9 * (size_t)i;
+    v3 c = add(add(mk(t[0], t[1], t[2]), mk(t[3], t[4], t[5])), mk(t[6], t[7], t[8]));
+    c = mk(c.x / 3.f, c.y / 3.f, c.z / 3.f);
+    gc = add(gc, c);
+  }
+  const float inv = 1.f / (float)n_faces;
+  gc = mk(gc.x * inv, gc.y * inv, gc.z * inv);
+  float r = 0;
+  for (int i = 1; i < n_faces; ++i) {
+    const float* t = faces + 9 * (size_t)i;
+    for (int k = 0; k < 3; ++k) {
+      v3 d = sub(gc, mk(t[3 * k], t[3 * k + 1], t[3 * k + 2]));
+      float n = std::sqrt(dot(d, d));
+      if (n > r) r = n;
+    }
+  }
+  center3[0] = gc.x;
+  center3[1] = gc.y;
+  center3[2] = gc.z;
+  *radius = r;
+}
+
+// parallel_rays_cpu::calculate (thermal_equation/parallel_rays_cpu.cpp:92-131):
+// rays from a disk (aperture 0) or square (1) of radius 1.1 * bsphere radius,
+// distance 1.1 + radius before the bsphere centre; each hit deposits
+// absorbance * ray_power on the hit face's mesh, front absorbance when
+// dot(normal, dir) < 0, ray_power = power * (pi * r * r) / n_rays. absorption
+// accumulates in float in ray order, absorption64 (may be NULL) in double.
+void or_sun(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* absorb2,
+            float power, const float* dir3, int n_rays, int rng, uint64_t seed, int aperture, int caster,
+            int n_threads, float* absorption, double* absorption64) {
+  std::vector<int> f2m(n_faces, -1);
+  for (int m = 0; m != n_meshes; ++m)
+    for (int f = 0; f != meshes[3 * m + 1]; ++f) f2m[meshes[3 * m] + f] = m;
+  float center[3], r0;
+  or_bsphere(faces, n_faces, center, &r0);
+  const float radius = r0 * 1.1f;
+  const float distance = 1.1f + radius;
+  std::vector<float> rays(6 * (size_t)std::max(n_rays, 1)), pts(3 * (size_t)std::max(n_rays, 1));
+  std::vector<int> idx(std::max(n_rays, 1));
+  or_parallel_emit2(dir3, center, distance, radius, aperture, n_rays, rng, seed, rays.data());
+  if (caster == 0) {
+    or_naive_cast(faces, n_faces, rays.data(), n_rays, idx.data(), pts.data(), n_threads);
+  } else {
+    or_zgrid* z = or_zgrid_build(faces, n_faces);
+    or_zgrid_cast(z, rays.data(), n_rays, idx.data(), pts.data(), n_threads);
+    or_zgrid_free(z);
+  }
+  const float ray_power = power * ((float)M_PI * radius * radius) / n_rays;
+  const v3 dir = mk(dir3[0], dir3[1], dir3[2]);
+  for (int r = 0; r != n_rays; ++r) {
+    const int h = idx[r];
+    if (h < 0 || f2m[h] < 0) continue;
+    const float* t = faces + 9 * (size_t)h;
+    v3 nrm = cross(sub(mk(t[3], t[4], t[5]), mk(t[0], t[1], t[2])), sub(mk(t[6], t[7], t[8]), mk(t[0], t[1], t[2])));
+    const float side = dot(nrm, dir);
+    const int m = f2m[h];
+    const float v = (side < 0 ? absorb2[2 * m] : absorb2[2 * m + 1]) * ray_power;
+    absorption[m] += v;
+    if (absorption64) absorption64[m] += (double)v;
+  }
+}
 
 // thermal_equation/radiance_cpu.cpp:96-122: emission weights T * eps * A per
 // face side, in float, accumulated in mesh order; faces outside every mesh

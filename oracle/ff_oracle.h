@@ -96,6 +96,12 @@ int or_ff_calculate(const float* faces, int n_faces, const int* meshes, int n_me
 
 /* ---- thermal (thermal_equation/*.cpp, thermal_solution/cpu_adams.cpp) -- */
 /* q_n += sum_m E_m F[m*N+n] with E_m = (eps_f+eps_r) sigma T^4 A_m, in double. */
+void or_parallel_emit2(const float* dir3, const float* center3, float distance, float radius,
+                       int aperture, int n_rays, int rng, uint64_t seed, float* out);
+void or_bsphere(const float* faces, int n_faces, float* center3, float* radius);
+void or_sun(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* absorb2,
+            float power, const float* dir3, int n_rays, int rng, uint64_t seed, int aperture, int caster,
+            int n_threads, float* absorption, double* absorption64);
 float or_radiance_weights(const float* faces, int n_faces, const int* meshes, int n_meshes,
                           const float* eps2, const float* T, float* weights);
 void or_radiance_walk(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* eps2,

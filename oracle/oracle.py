@@ -81,6 +81,12 @@ def lib():
         _sig(L, "or_ff_tally", C.c_int, [_i, C.c_int, C.c_int, _i, _i, C.c_long, _i, _f])
         _sig(L, "or_ff_calculate", C.c_int,
              [_f, C.c_int, _i, C.c_int, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int, C.c_int, _f])
+        _sig(L, "or_parallel_emit2", None,
+             [_f, _f, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int, C.c_uint64, _f])
+        _sig(L, "or_bsphere", None, [_f, C.c_int, _f, _f])
+        _sig(L, "or_sun", None,
+             [_f, C.c_int, _i, C.c_int, _f, C.c_float, _f, C.c_int, C.c_int, C.c_uint64, C.c_int, C.c_int,
+              C.c_int, _f, _d])
         _sig(L, "or_radiance_weights", C.c_float, [_f, C.c_int, _i, C.c_int, _f, _f, _f])
         _sig(L, "or_radiance_walk", None,
              [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, C.c_long, _i, _f, _f, _d])
@@ -124,6 +130,9 @@ def ref():
         _sig(R, "ffref_adams", C.c_int,
              [_f, C.c_int, _i, C.c_int, _f, _f, _f, _f, C.c_float, C.c_int, _f])
         _sig(R, "ffref_radiance", C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, C.c_int, _f, _f])
+        _sig(R, "ffref_bsphere", None, [_f, C.c_int, _f, _f])
+        _sig(R, "ffref_sun", C.c_int, [_f, C.c_int, _i, C.c_int, _f, C.c_float, _f, C.c_int, _f])
+        _sig(R, "ffref_heat_source", C.c_int, [_i, _f, C.c_int, _f, _f])
         _ref = R
     return _ref
 
@@ -276,6 +285,36 @@ def sbff_f64(F, T, area, eps_front, eps_rear):
     ab = np.zeros(n)
     lib().or_sbff_f64(n, _p(F, _d), *[_p(a, _d) for a in args], _p(em, _d), _p(ab, _d))
     return em, ab
+
+
+def parallel_emit2(direction, center, distance, radius, aperture, n_rays, rng=RNG_PHILOX, seed=0):
+    """Parallel rays from a disk (aperture 0) or a square of half side radius (1)."""
+    rays = np.zeros((max(n_rays, 1), 6), np.float32)
+    lib().or_parallel_emit2(_p(f32(direction), _f), _p(f32(center), _f), float(distance), float(radius),
+                            int(aperture), int(n_rays), rng, seed, _p(rays, _f))
+    return rays[:n_rays]
+
+
+def bsphere(faces):
+    """math::triangles_bsphere: (centre, radius)."""
+    faces = f32(faces).reshape(-1, 9)
+    c = np.zeros(3, np.float32)
+    r = np.zeros(1, np.float32)
+    lib().or_bsphere(_p(faces, _f), len(faces), _p(c, _f), _p(r, _f))
+    return c, np.float32(r[0])
+
+
+def sun(faces, meshes, absorb2, power, direction, n_rays, rng=RNG_PHILOX, seed=0, aperture=0, caster=0,
+        n_threads=1):
+    """parallel_rays_cpu::calculate: (absorption float ray order, absorption double)."""
+    faces, meshes = f32(faces).reshape(-1, 9), i32(meshes).reshape(-1, 3)
+    nm = len(meshes)
+    ab = np.zeros(nm, np.float32)
+    ab64 = np.zeros(nm, np.float64)
+    lib().or_sun(_p(faces, _f), len(faces), _p(meshes, _i), nm, _p(f32(absorb2), _f), float(power),
+                 _p(f32(direction), _f), int(n_rays), rng, seed, int(aperture), caster, n_threads, _p(ab, _f),
+                 _p(ab64, _d))
+    return ab, ab64
 
 
 def radiance_weights(faces, meshes, eps2, T):

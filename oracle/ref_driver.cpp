@@ -25,6 +25,8 @@
 #include "ray_caster/zgrid_cpu.h"
 #include "subject/system.h"
 #include "thermal_equation/conductive_cpu.h"
+#include "thermal_equation/heat_source_cpu.h"
+#include "thermal_equation/parallel_rays_cpu.h"
 #include "thermal_equation/radiance_cpu.h"
 #include "thermal_equation/system.h"
 #include "thermal_solution/cpu_adams.h"
@@ -423,5 +425,67 @@ EXPORT int ffref_radiance(const float* faces, int n_faces, const int* meshes, in
   thermal_equation::system_free(eq);
   emission::system_free(em);
   free_caster(rc);
+  return r;
+}
+
+/* math::triangles_bsphere (math/triangle.cpp:361-383). */
+EXPORT void ffref_bsphere(const float* faces, int n_faces, float* center3, float* radius) {
+  math::sphere_t sp = math::triangles_bsphere((const math::triangle_t*)faces, n_faces);
+  memcpy(center3, &sp.center, 12);
+  *radius = sp.radius;
+}
+
+namespace {
+parallel_rays_cpu::source_t sun_source(void* p) { return *(parallel_rays_cpu::source_t*)p; }
+}  // namespace
+
+/* parallel_rays_cpu (thermal_equation/parallel_rays_cpu.cpp) over
+ * PARALLEL_RAYS_CPU emission and the naive caster: one calculate() call.
+ * absorb2 holds the front/rear absorbance of each mesh. */
+EXPORT int ffref_sun(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* absorb2,
+                     float power, const float* dir3, int n_rays, float* absorption) {
+  std::vector<subject::material_t> mats(n_meshes);
+  std::vector<subject::mesh_t> ms(n_meshes);
+  for (int m = 0; m < n_meshes; ++m) {
+    mats[m] = subject::black_body();
+    mats[m].front.absorbance = absorb2[2 * m];
+    mats[m].rear.absorbance = absorb2[2 * m + 1];
+    ms[m] = subject::make_mesh(meshes[3 * m], meshes[3 * m + 1], m);
+  }
+  subject::scene_t scene = {n_faces, (subject::face_t*)faces, n_meshes, ms.data(), n_meshes, mats.data()};
+  ray_caster::system_t* rc = make_caster(0);
+  emission::system_t* em = emission::system_create(EMISSION_PARALLEL_RAYS_CPU, rc);
+  parallel_rays_cpu::source_t src;
+  src.power = power;
+  memcpy(&src.direction, dir3, 12);
+  parallel_rays_cpu::params_t params = {n_rays, em, &src, &sun_source};
+  thermal_equation::system_t* eq = parallel_rays_cpu::system_create();
+  thermal_equation::system_init(eq, &params);
+  int r = thermal_equation::system_set_scene(eq, &scene);
+  std::vector<float> T(n_meshes, 300.f), emission(n_meshes, 0.f);
+  memset(absorption, 0, n_meshes * sizeof(float));
+  thermal_equation::task_t task = {T.data(), emission.data(), absorption};
+  if (r == 0) r = thermal_equation::system_calculate(eq, &task);
+  thermal_equation::system_free(eq);
+  emission::system_free(em);
+  free_caster(rc);
+  return r;
+}
+
+/* heat_source_equation (thermal_equation/heat_source_cpu.cpp): one calculate()
+ * adding to the given arrays. */
+EXPORT int ffref_heat_source(const int* mesh_idx, const float* power, int n_sources, float* emission,
+                             float* absorption) {
+  std::vector<heat_source_equation::heat_source_t> src(n_sources);
+  for (int i = 0; i < n_sources; ++i) {
+    src[i].mesh_idx = mesh_idx[i];
+    src[i].power = power[i];
+  }
+  heat_source_equation::params_t params = {n_sources, src.data()};
+  thermal_equation::system_t* eq = heat_source_equation::system_create();
+  thermal_equation::system_init(eq, &params);
+  thermal_equation::task_t task = {nullptr, emission, absorption};
+  int r = thermal_equation::system_calculate(eq, &task);
+  thermal_equation::system_free(eq);
   return r;
 }

@@ -302,6 +302,40 @@ def gen_radiance(rng):
          absorption=ab)
 
 
+def gen_sun():
+    """parallel_rays_cpu (thermal) and triangles_bsphere; own random stream."""
+    R = O.ref()
+    r2 = np.random.default_rng(8)
+    out = {}
+    for k, (faces, _) in enumerate([scenes.confetti_scene(150, cells=2, seed=3), scenes.ff_box(),
+                                    (scenes.icosphere(2), None)]):
+        faces = np.ascontiguousarray(faces, np.float32)
+        c = np.zeros(3, np.float32)
+        r = np.zeros(1, np.float32)
+        R.ffref_bsphere(P(faces), len(faces), P(c), P(r))
+        out[f"bs{k}_faces"], out[f"bs{k}_center"], out[f"bs{k}_radius"] = faces, c, r
+    faces, meshes = scenes.confetti_scene(150, cells=2, seed=3)
+    faces = np.ascontiguousarray(faces, np.float32)
+    nm = len(meshes)
+    absorb = r2.uniform(0.1, 1.0, (nm, 2)).astype(np.float32)
+    d = r2.normal(size=3)
+    d = (d / np.linalg.norm(d)).astype(np.float32)
+    ab = np.zeros(nm, np.float32)
+    rc = R.ffref_sun(P(faces), len(faces), P(meshes, _i), nm, P(absorb), np.float32(1361.0), P(d), 20_000, P(ab))
+    assert rc == 0
+    out.update(faces=faces, meshes=meshes, absorb=absorb, power=np.float32(1361.0), direction=d,
+               n_rays=np.int32(20_000), absorption=ab)
+    mesh_idx = np.array([0, 3, 3, 7], np.int32)
+    power = np.array([5.0, -2.0, 1.5, -0.25], np.float32)
+    em = np.linspace(1, 2, 8).astype(np.float32)
+    ab = np.linspace(3, 4, 8).astype(np.float32)
+    em0, ab0 = em.copy(), ab.copy()
+    assert R.ffref_heat_source(P(mesh_idx, _i), P(power), 4, P(em), P(ab)) == 0
+    out.update(hs_mesh=mesh_idx, hs_power=power, hs_emission0=em0, hs_absorption0=ab0, hs_emission=em,
+               hs_absorption=ab)
+    save("sun.npz", **out)
+
+
 def gen_parallel(rng):
     R = O.ref()
     faces = scenes.icosphere(1)
@@ -356,6 +390,7 @@ def main():
     gen_radiance(rng)
     gen_parallel(rng)
     gen_conduction(rng)
+    gen_sun()
 
 
 if __name__ == "__main__":

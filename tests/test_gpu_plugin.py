@@ -380,3 +380,92 @@ def test_radiance_equation_plugin(P, oracle, caster_type):
     L.ffp_emission_free(em)
     L.ffp_ray_caster_free(rc)
     assert not L.ffp_thermal_equation_create(P.THERMAL_EQUATION_RADIANCE_HIP, None, 1000, 0)
+
+
+# ------------------ thermal_equation/{conductive,heat_source,parallel_rays}_cpu --
+
+def _materials(P, n, **fields):
+    mats = []
+    for m in range(n):
+        mat = P.black_body()
+        for k, v in fields.items():
+            if k in ("front_absorbance", "rear_absorbance"):
+                getattr(mat, k.split("_")[0])[2] = float(v[m])
+            else:
+                setattr(mat, k, float(v[m]))
+        mats.append(mat)
+    return mats
+
+
+@pytest.mark.parametrize("name", ["plate4", "box12", "two_plates6"])
+def test_conductive_equation_plugin(P, name):
+    """Conduction through the plugin table against the reference's
+    conductive_equation (its per-pair split into emission and absorption)."""
+    L = P.lib()
+    g = golden("conduction.npz")
+    faces, meshes = g[f"{name}_faces"], g[f"{name}_meshes"]
+    n = len(meshes)
+    mats = _materials(P, n, thermal_conductivity=g[f"{name}_k"], thickness=g[f"{name}_thickness"])
+    sm = np.ascontiguousarray(meshes, np.int32).reshape(-1, 3).copy()
+    sm[:, 2] = np.arange(n)
+    sc, keep = subject_scene(P, faces, sm, mats)
+    eq = L.ffp_thermal_equation_create(P.THERMAL_EQUATION_CONDUCTIVE_HIP, None, 0, 0)
+    assert eq and L.ffp_thermal_equation_set_scene(eq, C.byref(sc)) == 0
+    T = (C.c_float * n)(*[float(x) for x in g[f"{name}_T"]])
+    em, ab = (C.c_float * n)(), (C.c_float * n)()
+    assert L.ffp_thermal_equation_calculate(eq, C.byref(P.TeTask(T, em, ab))) == 0
+    scale = np.abs(g[f"{name}_absorption"]).max() + 1e-30
+    assert np.abs(np.frombuffer(em, np.float32) - g[f"{name}_emission"]).max() <= 1e-5 * scale
+    assert np.abs(np.frombuffer(ab, np.float32) - g[f"{name}_absorption"]).max() <= 1e-5 * scale
+    L.ffp_thermal_equation_free(eq)
+
+
+def test_heat_source_plugin(P):
+    L = P.lib()
+    g = golden("sun.npz")
+    src = (P.HeatSource * 4)(*[P.HeatSource(int(m), float(p)) for m, p in zip(g["hs_mesh"], g["hs_power"])])
+    eq = L.ffp_heat_source_create(4, src)
+    assert eq
+    em = (C.c_float * 8)(*[float(x) for x in g["hs_emission0"]])
+    ab = (C.c_float * 8)(*[float(x) for x in g["hs_absorption0"]])
+    T = (C.c_float * 8)()
+    assert L.ffp_thermal_equation_calculate(eq, C.byref(P.TeTask(T, em, ab))) == 0
+    assert np.array_equal(bits(np.frombuffer(em, np.float32)), bits(g["hs_emission"]))
+    assert np.array_equal(bits(np.frombuffer(ab, np.float32)), bits(g["hs_absorption"]))
+    L.ffp_thermal_equation_free(eq)
+
+
+@pytest.mark.parametrize("caster_type", CASTERS)
+def test_distant_source_plugin(P, oracle, caster_type):
+    """Distant source through the plugin table: grid caster = fused device
+    pass, naive caster = the emitter emits and casts, the deposit runs on the
+    host; both equal the exact sum of the reference's per-ray float values
+    (FP64 oracle on the identical rays), added once as float."""
+    L = P.lib()
+    g = golden("sun.npz")
+    faces, meshes, absorb = g["faces"], g["meshes"], g["absorb"]
+    n = len(meshes)
+    mats = _materials(P, n, front_absorbance=absorb[:, 0], rear_absorbance=absorb[:, 1])
+    sm = meshes.copy()
+    sm[:, 2] = np.arange(n)
+    sc, keep = subject_scene(P, faces, sm, mats)
+    rc = caster(P, caster_type)
+    em = L.ffp_emission_create(P.EMISSION_PARALLEL_RAYS_HIP, rc)
+    seed = 41
+    assert L.ffp_emission_set_seed(em, seed, 0) == 0
+    d = g["direction"]
+    source = P.Source(float(g["power"]), P.Vec3(*[float(x) for x in d]))
+    eq = L.ffp_distant_source_create(em, 30_000, C.byref(source))
+    assert eq and L.ffp_thermal_equation_set_scene(eq, C.byref(sc)) == 0
+    for call in range(2):
+        T = (C.c_float * n)()
+        emi, ab = (C.c_float * n)(), (C.c_float * n)()
+        assert L.ffp_thermal_equation_calculate(eq, C.byref(P.TeTask(T, emi, ab))) == 0
+        key = (seed + 0x9E3779B97F4A7C15 * call) % (1 << 64)
+        _, ab64 = oracle.sun(faces, meshes, absorb, float(g["power"]), d, 30_000, seed=key, n_threads=8)
+        got = np.frombuffer(ab, np.float32).astype(np.float64)
+        assert ab64.sum() > 0 and np.abs(got - ab64).max() <= 1e-6 * ab64.max()
+        assert not np.frombuffer(emi, np.float32).any()
+    L.ffp_thermal_equation_free(eq)
+    L.ffp_emission_free(em)
+    L.ffp_ray_caster_free(rc)

@@ -45,3 +45,13 @@ def test_radiance_powers_zero_for_silent_sides():
     assert rear.sum() == 0 and front[2:].sum() == 0
     p, em = ffk.radiance_powers(faces, meshes, eps, T, front, rear)
     assert (p[1::2] == 0).all() and (p[4:] == 0).all() and em[1] == 0
+
+
+def test_bsphere_host_matches_reference():
+    """ffk_bsphere (host) is math::triangles_bsphere bit for bit."""
+    from form_factors_amd import ffk
+
+    g = golden("sun.npz")
+    for k in range(3):
+        c, r = ffk.bsphere(g[f"bs{k}_faces"])
+        assert np.array_equal(bits(c), bits(g[f"bs{k}_center"])) and bits(r) == bits(g[f"bs{k}_radius"][0])

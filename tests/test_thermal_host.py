@@ -24,6 +24,18 @@ def test_conduction_operator_matches_reference(name):
     net_ref = g[f"{name}_absorption"].astype(np.float64) - g[f"{name}_emission"].astype(np.float64)
     scale = np.abs(g[f"{name}_absorption"]).max() + 1e-30
     assert np.abs(flow - net_ref).max() <= 1e-5 * scale, (name, np.abs(flow - net_ref).max(), scale)
+    # the reference's split: per neighbour, inflow is absorbed, outflow emitted
+    ab = np.zeros(n)
+    em = np.zeros(n)
+    for m in range(n):
+        for j in range(row[m], row[m + 1]):
+            f = G[j] * (T[cols[j]] - T[m])
+            if f > 0:
+                ab[m] += f
+            else:
+                em[m] -= f
+    assert np.abs(ab - g[f"{name}_absorption"]).max() <= 1e-5 * scale
+    assert np.abs(em - g[f"{name}_emission"]).max() <= 1e-5 * scale
     # symmetric operator, energy conserving: sum of flows is zero
     dense = np.zeros((n, n))
     for m in range(n):
