@@ -49,6 +49,7 @@ def parse():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-thermal", action="store_true")
     ap.add_argument("--no-batch-set", action="store_true")
+    ap.add_argument("--no-confetti", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--thermal-steps", type=int, default=10_000)
     ap.add_argument("--block", type=int, default=0)
@@ -163,6 +164,68 @@ def batch_set_leg(ffk, scenes, n_scenes):
                         "events_seconds": round(ev_b, 4),
                         "path": "persistent tile-queue launches per LDS class, one hipGraph"},
             "speedup": round(wall_p / wall_b, 4), "F_bits_identical": same, "unit": "Mrays/s"}
+
+
+def confetti_leg(ffk, scenes, steps):
+    """2000 confetti triangles (centroids uniform in the unit cube, edges in
+    [0.02, 0.1], random orientation) inside a 2028-triangle cube shell, one
+    mesh per triangle, 50k Malley rays per triangle; the fused kernel on the
+    default grid and after the autotuner; then 1e7 parallel rays from a fixed
+    direction through a 2 x 2 square aperture and through the reference's
+    disk aperture."""
+    import torch
+
+    faces, meshes = scenes.confetti_scene(2000, cells=13, seed=1)
+    sc = ffk.Scene(faces, meshes)
+    n_rays = sc.set_emission(n_rays=50_000 * len(faces), frame=ffk.FRAME_ORTHONORMAL)
+    sc.build_grid()
+    N = len(meshes)
+    F = torch.empty((N, N), dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def run(k):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        sc.ff_fused(0xC3, F=F)  # warm-up
+        torch.cuda.synchronize()
+        e0.record(stream)
+        for i in range(k):
+            sc.ff_fused(0xC3 + i, F=F)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / 1e3 / k
+
+    g0 = sc.grid_info()["dims"]
+    t_default = run(steps)
+    sum_default = int(F.view(torch.int32).sum(dtype=torch.int64))
+    t0 = time.perf_counter()
+    tuned = sc.autotune(sample_rays=1 << 21)
+    t_tune = time.perf_counter() - t0
+    t_tuned = run(steps)
+    same = int(F.view(torch.int32).sum(dtype=torch.int64)) == sum_default
+    log(f"confetti: default grid {g0} {n_rays / t_default / 1e6:.1f} Mrays/s, tuned {tuned} "
+        f"{n_rays / t_tuned / 1e6:.1f} Mrays/s (tuning {t_tune:.1f} s)")
+    c, r0 = ffk.bsphere(faces)
+    radius = np.float32(r0 * np.float32(1.1))
+    d = np.array([0.36, -0.48, -0.8], np.float32)
+    d /= np.linalg.norm(d)
+    sun = {}
+    for name, ap, rad in (("square_2x2", 1, np.float32(1.0)), ("disk_reference", 0, radius)):
+        sc.sun_counts(d, c, np.float32(1.1) + radius, rad, 10_000_000, 5, aperture=ap)  # warm-up
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        counts = sc.sun_counts(d, c, np.float32(1.1) + radius, rad, 10_000_000, 5, aperture=ap)
+        el = time.perf_counter() - t
+        sun[name] = {"rays": 10_000_000, "value": round(1e7 / el / 1e6, 2), "hits": int(counts.sum()),
+                     "seconds": round(el, 4)}
+        log(f"sun {name}: {1e7 / el / 1e6:.1f} Mrays/s, {int(counts.sum())} hits")
+    del F
+    torch.cuda.empty_cache()
+    return {"workload": f"{len(faces)} triangles (2000 confetti + cube shell), {n_rays} Malley rays, F {N}x{N}",
+            "n_rays": n_rays, "unit": "Mrays/s",
+            "default_grid": {"dims": list(g0), "value": round(n_rays / t_default / 1e6, 2)},
+            "autotuned": {"dims": list(tuned["dims"]), "block": tuned["block"],
+                          "value": round(n_rays / t_tuned / 1e6, 2), "tuning_seconds": round(t_tune, 2)},
+            "F_bits_identical": same, "sun": sun}
 
 
 def thermal_leg(ffk, scenes, n_steps):
@@ -356,6 +419,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batch_set:
         result["batch_set"] = batch_set_leg(ffk, scenes, args.batch_scenes)
+    if not args.no_confetti:
+        result["confetti"] = confetti_leg(ffk, scenes, 3)
     if not args.no_thermal:
         result["thermal"] = thermal_leg(ffk, scenes, args.thermal_steps)
 

@@ -445,6 +445,17 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
   return ffk::build_tiles(s, s->tune_rays_per_tile, st);
 }
 
+int ffk_scene_layout(const ffk_scene* s, float* h_faces, int* h_meshes, int* h_front, int* h_rear, int* frame) {
+  FFK_REQUIRE(s && s->has_faces && s->has_meshes && s->has_emission, FFK_ERR_ARG,
+              "ffk_scene_layout: faces, meshes and emission must be set");
+  if (h_faces) std::copy(s->h_faces.begin(), s->h_faces.end(), h_faces);
+  if (h_meshes) std::copy(s->h_meshes.begin(), s->h_meshes.end(), h_meshes);
+  if (h_front) std::copy(s->h_front.begin(), s->h_front.end(), h_front);
+  if (h_rear) std::copy(s->h_rear.begin(), s->h_rear.end(), h_rear);
+  if (frame) *frame = s->frame;
+  return FFK_OK;
+}
+
 int ffk_scene_counts(const ffk_scene* s, int* n_faces, int* n_meshes, long long* n_rays) {
   FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
   if (n_faces) *n_faces = s->n_faces;

@@ -39,6 +39,8 @@ SIGNATURES = {
     "ffk_scene_build_grid": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp]),
     "ffk_scene_grid_info": (C.c_int, [_vp, _i, _f, _f, C.POINTER(_ll)]),
     "ffk_scene_counts": (C.c_int, [_vp, _i, _i, C.POINTER(_ll)]),
+    "ffk_scene_layout": (C.c_int, [_vp, _f, _i, _i, _i, _i]),
+    "ffk_autotune": (C.c_int, [_vp, _ll, _i, _i, _vp]),
     "ffk_cast": (C.c_int, [_vp, C.c_int, _vp, _ll, _vp, _vp, _vp, _vp]),
     "ffk_emitted_counts": (C.c_int, [_f, C.c_int, C.c_int, C.c_float, _i, _i]),
     "ffk_area_weights": (C.c_int, [_f, C.c_int, _f, _f]),
@@ -303,6 +305,14 @@ class Scene:
         nr = C.c_longlong()
         check(lib().ffk_scene_grid_info(self.h, dims, lo, hi, C.byref(nr)), "ffk_scene_grid_info")
         return {"dims": tuple(dims), "lo": tuple(lo), "hi": tuple(hi), "n_refs": nr.value}
+
+    def autotune(self, sample_rays=1 << 20):
+        """Grid resolution and block size from timed trials (cached per scene
+        signature). Returns {"dims", "block", "cached"}."""
+        dims = (C.c_int * 3)()
+        block = C.c_int()
+        rc = check(lib().ffk_autotune(self.h, int(sample_rays), dims, C.byref(block), _stream()), "ffk_autotune")
+        return {"dims": tuple(dims), "block": block.value, "cached": rc == 1}
 
     def set_tuning(self, block=0, rays_per_tile=0, hist=-1):
         check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist), "ffk_set_tuning")

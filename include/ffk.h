@@ -90,6 +90,9 @@ int ffk_scene_build_grid(ffk_scene* s, int nx, int ny, int nz, void* stream);
 int ffk_scene_grid_info(const ffk_scene* s, int* dims3, float* lo3, float* hi3,
                         long long* n_refs);
 int ffk_scene_counts(const ffk_scene* s, int* n_faces, int* n_meshes, long long* n_rays);
+/* Host copies of the scene's faces (9 per face), meshes (3 per mesh), emitted
+ * front/rear ray counts and emission frame; any pointer may be NULL. */
+int ffk_scene_layout(const ffk_scene* s, float* h_faces, int* h_meshes, int* h_front, int* h_rear, int* frame);
 
 /* ---- casting ---------------------------------------------------------------- */
 /* Nearest hit for n_rays device rays: naive_cpu::cast semantics
@@ -162,6 +165,18 @@ int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long l
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
 /* h_flags[i] = 1 when scene i needs a tally buffer in ffk_batch_run, else 0. */
 int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags);
+
+/* ---- autotuning ------------------------------------------------------------------
+ * Picks the grid resolution (a ladder of 0.25 .. 16 cells per face, each
+ * timed by casting the scene's own Malley rays with the emission layout
+ * scaled to about sample_rays) and the fused kernel's block size (256, 512 or
+ * 1024, timed on the full layout when a run is estimated under 250 ms), then
+ * leaves the scene built and tuned with them. Results are cached per scene
+ * signature (faces, meshes, ray counts, frame) for the life of the process.
+ * Returns 0 when tuned, 1 when taken from the cache. The grid resolution
+ * never changes hit indices. out_dims3 / out_block may be NULL (block 0 = the
+ * automatic choice was kept). */
+int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_block, void* stream);
 
 /* ---- radiance equation --------------------------------------------------------
  * radiance_equation::calculate (thermal_equation/radiance_cpu.cpp:132-190) on
