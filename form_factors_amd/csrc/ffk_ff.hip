@@ -107,6 +107,10 @@ __global__ void k_zero_rows(int N, const int* __restrict__ rows, int n_rows,
 // tile's row; 2 = two uint16 counters per LDS word (tiles <= 65535 rays).
 // PERSIST: blocks pull tiles from a queue (batched launch) instead of one
 // tile per block (plain launch).
+// Idle lanes (of 64) that trigger a refill in the fused kernel; 0 = off.
+// 16 measured best on incoherent Malley rays (8..24 within 2 %).
+constexpr int kDefaultRefill = 16;
+
 struct fused_args {
   const scene_desc* descs;
   const tile* tiles;
@@ -117,18 +121,27 @@ struct fused_args {
   unsigned long long* const* tallies;
   float* const* Fs;
   unsigned long long* stats;
+  int refill;  // 0: one ray per lane at a time; else the idle-lane threshold of the refill loop
 };
 
-template <int HIST, bool PERSIST, bool STATS>
+// REFILL: lanes walk one grid cell per iteration and, once at least
+// `refill` lanes of the wavefront are idle, the wavefront reserves that many
+// rays of the tile with one LDS atomic and the idle lanes start them. Without
+// it each lane carries one ray at a time and the wavefront waits for its
+// slowest ray before the next round, which leaves most lanes idle on
+// incoherent rays. Rays are tallied as integers, so the order in which they
+// are cast does not change F.
+template <int HIST, bool PERSIST, bool STATS, bool REFILL>
 __global__ void __launch_bounds__(1024)
     k_ff_fused(const scene_desc* __restrict__ descs, const tile* __restrict__ tiles, int n_tiles,
                int* __restrict__ queue, const unsigned long long* __restrict__ seeds,
                unsigned long long* const* __restrict__ tallies, float* const* __restrict__ Fs,
-               unsigned long long* __restrict__ stats) {
+               unsigned long long* __restrict__ stats, int refill) {
   // Read-only, restrict-qualified tables indexed by wave-uniform values are
   // read with scalar loads: the scene descriptor lives in SGPRs.
   extern __shared__ uint32_t hist[];
   __shared__ int s_tile;
+  __shared__ int s_next;  // next ray of the tile (REFILL)
   uint32_t nt = 0, nc = 0;
   int t_idx = blockIdx.x;
   while (true) {
@@ -144,21 +157,15 @@ __global__ void __launch_bounds__(1024)
     const scene_desc S = descs[sc];
     const int N = S.n_meshes;
     const int words = HIST == 1 ? N : (N + 1) >> 1;
-    if (HIST != 0) {
+    if (HIST != 0)
       for (int w = threadIdx.x; w < words; w += blockDim.x) hist[w] = 0u;
-      __syncthreads();
-    }
+    if (REFILL && threadIdx.x == 0) s_next = 0;
+    if (HIST != 0 || REFILL) __syncthreads();
     const unsigned long long seed = seeds[sc];
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     unsigned long long* tally = tallies ? tallies[sc] : nullptr;
     int hint = find_face(S.face_off, S.n_faces, T.g0, 0);
-    for (long long g = T.g0 + threadIdx.x; g < T.g1; g += blockDim.x) {
-      const int f = find_face(S.face_off, S.n_faces, g, hint);
-      hint = f;
-      v3 o, e;
-      malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
-      hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
-      cast_grid<STATS>(S, o, e - o, h, nt, nc);
+    auto count_hit = [&](const hit_rec& h) {
       if (h.f >= 0) {
         const int m = S.face2mesh[h.f];
         if (m >= 0) {
@@ -166,6 +173,59 @@ __global__ void __launch_bounds__(1024)
           else if (HIST == 1) atomicAdd(&hist[m], 1u);
           else atomicAdd(&hist[m >> 1], (m & 1) ? 65536u : 1u);
         }
+      }
+    };
+    if (REFILL) {
+      const long long n = T.g1 - T.g0;
+      const int lane = threadIdx.x & 63;
+      bool have = false, more = true;  // `more` is wave-uniform
+      v3 o = mk(0.f, 0.f, 0.f), dir = mk(0.f, 0.f, 0.f);
+      float ix = 0.f, iy = 0.f, iz = 0.f, dlen = 0.f, tnx = 0.f, tny = 0.f, tnz = 0.f;
+      int cx = 0, cy = 0, cz = 0, cell = 0;
+      hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
+      mailbox mb;
+      mb.clear();
+      while (true) {
+        const unsigned long long idle = __ballot(!have);
+        const int n_idle = __popcll(idle);
+        if (more && n_idle >= refill) {
+          int base = 0;
+          if (lane == 0) base = atomicAdd(&s_next, n_idle);
+          base = __shfl(base, 0, 64);
+          if ((long long)base + n_idle >= n) more = false;
+          if (!have) {
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            const long long off = (long long)base + rank;
+            if (off < n) {
+              const long long g = T.g0 + off;
+              const int f = find_face(S.face_off, S.n_faces, g, hint);
+              hint = f;
+              v3 e;
+              malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+              dir = e - o;
+              h.d2 = 3.402823466e38f;
+              h.f = -1;
+              mb.clear();
+              have = walk_begin(S.g, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell);
+            }
+          }
+        }
+        if (!more && __ballot(have) == 0ull) break;
+        if (have && !walk_cell<STATS>(S, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell, h, mb, nt, nc)) {
+          count_hit(h);
+          have = false;
+        }
+      }
+    } else {
+      for (long long g = T.g0 + threadIdx.x; g < T.g1; g += blockDim.x) {
+        const int f = find_face(S.face_off, S.n_faces, g, hint);
+        hint = f;
+        v3 o, e;
+        malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+        hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
+        cast_grid<STATS>(S, o, e - o, h, nt, nc);
+        count_hit(h);
       }
     }
     if (HIST != 0) {
@@ -233,39 +293,43 @@ static size_t hist_bytes(int mode, int N) {
   return 0;
 }
 
-template <int H, bool P, bool S>
+template <int H, bool P, bool S, bool R>
 static void set_lds_attr(size_t bytes) {
   if (bytes > 65536)
-    (void)hipFuncSetAttribute((const void*)&k_ff_fused<H, P, S>,
+    (void)hipFuncSetAttribute((const void*)&k_ff_fused<H, P, S, R>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <bool P, bool S>
-static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStream_t st,
-                              const fused_args& A) {
+template <int H, bool P, bool S, bool R>
+static void launch_one(dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
+  set_lds_attr<H, P, S, R>(lds);
+  k_ff_fused<H, P, S, R><<<grid, block, H ? lds : 0, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds,
+                                                            A.tallies, A.Fs, A.stats, A.refill);
+}
+
+template <bool P, bool S, bool R>
+static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
   switch (hist) {
-    case 1:
-      set_lds_attr<1, P, S>(lds);
-      k_ff_fused<1, P, S><<<grid, block, lds, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
-      break;
-    case 2:
-      set_lds_attr<2, P, S>(lds);
-      k_ff_fused<2, P, S><<<grid, block, lds, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
-      break;
-    default:
-      k_ff_fused<0, P, S><<<grid, block, 0, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies, A.Fs, A.stats);
-      break;
+    case 1: launch_one<1, P, S, R>(grid, block, lds, st, A); break;
+    case 2: launch_one<2, P, S, R>(grid, block, lds, st, A); break;
+    default: launch_one<0, P, S, R>(grid, block, lds, st, A); break;
   }
   return hipGetLastError();
+}
+
+template <bool P, bool S>
+static hipError_t launch_refill(int hist, dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
+  return A.refill > 0 ? launch_mode<P, S, true>(hist, grid, block, lds, st, A)
+                      : launch_mode<P, S, false>(hist, grid, block, lds, st, A);
 }
 
 static hipError_t launch_fused(int hist, bool persist, bool stats, dim3 grid, int block, size_t lds,
                                hipStream_t st, const fused_args& A) {
   if (persist)
-    return stats ? launch_mode<true, true>(hist, grid, block, lds, st, A)
-                 : launch_mode<true, false>(hist, grid, block, lds, st, A);
-  return stats ? launch_mode<false, true>(hist, grid, block, lds, st, A)
-               : launch_mode<false, false>(hist, grid, block, lds, st, A);
+    return stats ? launch_refill<true, true>(hist, grid, block, lds, st, A)
+                 : launch_refill<true, false>(hist, grid, block, lds, st, A);
+  return stats ? launch_refill<false, true>(hist, grid, block, lds, st, A)
+               : launch_refill<false, false>(hist, grid, block, lds, st, A);
 }
 
 static long long auto_rays_per_tile(long long total) {
@@ -349,6 +413,7 @@ struct ffk_batch {
   float** d_Fs = nullptr;
   long long total_rays = 0;
   int block = 0, rpt = 0, n_blocks = 0;
+  int refill = kDefaultRefill;
   // graph state
   hipStream_t cap = nullptr;  // capture stream (default priority)
   hipGraph_t graph = nullptr;
@@ -551,6 +616,21 @@ int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mod
   return FFK_OK;
 }
 
+int ffk_set_refill(ffk_scene* s, int lanes) {
+  FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
+  FFK_REQUIRE(lanes >= 0 && lanes <= 64, FFK_ERR_ARG, "refill lanes %d outside [0, 64]", lanes);
+  s->tune_refill = lanes;
+  return FFK_OK;
+}
+
+int ffk_batch_set_refill(ffk_batch* b, int lanes) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  FFK_REQUIRE(lanes >= 0 && lanes <= 64, FFK_ERR_ARG, "refill lanes %d outside [0, 64]", lanes);
+  b->refill = lanes;
+  free_graph(b);
+  return FFK_OK;
+}
+
 int ffk_ff_needs_tally(const ffk_scene* s) {
   if (!s || !s->has_emission) return 0;
   const long long max_tile = std::min<long long>(s->tiles_rays_per_tile, 65535);
@@ -605,6 +685,7 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
   A.tallies = reinterpret_cast<unsigned long long* const*>(d_misc + 1);
   A.Fs = reinterpret_cast<float* const*>(d_misc + 2);
   A.stats = d_stats;
+  A.refill = s->tune_refill;
   const int block = s->tune_block > 0 ? s->tune_block : default_block(hist, hist_bytes(hist, N));
   if (s->n_tiles > 0)
     FFK_HIP(launch_fused(hist, false, d_stats != nullptr, dim3((unsigned)s->n_tiles), block,
@@ -685,24 +766,32 @@ int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_
   return FFK_OK;
 }
 
+}  // extern "C"
+
+template <bool R>
+static hipError_t persistent_occupancy(int hist, int block, size_t lds, int* per_cu) {
+  switch (hist) {
+    case 1:
+      set_lds_attr<1, true, false, R>(lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<1, true, false, R>, block, lds);
+    case 2:
+      set_lds_attr<2, true, false, R>(lds);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<2, true, false, R>, block, lds);
+    default:
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<0, true, false, R>, block, 0);
+  }
+}
+
+static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int n_tiles);
+
+extern "C" {
+
 static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int n_tiles) {
   int n_blocks = b->n_blocks;
   if (n_blocks <= 0) {
     int per_cu = 0;
-    hipError_t e;
-    switch (hist) {
-      case 1:
-        set_lds_attr<1, true, false>(lds);
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ff_fused<1, true, false>, block, lds);
-        break;
-      case 2:
-        set_lds_attr<2, true, false>(lds);
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ff_fused<2, true, false>, block, lds);
-        break;
-      default:
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_ff_fused<0, true, false>, block, 0);
-        break;
-    }
+    const hipError_t e = b->refill > 0 ? persistent_occupancy<true>(hist, block, lds, &per_cu)
+                                       : persistent_occupancy<false>(hist, block, lds, &per_cu);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     n_blocks = per_cu * device_cu_count();
   }
@@ -733,6 +822,7 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
     A.tallies = b->d_tallies;
     A.Fs = b->d_Fs;
     A.stats = d_stats;
+    A.refill = b->refill;
     const int n_blocks = batch_blocks(b, k.hist, k.lds, block, k.n);
     FFK_HIP(launch_fused(k.hist, true, stats, dim3((unsigned)n_blocks), block, k.lds, st, A));
   }

@@ -304,6 +304,7 @@ struct ffk_scene {
   std::vector<long long> h_face_off, h_mesh_off;
   // tuning (0 = automatic)
   int tune_block = 0, tune_rays_per_tile = 0, tune_hist = -1;
+  int tune_refill = 16;  // idle lanes that trigger a refill in the fused kernel (0 = off)
   int tiles_rays_per_tile = 0;  // value the tile list was built with
 
   ffk::scene_desc desc() const;

@@ -62,8 +62,152 @@ __device__ inline uint32_t test_ref(const scene_desc& S, int k, v3 o, v3 dir, hi
 // rounding of both), so the result equals testing every face. Empty cells
 // carry their Chebyshev distance r to the nearest occupied cell, and the
 // walk leaves the empty box of radius r-1 around them in one step.
-// (All walk state is kept in plain locals: selecting among struct members
-// here makes the compiler address the struct in scratch memory.)
+//
+// The walk is split in two so that a caller can interleave many rays per
+// lane: walk_begin clips the ray to the grid box and finds its first cell,
+// walk_cell visits one cell (tests, then one step or jump). All walk state
+// is kept in plain scalars passed by reference: selecting among struct
+// members here makes the compiler address the struct in scratch memory.
+
+// Ray parameter of the next boundary of cell c along one axis (s = step sign).
+__device__ inline float walk_boundary(float lo, float h, int c, int s, float o, float inv) {
+  return s ? (lo + (float)(c + (s > 0)) * h - o) * inv : 3.4e38f;
+}
+
+__device__ inline int walk_sign(float d) { return d > 0.f ? 1 : (d < 0.f ? -1 : 0); }
+
+// Returns false when the ray misses the grid box (nothing to walk).
+__device__ inline bool walk_begin(const grid_desc& g, v3 o, v3 dir, float& ix, float& iy, float& iz, float& dlen,
+                                  int& cx, int& cy, int& cz, float& tnx, float& tny, float& tnz, int& cell) {
+  // Clip to the grid box (slabs); t is in units of dir.
+  float t0 = 0.f, t1 = 3.4e38f;
+  ix = 0.f;
+  iy = 0.f;
+  iz = 0.f;
+  if (dir.x != 0.f) {
+    ix = 1.0f / dir.x;
+    float ta = (g.lo[0] - o.x) * ix, tb = (g.hi[0] - o.x) * ix;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.x < g.lo[0] || o.x > g.hi[0]) {
+    return false;
+  }
+  if (dir.y != 0.f) {
+    iy = 1.0f / dir.y;
+    float ta = (g.lo[1] - o.y) * iy, tb = (g.hi[1] - o.y) * iy;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.y < g.lo[1] || o.y > g.hi[1]) {
+    return false;
+  }
+  if (dir.z != 0.f) {
+    iz = 1.0f / dir.z;
+    float ta = (g.lo[2] - o.z) * iz, tb = (g.hi[2] - o.z) * iz;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.z < g.lo[2] || o.z > g.hi[2]) {
+    return false;
+  }
+  if (!(t0 <= t1)) return false;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  cx = min(max((int)floorf((o.x + dir.x * t0 - g.lo[0]) * g.inv_h[0]), 0), nx - 1);
+  cy = min(max((int)floorf((o.y + dir.y * t0 - g.lo[1]) * g.inv_h[1]), 0), ny - 1);
+  cz = min(max((int)floorf((o.z + dir.z * t0 - g.lo[2]) * g.inv_h[2]), 0), nz - 1);
+  tnx = walk_boundary(g.lo[0], g.h[0], cx, walk_sign(dir.x), o.x, ix);
+  tny = walk_boundary(g.lo[1], g.h[1], cy, walk_sign(dir.y), o.y, iy);
+  tnz = walk_boundary(g.lo[2], g.h[2], cz, walk_sign(dir.z), o.z, iz);
+  // Distances along the ray are compared as (t * |dir|)^2 so that neither
+  // factor overflows for very short or very long direction vectors.
+  dlen = sqrtf(dot(dir, dir));
+  cell = cx + nx * cy + nx * ny * cz;
+  return true;
+}
+
+// Visits the current cell: tests its references, then moves to the next
+// cell (or past an empty box). Returns false when the walk is over.
+template <bool STATS>
+__device__ inline bool walk_cell(const scene_desc& S, v3 o, v3 dir, float ix, float iy, float iz, float dlen,
+                                 int& cx, int& cy, int& cz, float& tnx, float& tny, float& tnz, int& cell,
+                                 hit_rec& h, mailbox& mb, uint32_t& n_tests, uint32_t& n_cells) {
+  const grid_desc& g = S.g;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int nxy = nx * ny;
+  const int sx = walk_sign(dir.x), sy = walk_sign(dir.y), sz = walk_sign(dir.z);
+  const int2 ci = S.cell_info[cell];
+  if (STATS) n_cells += 1;
+  for (int k = ci.x; k < ci.x + ci.y; ++k) {
+    const uint32_t t = test_ref(S, k, o, dir, h, mb);
+    if (STATS) n_tests += t;
+  }
+  const int R = ci.y > 0 ? 0 : ci.x - 1;
+  if (R >= 1) {
+    // Every cell within R of this one is empty: leave the box
+    // [c-R, c+R] (clipped to the grid) in one step.
+    const int hx = sx > 0 ? min(cx + R, nx - 1) : max(cx - R, 0);
+    const int hy = sy > 0 ? min(cy + R, ny - 1) : max(cy - R, 0);
+    const int hz = sz > 0 ? min(cz + R, nz - 1) : max(cz - R, 0);
+    const float bx = walk_boundary(g.lo[0], g.h[0], hx, sx, o.x, ix);
+    const float by = walk_boundary(g.lo[1], g.h[1], hy, sy, o.y, iy);
+    const float bz = walk_boundary(g.lo[2], g.h[2], hz, sz, o.z, iz);
+    const bool ex = (bx <= by) && (bx <= bz);
+    const bool ey = !ex && (by <= bz);
+    const float tb = ex ? bx : (ey ? by : bz);
+    if (!(tb < 3.4e38f)) return false;
+    const float de = tb * dlen;
+    if (h.f >= 0 && h.d2 < de * de * 0.9999f) return false;
+    // Cell just past the exit face; the other two coordinates come from
+    // the exit point, kept inside the box and never moved backwards.
+    const float px = o.x + dir.x * tb, py = o.y + dir.y * tb, pz = o.z + dir.z * tb;
+    int nxc = min(max((int)floorf((px - g.lo[0]) * g.inv_h[0]), max(cx - R, 0)), min(cx + R, nx - 1));
+    int nyc = min(max((int)floorf((py - g.lo[1]) * g.inv_h[1]), max(cy - R, 0)), min(cy + R, ny - 1));
+    int nzc = min(max((int)floorf((pz - g.lo[2]) * g.inv_h[2]), max(cz - R, 0)), min(cz + R, nz - 1));
+    nxc = sx > 0 ? max(nxc, cx) : (sx < 0 ? min(nxc, cx) : cx);
+    nyc = sy > 0 ? max(nyc, cy) : (sy < 0 ? min(nyc, cy) : cy);
+    nzc = sz > 0 ? max(nzc, cz) : (sz < 0 ? min(nzc, cz) : cz);
+    if (ex) nxc = hx + sx;
+    else if (ey) nyc = hy + sy;
+    else nzc = hz + sz;
+    if (nxc < 0 || nxc >= nx || nyc < 0 || nyc >= ny || nzc < 0 || nzc >= nz) return false;
+    cx = nxc;
+    cy = nyc;
+    cz = nzc;
+    tnx = walk_boundary(g.lo[0], g.h[0], cx, sx, o.x, ix);
+    tny = walk_boundary(g.lo[1], g.h[1], cy, sy, o.y, iy);
+    tnz = walk_boundary(g.lo[2], g.h[2], cz, sz, o.z, iz);
+    cell = cx + nx * cy + nxy * cz;
+    return true;
+  }
+  const bool ax = (tnx <= tny) && (tnx <= tnz);
+  const bool ay = !ax && (tny <= tnz);
+  const float t_exit = ax ? tnx : (ay ? tny : tnz);
+  const float de = t_exit * dlen;
+  if (h.f >= 0 && h.d2 < de * de * 0.9999f) return false;
+  const int st = ax ? sx : (ay ? sy : sz);
+  if (st == 0) return false;  // zero or NaN direction: the start cell was all
+  if (ax) {
+    cx += sx;
+    if (cx < 0 || cx >= nx) return false;
+    cell += sx;
+    tnx = walk_boundary(g.lo[0], g.h[0], cx, sx, o.x, ix);
+  } else if (ay) {
+    cy += sy;
+    if (cy < 0 || cy >= ny) return false;
+    cell += sy * nx;
+    tny = walk_boundary(g.lo[1], g.h[1], cy, sy, o.y, iy);
+  } else {
+    cz += sz;
+    if (cz < 0 || cz >= nz) return false;
+    cell += sz * nxy;
+    tnz = walk_boundary(g.lo[2], g.h[2], cz, sz, o.z, iz);
+  }
+  return true;
+}
+
+// One ray, walked to its end in a single loop (the same steps as
+// walk_begin + walk_cell). Callers that carry one ray per lane use this form:
+// written as begin + a loop over walk_cell, the one-ray-per-lane kernels ran
+// at half the speed on gfx950; the interleaved (refill) scheduling uses the
+// split form. Both give the same hits (tested against the oracle).
 template <bool STATS>
 __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, uint32_t& n_tests,
                                  uint32_t& n_cells) {

@@ -52,6 +52,8 @@ SIGNATURES = {
     "ffk_ff_fused": (C.c_int, [_vp, C.c_int, _ull, _vp, _vp, _vp, _vp]),
     "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ffk_ff_needs_tally": (C.c_int, [_vp]),
+    "ffk_set_refill": (C.c_int, [_vp, C.c_int]),
+    "ffk_batch_set_refill": (C.c_int, [_vp, C.c_int]),
     "ffk_batch_tally_flags": (C.c_int, [_vp, _i]),
     "ffk_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp]),
     "ffk_batch_destroy": (None, [_vp]),
@@ -317,6 +319,9 @@ class Scene:
     def set_tuning(self, block=0, rays_per_tile=0, hist=-1):
         check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist), "ffk_set_tuning")
 
+    def set_refill(self, lanes):
+        check(lib().ffk_set_refill(self.h, int(lanes)), "ffk_set_refill")
+
     # -- device calls (torch tensors in, torch tensors out) --
     def cast(self, rays, method=CAST_GRID, points=True, stats=None):
         import torch
@@ -405,6 +410,9 @@ class Batch:
 
     def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0):
         check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks), "ffk_batch_set_tuning")
+
+    def set_refill(self, lanes):
+        check(lib().ffk_batch_set_refill(self.h, int(lanes)), "ffk_batch_set_refill")
 
     def tally_flags(self):
         """Scenes that need a uint64 tally buffer in run()."""

@@ -141,6 +141,14 @@ int ffk_normalise(const ffk_scene* s, const unsigned long long* d_tally, float* 
 int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned long long* d_tally,
                  float* d_F, unsigned long long* d_stats, void* stream);
 
+/* Ray scheduling of the fused kernel: 0 = each lane casts one ray at a time
+ * (the wavefront waits for its slowest ray); 1..64 = lanes walk one grid
+ * cell per iteration and a wavefront starts new rays on its idle lanes once
+ * at least `lanes` of its 64 lanes are idle (default 16). Results are
+ * identical either way (integer tallies). */
+int ffk_set_refill(ffk_scene* s, int lanes);
+int ffk_batch_set_refill(ffk_batch* b, int lanes);
+
 /* 1 when ffk_ff_fused needs d_tally for this scene and tuning (the rays of a
  * mesh span several tiles, or the row histogram does not fit in LDS), else 0. */
 int ffk_ff_needs_tally(const ffk_scene* s);

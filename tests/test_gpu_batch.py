@@ -58,8 +58,10 @@ def test_batched_tuning_does_not_change_bits(gpu):
     seeds = [3, 4]
     plain = [sc.ff_fused(s).cpu().numpy() for sc, s in zip(scs, seeds)]
     b = gpu.Batch(scs)
-    for block, rays_per_tile, n_blocks in [(256, 1024, 0), (512, 4096, 64), (1024, 0, 7)]:
+    for block, rays_per_tile, n_blocks, refill in [(256, 1024, 0, 0), (512, 4096, 64, 16), (1024, 0, 7, 0),
+                                                    (512, 0, 0, 32)]:
         b.set_tuning(block, rays_per_tile, n_blocks)
+        b.set_refill(refill)
         Fs = [torch.empty((sc.n_meshes,) * 2, device="cuda") for sc in scs]
         tallies = [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda") for sc in scs]
         b.run(seeds, Fs, tallies=tallies, use_graph=True)

@@ -161,6 +161,24 @@ def test_fused_ff_tiling_and_histograms(gpu, oracle, hist, rays_per_tile):
     assert np.array_equal(bits(F), bits(want))
 
 
+@pytest.mark.parametrize("refill", [1, 16, 48, 64])
+@pytest.mark.parametrize("hist,rays_per_tile", [(0, 0), (2, 1000), (1, 0)])
+def test_fused_ff_refill_bit_exact(gpu, oracle, refill, hist, rays_per_tile):
+    """Lanes that take new rays as soon as enough of them are idle cast the
+    same rays: F is the oracle's bit for bit (global tally, split tiles and
+    direct rows)."""
+    from form_factors_amd import scenes
+
+    faces, meshes = scenes.confetti_scene(150, cells=2, seed=13)
+    meshes = np.ascontiguousarray(meshes)
+    sc = make_scene(gpu, faces, meshes, n_rays=120_000, frame=1)
+    sc.set_tuning(block=256, rays_per_tile=rays_per_tile, hist=hist)
+    sc.set_refill(refill)
+    F = sc.ff_fused(seed=17).cpu().numpy()
+    r, want = oracle.ff_calculate(faces, meshes, 120_000, seed=17, frame=1, n_threads=8)
+    assert np.array_equal(bits(F), bits(want))
+
+
 def test_unfused_pipeline_equals_fused(gpu, oracle):
     from form_factors_amd import scenes
 
