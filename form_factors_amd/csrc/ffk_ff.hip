@@ -108,8 +108,10 @@ __global__ void k_zero_rows(int N, const int* __restrict__ rows, int n_rows,
 // PERSIST: blocks pull tiles from a queue (batched launch) instead of one
 // tile per block (plain launch).
 // Idle lanes (of 64) that trigger a refill in the fused kernel; 0 = off.
-// 16 measured best on incoherent Malley rays (8..24 within 2 %).
 constexpr int kDefaultRefill = 16;
+// Pooled triangle tests run once waiting lanes x kDefaultPool >= active
+// lanes; 0 = each lane tests its own cells.
+constexpr int kDefaultPool = 4;
 
 struct fused_args {
   const scene_desc* descs;
@@ -122,6 +124,7 @@ struct fused_args {
   float* const* Fs;
   unsigned long long* stats;
   int refill;  // 0: one ray per lane at a time; else the idle-lane threshold of the refill loop
+  int pool;    // 0: per-lane tests; else pool the tests once waiting lanes x pool >= active lanes
 };
 
 // REFILL: lanes walk one grid cell per iteration and, once at least
@@ -131,15 +134,18 @@ struct fused_args {
 // slowest ray before the next round, which leaves most lanes idle on
 // incoherent rays. Rays are tallied as integers, so the order in which they
 // are cast does not change F.
-template <int HIST, bool PERSIST, bool STATS, bool REFILL>
+template <int HIST, bool PERSIST, bool STATS, int SCHED>
 __global__ void __launch_bounds__(1024)
     k_ff_fused(const scene_desc* __restrict__ descs, const tile* __restrict__ tiles, int n_tiles,
                int* __restrict__ queue, const unsigned long long* __restrict__ seeds,
                unsigned long long* const* __restrict__ tallies, float* const* __restrict__ Fs,
-               unsigned long long* __restrict__ stats, int refill) {
+               unsigned long long* __restrict__ stats, int refill, int pool) {
   // Read-only, restrict-qualified tables indexed by wave-uniform values are
   // read with scalar loads: the scene descriptor lives in SGPRs.
-  extern __shared__ uint32_t hist[];
+  // 16-byte aligned: the dynamic region follows the static words below, and
+  // the pooled-test best-hit words after the histogram take 64-bit LDS
+  // atomics, which need natural alignment.
+  extern __shared__ __align__(16) uint32_t hist[];
   __shared__ int s_tile;
   __shared__ int s_next;  // next ray of the tile (REFILL)
   uint32_t nt = 0, nc = 0;
@@ -159,8 +165,8 @@ __global__ void __launch_bounds__(1024)
     const int words = HIST == 1 ? N : (N + 1) >> 1;
     if (HIST != 0)
       for (int w = threadIdx.x; w < words; w += blockDim.x) hist[w] = 0u;
-    if (REFILL && threadIdx.x == 0) s_next = 0;
-    if (HIST != 0 || REFILL) __syncthreads();
+    if (SCHED != 0 && threadIdx.x == 0) s_next = 0;
+    if (HIST != 0 || SCHED != 0) __syncthreads();
     const unsigned long long seed = seeds[sc];
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     unsigned long long* tally = tallies ? tallies[sc] : nullptr;
@@ -175,7 +181,125 @@ __global__ void __launch_bounds__(1024)
         }
       }
     };
-    if (REFILL) {
+    if (SCHED == 2) {
+      // As SCHED 1, and the triangle tests of all lanes are pooled: the
+      // (lane, reference) pairs of the wavefront's current cells are spread
+      // over its 64 lanes, and each result is folded into the owning ray's
+      // best hit with a 64-bit LDS atomic minimum on hit_key (the rule of
+      // `consider`), so the lanes test together instead of each walking its
+      // own list.
+      const long long n = T.g1 - T.g0;
+      const int lane = threadIdx.x & 63;
+      unsigned long long* best =
+          reinterpret_cast<unsigned long long*>(hist + ((HIST == 0 ? 0 : (words + 1)) & ~1)) +
+          64 * (threadIdx.x >> 6);
+      bool have = false, more = true, pend = false;
+      int ref0 = 0, nref = 0;
+      v3 o = mk(0.f, 0.f, 0.f), dir = mk(0.f, 0.f, 0.f);
+      float ix = 0.f, iy = 0.f, iz = 0.f, dlen = 0.f, tnx = 0.f, tny = 0.f, tnz = 0.f;
+      int cx = 0, cy = 0, cz = 0, cell = 0;
+      auto decode = [&](unsigned long long key, float& d2, int& f) {
+        f = key == ~0ull ? -1 : (int)(uint32_t)key;
+        d2 = key == ~0ull ? 3.402823466e38f : __uint_as_float((uint32_t)(key >> 32));
+      };
+      auto finish = [&](unsigned long long key) {
+        hit_rec hh;
+        decode(key, hh.d2, hh.f);
+        count_hit(hh);
+      };
+      while (true) {
+        const unsigned long long idle = __ballot(!have);
+        const int n_idle = __popcll(idle);
+        if (more && n_idle >= refill) {
+          int base = 0;
+          if (lane == 0) base = atomicAdd(&s_next, n_idle);
+          base = __shfl(base, 0, 64);
+          if ((long long)base + n_idle >= n) more = false;
+          if (!have) {
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+            const long long off = (long long)base + rank;
+            if (off < n) {
+              const long long g = T.g0 + off;
+              const int f = find_face(S.face_off, S.n_faces, g, hint);
+              hint = f;
+              v3 e;
+              malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+              dir = e - o;
+              best[lane] = ~0ull;
+              pend = false;
+              have = walk_begin(S.g, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell);
+            }
+          }
+        }
+        if (!more && __ballot(have) == 0ull) break;
+        // Advance the lanes that are not waiting for tests by one cell.
+        if (have && !pend) {
+          const int2 ci = S.cell_info[cell];
+          if (STATS) nc += 1;
+          if (ci.y > 0) {
+            pend = true;
+            ref0 = ci.x;
+            nref = ci.y;
+          } else {
+            float bd;
+            int bf;
+            decode(best[lane], bd, bf);
+            if (!walk_leave(S.g, o, dir, ix, iy, iz, dlen, ci.x - 1, cx, cy, cz, tnx, tny, tnz, cell, bd, bf)) {
+              finish(best[lane]);
+              have = false;
+            }
+          }
+        }
+        // Pooled tests once a quarter of the active lanes wait for them (or
+        // nobody can advance any more).
+        const unsigned long long waiting = __ballot(have && pend);
+        const unsigned long long active = __ballot(have);
+        if (waiting != 0ull && __popcll(waiting) * pool >= __popcll(active)) {
+          const int cnt = (have && pend) ? nref : 0;
+          int incl = cnt;
+#pragma unroll
+          for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+          }
+          const int total = __shfl(incl, 63, 64);
+          for (int p0 = 0; p0 < total; p0 += 64) {
+            const int p = p0 + lane;
+            int owner = 0;  // first lane whose inclusive count exceeds p
+#pragma unroll
+            for (int step = 32; step >= 1; step >>= 1) {
+              const int v = __shfl(incl, owner + step - 1, 64);
+              if (v <= p) owner += step;
+            }
+            const int inc_o = __shfl(incl, owner, 64), cnt_o = __shfl(cnt, owner, 64);
+            const int ref_o = __shfl(ref0, owner, 64);
+            const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
+            const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));
+            if (p < total) {
+              const int f = S.refs[ref_o + (p - (inc_o - cnt_o))];
+              const float4* q = reinterpret_cast<const float4*>(S.tri + f);
+              v3 pt;
+              float d2;
+              if (hit_terms(q[0], q[1], q[2], q[3], oo, dd, &pt, &d2) && d2 < 3.402823466e38f)
+                atomicMin(&best[owner], hit_key(d2, f));
+              if (STATS) nt += 1;
+            }
+          }
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          if (have && pend) {
+            pend = false;
+            float bd;
+            int bf;
+            decode(best[lane], bd, bf);
+            if (!walk_leave(S.g, o, dir, ix, iy, iz, dlen, 0, cx, cy, cz, tnx, tny, tnz, cell, bd, bf)) {
+              finish(best[lane]);
+              have = false;
+            }
+          }
+        }
+      }
+    } else if (SCHED == 1) {
       const long long n = T.g1 - T.g0;
       const int lane = threadIdx.x & 63;
       bool have = false, more = true;  // `more` is wave-uniform
@@ -263,15 +387,20 @@ __global__ void __launch_bounds__(1024)
 
 // ---------------------------------------------------------- planning --
 
+// Dynamic LDS a fused launch may use: the block's LDS less the kernel's
+// static words (s_tile, s_next), padded to the 16-byte alignment of the
+// dynamic region.
+constexpr int kStaticLds = 16;
+
 static int max_lds_bytes() {
   static int cached = 0;
   if (!cached) {
     int dev = 0;
     hipDeviceProp_t p;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
-      cached = (int)std::max<size_t>(p.sharedMemPerBlock, 65536);
+      cached = (int)std::max<size_t>(p.sharedMemPerBlock, 65536) - kStaticLds;
     else
-      cached = 65536;
+      cached = 65536 - kStaticLds;
   }
   return cached;
 }
@@ -282,7 +411,7 @@ static int pick_hist(int N, long long max_tile, int requested) {
   if (requested == 0) return 0;
   if (requested == 1 && 4LL * N <= lds) return 1;
   if (requested == 2 && 2LL * ((N + 1) / 2) * 2 <= lds && max_tile <= 65535) return 2;
-  if (4LL * N <= 65536) return 1;
+  if (4LL * N <= std::min(65536, lds)) return 1;
   if (4LL * ((N + 1) / 2) <= lds && max_tile <= 65535) return 2;
   return 0;
 }
@@ -293,21 +422,30 @@ static size_t hist_bytes(int mode, int N) {
   return 0;
 }
 
-template <int H, bool P, bool S, bool R>
+template <int H, bool P, bool S, int R>
 static void set_lds_attr(size_t bytes) {
-  if (bytes > 65536)
+  if (bytes + kStaticLds > 65536)
     (void)hipFuncSetAttribute((const void*)&k_ff_fused<H, P, S, R>,
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-template <int H, bool P, bool S, bool R>
-static void launch_one(dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
-  set_lds_attr<H, P, S, R>(lds);
-  k_ff_fused<H, P, S, R><<<grid, block, H ? lds : 0, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds,
-                                                            A.tallies, A.Fs, A.stats, A.refill);
+// Dynamic LDS of a launch: the row histogram, then (pooled tests) one
+// 64-bit best-hit word per lane.
+static size_t fused_lds(int hist, size_t hist_lds, int sched, int block) {
+  size_t b = hist ? hist_lds : 0;
+  if (sched == 2) b = ((b + 7) & ~(size_t)7) + (size_t)block * 8;
+  return b;
 }
 
-template <bool P, bool S, bool R>
+template <int H, bool P, bool S, int R>
+static void launch_one(dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
+  const size_t bytes = fused_lds(H, lds, R, block);
+  set_lds_attr<H, P, S, R>(bytes);
+  k_ff_fused<H, P, S, R><<<grid, block, bytes, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies,
+                                                     A.Fs, A.stats, A.refill, A.pool);
+}
+
+template <bool P, bool S, int R>
 static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
   switch (hist) {
     case 1: launch_one<1, P, S, R>(grid, block, lds, st, A); break;
@@ -317,10 +455,21 @@ static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStr
   return hipGetLastError();
 }
 
+// Scheduling: 0 = one ray per lane, 1 = lane refill, 2 = lane refill with
+// pooled tests (when the best-hit words fit beside the histogram; the three
+// cast the same rays and tally the same hits).
+static int sched_of(int refill, int pool, int hist, size_t hist_lds, int block) {
+  if (refill <= 0) return 0;
+  if (pool <= 0 || fused_lds(hist, hist_lds, 2, block) > (size_t)max_lds_bytes()) return 1;
+  return 2;
+}
+
 template <bool P, bool S>
 static hipError_t launch_refill(int hist, dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
-  return A.refill > 0 ? launch_mode<P, S, true>(hist, grid, block, lds, st, A)
-                      : launch_mode<P, S, false>(hist, grid, block, lds, st, A);
+  const int sched = sched_of(A.refill, A.pool, hist, lds, block);
+  return sched == 2   ? launch_mode<P, S, 2>(hist, grid, block, lds, st, A)
+         : sched == 1 ? launch_mode<P, S, 1>(hist, grid, block, lds, st, A)
+                      : launch_mode<P, S, 0>(hist, grid, block, lds, st, A);
 }
 
 static hipError_t launch_fused(int hist, bool persist, bool stats, dim3 grid, int block, size_t lds,
@@ -414,6 +563,7 @@ struct ffk_batch {
   long long total_rays = 0;
   int block = 0, rpt = 0, n_blocks = 0;
   int refill = kDefaultRefill;
+  int pool = kDefaultPool;
   // graph state
   hipStream_t cap = nullptr;  // capture stream (default priority)
   hipGraph_t graph = nullptr;
@@ -616,17 +766,21 @@ int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mod
   return FFK_OK;
 }
 
-int ffk_set_refill(ffk_scene* s, int lanes) {
+int ffk_set_refill(ffk_scene* s, int lanes, int pool) {
   FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
   FFK_REQUIRE(lanes >= 0 && lanes <= 64, FFK_ERR_ARG, "refill lanes %d outside [0, 64]", lanes);
+  FFK_REQUIRE(pool >= 0 && pool <= 64, FFK_ERR_ARG, "pool ratio %d outside [0, 64]", pool);
   s->tune_refill = lanes;
+  s->tune_pool = pool;
   return FFK_OK;
 }
 
-int ffk_batch_set_refill(ffk_batch* b, int lanes) {
+int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool) {
   FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
   FFK_REQUIRE(lanes >= 0 && lanes <= 64, FFK_ERR_ARG, "refill lanes %d outside [0, 64]", lanes);
+  FFK_REQUIRE(pool >= 0 && pool <= 64, FFK_ERR_ARG, "pool ratio %d outside [0, 64]", pool);
   b->refill = lanes;
+  b->pool = pool;
   free_graph(b);
   return FFK_OK;
 }
@@ -686,6 +840,7 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
   A.Fs = reinterpret_cast<float* const*>(d_misc + 2);
   A.stats = d_stats;
   A.refill = s->tune_refill;
+  A.pool = s->tune_pool;
   const int block = s->tune_block > 0 ? s->tune_block : default_block(hist, hist_bytes(hist, N));
   if (s->n_tiles > 0)
     FFK_HIP(launch_fused(hist, false, d_stats != nullptr, dim3((unsigned)s->n_tiles), block,
@@ -768,17 +923,19 @@ int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_
 
 }  // extern "C"
 
-template <bool R>
+template <int R>
 static hipError_t persistent_occupancy(int hist, int block, size_t lds, int* per_cu) {
+  const size_t bytes = fused_lds(hist, lds, R, block);
   switch (hist) {
     case 1:
-      set_lds_attr<1, true, false, R>(lds);
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<1, true, false, R>, block, lds);
+      set_lds_attr<1, true, false, R>(bytes);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<1, true, false, R>, block, bytes);
     case 2:
-      set_lds_attr<2, true, false, R>(lds);
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<2, true, false, R>, block, lds);
+      set_lds_attr<2, true, false, R>(bytes);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<2, true, false, R>, block, bytes);
     default:
-      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<0, true, false, R>, block, 0);
+      set_lds_attr<0, true, false, R>(bytes);
+      return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, k_ff_fused<0, true, false, R>, block, bytes);
   }
 }
 
@@ -790,8 +947,10 @@ static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int
   int n_blocks = b->n_blocks;
   if (n_blocks <= 0) {
     int per_cu = 0;
-    const hipError_t e = b->refill > 0 ? persistent_occupancy<true>(hist, block, lds, &per_cu)
-                                       : persistent_occupancy<false>(hist, block, lds, &per_cu);
+    const int sched = sched_of(b->refill, b->pool, hist, lds, block);
+    const hipError_t e = sched == 2   ? persistent_occupancy<2>(hist, block, lds, &per_cu)
+                         : sched == 1 ? persistent_occupancy<1>(hist, block, lds, &per_cu)
+                                      : persistent_occupancy<0>(hist, block, lds, &per_cu);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
     n_blocks = per_cu * device_cu_count();
   }
@@ -823,6 +982,7 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
     A.Fs = b->d_Fs;
     A.stats = d_stats;
     A.refill = b->refill;
+    A.pool = b->pool;
     const int n_blocks = batch_blocks(b, k.hist, k.lds, block, k.n);
     FFK_HIP(launch_fused(k.hist, true, stats, dim3((unsigned)n_blocks), block, k.lds, st, A));
   }

@@ -203,6 +203,81 @@ __device__ inline bool walk_cell(const scene_desc& S, v3 o, v3 dir, float ix, fl
   return true;
 }
 
+// Leaves the current cell after its tests (the second half of walk_cell):
+// past the empty box of radius R (R >= 1) or to the next cell of the DDA
+// (R = 0). Returns false when the walk is over.
+__device__ inline bool walk_leave(const grid_desc& g, v3 o, v3 dir, float ix, float iy, float iz, float dlen, int R,
+                                  int& cx, int& cy, int& cz, float& tnx, float& tny, float& tnz, int& cell,
+                                  float best_d2, int best_f) {
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int nxy = nx * ny;
+  const int sx = walk_sign(dir.x), sy = walk_sign(dir.y), sz = walk_sign(dir.z);
+  if (R >= 1) {
+    const int hx = sx > 0 ? min(cx + R, nx - 1) : max(cx - R, 0);
+    const int hy = sy > 0 ? min(cy + R, ny - 1) : max(cy - R, 0);
+    const int hz = sz > 0 ? min(cz + R, nz - 1) : max(cz - R, 0);
+    const float bx = walk_boundary(g.lo[0], g.h[0], hx, sx, o.x, ix);
+    const float by = walk_boundary(g.lo[1], g.h[1], hy, sy, o.y, iy);
+    const float bz = walk_boundary(g.lo[2], g.h[2], hz, sz, o.z, iz);
+    const bool ex = (bx <= by) && (bx <= bz);
+    const bool ey = !ex && (by <= bz);
+    const float tb = ex ? bx : (ey ? by : bz);
+    if (!(tb < 3.4e38f)) return false;
+    const float de = tb * dlen;
+    if (best_f >= 0 && best_d2 < de * de * 0.9999f) return false;
+    const float px = o.x + dir.x * tb, py = o.y + dir.y * tb, pz = o.z + dir.z * tb;
+    int nxc = min(max((int)floorf((px - g.lo[0]) * g.inv_h[0]), max(cx - R, 0)), min(cx + R, nx - 1));
+    int nyc = min(max((int)floorf((py - g.lo[1]) * g.inv_h[1]), max(cy - R, 0)), min(cy + R, ny - 1));
+    int nzc = min(max((int)floorf((pz - g.lo[2]) * g.inv_h[2]), max(cz - R, 0)), min(cz + R, nz - 1));
+    nxc = sx > 0 ? max(nxc, cx) : (sx < 0 ? min(nxc, cx) : cx);
+    nyc = sy > 0 ? max(nyc, cy) : (sy < 0 ? min(nyc, cy) : cy);
+    nzc = sz > 0 ? max(nzc, cz) : (sz < 0 ? min(nzc, cz) : cz);
+    if (ex) nxc = hx + sx;
+    else if (ey) nyc = hy + sy;
+    else nzc = hz + sz;
+    if (nxc < 0 || nxc >= nx || nyc < 0 || nyc >= ny || nzc < 0 || nzc >= nz) return false;
+    cx = nxc;
+    cy = nyc;
+    cz = nzc;
+    tnx = walk_boundary(g.lo[0], g.h[0], cx, sx, o.x, ix);
+    tny = walk_boundary(g.lo[1], g.h[1], cy, sy, o.y, iy);
+    tnz = walk_boundary(g.lo[2], g.h[2], cz, sz, o.z, iz);
+    cell = cx + nx * cy + nxy * cz;
+    return true;
+  }
+  const bool ax = (tnx <= tny) && (tnx <= tnz);
+  const bool ay = !ax && (tny <= tnz);
+  const float t_exit = ax ? tnx : (ay ? tny : tnz);
+  const float de = t_exit * dlen;
+  if (best_f >= 0 && best_d2 < de * de * 0.9999f) return false;
+  const int st = ax ? sx : (ay ? sy : sz);
+  if (st == 0) return false;
+  if (ax) {
+    cx += sx;
+    if (cx < 0 || cx >= nx) return false;
+    cell += sx;
+    tnx = walk_boundary(g.lo[0], g.h[0], cx, sx, o.x, ix);
+  } else if (ay) {
+    cy += sy;
+    if (cy < 0 || cy >= ny) return false;
+    cell += sy * nx;
+    tny = walk_boundary(g.lo[1], g.h[1], cy, sy, o.y, iy);
+  } else {
+    cz += sz;
+    if (cz < 0 || cz >= nz) return false;
+    cell += sz * nxy;
+    tnz = walk_boundary(g.lo[2], g.h[2], cz, sz, o.z, iz);
+  }
+  return true;
+}
+
+// The nearest-hit rule of `consider` as one 64-bit key: the squared distance's
+// bits (non-negative floats order as unsigned integers) above the face index,
+// so the smallest key is the nearest hit with ties to the lower face.
+__device__ inline unsigned long long hit_key(float d2, int f) {
+  return ((unsigned long long)__float_as_uint(d2) << 32) | (uint32_t)f;
+}
+
 // One ray, walked to its end in a single loop (the same steps as
 // walk_begin + walk_cell). Callers that carry one ray per lane use this form:
 // written as begin + a loop over walk_cell, the one-ray-per-lane kernels ran

@@ -52,8 +52,8 @@ SIGNATURES = {
     "ffk_ff_fused": (C.c_int, [_vp, C.c_int, _ull, _vp, _vp, _vp, _vp]),
     "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ffk_ff_needs_tally": (C.c_int, [_vp]),
-    "ffk_set_refill": (C.c_int, [_vp, C.c_int]),
-    "ffk_batch_set_refill": (C.c_int, [_vp, C.c_int]),
+    "ffk_set_refill": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ffk_batch_set_refill": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ffk_batch_tally_flags": (C.c_int, [_vp, _i]),
     "ffk_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp]),
     "ffk_batch_destroy": (None, [_vp]),
@@ -319,8 +319,10 @@ class Scene:
     def set_tuning(self, block=0, rays_per_tile=0, hist=-1):
         check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist), "ffk_set_tuning")
 
-    def set_refill(self, lanes):
-        check(lib().ffk_set_refill(self.h, int(lanes)), "ffk_set_refill")
+    def set_refill(self, lanes, pool=4):
+        """Fused-kernel ray scheduling: refill threshold (idle lanes of 64,
+        0 = one ray per lane) and pooled-test ratio (0 = per-lane tests)."""
+        check(lib().ffk_set_refill(self.h, int(lanes), int(pool)), "ffk_set_refill")
 
     # -- device calls (torch tensors in, torch tensors out) --
     def cast(self, rays, method=CAST_GRID, points=True, stats=None):
@@ -411,8 +413,8 @@ class Batch:
     def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0):
         check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks), "ffk_batch_set_tuning")
 
-    def set_refill(self, lanes):
-        check(lib().ffk_batch_set_refill(self.h, int(lanes)), "ffk_batch_set_refill")
+    def set_refill(self, lanes, pool=4):
+        check(lib().ffk_batch_set_refill(self.h, int(lanes), int(pool)), "ffk_batch_set_refill")
 
     def tally_flags(self):
         """Scenes that need a uint64 tally buffer in run()."""

@@ -141,13 +141,17 @@ int ffk_normalise(const ffk_scene* s, const unsigned long long* d_tally, float* 
 int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned long long* d_tally,
                  float* d_F, unsigned long long* d_stats, void* stream);
 
-/* Ray scheduling of the fused kernel: 0 = each lane casts one ray at a time
- * (the wavefront waits for its slowest ray); 1..64 = lanes walk one grid
- * cell per iteration and a wavefront starts new rays on its idle lanes once
- * at least `lanes` of its 64 lanes are idle (default 16). Results are
- * identical either way (integer tallies). */
-int ffk_set_refill(ffk_scene* s, int lanes);
-int ffk_batch_set_refill(ffk_batch* b, int lanes);
+/* Ray scheduling of the fused kernel. lanes: 0 = each lane casts one ray at
+ * a time (the wavefront waits for its slowest ray); 1..64 = lanes walk one
+ * grid cell per iteration and a wavefront starts new rays on its idle lanes
+ * once at least `lanes` of its 64 lanes are idle (default 16). pool (with
+ * lanes > 0): 0 = each lane tests its own cells' triangles; 1..64 = lanes
+ * that reach an occupied cell wait, and once waiting lanes x pool >= active
+ * lanes the wavefront spreads all their (ray, triangle) tests over its 64
+ * lanes (default 4). Results are identical in every setting (integer
+ * tallies, the same nearest-hit rule). */
+int ffk_set_refill(ffk_scene* s, int lanes, int pool);
+int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool);
 
 /* 1 when ffk_ff_fused needs d_tally for this scene and tuning (the rays of a
  * mesh span several tiles, or the row histogram does not fit in LDS), else 0. */

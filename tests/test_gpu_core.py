@@ -161,10 +161,12 @@ def test_fused_ff_tiling_and_histograms(gpu, oracle, hist, rays_per_tile):
     assert np.array_equal(bits(F), bits(want))
 
 
-@pytest.mark.parametrize("refill", [1, 16, 48, 64])
+@pytest.mark.parametrize("refill,pool", [(1, 0), (16, 0), (48, 0), (64, 0), (1, 1), (16, 4), (16, 1), (64, 64),
+                                         (8, 2)])
 @pytest.mark.parametrize("hist,rays_per_tile", [(0, 0), (2, 1000), (1, 0)])
-def test_fused_ff_refill_bit_exact(gpu, oracle, refill, hist, rays_per_tile):
-    """Lanes that take new rays as soon as enough of them are idle cast the
+def test_fused_ff_refill_bit_exact(gpu, oracle, refill, pool, hist, rays_per_tile):
+    """Lanes that take new rays as soon as enough of them are idle, with the
+    triangle tests of waiting lanes pooled over the wavefront or not, cast the
     same rays: F is the oracle's bit for bit (global tally, split tiles and
     direct rows)."""
     from form_factors_amd import scenes
@@ -173,7 +175,7 @@ def test_fused_ff_refill_bit_exact(gpu, oracle, refill, hist, rays_per_tile):
     meshes = np.ascontiguousarray(meshes)
     sc = make_scene(gpu, faces, meshes, n_rays=120_000, frame=1)
     sc.set_tuning(block=256, rays_per_tile=rays_per_tile, hist=hist)
-    sc.set_refill(refill)
+    sc.set_refill(refill, pool)
     F = sc.ff_fused(seed=17).cpu().numpy()
     r, want = oracle.ff_calculate(faces, meshes, 120_000, seed=17, frame=1, n_threads=8)
     assert np.array_equal(bits(F), bits(want))
