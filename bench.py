@@ -198,7 +198,7 @@ def confetti_leg(ffk, scenes, steps):
     t_default = run(steps)
     sum_default = int(F.view(torch.int32).sum(dtype=torch.int64))
     t0 = time.perf_counter()
-    tuned = sc.autotune(sample_rays=1 << 21)
+    tuned = sc.autotune(sample_rays=1 << 22)
     t_tune = time.perf_counter() - t0
     t_tuned = run(steps)
     same = int(F.view(torch.int32).sum(dtype=torch.int64)) == sum_default

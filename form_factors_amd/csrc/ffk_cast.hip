@@ -1,4 +1,6 @@
 // Nearest-hit casting of a device ray array (ray_caster::system_cast).
+#include <algorithm>
+
 #include "ffk_trace.hpp"
 
 namespace ffk {
@@ -46,6 +48,39 @@ __global__ void __launch_bounds__(256) k_cast_grid(scene_desc S, const float* __
     cast_grid<STATS>(S, o, dir, h, nt, nc);
     store_hit(h, i, idx, pts);
   }
+  if (STATS) wave_add_stats(stats, nt, nc);
+}
+
+// The same cast with lane refill and pooled triangle tests (pooled_walk):
+// block b casts rays [b * chunk, (b + 1) * chunk). The hit point of the
+// winning face is recomputed with the same test, so points are those of
+// k_cast_grid bit for bit.
+template <bool STATS>
+__global__ void __launch_bounds__(256) k_cast_pooled(scene_desc S, const float* __restrict__ rays, long long n,
+                                                     long long chunk, int* __restrict__ idx,
+                                                     float* __restrict__ pts, unsigned long long* __restrict__ stats,
+                                                     int refill, int pool) {
+  __shared__ unsigned long long best[256];
+  __shared__ int s_next;
+  const long long r0 = (long long)blockIdx.x * chunk;
+  const long long cnt = min(chunk, n - r0);
+  if (threadIdx.x == 0) s_next = 0;
+  __syncthreads();
+  uint32_t nt = 0, nc = 0;
+  pooled_walk<STATS>(
+      S, best + 64 * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
+      [&](long long i, v3& o, v3& dir) { load_ray(rays, r0 + i, &o, &dir); },
+      [&](long long i, v3 o, v3 dir, unsigned long long key) {
+        hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
+        decode_key(key, h.d2, h.f);
+        if (h.f >= 0 && pts) {
+          const float4* q = reinterpret_cast<const float4*>(S.tri + h.f);
+          float d2;
+          (void)hit_terms(q[0], q[1], q[2], q[3], o, dir, &h.p, &d2);
+        }
+        store_hit(h, r0 + i, idx, pts);
+      },
+      nt, nc);
   if (STATS) wave_add_stats(stats, nt, nc);
 }
 
@@ -120,11 +155,24 @@ extern "C" int ffk_cast(ffk_scene* s, int method, const float* d_rays, long long
   hipStream_t st = as_stream(stream);
   if (method == FFK_CAST_GRID) {
     FFK_REQUIRE(s->has_grid, FFK_ERR_ARG, "ffk_cast: grid not built");
-    const unsigned blocks = (unsigned)((n_rays + 255) / 256);
-    if (d_stats)
-      k_cast_grid<true><<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, d_hit_idx, d_hit_pts, d_stats);
-    else
-      k_cast_grid<false><<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, d_hit_idx, d_hit_pts, nullptr);
+    if (s->tune_refill > 0 && s->tune_pool > 0) {
+      // About eight blocks per CU, at least a ray per lane and at most 64.
+      const long long target = 8LL * ffk::device_cu_count();
+      const long long chunk = std::min<long long>(16384, std::max<long long>(256, (n_rays + target - 1) / target));
+      const unsigned blocks = (unsigned)((n_rays + chunk - 1) / chunk);
+      if (d_stats)
+        k_cast_pooled<true><<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, chunk, d_hit_idx, d_hit_pts, d_stats,
+                                                    s->tune_refill, s->tune_pool);
+      else
+        k_cast_pooled<false><<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, chunk, d_hit_idx, d_hit_pts,
+                                                     nullptr, s->tune_refill, s->tune_pool);
+    } else {
+      const unsigned blocks = (unsigned)((n_rays + 255) / 256);
+      if (d_stats)
+        k_cast_grid<true><<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, d_hit_idx, d_hit_pts, d_stats);
+      else
+        k_cast_grid<false><<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, d_hit_idx, d_hit_pts, nullptr);
+    }
   } else if (method == FFK_CAST_NAIVE) {
     // Few rays: a wavefront per ray keeps every CU busy on the face loop.
     if (n_rays * 64 <= 4LL * ffk::device_cu_count() * 1024) {

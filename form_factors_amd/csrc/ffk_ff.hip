@@ -182,123 +182,27 @@ __global__ void __launch_bounds__(1024)
       }
     };
     if (SCHED == 2) {
-      // As SCHED 1, and the triangle tests of all lanes are pooled: the
-      // (lane, reference) pairs of the wavefront's current cells are spread
-      // over its 64 lanes, and each result is folded into the owning ray's
-      // best hit with a 64-bit LDS atomic minimum on hit_key (the rule of
-      // `consider`), so the lanes test together instead of each walking its
-      // own list.
-      const long long n = T.g1 - T.g0;
-      const int lane = threadIdx.x & 63;
+      // As SCHED 1, with the triangle tests of the wavefront's waiting lanes
+      // pooled over its 64 lanes (pooled_walk).
       unsigned long long* best =
           reinterpret_cast<unsigned long long*>(hist + ((HIST == 0 ? 0 : (words + 1)) & ~1)) +
           64 * (threadIdx.x >> 6);
-      bool have = false, more = true, pend = false;
-      int ref0 = 0, nref = 0;
-      v3 o = mk(0.f, 0.f, 0.f), dir = mk(0.f, 0.f, 0.f);
-      float ix = 0.f, iy = 0.f, iz = 0.f, dlen = 0.f, tnx = 0.f, tny = 0.f, tnz = 0.f;
-      int cx = 0, cy = 0, cz = 0, cell = 0;
-      auto decode = [&](unsigned long long key, float& d2, int& f) {
-        f = key == ~0ull ? -1 : (int)(uint32_t)key;
-        d2 = key == ~0ull ? 3.402823466e38f : __uint_as_float((uint32_t)(key >> 32));
-      };
-      auto finish = [&](unsigned long long key) {
-        hit_rec hh;
-        decode(key, hh.d2, hh.f);
-        count_hit(hh);
-      };
-      while (true) {
-        const unsigned long long idle = __ballot(!have);
-        const int n_idle = __popcll(idle);
-        if (more && n_idle >= refill) {
-          int base = 0;
-          if (lane == 0) base = atomicAdd(&s_next, n_idle);
-          base = __shfl(base, 0, 64);
-          if ((long long)base + n_idle >= n) more = false;
-          if (!have) {
-            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-            const long long off = (long long)base + rank;
-            if (off < n) {
-              const long long g = T.g0 + off;
-              const int f = find_face(S.face_off, S.n_faces, g, hint);
-              hint = f;
-              v3 e;
-              malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
-              dir = e - o;
-              best[lane] = ~0ull;
-              pend = false;
-              have = walk_begin(S.g, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell);
-            }
-          }
-        }
-        if (!more && __ballot(have) == 0ull) break;
-        // Advance the lanes that are not waiting for tests by one cell.
-        if (have && !pend) {
-          const int2 ci = S.cell_info[cell];
-          if (STATS) nc += 1;
-          if (ci.y > 0) {
-            pend = true;
-            ref0 = ci.x;
-            nref = ci.y;
-          } else {
-            float bd;
-            int bf;
-            decode(best[lane], bd, bf);
-            if (!walk_leave(S.g, o, dir, ix, iy, iz, dlen, ci.x - 1, cx, cy, cz, tnx, tny, tnz, cell, bd, bf)) {
-              finish(best[lane]);
-              have = false;
-            }
-          }
-        }
-        // Pooled tests once a quarter of the active lanes wait for them (or
-        // nobody can advance any more).
-        const unsigned long long waiting = __ballot(have && pend);
-        const unsigned long long active = __ballot(have);
-        if (waiting != 0ull && __popcll(waiting) * pool >= __popcll(active)) {
-          const int cnt = (have && pend) ? nref : 0;
-          int incl = cnt;
-#pragma unroll
-          for (int d = 1; d < 64; d <<= 1) {
-            const int v = __shfl_up(incl, d, 64);
-            if (lane >= d) incl += v;
-          }
-          const int total = __shfl(incl, 63, 64);
-          for (int p0 = 0; p0 < total; p0 += 64) {
-            const int p = p0 + lane;
-            int owner = 0;  // first lane whose inclusive count exceeds p
-#pragma unroll
-            for (int step = 32; step >= 1; step >>= 1) {
-              const int v = __shfl(incl, owner + step - 1, 64);
-              if (v <= p) owner += step;
-            }
-            const int inc_o = __shfl(incl, owner, 64), cnt_o = __shfl(cnt, owner, 64);
-            const int ref_o = __shfl(ref0, owner, 64);
-            const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
-            const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));
-            if (p < total) {
-              const int f = S.refs[ref_o + (p - (inc_o - cnt_o))];
-              const float4* q = reinterpret_cast<const float4*>(S.tri + f);
-              v3 pt;
-              float d2;
-              if (hit_terms(q[0], q[1], q[2], q[3], oo, dd, &pt, &d2) && d2 < 3.402823466e38f)
-                atomicMin(&best[owner], hit_key(d2, f));
-              if (STATS) nt += 1;
-            }
-          }
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          if (have && pend) {
-            pend = false;
-            float bd;
-            int bf;
-            decode(best[lane], bd, bf);
-            if (!walk_leave(S.g, o, dir, ix, iy, iz, dlen, 0, cx, cy, cz, tnx, tny, tnz, cell, bd, bf)) {
-              finish(best[lane]);
-              have = false;
-            }
-          }
-        }
-      }
+      pooled_walk<STATS>(
+          S, best, &s_next, T.g1 - T.g0, refill, pool,
+          [&](long long i, v3& o, v3& dir) {
+            const long long g = T.g0 + i;
+            const int f = find_face(S.face_off, S.n_faces, g, hint);
+            hint = f;
+            v3 e;
+            malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+            dir = e - o;
+          },
+          [&](long long, v3, v3, unsigned long long key) {
+            hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
+            decode_key(key, h.d2, h.f);
+            count_hit(h);
+          },
+          nt, nc);
     } else if (SCHED == 1) {
       const long long n = T.g1 - T.g0;
       const int lane = threadIdx.x & 63;

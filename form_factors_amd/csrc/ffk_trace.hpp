@@ -278,6 +278,129 @@ __device__ inline unsigned long long hit_key(float d2, int f) {
   return ((unsigned long long)__float_as_uint(d2) << 32) | (uint32_t)f;
 }
 
+constexpr unsigned long long kNoHit = ~0ull;  // above every hit_key
+
+__device__ inline void decode_key(unsigned long long key, float& d2, int& f) {
+  f = key == kNoHit ? -1 : (int)(uint32_t)key;
+  d2 = key == kNoHit ? 3.402823466e38f : __uint_as_float((uint32_t)(key >> 32));
+}
+
+// Casts rays 0..n-1 of a work list with a wavefront's 64 lanes working as a
+// pool. Each lane walks its own ray one grid cell per iteration; a lane that
+// reaches an occupied cell waits, and once waiting lanes x `pool` >= active
+// lanes the wavefront spreads the (ray, reference) pairs of all waiting
+// lanes over its 64 lanes: each lane tests one pair and folds the result
+// into the owning ray's best hit with a 64-bit LDS atomic minimum on hit_key
+// (the rule of `consider`). Once at least `refill` lanes are idle, they take
+// the next rays of the list (one LDS atomic on *next per refill). The hits
+// are those of cast_grid, whatever the order.
+//   best:  this wavefront's 64 words of LDS (8-byte aligned)
+//   next:  the list's LDS ray counter, zero at the start, shared by the
+//          block's wavefronts
+//   start(i, o, dir):            sets up ray i
+//   finish(i, o, dir, key):      ray i is done; key = hit_key of its hit or kNoHit
+// Every lane of the wavefront must call this together.
+template <bool STATS, class Start, class Finish>
+__device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best, int* next, long long n, int refill,
+                                   int pool, Start start, Finish finish, uint32_t& n_tests, uint32_t& n_cells) {
+  const int lane = threadIdx.x & 63;
+  bool have = false, more = true, pend = false;  // `more` is wave-uniform
+  long long cur = 0;
+  int ref0 = 0, nref = 0;
+  v3 o = mk(0.f, 0.f, 0.f), dir = mk(0.f, 0.f, 0.f);
+  float ix = 0.f, iy = 0.f, iz = 0.f, dlen = 0.f, tnx = 0.f, tny = 0.f, tnz = 0.f;
+  int cx = 0, cy = 0, cz = 0, cell = 0;
+  while (true) {
+    const unsigned long long idle = __ballot(!have);
+    const int n_idle = __popcll(idle);
+    if (more && n_idle >= refill) {
+      int base = 0;
+      if (lane == 0) base = atomicAdd(next, n_idle);
+      base = __shfl(base, 0, 64);
+      if ((long long)base + n_idle >= n) more = false;
+      if (!have) {
+        const int rank =
+            (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+        const long long i = (long long)base + rank;
+        if (i < n) {
+          cur = i;
+          start(i, o, dir);
+          best[lane] = kNoHit;
+          pend = false;
+          have = walk_begin(S.g, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell);
+          if (!have) finish(i, o, dir, kNoHit);
+        }
+      }
+    }
+    if (!more && __ballot(have) == 0ull) break;
+    // Lanes that are not waiting for tests advance by one cell.
+    if (have && !pend) {
+      const int2 ci = S.cell_info[cell];
+      if (STATS) n_cells += 1;
+      if (ci.y > 0) {
+        pend = true;
+        ref0 = ci.x;
+        nref = ci.y;
+      } else {
+        const unsigned long long key = best[lane];
+        float bd;
+        int bf;
+        decode_key(key, bd, bf);
+        if (!walk_leave(S.g, o, dir, ix, iy, iz, dlen, ci.x - 1, cx, cy, cz, tnx, tny, tnz, cell, bd, bf)) {
+          finish(cur, o, dir, key);
+          have = false;
+        }
+      }
+    }
+    const unsigned long long waiting = __ballot(have && pend);
+    const unsigned long long active = __ballot(have);
+    if (waiting != 0ull && __popcll(waiting) * pool >= __popcll(active)) {
+      const int cnt = (have && pend) ? nref : 0;
+      int incl = cnt;  // inclusive prefix sum of the waiting lanes' reference counts
+#pragma unroll
+      for (int d = 1; d < 64; d <<= 1) {
+        const int v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+      }
+      const int total = __shfl(incl, 63, 64);
+      for (int p0 = 0; p0 < total; p0 += 64) {
+        const int p = p0 + lane;
+        int owner = 0;  // first lane whose inclusive count exceeds p
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+          const int v = __shfl(incl, owner + step - 1, 64);
+          if (v <= p) owner += step;
+        }
+        const int inc_o = __shfl(incl, owner, 64), cnt_o = __shfl(cnt, owner, 64);
+        const int ref_o = __shfl(ref0, owner, 64);
+        const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
+        const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));
+        if (p < total) {
+          const int f = S.refs[ref_o + (p - (inc_o - cnt_o))];
+          const float4* q = reinterpret_cast<const float4*>(S.tri + f);
+          v3 pt;
+          float d2;
+          if (hit_terms(q[0], q[1], q[2], q[3], oo, dd, &pt, &d2) && d2 < 3.402823466e38f)
+            atomicMin(&best[owner], hit_key(d2, f));
+          if (STATS) n_tests += 1;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (have && pend) {
+        pend = false;
+        const unsigned long long key = best[lane];
+        float bd;
+        int bf;
+        decode_key(key, bd, bf);
+        if (!walk_leave(S.g, o, dir, ix, iy, iz, dlen, 0, cx, cy, cz, tnx, tny, tnz, cell, bd, bf)) {
+          finish(cur, o, dir, key);
+          have = false;
+        }
+      }
+    }
+  }
+}
+
 // One ray, walked to its end in a single loop (the same steps as
 // walk_begin + walk_cell). Callers that carry one ray per lane use this form:
 // written as begin + a loop over walk_cell, the one-ray-per-lane kernels ran

@@ -5,12 +5,15 @@
 // (triangle sizes, empty space, ray coherence). The tuner casts a fixed
 // sample of the scene's own Malley rays (the emission layout scaled down to
 // about `sample_rays`, same faces and directions) over a ladder of
-// resolutions, keeps the fastest, then times the fused kernel's block size
-// on the full layout when a run is short. The choice is cached per scene
+// resolutions with the scene's ray scheduling. When a full run is short
+// (estimated from the sample) and F fits, the three fastest resolutions are
+// then timed with the fused kernel itself on the full layout, and the
+// fused kernel's block size after that. The choice is cached per scene
 // signature (faces, meshes, ray counts, frame) for the life of the process.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -122,8 +125,12 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
     }
     r = ffk_emit_malley(s, 0x7E57ull, d_rays, stream);
     // 2. grid resolution ladder (cells per face), casting the sample
-    int best_dims[3] = {0, 0, 0};
-    float best = 3.0e38f;
+    struct trial {
+      float ms;
+      double cpf;
+      int dims[3];
+    };
+    std::vector<trial> trials;
     if (r == FFK_OK) r = ffk_scene_build_grid(s, 0, 0, 0, stream);
     float lo[3], hi[3];
     int d0[3];
@@ -133,18 +140,15 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
     int prev[3] = {-1, -1, -1};
     for (double cpf : ladder) {
       if (r != FFK_OK) break;
-      int dims[3];
-      dims_for(lo, hi, cpf, nf, dims);
-      if (std::equal(dims, dims + 3, prev)) continue;
-      std::copy(dims, dims + 3, prev);
-      if ((r = ffk_scene_build_grid(s, dims[0], dims[1], dims[2], stream)) < 0) break;
-      float ms = 0.f;
+      trial tr;
+      tr.cpf = cpf;
+      dims_for(lo, hi, cpf, nf, tr.dims);
+      if (std::equal(tr.dims, tr.dims + 3, prev)) continue;
+      std::copy(tr.dims, tr.dims + 3, prev);
+      if ((r = ffk_scene_build_grid(s, tr.dims[0], tr.dims[1], tr.dims[2], stream)) < 0) break;
       r = time_ms(st, 2, [&] { return ffk_cast(s, FFK_CAST_GRID, d_rays, n_sample, d_idx, nullptr, nullptr, stream); },
-                  &ms);
-      if (r == FFK_OK && ms < best) {
-        best = ms;
-        std::copy(dims, dims + 3, best_dims);
-      }
+                  &tr.ms);
+      if (r == FFK_OK) trials.push_back(tr);
     }
     (void)hipStreamSynchronize(st);
     (void)hipFree(d_rays);
@@ -152,19 +156,54 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
     const int r2 = ffk_scene_set_emission(s, front.data(), rear.data(), frame, nullptr, stream);
     if (r == FFK_OK) r = r2;
     if (r < 0) return r;
-    std::copy(best_dims, best_dims + 3, t.dims);
-    if ((r = ffk_scene_build_grid(s, t.dims[0], t.dims[1], t.dims[2], stream)) < 0) return r;
-    // 3. block size of the fused kernel on the full layout, when a run is short
-    // (estimated from the sample) and F is small enough to allocate here
+    if (trials.empty()) return FFK_ERR_ARG;
+    std::stable_sort(trials.begin(), trials.end(), [](const trial& a, const trial& b) { return a.ms < b.ms; });
+    std::copy(trials[0].dims, trials[0].dims + 3, t.dims);
+    // 3. the fused kernel on the full layout, when a run is short (estimated
+    // from the sample) and F (plus the tally, if the scene needs one) takes
+    // under half of the free device memory: the two fastest resolutions and
+    // the resolutions a factor sqrt(2) in cells either side of the fastest,
+    // then the block size
     t.block = 0;
-    const double est_ms = best * (double)n_rays / (double)std::max(1LL, n_sample);
+    const double est_ms = trials[0].ms * (double)n_rays / (double)std::max(1LL, n_sample);
     const size_t nn = (size_t)nm * nm;
-    if (est_ms < 250.0 && nn * 12 < (size_t(1) << 31)) {
+    const bool need_tally = ffk_ff_needs_tally(s) != 0;
+    const size_t scratch = 4 * nn + (need_tally ? 8 * nn : 0);
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    bool fused_timed = false;
+    if (est_ms < 250.0 && scratch <= free_b / 2) {
       float* d_F = nullptr;
       unsigned long long* d_tally = nullptr;
-      if (hipMalloc(&d_F, 4 * nn) == hipSuccess && hipMalloc(&d_tally, 8 * nn) == hipSuccess) {
+      if (hipMalloc(&d_F, 4 * nn) == hipSuccess && (!need_tally || hipMalloc(&d_tally, 8 * nn) == hipSuccess)) {
+        fused_timed = true;
+        std::vector<std::array<int, 3>> cand;
+        auto add = [&](const int* d) {
+          std::array<int, 3> a = {d[0], d[1], d[2]};
+          if (std::find(cand.begin(), cand.end(), a) == cand.end()) cand.push_back(a);
+        };
+        add(trials[0].dims);
+        if (trials.size() > 1) add(trials[1].dims);
+        for (double f : {std::sqrt(2.0), std::sqrt(0.5)}) {
+          int d[3];
+          dims_for(lo, hi, trials[0].cpf * f, nf, d);
+          add(d);
+        }
+        float best_g = 3.0e38f;
+        for (size_t k = 0; k < cand.size() && r == FFK_OK; ++k) {
+          const int* dims = cand[k].data();
+          if ((r = ffk_scene_build_grid(s, dims[0], dims[1], dims[2], stream)) < 0) break;
+          float ms = 0.f;
+          r = time_ms(st, 2, [&] { return ffk_ff_fused(s, FFK_CAST_GRID, 1, d_tally, d_F, nullptr, stream); }, &ms);
+          if (r == FFK_OK && ms < best_g) {
+            best_g = ms;
+            std::copy(dims, dims + 3, t.dims);
+          }
+        }
+        if (r == FFK_OK) r = ffk_scene_build_grid(s, t.dims[0], t.dims[1], t.dims[2], stream);
         float best_b = 3.0e38f;
         for (int block : {256, 512, 1024}) {
+          if (r != FFK_OK) break;
           if ((r = ffk_set_tuning(s, block, 0, -1)) < 0) break;
           float ms = 0.f;
           r = time_ms(st, 2, [&] { return ffk_ff_fused(s, FFK_CAST_GRID, 1, d_tally, d_F, nullptr, stream); }, &ms);
@@ -180,6 +219,7 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
       (void)hipFree(d_tally);
       if (r < 0) return r;
     }
+    if (!fused_timed && (r = ffk_scene_build_grid(s, t.dims[0], t.dims[1], t.dims[2], stream)) < 0) return r;
     std::lock_guard<std::mutex> lk(g_mu);
     g_cache[key] = t;
   } else {

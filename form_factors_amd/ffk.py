@@ -308,7 +308,7 @@ class Scene:
         check(lib().ffk_scene_grid_info(self.h, dims, lo, hi, C.byref(nr)), "ffk_scene_grid_info")
         return {"dims": tuple(dims), "lo": tuple(lo), "hi": tuple(hi), "n_refs": nr.value}
 
-    def autotune(self, sample_rays=1 << 20):
+    def autotune(self, sample_rays=1 << 22):
         """Grid resolution and block size from timed trials (cached per scene
         signature). Returns {"dims", "block", "cached"}."""
         dims = (C.c_int * 3)()

@@ -55,6 +55,40 @@ def test_grid_cast_independent_of_resolution(gpu, oracle, dims):
     assert np.array_equal(bits(pts[idx >= 0]), bits(ref_p[ref_i >= 0]))
 
 
+@pytest.mark.parametrize("n", [1, 100, 5000, 300_000])
+def test_grid_cast_pooled_equals_per_ray(gpu, oracle, n):
+    """The pooled-test cast (lane refill, tests spread over the wavefront)
+    returns the per-ray cast's indices and points bit for bit, for every
+    setting and ray count (one partial block up to many full ones); rays that
+    miss the grid box, zero directions and rays starting on faces included."""
+    from form_factors_amd import scenes
+
+    rng = np.random.default_rng(n)
+    faces = np.concatenate([scenes.confetti(400, rng), scenes.cube_shell(4)]).astype(np.float32)
+    o = rng.uniform(-0.5, 1.5, (n, 3))
+    e = o + rng.normal(0, 1, (n, 3))
+    m = min(n, 10)
+    e[:m // 2] = o[:m // 2]                                   # zero direction
+    o[m // 2:m] = faces[:m - m // 2, :3] * 0.5 + faces[:m - m // 2, 3:6] * 0.5  # on a face edge
+    rays = np.concatenate([o, e], 1).astype(np.float32)
+    sc = make_scene(gpu, faces, dims=(24, 24, 24))
+    sc.set_refill(0, 0)
+    ref_i, ref_p = sc.cast(t(rays), method=1)
+    ref_i, ref_p = ref_i.cpu().numpy(), ref_p.cpu().numpy()
+    if n <= 5000:
+        want_i, _ = oracle.naive_cast(faces, rays, n_threads=8)
+        assert np.array_equal(ref_i, want_i)
+    for refill, pool in [(16, 4), (1, 1), (64, 64), (8, 2), (32, 16)]:
+        print(f"pooled cast n={n} refill={refill} pool={pool}", flush=True)
+        sc.set_refill(refill, pool)
+        idx, pts = sc.cast(t(rays), method=1)
+        idx, pts = idx.cpu().numpy(), pts.cpu().numpy()
+        assert np.array_equal(idx, ref_i), (refill, pool)
+        hit = idx >= 0
+        assert np.array_equal(bits(pts[hit]), bits(ref_p[hit])), (refill, pool)
+    sc.close()
+
+
 def test_cast_reference_kats(gpu):
     from test_oracle import reference_cast_kats
 
