@@ -50,6 +50,7 @@ def parse():
     ap.add_argument("--no-thermal", action="store_true")
     ap.add_argument("--no-batch-set", action="store_true")
     ap.add_argument("--no-confetti", action="store_true")
+    ap.add_argument("--no-radiance", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--thermal-steps", type=int, default=10_000)
     ap.add_argument("--block", type=int, default=0)
@@ -226,6 +227,50 @@ def confetti_leg(ffk, scenes, steps):
             "autotuned": {"dims": list(tuned["dims"]), "block": tuned["block"],
                           "value": round(n_rays / t_tuned / 1e6, 2), "tuning_seconds": round(t_tune, 2)},
             "F_bits_identical": same, "sun": sun}
+
+
+def radiance_leg(ffk, scenes):
+    """radiance_equation::calculate per step on the two 100 x 100-cell plates
+    (40,000 triangles, 20,000 meshes, temperatures 250-350 K): radiance
+    weights T * eps * A, the emission layout, ray powers, then one device pass
+    that emits, casts and deposits every ray (64-bit fixed-point sums). At the
+    reference controller's 1e6 rays per step and at 1e8."""
+    import torch
+
+    prob = scenes.thermal_plates(cells=100)
+    n = len(prob["meshes"])
+    rng = np.random.default_rng(0x5AD)
+    T = rng.uniform(250.0, 350.0, n).astype(np.float32)
+    eps = np.stack([prob["eps_front"], prob["eps_rear"]], 1).astype(np.float32)
+    sc = ffk.Scene(prob["faces"], prob["meshes"])
+    sc.build_grid()
+    out = {"workload": f"two 100x100-cell plates, {len(prob['faces'])} triangles, {n} meshes, T in [250, 350] K, "
+                       "one radiance_equation step (weights, layout, powers, emit-cast-deposit pass)",
+           "unit": "Mrays/s"}
+    for n_rays, reps in ((1_000_000, 20), (100_000_000, 3)):
+        sc.radiance_step(eps, T, n_rays, 1)  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(reps):
+            em, ab = sc.radiance_step(eps, T, n_rays, 2 + k)
+        step_s = (time.perf_counter() - t0) / reps
+        w, total = ffk.radiance_weights(sc.faces, sc.meshes, eps, T)
+        front, rear = ffk.emitted_counts(w, n_rays, total)
+        cast = int(front.sum() + rear.sum())
+        p, _ = ffk.radiance_powers(sc.faces, sc.meshes, eps, T, front, rear)
+        t0 = time.perf_counter()
+        for k in range(reps):
+            sc.radiance(2 + k, p)
+        dep_s = (time.perf_counter() - t0) / reps
+        frac = float(ab.sum() / em.astype(np.float64).sum())
+        out[f"rays_{n_rays}"] = {"rays_cast": cast, "ms_per_step": round(step_s * 1e3, 3),
+                                 "value": round(cast / step_s / 1e6, 2),
+                                 "deposit_ms": round(dep_s * 1e3, 3), "deposit_value": round(cast / dep_s / 1e6, 2),
+                                 "absorbed_over_emitted": round(frac, 5)}
+        log(f"radiance {n_rays} rays: {step_s * 1e3:.3f} ms/step ({cast / step_s / 1e6:.1f} Mrays/s), "
+            f"deposit pass {dep_s * 1e3:.3f} ms ({cast / dep_s / 1e6:.1f} Mrays/s), absorbed/emitted {frac:.4f}")
+    sc.close()
+    return out
 
 
 def thermal_leg(ffk, scenes, n_steps):
@@ -421,6 +466,8 @@ def main():
         result["batch_set"] = batch_set_leg(ffk, scenes, args.batch_scenes)
     if not args.no_confetti:
         result["confetti"] = confetti_leg(ffk, scenes, 3)
+    if not args.no_radiance:
+        result["radiance"] = radiance_leg(ffk, scenes)
     if not args.no_thermal:
         result["thermal"] = thermal_leg(ffk, scenes, args.thermal_steps)
 

@@ -9,6 +9,7 @@
 // their first walk position), and the powers are summed per mesh in 64-bit
 // fixed point: integer additions, so the result does not depend on the order
 // of the atomics and carries no float accumulation error.
+#include <algorithm>
 #include <cmath>
 
 #include "ffk_trace.hpp"
@@ -45,6 +46,61 @@ __global__ void __launch_bounds__(256) k_radiance(scene_desc S, const long long*
     const unsigned long long v = q[2 * k + ((g - walk_start[k]) < S.front[wf] ? 0 : 1)];
     if (v) atomicAdd(&acc[m], v);
   }
+  if (STATS) {
+    unsigned long long t = nt, c = nc;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      t += __shfl_xor(t, off, 64);
+      c += __shfl_xor(c, off, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+      atomicAdd(&stats[0], t);
+      atomicAdd(&stats[1], c);
+    }
+  }
+}
+
+// The same deposit with lane refill and pooled triangle tests
+// (pooled_walk): block b casts walk positions [b * chunk, (b + 1) * chunk).
+template <bool STATS>
+__global__ void __launch_bounds__(256)
+    k_radiance_pooled(scene_desc S, const long long* __restrict__ walk_start, const int* __restrict__ walk_face,
+                      int n_walk, const unsigned long long* __restrict__ q, long long n_rays, long long chunk,
+                      uint32_t k0, uint32_t k1, unsigned long long* __restrict__ acc,
+                      unsigned long long* __restrict__ stats, int refill, int pool) {
+  __shared__ unsigned long long best[256];
+  __shared__ int s_next;
+  const long long r0 = (long long)blockIdx.x * chunk;
+  const long long cnt = min(chunk, n_rays - r0);
+  if (threadIdx.x == 0) s_next = 0;
+  __syncthreads();
+  uint32_t nt = 0, nc = 0;
+  int hint = find_face(S.face_off, S.n_faces, r0, 0), whint = 0;
+  pooled_walk<STATS>(
+      S, best + 64 * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
+      [&](long long i, v3& o, v3& dir) {
+        const long long g = r0 + i;
+        const int f = find_face(S.face_off, S.n_faces, g, hint);
+        hint = f;
+        v3 e;
+        malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+        dir = e - o;
+      },
+      [&](long long i, v3, v3, unsigned long long key) {
+        float d2;
+        int hf;
+        decode_key(key, d2, hf);
+        if (hf < 0) return;
+        const int m = S.face2mesh[hf];
+        if (m < 0) return;
+        const long long g = r0 + i;
+        const int k = find_face(walk_start, n_walk, g, whint);
+        whint = k;
+        const int wf = walk_face[k];
+        const unsigned long long v = q[2 * k + ((g - walk_start[k]) < S.front[wf] ? 0 : 1)];
+        if (v) atomicAdd(&acc[m], v);
+      },
+      nt, nc);
   if (STATS) {
     unsigned long long t = nt, c = nc;
 #pragma unroll
@@ -201,15 +257,30 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   FFK_HIP(hipMemcpyAsync(s->d_walk_face, face.data(), 4 * face.size(), hipMemcpyHostToDevice, st));
   FFK_HIP(hipMemcpyAsync(s->d_walk_q, q.data(), 8 * q.size(), hipMemcpyHostToDevice, st));
   k_fill_acc<<<(nm + 255) / 256, 256, 0, st>>>(s->d_rad_acc, nm);
-  const long long blocks_needed = (walk_total + 255) / 256;
-  const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-  if (d_stats)
-    k_radiance<true><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q,
-                                             walk_total, k0, k1, s->d_rad_acc, d_stats);
-  else
-    k_radiance<false><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q,
-                                              walk_total, k0, k1, s->d_rad_acc, nullptr);
+  if (walk_total > 0 && s->tune_refill > 0 && s->tune_pool > 0) {
+    // About eight blocks per CU, at least a ray per lane and at most 64.
+    const long long target = 8LL * device_cu_count();
+    const long long chunk = std::min<long long>(16384, std::max<long long>(256, (walk_total + target - 1) / target));
+    const unsigned blocks = (unsigned)((walk_total + chunk - 1) / chunk);
+    if (d_stats)
+      k_radiance_pooled<true><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk,
+                                                      s->d_walk_q, walk_total, chunk, k0, k1, s->d_rad_acc, d_stats,
+                                                      s->tune_refill, s->tune_pool);
+    else
+      k_radiance_pooled<false><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk,
+                                                       s->d_walk_q, walk_total, chunk, k0, k1, s->d_rad_acc,
+                                                       nullptr, s->tune_refill, s->tune_pool);
+  } else if (walk_total > 0) {
+    const long long blocks_needed = (walk_total + 255) / 256;
+    const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));
+    if (d_stats)
+      k_radiance<true><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q,
+                                               walk_total, k0, k1, s->d_rad_acc, d_stats);
+    else
+      k_radiance<false><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q,
+                                                walk_total, k0, k1, s->d_rad_acc, nullptr);
+  }
   FFK_HIP(hipGetLastError());
   std::vector<unsigned long long> acc(nm);
   FFK_HIP(hipMemcpyAsync(acc.data(), s->d_rad_acc, 8 * (size_t)nm, hipMemcpyDeviceToHost, st));

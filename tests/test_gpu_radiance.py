@@ -75,6 +75,24 @@ def test_radiance_homogeneous_and_repeatable(gpu):
     assert abs(ab1.sum() / em1.astype(np.float64).sum() - 0.5) < 1e-3
 
 
+@pytest.mark.parametrize("case", [0, 3])
+def test_radiance_schedules_agree(gpu, case):
+    """Per-ray casting, lane refill and pooled tests deposit the same fixed
+    point sums: identical absorption bits in every setting."""
+    name, faces, meshes, eps, T, n_rays = _cases()[case]
+    sc = gpu.Scene(faces, meshes)
+    sc.build_grid()
+    out = []
+    for refill, pool in [(0, 0), (16, 0), (16, 4), (1, 1), (64, 64)]:
+        print(f"radiance {name} refill={refill} pool={pool}", flush=True)
+        sc.set_refill(refill, pool)
+        em, ab = sc.radiance_step(eps, T, n_rays, 99)
+        out.append(ab)
+    for ab in out[1:]:
+        assert np.array_equal(bits(ab), bits(out[0]))
+    sc.close()
+
+
 def test_radiance_errors(gpu):
     from form_factors_amd import ffk, scenes
 
