@@ -14,6 +14,9 @@
 #include "emission/malley_emission.h"
 #include "emission/parallel_rays_cpu.h"
 #include "emission/system.h"
+#include "import_export/csv_export.h"
+#include "import_export/obj_export.h"
+#include "import_export/obj_import.h"
 #include "form_factors/system.h"
 #include "math/grid.h"
 #include "math/mat.h"
@@ -488,4 +491,105 @@ EXPORT int ffref_heat_source(const int* mesh_idx, const float* power, int n_sour
   int r = thermal_equation::system_calculate(eq, &task);
   thermal_equation::system_free(eq);
   return r;
+}
+
+// ---- import_export: obj_import::scene / task, obj_export, csv_export ----
+
+// obj_import::scene; on success *handle holds the scene for ffref_obj_copy.
+EXPORT int ffref_obj_import(const char* path, int* n_faces, int* n_meshes, int* n_materials, void** handle) {
+  subject::scene_t* scene = nullptr;
+  const int r = obj_import::scene(path, &scene);
+  *handle = scene;
+  if (r != OBJ_IMPORT_OK || !scene) return r;
+  *n_faces = scene->n_faces;
+  *n_meshes = scene->n_meshes;
+  *n_materials = scene->n_materials;
+  return r;
+}
+
+// faces: 9 floats per face; meshes: (first, n_faces, material); materials:
+// 14 floats (shell 4, front 5, rear 5) and a 128-byte name each.
+EXPORT void ffref_obj_copy(void* handle, float* faces, int* meshes, float* props, char* names) {
+  const subject::scene_t* s = static_cast<const subject::scene_t*>(handle);
+  std::memcpy(faces, s->faces, sizeof(float) * 9 * (size_t)s->n_faces);
+  for (int m = 0; m < s->n_meshes; ++m) {
+    meshes[3 * m] = s->meshes[m].first_idx;
+    meshes[3 * m + 1] = s->meshes[m].n_faces;
+    meshes[3 * m + 2] = s->meshes[m].material_idx;
+  }
+  for (int k = 0; k < s->n_materials; ++k) {
+    const subject::material_t& mt = s->materials[k];
+    std::memcpy(props + 14 * k, &mt.shell, 4 * sizeof(float));
+    std::memcpy(props + 14 * k + 4, &mt.front, 5 * sizeof(float));
+    std::memcpy(props + 14 * k + 9, &mt.rear, 5 * sizeof(float));
+    std::memcpy(names + 128 * k, mt.name, 128);
+  }
+}
+
+EXPORT void ffref_obj_free(void* handle) { subject::scene_free(static_cast<subject::scene_t*>(handle)); }
+
+// obj_import::task called n_calls times on one open file. Per call: the
+// return code, the step and the first n_meshes temperatures; after the last
+// call the heat sources (up to src_cap) and the distant source.
+EXPORT int ffref_task_import(const char* path, int n_meshes, int n_calls, int* codes, float* dts, float* temps,
+                             int* n_src, int* src_mesh, float* src_power, int src_cap, float* dst4, int* has_dst) {
+  FILE* f = fopen(path, "r");
+  if (!f) return -1;
+  thermal_solution::task_t* t = thermal_solution::task_create(0);
+  heat_source_equation::params_t hs = {0, 0};
+  parallel_rays_cpu::params_t ds = {1, 0, 0, 0};
+  for (int c = 0; c < n_calls; ++c) {
+    codes[c] = obj_import::task(f, n_meshes, t, &hs, &ds);
+    dts[c] = t->time_delta;
+    for (int m = 0; m < n_meshes && m < (codes[c] > 0 ? codes[c] : 0); ++m) temps[(size_t)c * n_meshes + m] = t->temperatures[m];
+  }
+  fclose(f);
+  *n_src = hs.n_sources;
+  for (int i = 0; i < hs.n_sources && i < src_cap; ++i) {
+    src_mesh[i] = hs.sources[i].mesh_idx;
+    src_power[i] = hs.sources[i].power;
+  }
+  *has_dst = ds.source != 0;
+  if (ds.source) {
+    parallel_rays_cpu::source_t src = ds.source(ds.source_param);
+    dst4[0] = src.power;
+    dst4[1] = src.direction.x;
+    dst4[2] = src.direction.y;
+    dst4[3] = src.direction.z;
+  }
+  free(hs.sources);
+  thermal_solution::task_free(t);
+  return 0;
+}
+
+// obj_import::scene then obj_export::scene into out_path.
+EXPORT int ffref_obj_reexport(const char* obj_path, const char* out_path) {
+  subject::scene_t* scene = nullptr;
+  int r = obj_import::scene(obj_path, &scene);
+  if (r != OBJ_IMPORT_OK) return r;
+  FILE* f = fopen(out_path, "w");
+  if (!f) return -1;
+  r = obj_export::scene(f, scene);
+  fclose(f);
+  subject::scene_free(scene);
+  return r;
+}
+
+EXPORT int ffref_task_export(const char* out_path, int n_step, float dt, const float* temps, int n_meshes,
+                             int binary) {
+  FILE* f = fopen(out_path, "w");
+  if (!f) return -1;
+  thermal_solution::task_t t;
+  t.n_step = n_step;
+  t.time_delta = dt;
+  t.temperatures = const_cast<float*>(temps);
+  const int r = binary ? obj_export::task_binary(f, n_meshes, &t) : obj_export::task(f, n_meshes, &t);
+  fclose(f);
+  return r;
+}
+
+EXPORT int ffref_csv_export(const char* path, const float* F, int n_meshes) {
+  form_factors::scene_t scene = {0, 0, n_meshes, 0};
+  form_factors::task_t task = {0, const_cast<float*>(F)};
+  return csv_export::export_csv(path, &scene, &task);
 }

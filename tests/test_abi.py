@@ -20,7 +20,7 @@ def declared_symbols():
             continue
         text = open(os.path.join(ROOT, "include", hdr)).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-        for m in re.finditer(r"^\s*(?:const\s+)?[a-zA-Z_][\w\s\*]*?\b((?:ffk|ffp)_\w+)\s*\(", text, re.M):
+        for m in re.finditer(r"^\s*(?:const\s+)?[a-zA-Z_][\w\s\*]*?\b((?:ffk|ffp|ffio)_\w+)\s*\(", text, re.M):
             names.add(m.group(1))
     return sorted(names)
 
@@ -47,6 +47,9 @@ def test_python_signatures_cover_the_header():
 
     ffk_names = [n for n in declared_symbols() if n.startswith("ffk_")]
     assert sorted(ffk.SIGNATURES) == ffk_names
+    from form_factors_amd import formats
+
+    assert sorted(formats.SIGNATURES) == [n for n in declared_symbols() if n.startswith("ffio_")]
 
 
 def test_host_area_weights_and_counts_match_oracle(oracle):
