@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--no-batch-set", action="store_true")
     ap.add_argument("--no-confetti", action="store_true")
     ap.add_argument("--no-radiance", action="store_true")
+    ap.add_argument("--no-controller", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--thermal-steps", type=int, default=10_000)
     ap.add_argument("--block", type=int, default=0)
@@ -273,6 +274,43 @@ def radiance_leg(ffk, scenes):
     return out
 
 
+def controller_leg():
+    """The reference controller's thermal run on its confetti_2k model (2000
+    meshes, 100 Adams steps of 0.01 s, 1e6 radiance rays per step; the
+    reference recorded 94.84 s on a GTX 780 Ti in models/confetti_2k.frames)
+    through the plugin and file ABIs: OBJ and .task import, HIP radiance,
+    conduction, heat-source and distant-source equations, FP64 Adams, frame
+    export. Timed as the controller times it (set-up of the solution and the
+    steps). Unit heat capacity, as in the recorded run; the frames' mean
+    temperatures are compared with the recorded ones."""
+    from form_factors_amd import controller, formats
+
+    models = os.path.join(ROOT, "tests", "golden", "models")
+    g = np.load(os.path.join(ROOT, "tests", "golden", "controller_confetti_2k.npz"))
+    sc = formats.obj_import(os.path.join(models, "confetti_2k.obj"), inline_materials=True)
+    sc["materials"][:, 1] = 1.0
+    task = os.path.join(models, "confetti_2k.task")
+    controller.run(None, task, n_rays=1_000_000, n_steps=5, seed=99, scene=sc)  # warm-up
+    runs = []
+    for seed in range(3):
+        t0 = time.perf_counter()
+        out = controller.run(None, task, n_rays=1_000_000, n_steps=100, seed=seed, scene=sc)
+        wall = time.perf_counter() - t0
+        T = np.array([f[2] for f in out["frames"]])
+        runs.append((out["seconds"], wall, float(np.abs(T.mean(1) - g["mean"]).max())))
+    best = min(r[0] for r in runs)
+    ref = float(g["seconds"])
+    log(f"controller confetti_2k: {best:.3f} s for 100 steps (reference {ref:.2f} s, {ref / best:.0f}x); "
+        f"max |mean T - recorded| {max(r[2] for r in runs):.4f} K")
+    return {"workload": "reference controller run, confetti_2k model (2000 meshes), 100 Adams steps dt=0.01 s, "
+                        "1e6 radiance rays per step, plugin ABI end to end",
+            "seconds": round(best, 4), "seconds_all": [round(r[0], 4) for r in runs],
+            "wall_with_import_seconds": round(min(r[1] for r in runs), 4),
+            "reference_seconds": ref, "reference_hardware": "GeForce GTX 780 Ti (recorded in the model's frames file)",
+            "speedup_vs_recorded": round(ref / best, 1),
+            "max_abs_mean_T_diff_K": round(max(r[2] for r in runs), 4)}
+
+
 def thermal_leg(ffk, scenes, n_steps):
     """Two 100 x 100-cell plates (40,000 triangles, 20,000 meshes): F from the
     fused kernel, then n_steps Adams steps of dt = 0.1 s with the exchange
@@ -468,6 +506,8 @@ def main():
         result["confetti"] = confetti_leg(ffk, scenes, 3)
     if not args.no_radiance:
         result["radiance"] = radiance_leg(ffk, scenes)
+    if not args.no_controller:
+        result["controller"] = controller_leg()
     if not args.no_thermal:
         result["thermal"] = thermal_leg(ffk, scenes, args.thermal_steps)
 

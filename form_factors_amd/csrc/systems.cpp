@@ -1467,10 +1467,11 @@ parallel_rays_cpu::source_t read_source(void* p) { return *(parallel_rays_cpu::s
 }  // namespace
 
 void* ffp_distant_source_create(void* emitter, int n_rays, ffp_source* source) {
-  if (!emitter || n_rays <= 0 || !source) return nullptr;
+  if (!emitter || n_rays <= 0) return nullptr;
   thermal_equation::system_t* s = hip_distant_source::system_create();
   if (!s) return nullptr;
-  parallel_rays_cpu::params_t p = {n_rays, (emission::system_t*)emitter, source, &read_source};
+  // No source: the equation adds nothing (parallel_rays_cpu.cpp:94-95).
+  parallel_rays_cpu::params_t p = {n_rays, (emission::system_t*)emitter, source, source ? &read_source : nullptr};
   if (s->methods->init(s, &p) < 0) {
     s->methods->shutdown(s);
     free(s);

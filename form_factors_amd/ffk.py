@@ -94,6 +94,11 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise FFKError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # torch ships its own HIP runtime under the same soname as the one
+        # this library links: load torch's first, so that one runtime (and
+        # one device context) serves both in the process.
+        import torch  # noqa: F401
+
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -124,11 +124,12 @@ def obj_import(path, inline_materials=False):
         lib().ffio_scene_free(sp)
 
 
-def task_import(path, n_meshes, n_calls=1):
-    """obj_import::task called n_calls times on one open file. Returns
-    {"frames": [(count, step, temperatures[:count])], "heat_sources":
-    [(mesh, power)], "distant_source": (power, x, y, z) or None}."""
-    t = P.TsTask(0, 0.0, None)
+def task_import(path, n_meshes, n_calls=1, step=0.0):
+    """obj_import::task called n_calls times on one open file (the step is
+    `step` until a "step" line sets it). Returns {"frames": [(count, step,
+    temperatures[:count])], "heat_sources": [(mesh, power)],
+    "distant_source": (power, x, y, z) or None}."""
+    t = P.TsTask(0, float(step), None)
     hs = HeatSourceParams(0, None)
     ds = C.POINTER(P.Source)()
     frames = []

@@ -104,7 +104,8 @@ void* ffp_heat_source_create(int n_sources, ffp_heat_source* sources);
 /* type 10 = distant source (thermal_equation/parallel_rays_cpu.h:26-39) over
  * a parallel-rays emitter (fused on the device for the HIP emitter over the
  * grid caster); the caller keeps `source` alive and may change it between
- * calls (the reference reads it through its source callback each step). */
+ * calls (the reference reads it through its source callback each step).
+ * source NULL: no distant source, the equation adds nothing. */
 typedef struct ffp_source {
   float power; /* W/m^2 */
   ffp_vec3 direction;
