@@ -57,6 +57,8 @@ def parse():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--rays-per-tile", type=int, default=0)
     ap.add_argument("--hist", type=int, default=-1)
+    ap.add_argument("--refill", type=int, default=-1, help="fused-kernel refill threshold (lanes; -1 = default)")
+    ap.add_argument("--pool", type=int, default=-1, help="pooled-test ratio (0 = per-lane tests; -1 = default)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one untimed device copy of this size (HBM counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -393,6 +395,8 @@ def main():
     n_rays = sc.set_emission(n_rays=n_req, frame=ffk.FRAME_ORTHONORMAL)
     g = sc.build_grid((args.grid,) * 3)
     sc.set_tuning(block=args.block, rays_per_tile=args.rays_per_tile, hist=args.hist)
+    if args.refill >= 0 or args.pool >= 0:
+        sc.set_refill(16 if args.refill < 0 else args.refill, 4 if args.pool < 0 else args.pool)
     N = len(meshes)
     log(f"scene: {len(faces)} faces, {N} meshes, {n_rays} rays, grid {g['dims']} refs {g['n_refs']}")
     F = torch.empty((N, N), dtype=torch.float32, device="cuda")
