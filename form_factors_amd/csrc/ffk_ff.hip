@@ -718,11 +718,11 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
   // per-launch argument block in device memory
   scene_desc d = s->desc();
   FFK_HIP(hipMemcpyAsync(s->d_desc, &d, sizeof(d), hipMemcpyHostToDevice, st));
-  static thread_local unsigned long long* d_misc = nullptr;  // seed, tally ptr, F ptr
-  if (!d_misc) {
+  if (!s->d_misc) {  // seed, tally pointer, F pointer
     int r;
-    if ((r = dalloc(&d_misc, 4)) < 0) return r;
+    if ((r = dalloc(&s->d_misc, 4)) < 0) return r;
   }
+  unsigned long long* d_misc = s->d_misc;
   unsigned long long misc[3] = {seed, (unsigned long long)(uintptr_t)d_tally,
                                 (unsigned long long)(uintptr_t)d_F};
   FFK_HIP(hipMemcpyAsync(d_misc, misc, sizeof(misc), hipMemcpyHostToDevice, st));

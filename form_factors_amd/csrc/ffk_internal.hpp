@@ -292,6 +292,10 @@ struct ffk_scene {
   int* d_indirect_rows = nullptr;  // meshes tallied through the global tally
   int n_indirect_rows = 0;
   ffk::scene_desc* d_desc = nullptr;
+  // per-launch arguments of the fused kernel (seed, tally and F pointers),
+  // one block per scene so that launches of different scenes on different
+  // streams do not share it
+  unsigned long long* d_misc = nullptr;
   // radiance work buffers, grown on demand
   long long* d_walk_start = nullptr;
   int* d_walk_face = nullptr;

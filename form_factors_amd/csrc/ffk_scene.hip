@@ -340,6 +340,7 @@ void ffk_scene_destroy(ffk_scene* s) {
   dfree(s->d_tri);
   dfree(s->d_face2mesh);
   dfree(s->d_desc);
+  dfree(s->d_misc);
   dfree(s->d_walk_start);
   dfree(s->d_walk_face);
   dfree(s->d_walk_q);

@@ -11,6 +11,11 @@
  * on top of these calls, see include/ff_plugin.h. Each entry below names the
  * reference function whose work it performs.
  *
+ * Streams: work is queued on the stream passed to each call. Different
+ * scenes (and batches) keep their own device buffers and may run at the same
+ * time on different streams; calls on one scene are issued one stream at a
+ * time (they share that scene's argument and work buffers).
+ *
  * Layouts in HBM:
  *   faces : float[9*n]  p0 p1 p2 (AoS, math/types.h:51-54)
  *   rays  : float[6*n]  origin, END POINT (math/types.h:72-76; the direction

@@ -51,6 +51,28 @@ def test_batched_equals_plain_and_oracle(gpu, oracle):
     b.close()
 
 
+def test_scenes_on_two_streams(gpu):
+    """Fused launches of different scenes queued back to back on two streams
+    keep their own arguments: each F equals its single-stream result."""
+    import torch
+
+    scs = build_scenes(gpu, [300, 200, 500, 100], [300, 400, 200, 600], seed0=21)
+    want = [sc.ff_fused(40 + k).cpu().numpy() for k, sc in enumerate(scs)]
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    Fs = [torch.full((sc.n_meshes,) * 2, -1.0, device="cuda") for sc in scs]
+    torch.cuda.synchronize()
+    for rep in range(3):
+        for k, sc in enumerate(scs):
+            with torch.cuda.stream(streams[k % 2]):
+                sc.ff_fused(40 + k, F=Fs[k])
+        torch.cuda.synchronize()
+        for F, w in zip(Fs, want):
+            assert np.array_equal(bits(F.cpu().numpy()), bits(w)), rep
+        for F in Fs:
+            F.fill_(-1.0)
+        torch.cuda.synchronize()
+
+
 def test_batched_tuning_does_not_change_bits(gpu):
     import torch
 
