@@ -644,24 +644,12 @@ int ffk_pack_exchange(const float* d_F, int n, int dtype, const double* d_row_mu
 
 int ffk_sbff(int n, const void* d_S, int dtype, const double* d_row_scale, const double* d_T,
              const double* d_area, const double* d_eps_front, const double* d_eps_rear, double* d_emission,
-             double* d_absorption, void* stream) {
-  FFK_REQUIRE(n > 0 && d_S && d_T && d_area && d_eps_front && d_eps_rear && d_emission && d_absorption,
+             double* d_absorption, float* d_work, void* stream) {
+  FFK_REQUIRE(n > 0 && d_S && d_T && d_area && d_eps_front && d_eps_rear && d_emission && d_absorption && d_work,
               FFK_ERR_ARG, "ffk_sbff: bad args");
   FFK_REQUIRE(dtype == FFK_F32 || dtype == FFK_BF16 || dtype == FFK_F16, FFK_ERR_ARG, "dtype %d", dtype);
   hipStream_t st = as_stream(stream);
-  // Scratch for E (FP32), kept per thread and grown on demand.
-  static thread_local float* d_scratch = nullptr;
-  static thread_local size_t scratch_n = 0;
-  if ((size_t)n > scratch_n) {
-    FFK_HIP(hipStreamSynchronize(st));
-    dfree(d_scratch);
-    d_scratch = nullptr;
-    scratch_n = 0;
-    int r;
-    if ((r = dalloc(&d_scratch, (size_t)n)) < 0) return r;
-    scratch_n = (size_t)n;
-  }
-  float* E32 = d_scratch;
+  float* E32 = d_work;
   k_sb_emission<<<(n + 255) / 256, 256, 0, st>>>(n, d_T, d_area, d_eps_front, d_eps_rear, d_row_scale, d_emission,
                                                  E32);
   if (dtype == FFK_BF16) launch_rows<FFK_BF16>(n, d_S, E32, d_absorption, st);

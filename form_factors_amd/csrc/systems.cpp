@@ -606,7 +606,7 @@ struct state {
   std::vector<double> T, area, ef, er, em, ab;
   dbuf<float> dF32;
   dbuf<unsigned short> dS;  // packed exchange storage (2 or 4 bytes per cell)
-  dbuf<double> dvec;  // T, area, ef, er, em, ab, row scale
+  dbuf<double> dvec;  // T, area, ef, er, em, ab, row scale, E in FP32 (n/2 + 1 doubles)
   hipStream_t st = nullptr;
 };
 struct system : thermal_equation::system_t {
@@ -662,7 +662,7 @@ int set_scene(system* sys, subject::scene_t* scene) {
   if ((r = ff->methods->calculate(ff, &task)) < 0) return r;
   // F in the chosen storage type on the device
   const size_t NN = (size_t)N * N;
-  if (!s->dF32.reserve(NN) || !s->dvec.reserve(7 * (size_t)N)) return -THERMAL_EQUATION_ERROR;
+  if (!s->dF32.reserve(NN) || !s->dvec.reserve(7 * (size_t)N + (size_t)N / 2 + 1)) return -THERMAL_EQUATION_ERROR;
   if (hipMemcpyAsync(s->dF32.p, s->F.data(), 4 * NN, hipMemcpyHostToDevice, s->st) != hipSuccess)
     return -THERMAL_EQUATION_ERROR;
   double* d = s->dvec.p;
@@ -690,7 +690,9 @@ int calculate(system* sys, thermal_equation::task_t* task) {
       hipMemsetAsync(d + 4 * N, 0, 16 * (size_t)N, s->st) != hipSuccess)
     return -THERMAL_EQUATION_ERROR;
   const double* scale = d + 6 * N;
-  if (ffk_sbff(N, F, s->params.f_dtype, scale, d, d + N, d + 2 * N, d + 3 * N, d + 4 * N, d + 5 * N, s->st) < 0)
+  float* work = reinterpret_cast<float*>(d + 7 * N);
+  if (ffk_sbff(N, F, s->params.f_dtype, scale, d, d + N, d + 2 * N, d + 3 * N, d + 4 * N, d + 5 * N, work, s->st) <
+      0)
     return -THERMAL_EQUATION_ERROR;
   s->em.resize(N);
   s->ab.resize(N);

@@ -265,10 +265,11 @@ int ffk_pack_exchange(const float* d_F, int n, int dtype, const double* d_row_mu
  *   E_m = (eps_f_m + eps_r_m) * sigma * T_m^4 * A_m        (emission[m] += E_m)
  *   absorption[n] += sum_m E_m * F[m*N + n]
  * d_S is the packed storage of ffk_pack_exchange (d_row_scale from the same
- * call), accumulated in FP32; T/area/eps/outputs in FP64. */
+ * call), accumulated in FP32; T/area/eps/outputs in FP64. d_work: n floats
+ * of device scratch (the scaled E in FP32), owned by the caller. */
 int ffk_sbff(int n, const void* d_S, int dtype, const double* d_row_scale, const double* d_T,
              const double* d_area, const double* d_eps_front, const double* d_eps_rear,
-             double* d_emission, double* d_absorption, void* stream);
+             double* d_emission, double* d_absorption, float* d_work, void* stream);
 /* Conduction (thermal_equation/conductive_cpu.cpp:83-134) as a CSR operator
  * over mesh adjacency. For each neighbour k of mesh m, f = G_mk (T_k - T_m):
  * f > 0 is added to absorption[m], else -f to emission[m] (the reference's

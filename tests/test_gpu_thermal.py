@@ -134,7 +134,8 @@ def test_pack_exchange_rows(gpu):
     em = torch.zeros(n, dtype=torch.float64, device="cuda")
     ab = torch.zeros(n, dtype=torch.float64, device="cuda")
     p = gpu._ptr
-    gpu.check(gpu.lib().ffk_sbff(n, p(Sd), gpu.BF16, p(scale_d), p(T), p(A), p(ef), p(er), p(em), p(ab),
+    work = torch.empty(n, dtype=torch.float32, device="cuda")
+    gpu.check(gpu.lib().ffk_sbff(n, p(Sd), gpu.BF16, p(scale_d), p(T), p(A), p(ef), p(er), p(em), p(ab), p(work),
                                  gpu._stream()), "ffk_sbff")
     torch.cuda.synchronize()
     E = (0.8 + 0.3) * 5.670400e-8 * T.cpu().numpy() ** 4 * A.cpu().numpy()
