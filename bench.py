@@ -59,6 +59,7 @@ def parse():
     ap.add_argument("--hist", type=int, default=-1)
     ap.add_argument("--refill", type=int, default=-1, help="fused-kernel refill threshold (lanes; -1 = default)")
     ap.add_argument("--pool", type=int, default=-1, help="pooled-test ratio (0 = per-lane tests; -1 = default)")
+    ap.add_argument("--walk", type=int, default=-1, help="empty-cell walk mode (0 per wavefront, 1 jump; -1 = default)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one untimed device copy of this size (HBM counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -227,7 +228,7 @@ def confetti_leg(ffk, scenes, steps):
     return {"workload": f"{len(faces)} triangles (2000 confetti + cube shell), {n_rays} Malley rays, F {N}x{N}",
             "n_rays": n_rays, "unit": "Mrays/s",
             "default_grid": {"dims": list(g0), "value": round(n_rays / t_default / 1e6, 2)},
-            "autotuned": {"dims": list(tuned["dims"]), "block": tuned["block"],
+            "autotuned": {"dims": list(tuned["dims"]), "block": tuned["block"], "walk": tuned["walk"],
                           "value": round(n_rays / t_tuned / 1e6, 2), "tuning_seconds": round(t_tune, 2)},
             "F_bits_identical": same, "sun": sun}
 
@@ -397,6 +398,8 @@ def main():
     sc.set_tuning(block=args.block, rays_per_tile=args.rays_per_tile, hist=args.hist)
     if args.refill >= 0 or args.pool >= 0:
         sc.set_refill(16 if args.refill < 0 else args.refill, 4 if args.pool < 0 else args.pool)
+    if args.walk >= 0:
+        sc.set_walk(args.walk)
     N = len(meshes)
     log(f"scene: {len(faces)} faces, {N} meshes, {n_rays} rays, grid {g['dims']} refs {g['n_refs']}")
     F = torch.empty((N, N), dtype=torch.float32, device="cuda")

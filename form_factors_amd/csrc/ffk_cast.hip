@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) k_cast_grid(scene_desc S, const float* __
 // block b casts rays [b * chunk, (b + 1) * chunk). The hit point of the
 // winning face is recomputed with the same test, so points are those of
 // k_cast_grid bit for bit.
-template <bool STATS>
+template <bool STATS, int WALK>
 __global__ void __launch_bounds__(256) k_cast_pooled(scene_desc S, const float* __restrict__ rays, long long n,
                                                      long long chunk, int* __restrict__ idx,
                                                      float* __restrict__ pts, unsigned long long* __restrict__ stats,
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) k_cast_pooled(scene_desc S, const float* 
   if (threadIdx.x == 0) s_next = 0;
   __syncthreads();
   uint32_t nt = 0, nc = 0;
-  pooled_walk<STATS>(
+  pooled_walk<STATS, WALK>(
       S, best + 64 * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
       [&](long long i, v3& o, v3& dir) { load_ray(rays, r0 + i, &o, &dir); },
       [&](long long i, v3 o, v3 dir, unsigned long long key) {
@@ -160,12 +160,12 @@ extern "C" int ffk_cast(ffk_scene* s, int method, const float* d_rays, long long
       const long long target = 8LL * ffk::device_cu_count();
       const long long chunk = std::min<long long>(16384, std::max<long long>(256, (n_rays + target - 1) / target));
       const unsigned blocks = (unsigned)((n_rays + chunk - 1) / chunk);
-      if (d_stats)
-        k_cast_pooled<true><<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, chunk, d_hit_idx, d_hit_pts, d_stats,
-                                                    s->tune_refill, s->tune_pool);
-      else
-        k_cast_pooled<false><<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, chunk, d_hit_idx, d_hit_pts,
-                                                     nullptr, s->tune_refill, s->tune_pool);
+      auto go = [&](auto kernel) {
+        kernel<<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, chunk, d_hit_idx, d_hit_pts, d_stats,
+                                       s->tune_refill, s->tune_pool);
+      };
+      if (d_stats) s->tune_walk == 1 ? go(k_cast_pooled<true, 1>) : go(k_cast_pooled<true, 2>);
+      else s->tune_walk == 1 ? go(k_cast_pooled<false, 1>) : go(k_cast_pooled<false, 2>);
     } else {
       const unsigned blocks = (unsigned)((n_rays + 255) / 256);
       if (d_stats)

@@ -125,6 +125,7 @@ struct fused_args {
   unsigned long long* stats;
   int refill;  // 0: one ray per lane at a time; else the idle-lane threshold of the refill loop
   int pool;    // 0: per-lane tests; else pool the tests once waiting lanes x pool >= active lanes
+  int walk;    // 0: per-wavefront choice of the empty-cell step; 1: always the jump code
 };
 
 // REFILL: lanes walk one grid cell per iteration and, once at least
@@ -181,13 +182,14 @@ __global__ void __launch_bounds__(1024)
         }
       }
     };
-    if (SCHED == 2) {
+    if (SCHED >= 2) {
       // As SCHED 1, with the triangle tests of the wavefront's waiting lanes
-      // pooled over its 64 lanes (pooled_walk).
+      // pooled over its 64 lanes (pooled_walk); SCHED 3 leaves every empty
+      // cell by the jump code.
       unsigned long long* best =
           reinterpret_cast<unsigned long long*>(hist + ((HIST == 0 ? 0 : (words + 1)) & ~1)) +
           64 * (threadIdx.x >> 6);
-      pooled_walk<STATS>(
+      pooled_walk<STATS, SCHED == 3 ? 1 : 2>(
           S, best, &s_next, T.g1 - T.g0, refill, pool,
           [&](long long i, v3& o, v3& dir) {
             const long long g = T.g0 + i;
@@ -337,7 +339,7 @@ static void set_lds_attr(size_t bytes) {
 // 64-bit best-hit word per lane.
 static size_t fused_lds(int hist, size_t hist_lds, int sched, int block) {
   size_t b = hist ? hist_lds : 0;
-  if (sched == 2) b = ((b + 7) & ~(size_t)7) + (size_t)block * 8;
+  if (sched >= 2) b = ((b + 7) & ~(size_t)7) + (size_t)block * 8;
   return b;
 }
 
@@ -360,18 +362,20 @@ static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStr
 }
 
 // Scheduling: 0 = one ray per lane, 1 = lane refill, 2 = lane refill with
-// pooled tests (when the best-hit words fit beside the histogram; the three
-// cast the same rays and tally the same hits).
-static int sched_of(int refill, int pool, int hist, size_t hist_lds, int block) {
+// pooled tests (when the best-hit words fit beside the histogram), 3 = as 2
+// with the jump code for every empty cell (walk 1). All cast the same rays
+// and tally the same hits.
+static int sched_of(int refill, int pool, int walk, int hist, size_t hist_lds, int block) {
   if (refill <= 0) return 0;
   if (pool <= 0 || fused_lds(hist, hist_lds, 2, block) > (size_t)max_lds_bytes()) return 1;
-  return 2;
+  return walk == 1 ? 3 : 2;
 }
 
 template <bool P, bool S>
 static hipError_t launch_refill(int hist, dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
-  const int sched = sched_of(A.refill, A.pool, hist, lds, block);
-  return sched == 2   ? launch_mode<P, S, 2>(hist, grid, block, lds, st, A)
+  const int sched = sched_of(A.refill, A.pool, A.walk, hist, lds, block);
+  return sched == 3   ? launch_mode<P, S, 3>(hist, grid, block, lds, st, A)
+         : sched == 2 ? launch_mode<P, S, 2>(hist, grid, block, lds, st, A)
          : sched == 1 ? launch_mode<P, S, 1>(hist, grid, block, lds, st, A)
                       : launch_mode<P, S, 0>(hist, grid, block, lds, st, A);
 }
@@ -468,6 +472,7 @@ struct ffk_batch {
   int block = 0, rpt = 0, n_blocks = 0;
   int refill = kDefaultRefill;
   int pool = kDefaultPool;
+  int walk = 0;
   // graph state
   hipStream_t cap = nullptr;  // capture stream (default priority)
   hipGraph_t graph = nullptr;
@@ -679,6 +684,21 @@ int ffk_set_refill(ffk_scene* s, int lanes, int pool) {
   return FFK_OK;
 }
 
+int ffk_set_walk(ffk_scene* s, int mode) {
+  FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
+  FFK_REQUIRE(mode == 0 || mode == 1, FFK_ERR_ARG, "walk mode %d is not 0 or 1", mode);
+  s->tune_walk = mode;
+  return FFK_OK;
+}
+
+int ffk_batch_set_walk(ffk_batch* b, int mode) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  FFK_REQUIRE(mode == 0 || mode == 1, FFK_ERR_ARG, "walk mode %d is not 0 or 1", mode);
+  b->walk = mode;
+  free_graph(b);
+  return FFK_OK;
+}
+
 int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool) {
   FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
   FFK_REQUIRE(lanes >= 0 && lanes <= 64, FFK_ERR_ARG, "refill lanes %d outside [0, 64]", lanes);
@@ -745,6 +765,7 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
   A.stats = d_stats;
   A.refill = s->tune_refill;
   A.pool = s->tune_pool;
+  A.walk = s->tune_walk;
   const int block = s->tune_block > 0 ? s->tune_block : default_block(hist, hist_bytes(hist, N));
   if (s->n_tiles > 0)
     FFK_HIP(launch_fused(hist, false, d_stats != nullptr, dim3((unsigned)s->n_tiles), block,
@@ -851,8 +872,9 @@ static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int
   int n_blocks = b->n_blocks;
   if (n_blocks <= 0) {
     int per_cu = 0;
-    const int sched = sched_of(b->refill, b->pool, hist, lds, block);
-    const hipError_t e = sched == 2   ? persistent_occupancy<2>(hist, block, lds, &per_cu)
+    const int sched = sched_of(b->refill, b->pool, b->walk, hist, lds, block);
+    const hipError_t e = sched == 3   ? persistent_occupancy<3>(hist, block, lds, &per_cu)
+                         : sched == 2 ? persistent_occupancy<2>(hist, block, lds, &per_cu)
                          : sched == 1 ? persistent_occupancy<1>(hist, block, lds, &per_cu)
                                       : persistent_occupancy<0>(hist, block, lds, &per_cu);
     if (e != hipSuccess || per_cu < 1) per_cu = 1;
@@ -887,6 +909,7 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
     A.stats = d_stats;
     A.refill = b->refill;
     A.pool = b->pool;
+    A.walk = b->walk;
     const int n_blocks = batch_blocks(b, k.hist, k.lds, block, k.n);
     FFK_HIP(launch_fused(k.hist, true, stats, dim3((unsigned)n_blocks), block, k.lds, st, A));
   }

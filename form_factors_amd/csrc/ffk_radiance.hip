@@ -62,7 +62,7 @@ __global__ void __launch_bounds__(256) k_radiance(scene_desc S, const long long*
 
 // The same deposit with lane refill and pooled triangle tests
 // (pooled_walk): block b casts walk positions [b * chunk, (b + 1) * chunk).
-template <bool STATS>
+template <bool STATS, int WALK>
 __global__ void __launch_bounds__(256)
     k_radiance_pooled(scene_desc S, const long long* __restrict__ walk_start, const int* __restrict__ walk_face,
                       int n_walk, const unsigned long long* __restrict__ q, long long n_rays, long long chunk,
@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256)
   __syncthreads();
   uint32_t nt = 0, nc = 0;
   int hint = find_face(S.face_off, S.n_faces, r0, 0), whint = 0;
-  pooled_walk<STATS>(
+  pooled_walk<STATS, WALK>(
       S, best + 64 * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
       [&](long long i, v3& o, v3& dir) {
         const long long g = r0 + i;
@@ -263,14 +263,12 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
     const long long target = 8LL * device_cu_count();
     const long long chunk = std::min<long long>(16384, std::max<long long>(256, (walk_total + target - 1) / target));
     const unsigned blocks = (unsigned)((walk_total + chunk - 1) / chunk);
-    if (d_stats)
-      k_radiance_pooled<true><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk,
-                                                      s->d_walk_q, walk_total, chunk, k0, k1, s->d_rad_acc, d_stats,
-                                                      s->tune_refill, s->tune_pool);
-    else
-      k_radiance_pooled<false><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk,
-                                                       s->d_walk_q, walk_total, chunk, k0, k1, s->d_rad_acc,
-                                                       nullptr, s->tune_refill, s->tune_pool);
+    auto go = [&](auto kernel) {
+      kernel<<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q, walk_total,
+                                     chunk, k0, k1, s->d_rad_acc, d_stats, s->tune_refill, s->tune_pool);
+    };
+    if (d_stats) s->tune_walk == 1 ? go(k_radiance_pooled<true, 1>) : go(k_radiance_pooled<true, 2>);
+    else s->tune_walk == 1 ? go(k_radiance_pooled<false, 1>) : go(k_radiance_pooled<false, 2>);
   } else if (walk_total > 0) {
     const long long blocks_needed = (walk_total + 255) / 256;
     const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));

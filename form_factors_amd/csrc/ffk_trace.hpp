@@ -205,20 +205,31 @@ __device__ inline bool walk_cell(const scene_desc& S, v3 o, v3 dir, float ix, fl
 
 // Leaves the current cell after its tests (the second half of walk_cell):
 // past the empty box of radius R (R >= 1) or to the next cell of the DDA
-// (R = 0). Returns false when the walk is over.
+// (R = 0). Returns false when the walk is over. UNIFIED takes the box code
+// for every R >= 0: with R = 0 the box is the cell itself and the next cell,
+// the exit test and the new boundaries are those of the DDA step, so a
+// caller may run one code path for lanes that jump and lanes that step. The
+// grid fields are read into scalars and the moving axis is chosen with
+// selects: written with arrays or a branch per axis, the compiler keeps the
+// scene descriptor and the walk state in scratch memory.
+template <bool UNIFIED = false>
 __device__ inline bool walk_leave(const grid_desc& g, v3 o, v3 dir, float ix, float iy, float iz, float dlen, int R,
                                   int& cx, int& cy, int& cz, float& tnx, float& tny, float& tnz, int& cell,
                                   float best_d2, int best_f) {
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int nxy = nx * ny;
   const int sx = walk_sign(dir.x), sy = walk_sign(dir.y), sz = walk_sign(dir.z);
-  if (R >= 1) {
+  const float lo0 = g.lo[0], lo1 = g.lo[1], lo2 = g.lo[2];
+  const float h0 = g.h[0], h1 = g.h[1], h2 = g.h[2];
+  const float ih0 = g.inv_h[0], ih1 = g.inv_h[1], ih2 = g.inv_h[2];
+  if (UNIFIED || R >= 1) {
+    R = R < 0 ? 0 : R;
     const int hx = sx > 0 ? min(cx + R, nx - 1) : max(cx - R, 0);
     const int hy = sy > 0 ? min(cy + R, ny - 1) : max(cy - R, 0);
     const int hz = sz > 0 ? min(cz + R, nz - 1) : max(cz - R, 0);
-    const float bx = walk_boundary(g.lo[0], g.h[0], hx, sx, o.x, ix);
-    const float by = walk_boundary(g.lo[1], g.h[1], hy, sy, o.y, iy);
-    const float bz = walk_boundary(g.lo[2], g.h[2], hz, sz, o.z, iz);
+    const float bx = walk_boundary(lo0, h0, hx, sx, o.x, ix);
+    const float by = walk_boundary(lo1, h1, hy, sy, o.y, iy);
+    const float bz = walk_boundary(lo2, h2, hz, sz, o.z, iz);
     const bool ex = (bx <= by) && (bx <= bz);
     const bool ey = !ex && (by <= bz);
     const float tb = ex ? bx : (ey ? by : bz);
@@ -226,22 +237,24 @@ __device__ inline bool walk_leave(const grid_desc& g, v3 o, v3 dir, float ix, fl
     const float de = tb * dlen;
     if (best_f >= 0 && best_d2 < de * de * 0.9999f) return false;
     const float px = o.x + dir.x * tb, py = o.y + dir.y * tb, pz = o.z + dir.z * tb;
-    int nxc = min(max((int)floorf((px - g.lo[0]) * g.inv_h[0]), max(cx - R, 0)), min(cx + R, nx - 1));
-    int nyc = min(max((int)floorf((py - g.lo[1]) * g.inv_h[1]), max(cy - R, 0)), min(cy + R, ny - 1));
-    int nzc = min(max((int)floorf((pz - g.lo[2]) * g.inv_h[2]), max(cz - R, 0)), min(cz + R, nz - 1));
+    int nxc = min(max((int)floorf((px - lo0) * ih0), max(cx - R, 0)), min(cx + R, nx - 1));
+    int nyc = min(max((int)floorf((py - lo1) * ih1), max(cy - R, 0)), min(cy + R, ny - 1));
+    int nzc = min(max((int)floorf((pz - lo2) * ih2), max(cz - R, 0)), min(cz + R, nz - 1));
     nxc = sx > 0 ? max(nxc, cx) : (sx < 0 ? min(nxc, cx) : cx);
     nyc = sy > 0 ? max(nyc, cy) : (sy < 0 ? min(nyc, cy) : cy);
     nzc = sz > 0 ? max(nzc, cz) : (sz < 0 ? min(nzc, cz) : cz);
-    if (ex) nxc = hx + sx;
-    else if (ey) nyc = hy + sy;
-    else nzc = hz + sz;
+    // (selects, not branches: a branch per axis is turned into an indexed
+    // store to a scratch array)
+    nxc = ex ? hx + sx : nxc;
+    nyc = (!ex && ey) ? hy + sy : nyc;
+    nzc = (!ex && !ey) ? hz + sz : nzc;
     if (nxc < 0 || nxc >= nx || nyc < 0 || nyc >= ny || nzc < 0 || nzc >= nz) return false;
     cx = nxc;
     cy = nyc;
     cz = nzc;
-    tnx = walk_boundary(g.lo[0], g.h[0], cx, sx, o.x, ix);
-    tny = walk_boundary(g.lo[1], g.h[1], cy, sy, o.y, iy);
-    tnz = walk_boundary(g.lo[2], g.h[2], cz, sz, o.z, iz);
+    tnx = walk_boundary(lo0, h0, cx, sx, o.x, ix);
+    tny = walk_boundary(lo1, h1, cy, sy, o.y, iy);
+    tnz = walk_boundary(lo2, h2, cz, sz, o.z, iz);
     cell = cx + nx * cy + nxy * cz;
     return true;
   }
@@ -252,22 +265,19 @@ __device__ inline bool walk_leave(const grid_desc& g, v3 o, v3 dir, float ix, fl
   if (best_f >= 0 && best_d2 < de * de * 0.9999f) return false;
   const int st = ax ? sx : (ay ? sy : sz);
   if (st == 0) return false;
-  if (ax) {
-    cx += sx;
-    if (cx < 0 || cx >= nx) return false;
-    cell += sx;
-    tnx = walk_boundary(g.lo[0], g.h[0], cx, sx, o.x, ix);
-  } else if (ay) {
-    cy += sy;
-    if (cy < 0 || cy >= ny) return false;
-    cell += sy * nx;
-    tny = walk_boundary(g.lo[1], g.h[1], cy, sy, o.y, iy);
-  } else {
-    cz += sz;
-    if (cz < 0 || cz >= nz) return false;
-    cell += sz * nxy;
-    tnz = walk_boundary(g.lo[2], g.h[2], cz, sz, o.z, iz);
-  }
+  // One axis moves; its operands are selected rather than branched on (a
+  // branch per axis is turned into an indexed store to a scratch array).
+  const int c_new = (ax ? cx : (ay ? cy : cz)) + st;
+  if (c_new < 0 || c_new >= (ax ? nx : (ay ? ny : nz))) return false;
+  const float tb_new = walk_boundary(ax ? lo0 : (ay ? lo1 : lo2), ax ? h0 : (ay ? h1 : h2), c_new, st,
+                                     ax ? o.x : (ay ? o.y : o.z), ax ? ix : (ay ? iy : iz));
+  cx = ax ? c_new : cx;
+  cy = (!ax && ay) ? c_new : cy;
+  cz = (!ax && !ay) ? c_new : cz;
+  tnx = ax ? tb_new : tnx;
+  tny = (!ax && ay) ? tb_new : tny;
+  tnz = (!ax && !ay) ? tb_new : tnz;
+  cell += ax ? st : (ay ? st * nx : st * nxy);
   return true;
 }
 
@@ -293,14 +303,18 @@ __device__ inline void decode_key(unsigned long long key, float& d2, int& f) {
 // into the owning ray's best hit with a 64-bit LDS atomic minimum on hit_key
 // (the rule of `consider`). Once at least `refill` lanes are idle, they take
 // the next rays of the list (one LDS atomic on *next per refill). The hits
-// are those of cast_grid, whatever the order.
+// are those of cast_grid, whatever the order. Lanes in an empty cell leave
+// it by the jump past its empty box or, next to occupied cells, by one DDA
+// step; WALK 2 runs the jump code for all of a wavefront's empty-cell lanes
+// when any of them can jump and the DDA step otherwise (never both), WALK 1
+// always runs the jump code (walk_leave<true>: the same cells).
 //   best:  this wavefront's 64 words of LDS (8-byte aligned)
 //   next:  the list's LDS ray counter, zero at the start, shared by the
 //          block's wavefronts
 //   start(i, o, dir):            sets up ray i
 //   finish(i, o, dir, key):      ray i is done; key = hit_key of its hit or kNoHit
 // Every lane of the wavefront must call this together.
-template <bool STATS, class Start, class Finish>
+template <bool STATS, int WALK, class Start, class Finish>
 __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best, int* next, long long n, int refill,
                                    int pool, Start start, Finish finish, uint32_t& n_tests, uint32_t& n_cells) {
   const int lane = threadIdx.x & 63;
@@ -334,22 +348,31 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
     }
     if (!more && __ballot(have) == 0ull) break;
     // Lanes that are not waiting for tests advance by one cell.
-    if (have && !pend) {
-      const int2 ci = S.cell_info[cell];
+    int2 ci = make_int2(0, 0);
+    const bool step = have && !pend;
+    if (step) {
+      ci = S.cell_info[cell];
       if (STATS) n_cells += 1;
       if (ci.y > 0) {
         pend = true;
         ref0 = ci.x;
         nref = ci.y;
-      } else {
-        const unsigned long long key = best[lane];
-        float bd;
-        int bf;
-        decode_key(key, bd, bf);
-        if (!walk_leave(S.g, o, dir, ix, iy, iz, dlen, ci.x - 1, cx, cy, cz, tnx, tny, tnz, cell, bd, bf)) {
-          finish(cur, o, dir, key);
-          have = false;
-        }
+      }
+    }
+    const bool empty = step && ci.y <= 0;  // then ci.x - 1 is the empty radius
+    const bool jump = WALK == 1 || __ballot(empty && ci.x >= 2) != 0ull;
+    if (empty) {
+      const unsigned long long key = best[lane];
+      float bd;
+      int bf;
+      decode_key(key, bd, bf);
+      const bool on = jump ? walk_leave<true>(S.g, o, dir, ix, iy, iz, dlen, ci.x - 1, cx, cy, cz, tnx, tny, tnz,
+                                              cell, bd, bf)
+                           : walk_leave<false>(S.g, o, dir, ix, iy, iz, dlen, 0, cx, cy, cz, tnx, tny, tnz, cell,
+                                               bd, bf);
+      if (!on) {
+        finish(cur, o, dir, key);
+        have = false;
       }
     }
     const unsigned long long waiting = __ballot(have && pend);

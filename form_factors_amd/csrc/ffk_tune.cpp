@@ -8,7 +8,7 @@
 // resolutions with the scene's ray scheduling. When a full run is short
 // (estimated from the sample) and F fits, the three fastest resolutions are
 // then timed with the fused kernel itself on the full layout, and the
-// fused kernel's block size after that. The choice is cached per scene
+// fused kernel's block size and walk mode after that. The choice is cached per scene
 // signature (faces, meshes, ray counts, frame) for the life of the process.
 #include <hip/hip_runtime_api.h>
 
@@ -27,6 +27,7 @@ namespace {
 struct tuned {
   int dims[3];
   int block;
+  int walk;
 };
 
 std::mutex g_mu;
@@ -78,7 +79,7 @@ void dims_for(const float lo[3], const float hi[3], double cells_per_face, int n
 
 extern "C" {
 
-int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_block, void* stream) {
+int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_block, int* out_walk, void* stream) {
   if (!s || sample_rays <= 0) return FFK_ERR_ARG;
   int nf = 0, nm = 0;
   long long n_rays = 0;
@@ -165,6 +166,7 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
     // the resolutions a factor sqrt(2) in cells either side of the fastest,
     // then the block size
     t.block = 0;
+    t.walk = 0;
     const double est_ms = trials[0].ms * (double)n_rays / (double)std::max(1LL, n_sample);
     const size_t nn = (size_t)nm * nm;
     const bool need_tally = ffk_ff_needs_tally(s) != 0;
@@ -213,7 +215,19 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
             t.block = block;
           }
         }
+        float best_w = 3.0e38f;
+        for (int walk : {0, 1}) {
+          if (r != FFK_OK) break;
+          if ((r = ffk_set_tuning(s, t.block, 0, -1)) < 0 || (r = ffk_set_walk(s, walk)) < 0) break;
+          float ms = 0.f;
+          r = time_ms(st, 2, [&] { return ffk_ff_fused(s, FFK_CAST_GRID, 1, d_tally, d_F, nullptr, stream); }, &ms);
+          if (r == FFK_OK && ms < best_w) {
+            best_w = ms;
+            t.walk = walk;
+          }
+        }
         (void)ffk_set_tuning(s, 0, 0, -1);
+        (void)ffk_set_walk(s, 0);
       }
       (void)hipFree(d_F);
       (void)hipFree(d_tally);
@@ -226,8 +240,10 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
     if ((r = ffk_scene_build_grid(s, t.dims[0], t.dims[1], t.dims[2], stream)) < 0) return r;
   }
   if ((r = ffk_set_tuning(s, t.block, 0, -1)) < 0) return r;
+  if ((r = ffk_set_walk(s, t.walk)) < 0) return r;
   if (out_dims3) std::copy(t.dims, t.dims + 3, out_dims3);
   if (out_block) *out_block = t.block;
+  if (out_walk) *out_walk = t.walk;
   return hit ? 1 : FFK_OK;
 }
 

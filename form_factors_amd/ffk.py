@@ -40,7 +40,7 @@ SIGNATURES = {
     "ffk_scene_grid_info": (C.c_int, [_vp, _i, _f, _f, C.POINTER(_ll)]),
     "ffk_scene_counts": (C.c_int, [_vp, _i, _i, C.POINTER(_ll)]),
     "ffk_scene_layout": (C.c_int, [_vp, _f, _i, _i, _i, _i]),
-    "ffk_autotune": (C.c_int, [_vp, _ll, _i, _i, _vp]),
+    "ffk_autotune": (C.c_int, [_vp, _ll, _i, _i, _i, _vp]),
     "ffk_cast": (C.c_int, [_vp, C.c_int, _vp, _ll, _vp, _vp, _vp, _vp]),
     "ffk_emitted_counts": (C.c_int, [_f, C.c_int, C.c_int, C.c_float, _i, _i]),
     "ffk_area_weights": (C.c_int, [_f, C.c_int, _f, _f]),
@@ -54,6 +54,8 @@ SIGNATURES = {
     "ffk_ff_needs_tally": (C.c_int, [_vp]),
     "ffk_set_refill": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ffk_batch_set_refill": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ffk_set_walk": (C.c_int, [_vp, C.c_int]),
+    "ffk_batch_set_walk": (C.c_int, [_vp, C.c_int]),
     "ffk_batch_tally_flags": (C.c_int, [_vp, _i]),
     "ffk_batch_create": (C.c_int, [C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp]),
     "ffk_batch_destroy": (None, [_vp]),
@@ -314,12 +316,14 @@ class Scene:
         return {"dims": tuple(dims), "lo": tuple(lo), "hi": tuple(hi), "n_refs": nr.value}
 
     def autotune(self, sample_rays=1 << 22):
-        """Grid resolution and block size from timed trials (cached per scene
-        signature). Returns {"dims", "block", "cached"}."""
+        """Grid resolution, block size and walk mode from timed trials (cached
+        per scene signature). Returns {"dims", "block", "walk", "cached"}."""
         dims = (C.c_int * 3)()
         block = C.c_int()
-        rc = check(lib().ffk_autotune(self.h, int(sample_rays), dims, C.byref(block), _stream()), "ffk_autotune")
-        return {"dims": tuple(dims), "block": block.value, "cached": rc == 1}
+        walk = C.c_int()
+        rc = check(lib().ffk_autotune(self.h, int(sample_rays), dims, C.byref(block), C.byref(walk), _stream()),
+                   "ffk_autotune")
+        return {"dims": tuple(dims), "block": block.value, "walk": walk.value, "cached": rc == 1}
 
     def set_tuning(self, block=0, rays_per_tile=0, hist=-1):
         check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist), "ffk_set_tuning")
@@ -328,6 +332,10 @@ class Scene:
         """Fused-kernel ray scheduling: refill threshold (idle lanes of 64,
         0 = one ray per lane) and pooled-test ratio (0 = per-lane tests)."""
         check(lib().ffk_set_refill(self.h, int(lanes), int(pool)), "ffk_set_refill")
+
+    def set_walk(self, mode):
+        """Empty-cell step of the pooled walk: 0 = per-wavefront choice, 1 = always the jump."""
+        check(lib().ffk_set_walk(self.h, int(mode)), "ffk_set_walk")
 
     # -- device calls (torch tensors in, torch tensors out) --
     def cast(self, rays, method=CAST_GRID, points=True, stats=None):
@@ -420,6 +428,9 @@ class Batch:
 
     def set_refill(self, lanes, pool=4):
         check(lib().ffk_batch_set_refill(self.h, int(lanes), int(pool)), "ffk_batch_set_refill")
+
+    def set_walk(self, mode):
+        check(lib().ffk_batch_set_walk(self.h, int(mode)), "ffk_batch_set_walk")
 
     def tally_flags(self):
         """Scenes that need a uint64 tally buffer in run()."""

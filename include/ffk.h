@@ -157,6 +157,13 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
  * tallies, the same nearest-hit rule). */
 int ffk_set_refill(ffk_scene* s, int lanes, int pool);
 int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool);
+/* How pooled lanes leave an empty grid cell: past its empty box (the
+ * distance-field jump) or, next to occupied cells, by one DDA step. mode 0
+ * (default): a wavefront runs the jump code for all its empty-cell lanes when
+ * any of them can jump, the DDA step otherwise; mode 1: always the jump code
+ * (fastest when rays cross large empty regions). Same cells, same hits. */
+int ffk_set_walk(ffk_scene* s, int mode);
+int ffk_batch_set_walk(ffk_batch* b, int mode);
 
 /* 1 when ffk_ff_fused needs d_tally for this scene and tuning (the rays of a
  * mesh span several tiles, or the row histogram does not fit in LDS), else 0. */
@@ -186,14 +193,16 @@ int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags);
 /* ---- autotuning ------------------------------------------------------------------
  * Picks the grid resolution (a ladder of 0.25 .. 16 cells per face, each
  * timed by casting the scene's own Malley rays with the emission layout
- * scaled to about sample_rays) and the fused kernel's block size (256, 512 or
- * 1024, timed on the full layout when a run is estimated under 250 ms), then
- * leaves the scene built and tuned with them. Results are cached per scene
- * signature (faces, meshes, ray counts, frame) for the life of the process.
- * Returns 0 when tuned, 1 when taken from the cache. The grid resolution
- * never changes hit indices. out_dims3 / out_block may be NULL (block 0 = the
- * automatic choice was kept). */
-int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_block, void* stream);
+ * scaled to about sample_rays; when a full run is estimated under 250 ms and
+ * F fits in half the free memory, the best candidates are re-timed with the
+ * fused kernel on the full layout), the fused kernel's block size (256, 512
+ * or 1024) and the walk mode (ffk_set_walk), then leaves the scene built and
+ * tuned with them. Results are cached per scene signature (faces, meshes, ray
+ * counts, frame) for the life of the process. Returns 0 when tuned, 1 when
+ * taken from the cache. None of the choices changes hit indices. out_dims3 /
+ * out_block / out_walk may be NULL (block 0 = the automatic choice was kept). */
+int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_block, int* out_walk,
+                 void* stream);
 
 /* ---- radiance equation --------------------------------------------------------
  * radiance_equation::calculate (thermal_equation/radiance_cpu.cpp:132-190) on

@@ -80,11 +80,13 @@ def test_batched_tuning_does_not_change_bits(gpu):
     seeds = [3, 4]
     plain = [sc.ff_fused(s).cpu().numpy() for sc, s in zip(scs, seeds)]
     b = gpu.Batch(scs)
-    for block, rays_per_tile, n_blocks, refill, pool in [(256, 1024, 0, 0, 0), (512, 4096, 64, 16, 0),
-                                                          (1024, 0, 7, 0, 0), (512, 0, 0, 32, 0), (512, 0, 0, 16, 4),
-                                                          (256, 4096, 9, 8, 1), (1024, 0, 0, 16, 8)]:
+    for block, rays_per_tile, n_blocks, refill, pool, walk in [
+            (256, 1024, 0, 0, 0, 0), (512, 4096, 64, 16, 0, 0), (1024, 0, 7, 0, 0, 0), (512, 0, 0, 32, 0, 0),
+            (512, 0, 0, 16, 4, 0), (256, 4096, 9, 8, 1, 0), (1024, 0, 0, 16, 8, 0), (512, 0, 0, 16, 4, 1),
+            (256, 4096, 9, 8, 1, 1)]:
         b.set_tuning(block, rays_per_tile, n_blocks)
         b.set_refill(refill, pool)
+        b.set_walk(walk)
         Fs = [torch.empty((sc.n_meshes,) * 2, device="cuda") for sc in scs]
         tallies = [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda") for sc in scs]
         b.run(seeds, Fs, tallies=tallies, use_graph=True)

@@ -78,9 +78,10 @@ def test_grid_cast_pooled_equals_per_ray(gpu, oracle, n):
     if n <= 5000:
         want_i, _ = oracle.naive_cast(faces, rays, n_threads=8)
         assert np.array_equal(ref_i, want_i)
-    for refill, pool in [(16, 4), (1, 1), (64, 64), (8, 2), (32, 16)]:
-        print(f"pooled cast n={n} refill={refill} pool={pool}", flush=True)
+    for refill, pool, walk in [(16, 4, 0), (1, 1, 0), (64, 64, 0), (8, 2, 0), (32, 16, 0), (16, 4, 1), (1, 1, 1)]:
+        print(f"pooled cast n={n} refill={refill} pool={pool} walk={walk}", flush=True)
         sc.set_refill(refill, pool)
+        sc.set_walk(walk)
         idx, pts = sc.cast(t(rays), method=1)
         idx, pts = idx.cpu().numpy(), pts.cpu().numpy()
         assert np.array_equal(idx, ref_i), (refill, pool)
@@ -195,10 +196,10 @@ def test_fused_ff_tiling_and_histograms(gpu, oracle, hist, rays_per_tile):
     assert np.array_equal(bits(F), bits(want))
 
 
-@pytest.mark.parametrize("refill,pool", [(1, 0), (16, 0), (48, 0), (64, 0), (1, 1), (16, 4), (16, 1), (64, 64),
-                                         (8, 2)])
+@pytest.mark.parametrize("refill,pool,walk", [(1, 0, 0), (16, 0, 0), (48, 0, 0), (64, 0, 0), (1, 1, 0), (16, 4, 0),
+                                              (16, 1, 0), (64, 64, 0), (8, 2, 0), (16, 4, 1), (1, 1, 1), (64, 8, 1)])
 @pytest.mark.parametrize("hist,rays_per_tile", [(0, 0), (2, 1000), (1, 0)])
-def test_fused_ff_refill_bit_exact(gpu, oracle, refill, pool, hist, rays_per_tile):
+def test_fused_ff_refill_bit_exact(gpu, oracle, refill, pool, walk, hist, rays_per_tile):
     """Lanes that take new rays as soon as enough of them are idle, with the
     triangle tests of waiting lanes pooled over the wavefront or not, cast the
     same rays: F is the oracle's bit for bit (global tally, split tiles and
@@ -210,6 +211,7 @@ def test_fused_ff_refill_bit_exact(gpu, oracle, refill, pool, hist, rays_per_til
     sc = make_scene(gpu, faces, meshes, n_rays=120_000, frame=1)
     sc.set_tuning(block=256, rays_per_tile=rays_per_tile, hist=hist)
     sc.set_refill(refill, pool)
+    sc.set_walk(walk)
     F = sc.ff_fused(seed=17).cpu().numpy()
     r, want = oracle.ff_calculate(faces, meshes, 120_000, seed=17, frame=1, n_threads=8)
     assert np.array_equal(bits(F), bits(want))

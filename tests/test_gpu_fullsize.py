@@ -37,13 +37,15 @@ def test_full_size_f_is_schedule_and_launch_independent(gpu, sphere):
     G = torch.empty((N, N), device="cuda")
     sc.set_refill(16, 4)
     sc.ff_fused(SEED, F=F)
-    for refill, pool in [(0, 0), (16, 0)]:
-        print(f"full size refill={refill} pool={pool}", flush=True)
+    for refill, pool, walk in [(0, 0, 0), (16, 0, 0), (16, 4, 1)]:
+        print(f"full size refill={refill} pool={pool} walk={walk}", flush=True)
         sc.set_refill(refill, pool)
+        sc.set_walk(walk)
         sc.ff_fused(SEED, F=G)
         torch.cuda.synchronize()
-        assert torch.equal(F.view(torch.int32), G.view(torch.int32)), (refill, pool)
+        assert torch.equal(F.view(torch.int32), G.view(torch.int32)), (refill, pool, walk)
     sc.set_refill(16, 4)
+    sc.set_walk(0)
     b = gpu.Batch([sc])
     G.fill_(-1.0)
     b.run([SEED], [G], use_graph=True)

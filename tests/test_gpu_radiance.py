@@ -83,9 +83,10 @@ def test_radiance_schedules_agree(gpu, case):
     sc = gpu.Scene(faces, meshes)
     sc.build_grid()
     out = []
-    for refill, pool in [(0, 0), (16, 0), (16, 4), (1, 1), (64, 64)]:
-        print(f"radiance {name} refill={refill} pool={pool}", flush=True)
+    for refill, pool, walk in [(0, 0, 0), (16, 0, 0), (16, 4, 0), (1, 1, 0), (64, 64, 0), (16, 4, 1)]:
+        print(f"radiance {name} refill={refill} pool={pool} walk={walk}", flush=True)
         sc.set_refill(refill, pool)
+        sc.set_walk(walk)
         em, ab = sc.radiance_step(eps, T, n_rays, 99)
         out.append(ab)
     for ab in out[1:]:
