@@ -420,6 +420,24 @@ def main():
     flops = FLOP_EMIT * n_rays + FLOP_TEST * n_tests + FLOP_STEP * n_cells
     log(f"work per step: {n_tests / n_rays:.2f} tests/ray, {n_cells / n_rays:.2f} cells/ray, {flops / 1e12:.3f} TFLOP")
 
+    # Walk mode (same hits either way): unless given, one untimed launch of
+    # each, the faster kept -- what ffk_autotune does for the walk mode,
+    # without changing the grid this configuration fixes.
+    walk_times = {}
+    if args.walk < 0:
+        for m in (0, 1):
+            sc.set_walk(m)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            sc.ff_fused(seed, F=F, need_tally=False)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            walk_times[m] = e0.elapsed_time(e1)
+        walk = min(walk_times, key=walk_times.get)
+        sc.set_walk(walk)
+        log(f"walk mode {walk} (one launch each: {walk_times[0]:.2f} / {walk_times[1]:.2f} ms)")
+    else:
+        walk = args.walk
     for _ in range(args.warmup):
         sc.ff_fused(seed, F=F, need_tally=False)
     torch.cuda.synchronize()
@@ -471,7 +489,8 @@ def main():
         "config": {"workload": f"closed icosphere interior, {len(faces)} triangles (inward), one mesh per "
                                f"triangle, {n_rays} Malley rays (~{args.rays_per_face}/triangle), "
                                f"grid {args.grid}^3, plain fused emit-cast-tally kernel, F {N}x{N} fp32",
-                   "n_faces": len(faces), "n_rays": n_rays, "grid": list(g["dims"]), "parallelism": "single-gpu"},
+                   "n_faces": len(faces), "n_rays": n_rays, "grid": list(g["dims"]), "parallelism": "single-gpu",
+                   "walk_mode": walk, "walk_probe_ms": {str(k): round(v, 3) for k, v in walk_times.items()}},
         "roofline": {"bound": "mfma", "achieved": round(flops / (kern_ms * 1e-3) / 1e12, 3), "peak": PEAK_FP32,
                      "unit": "TFLOP/s", "frac": round(flops / (kern_ms * 1e-3) / 1e12 / PEAK_FP32, 4),
                      "traffic": traffic_from_profiles("k_ff_fused_icosphere20k_traffic.json"),
@@ -490,6 +509,7 @@ def main():
         # The same scene through the persistent, graph-replayed launch.
         try:
             b = ffk.Batch([sc])
+            b.set_walk(walk)
             b.run([seed], [F], use_graph=True)
             torch.cuda.synchronize()
             reps = max(2, args.steps // 2)
