@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256) k_cast_pooled(scene_desc S, const float* 
         if (h.f >= 0 && pts) {
           const float4* q = reinterpret_cast<const float4*>(S.tri + h.f);
           float d2;
-          (void)hit_terms(q[0], q[1], q[2], q[3], o, dir, &h.p, &d2);
+          (void)hit_terms(gld4(q), gld4(q + 1), gld4(q + 2), gld4(q + 3), o, dir, &h.p, &d2);
         }
         store_hit(h, r0 + i, idx, pts);
       },

@@ -13,7 +13,7 @@ __global__ void __launch_bounds__(256) k_emit_malley(scene_desc S, uint32_t k0, 
   long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= S.n_rays) return;
   int f = find_face(S.face_off, S.n_faces, g, 0);
-  int j = (int)(g - S.face_off[f]);
+  int j = (int)(g - gld(S.face_off + f));
   v3 o, e;
   malley_ray(S, f, j, k0, k1, &o, &e);
   float* r = rays + 6 * g;
@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256) k_tally(scene_desc S, const int* __restri
   if (g >= S.mesh_off[S.n_meshes]) return;  // rays past the mesh walk are not counted
   int h = hit[g];
   if (h < 0) return;
-  int dst = S.face2mesh[h];
+  int dst = gld(S.face2mesh + h);
   if (dst < 0) return;
   int src = find_mesh(S.mesh_off, S.n_meshes, g);
   atomicAdd(&tally[(size_t)src * S.n_meshes + dst], 1ull);
@@ -174,7 +174,7 @@ __global__ void __launch_bounds__(1024)
     int hint = find_face(S.face_off, S.n_faces, T.g0, 0);
     auto count_hit = [&](const hit_rec& h) {
       if (h.f >= 0) {
-        const int m = S.face2mesh[h.f];
+        const int m = gld(S.face2mesh + h.f);
         if (m >= 0) {
           if (HIST == 0) atomicAdd(&tally[(size_t)T.mesh * N + m], 1ull);
           else if (HIST == 1) atomicAdd(&hist[m], 1u);
@@ -196,7 +196,7 @@ __global__ void __launch_bounds__(1024)
             const int f = find_face(S.face_off, S.n_faces, g, hint);
             hint = f;
             v3 e;
-            malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+            malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
             dir = e - o;
           },
           [&](long long, v3, v3, unsigned long long key) {
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(1024)
               const int f = find_face(S.face_off, S.n_faces, g, hint);
               hint = f;
               v3 e;
-              malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+              malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
               dir = e - o;
               h.d2 = 3.402823466e38f;
               h.f = -1;
@@ -252,7 +252,7 @@ __global__ void __launch_bounds__(1024)
         const int f = find_face(S.face_off, S.n_faces, g, hint);
         hint = f;
         v3 o, e;
-        malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+        malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
         hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
         cast_grid<STATS>(S, o, e - o, h, nt, nc);
         count_hit(h);

@@ -163,6 +163,36 @@ __host__ __device__ inline void rotate_towards(v3 s, v3 to, float* m) {
   for (int i = 0; i < 9; ++i) m[i] = r[i];
 }
 
+// Loads through the global address space. The scene tables are reached
+// through a descriptor read from memory, so the compiler cannot prove their
+// pointers global and emits flat instructions, which count against both the
+// vector-memory and the LDS wait counters: every table load then also waits
+// for the walk's cross-lane LDS shuffles. The casts make them global loads.
+#define FFK_GMEM __attribute__((address_space(1)))
+#ifdef __HIP_DEVICE_COMPILE__
+template <class T>
+__device__ inline T gld(const T* p) {
+  return *(const FFK_GMEM T*)p;
+}
+__device__ inline int2 gld2(const int2* p) {
+  typedef int i2v __attribute__((ext_vector_type(2)));
+  const i2v v = *(const FFK_GMEM i2v*)p;
+  return make_int2(v.x, v.y);
+}
+__device__ inline float4 gld4(const float4* p) {
+  typedef float f4v __attribute__((ext_vector_type(4)));
+  const f4v v = *(const FFK_GMEM f4v*)p;
+  return make_float4(v.x, v.y, v.z, v.w);
+}
+#else
+template <class T>
+__host__ __device__ inline T gld(const T* p) {
+  return *p;
+}
+__host__ __device__ inline int2 gld2(const int2* p) { return *p; }
+__host__ __device__ inline float4 gld4(const float4* p) { return *p; }
+#endif
+
 // mat33 * vec3 with the reference's column access (math/mat.h:30-37).
 __host__ __device__ inline v3 mat_vec(const float* m, v3 v) {
   return mk((m[0] * v.x) + (m[3] * v.y) + (m[6] * v.z), (m[1] * v.x) + (m[4] * v.y) + (m[7] * v.z),

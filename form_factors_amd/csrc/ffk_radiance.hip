@@ -34,16 +34,16 @@ __global__ void __launch_bounds__(256) k_radiance(scene_desc S, const long long*
     const int f = find_face(S.face_off, S.n_faces, g, hint);
     hint = f;
     v3 o, e;
-    malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+    malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
     hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
     cast_grid<STATS>(S, o, e - o, h, nt, nc);
     if (h.f < 0) continue;
-    const int m = S.face2mesh[h.f];
+    const int m = gld(S.face2mesh + h.f);
     if (m < 0) continue;
     const int k = find_face(walk_start, n_walk, g, whint);
     whint = k;
     const int wf = walk_face[k];
-    const unsigned long long v = q[2 * k + ((g - walk_start[k]) < S.front[wf] ? 0 : 1)];
+    const unsigned long long v = q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)];
     if (v) atomicAdd(&acc[m], v);
   }
   if (STATS) {
@@ -83,7 +83,7 @@ __global__ void __launch_bounds__(256)
         const int f = find_face(S.face_off, S.n_faces, g, hint);
         hint = f;
         v3 e;
-        malley_ray(S, f, (int)(g - S.face_off[f]), k0, k1, &o, &e);
+        malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
         dir = e - o;
       },
       [&](long long i, v3, v3, unsigned long long key) {
@@ -91,13 +91,13 @@ __global__ void __launch_bounds__(256)
         int hf;
         decode_key(key, d2, hf);
         if (hf < 0) return;
-        const int m = S.face2mesh[hf];
+        const int m = gld(S.face2mesh + hf);
         if (m < 0) return;
         const long long g = r0 + i;
         const int k = find_face(walk_start, n_walk, g, whint);
         whint = k;
         const int wf = walk_face[k];
-        const unsigned long long v = q[2 * k + ((g - walk_start[k]) < S.front[wf] ? 0 : 1)];
+        const unsigned long long v = q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)];
         if (v) atomicAdd(&acc[m], v);
       },
       nt, nc);

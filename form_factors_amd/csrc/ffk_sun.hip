@@ -41,11 +41,11 @@ __global__ void __launch_bounds__(256) k_sun(scene_desc S, v3 dir, v3 center, fl
     hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
     cast_grid<STATS>(S, o, e - o, h, nt, nc);
     if (h.f < 0) continue;
-    const int m = S.face2mesh[h.f];
+    const int m = gld(S.face2mesh + h.f);
     if (m < 0) continue;
     // math::triangle_normal = (p1 - p0) x (p2 - p0), the per-face u x v
     const float4* q = reinterpret_cast<const float4*>(S.tri + h.f);
-    const float4 q1 = q[1], q2 = q[2];
+    const float4 q1 = gld4(q + 1), q2 = gld4(q + 2);
     const v3 nrm = cross(mk(q1.x, q1.y, q1.z), mk(q2.x, q2.y, q2.z));
     const int side = dot(nrm, dir) < 0 ? 0 : 1;
     atomicAdd(&counts[2 * (size_t)m + side], 1ull);

@@ -24,7 +24,7 @@ __device__ inline void consider(hit_rec& h, int f, float d2, v3 p) {
 
 __device__ inline void test_face(const tri_terms* __restrict__ tri, int f, v3 o, v3 dir, hit_rec& h) {
   const float4* q = reinterpret_cast<const float4*>(tri + f);
-  float4 q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3];
+  float4 q0 = gld4(q), q1 = gld4(q + 1), q2 = gld4(q + 2), q3 = gld4(q + 3);
   v3 p;
   float d2;
   if (hit_terms(q0, q1, q2, q3, o, dir, &p, &d2)) consider(h, f, d2, p);
@@ -48,7 +48,7 @@ struct mailbox {
 // on that face's terms, unless the ray met the face in an earlier cell.
 // Returns 1 if a test was run.
 __device__ inline uint32_t test_ref(const scene_desc& S, int k, v3 o, v3 dir, hit_rec& h, mailbox& mb) {
-  const int f = S.refs[k];
+  const int f = gld(S.refs + k);
   if (mb.seen(f)) return 0;
   mb.put(f);
   test_face(S.tri, f, o, dir, h);
@@ -133,7 +133,7 @@ __device__ inline bool walk_cell(const scene_desc& S, v3 o, v3 dir, float ix, fl
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int nxy = nx * ny;
   const int sx = walk_sign(dir.x), sy = walk_sign(dir.y), sz = walk_sign(dir.z);
-  const int2 ci = S.cell_info[cell];
+  const int2 ci = gld2(S.cell_info + cell);
   if (STATS) n_cells += 1;
   for (int k = ci.x; k < ci.x + ci.y; ++k) {
     const uint32_t t = test_ref(S, k, o, dir, h, mb);
@@ -351,7 +351,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
     int2 ci = make_int2(0, 0);
     const bool step = have && !pend;
     if (step) {
-      ci = S.cell_info[cell];
+      ci = gld2(S.cell_info + cell);
       if (STATS) n_cells += 1;
       if (ci.y > 0) {
         pend = true;
@@ -399,11 +399,11 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
         const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));
         if (p < total) {
-          const int f = S.refs[ref_o + (p - (inc_o - cnt_o))];
+          const int f = gld(S.refs + ref_o + (p - (inc_o - cnt_o)));
           const float4* q = reinterpret_cast<const float4*>(S.tri + f);
           v3 pt;
           float d2;
-          if (hit_terms(q[0], q[1], q[2], q[3], oo, dd, &pt, &d2) && d2 < 3.402823466e38f)
+          if (hit_terms(gld4(q), gld4(q + 1), gld4(q + 2), gld4(q + 3), oo, dd, &pt, &d2) && d2 < 3.402823466e38f)
             atomicMin(&best[owner], hit_key(d2, f));
           if (STATS) n_tests += 1;
         }
@@ -481,7 +481,7 @@ __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, 
   mailbox mb;
   mb.clear();
   while (true) {
-    const int2 ci = S.cell_info[cell];
+    const int2 ci = gld2(S.cell_info + cell);
     if (STATS) n_cells += 1;
     for (int k = ci.x; k < ci.x + ci.y; ++k) {
       const uint32_t t = test_ref(S, k, o, dir, h, mb);
@@ -551,12 +551,13 @@ __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, 
 
 // First face f with face_off[f+1] > g, starting the search at `hint`.
 __device__ inline int find_face(const long long* __restrict__ off, int n_faces, long long g, int hint) {
-  if (off[hint] <= g && g < off[hint + 1]) return hint;
-  int lo = (off[hint] <= g) ? hint + 1 : 0;
-  int hi = (off[hint] <= g) ? n_faces - 1 : hint;
+  const long long oh = gld(off + hint);
+  if (oh <= g && g < gld(off + hint + 1)) return hint;
+  int lo = (oh <= g) ? hint + 1 : 0;
+  int hi = (oh <= g) ? n_faces - 1 : hint;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    if (off[mid + 1] > g) hi = mid;
+    if (gld(off + mid + 1) > g) hi = mid;
     else lo = mid + 1;
   }
   return lo;
@@ -577,13 +578,16 @@ __device__ inline void malley_ray(const scene_desc& S, int f, int j, uint32_t k0
     b = 1.f - b;
   }
   const float4* q = reinterpret_cast<const float4*>(S.tri + f);
-  float4 q0 = q[0], q1 = q[1], q2 = q[2];
+  float4 q0 = gld4(q), q1 = gld4(q + 1), q2 = gld4(q + 2);
   v3 p0 = mk(q0.x, q0.y, q0.z), v0 = mk(q1.x, q1.y, q1.z), v1 = mk(q2.x, q2.y, q2.z);
   v3 origin = (p0 + v0 * a) + v1 * b;
   float rad = sqrtf(rr);
   v3 m = mk(rad * cphi, rad * sphi, sqrtf(1 - rr));
-  if (j >= S.front[f]) m.z = -m.z;
-  v3 d = mat_vec(S.rot + 9 * (size_t)f, m);
+  if (j >= gld(S.front + f)) m.z = -m.z;
+  float R[9];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) R[i] = gld(S.rot + 9 * (size_t)f + i);
+  v3 d = mat_vec(R, m);
   *origin_out = origin + d * 0.0001f;
   *end_out = origin + d;
 }
