@@ -276,8 +276,8 @@ struct scene_desc {
   const long long* mesh_off;   // n_meshes+1: mesh-order tally ranges
   const int* cell_start;
   const int* refs;             // face of each reference, in cell order
-  const int2* cell_info;       // (first ref, count) or, for an empty cell, (r, 0):
-                               // every cell within Chebyshev distance r-1 is empty
+  const int2* cell_info;       // (first ref, count) or, for an empty cell, (boxes, 0):
+                               // nibble o of boxes = size of the empty box toward octant o
   grid_desc g;
   int n_faces;
   int n_meshes;

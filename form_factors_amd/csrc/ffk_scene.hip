@@ -220,33 +220,60 @@ __global__ void k_cell_count(const uint32_t* __restrict__ keys, long long n, int
   atomicAdd(&cnt[keys[i]], 1);
 }
 
-// Chessboard distance (in cells) from each cell to the nearest non-empty
-// cell: 0 for non-empty cells, relaxed over the 26 neighbours one ring per
-// pass, capped at 255. Cells outside the grid count as empty.
-__global__ void k_empty_init(const int* __restrict__ cs, long long nc, uint8_t* __restrict__ d) {
+// Empty boxes per ray octant. For octant o (bit 0: x decreasing, bit 1: y,
+// bit 2: z) nibble o of a cell's word is the largest k <= 15 such that the k
+// cells per axis from this cell toward that octant, [c, c + s*(k-1)] clipped
+// to the grid, are all empty; 0 for a non-empty cell. A ray whose direction
+// lies in octant o leaves such a box only through its three far faces, so it
+// crosses no occupied cell before leaving it. Cells outside the grid count as
+// empty. Relaxed from 15 down: k = 1 + min over the 7 neighbours toward the
+// octant, exact after 16 passes (the values are capped at 15).
+__global__ void k_empty_init(const int* __restrict__ cs, long long nc, uint32_t* __restrict__ d) {
   long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c < nc) d[c] = cs[c + 1] > cs[c] ? 0 : 255;
+  if (c < nc) d[c] = cs[c + 1] > cs[c] ? 0u : 0xffffffffu;
 }
 
-__global__ void k_empty_relax(grid_desc g, const uint8_t* __restrict__ in, uint8_t* __restrict__ out) {
+__global__ void k_empty_relax(grid_desc g, const uint32_t* __restrict__ in, uint32_t* __restrict__ out) {
   const long long nc = (long long)g.n[0] * g.n[1] * g.n[2];
   long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
+  const uint32_t self = in[c];
+  if (self == 0u) {  // non-empty (an empty cell has k >= 1 in every octant)
+    out[c] = 0u;
+    return;
+  }
   const int x = (int)(c % g.n[0]);
   const int y = (int)((c / g.n[0]) % g.n[1]);
   const int z = (int)(c / ((long long)g.n[0] * g.n[1]));
-  int best = in[c];
+  uint32_t w[27];  // the 3x3x3 neighbourhood, index (dz+1)*9 + (dy+1)*3 + (dx+1)
+#pragma unroll
   for (int dz = -1; dz <= 1; ++dz)
+#pragma unroll
     for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
       for (int dx = -1; dx <= 1; ++dx) {
         const int X = x + dx, Y = y + dy, Z = z + dz;
-        if (X < 0 || Y < 0 || Z < 0 || X >= g.n[0] || Y >= g.n[1] || Z >= g.n[2]) continue;
-        best = min(best, (int)in[X + (long long)g.n[0] * (Y + (long long)g.n[1] * Z)] + 1);
+        const bool in_grid = X >= 0 && Y >= 0 && Z >= 0 && X < g.n[0] && Y < g.n[1] && Z < g.n[2];
+        w[(dz + 1) * 9 + (dy + 1) * 3 + (dx + 1)] =
+            in_grid ? in[X + (long long)g.n[0] * (Y + (long long)g.n[1] * Z)] : 0xffffffffu;
       }
-  out[c] = (uint8_t)min(best, 255);
+  uint32_t res = 0u;
+#pragma unroll
+  for (int o = 0; o < 8; ++o) {
+    // index of the neighbour toward the octant along each axis (1 = this cell)
+    const int fx = (o & 1) ? 0 : 2, fy = (o & 2) ? 0 : 2, fz = (o & 4) ? 0 : 2;
+    uint32_t m = 15u;
+#pragma unroll
+    for (int e = 1; e < 8; ++e) {
+      const int ix = (e & 1) ? fx : 1, iy = (e & 2) ? fy : 1, iz = (e & 4) ? fz : 1;
+      m = min(m, (w[iz * 9 + iy * 3 + ix] >> (4 * o)) & 15u);
+    }
+    res |= min(m + 1u, 15u) << (4 * o);
+  }
+  out[c] = res;
 }
 
-__global__ void k_cell_info(const int* __restrict__ cs, const uint8_t* __restrict__ d, long long nc,
+__global__ void k_cell_info(const int* __restrict__ cs, const uint32_t* __restrict__ d, long long nc,
                             int2* __restrict__ info) {
   long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
@@ -569,15 +596,14 @@ int ffk_scene_build_grid(ffk_scene* s, int nx, int ny, int nz, void* stream) {
   void* d_tmp2 = nullptr;
   if ((r = dmalloc(&d_tmp2, tmp2)) < 0) return r;
   FFK_HIP(hipcub::DeviceScan::ExclusiveSum(d_tmp2, tmp2, d_ccnt, s->d_cell_start, nc + 1, st));
-  // 5. the per-cell (range | empty radius) words
+  // 5. the per-cell (range | empty boxes per octant) words
   if ((r = dalloc(&s->d_cell_info, (size_t)nc)) < 0) return r;
-  uint8_t *d_e0 = nullptr, *d_e1 = nullptr;
+  uint32_t *d_e0 = nullptr, *d_e1 = nullptr;
   if ((r = dalloc(&d_e0, (size_t)nc)) < 0) return r;
   if ((r = dalloc(&d_e1, (size_t)nc)) < 0) return r;
   const unsigned cblocks = (unsigned)((nc + 255) / 256);
   k_empty_init<<<cblocks, 256, 0, st>>>(s->d_cell_start, nc, d_e0);
-  const int passes = std::min(255, std::max(dims[0], std::max(dims[1], dims[2])));
-  for (int p = 0; p < passes; ++p) {
+  for (int p = 0; p < 16; ++p) {
     k_empty_relax<<<cblocks, 256, 0, st>>>(g, d_e0, d_e1);
     std::swap(d_e0, d_e1);
   }

@@ -59,15 +59,26 @@ __device__ inline uint32_t test_ref(const scene_desc& S, int k, v3 o, v3 dir, hi
 // footprint overlaps a visited cell is tested with the reference test; the
 // walk stops once the best squared distance is below the squared distance
 // to the current cell's exit (with a 1e-4 relative margin, far above the
-// rounding of both), so the result equals testing every face. Empty cells
-// carry their Chebyshev distance r to the nearest occupied cell, and the
-// walk leaves the empty box of radius r-1 around them in one step.
+// rounding of both), so the result equals testing every face. An empty cell
+// carries, for each of the 8 direction octants, the size k of the empty box
+// [c, c + s*(k-1)] toward that octant (4 bits each); the walk leaves that box
+// in one step through one of its far faces.
 //
 // The walk is split in two so that a caller can interleave many rays per
 // lane: walk_begin clips the ray to the grid box and finds its first cell,
 // walk_cell visits one cell (tests, then one step or jump). All walk state
 // is kept in plain scalars passed by reference: selecting among struct
 // members here makes the compiler address the struct in scratch memory.
+
+// Bit offset of the direction's octant nibble in an empty cell's word (bit 0
+// of the octant: x decreasing, bit 1: y, bit 2: z). A zero component counts as
+// increasing: the ray crosses no face of that axis, so either box side holds.
+__device__ inline int octant_shift(v3 dir) {
+  return (dir.x < 0.f ? 4 : 0) | (dir.y < 0.f ? 8 : 0) | (dir.z < 0.f ? 16 : 0);
+}
+
+// Size k >= 1 of an empty cell's box toward the octant at bit offset `shift`.
+__device__ inline int empty_box(int word, int shift) { return (int)(((uint32_t)word >> shift) & 15u); }
 
 // Ray parameter of the next boundary of cell c along one axis (s = step sign).
 __device__ inline float walk_boundary(float lo, float h, int c, int s, float o, float inv) {
@@ -139,10 +150,10 @@ __device__ inline bool walk_cell(const scene_desc& S, v3 o, v3 dir, float ix, fl
     const uint32_t t = test_ref(S, k, o, dir, h, mb);
     if (STATS) n_tests += t;
   }
-  const int R = ci.y > 0 ? 0 : ci.x - 1;
+  const int R = ci.y > 0 ? 0 : empty_box(ci.x, octant_shift(dir)) - 1;
   if (R >= 1) {
-    // Every cell within R of this one is empty: leave the box
-    // [c-R, c+R] (clipped to the grid) in one step.
+    // The box [c, c + s*R] toward the ray's octant (clipped to the grid) is
+    // empty: leave it in one step.
     const int hx = sx > 0 ? min(cx + R, nx - 1) : max(cx - R, 0);
     const int hy = sy > 0 ? min(cy + R, ny - 1) : max(cy - R, 0);
     const int hz = sz > 0 ? min(cz + R, nz - 1) : max(cz - R, 0);
@@ -204,8 +215,10 @@ __device__ inline bool walk_cell(const scene_desc& S, v3 o, v3 dir, float ix, fl
 }
 
 // Leaves the current cell after its tests (the second half of walk_cell):
-// past the empty box of radius R (R >= 1) or to the next cell of the DDA
-// (R = 0). Returns false when the walk is over. UNIFIED takes the box code
+// past the empty box [c, c + s*R] toward the ray's octant (R >= 1) or to the
+// next cell of the DDA (R = 0). The box code clamps the exit cell to
+// [c-R, c+R] and then to the ray's side of c, which is that box. Returns
+// false when the walk is over. UNIFIED takes the box code
 // for every R >= 0: with R = 0 the box is the cell itself and the next cell,
 // the exit test and the new boundaries are those of the DDA step, so a
 // caller may run one code path for lanes that jump and lanes that step. The
@@ -323,7 +336,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
   int ref0 = 0, nref = 0;
   v3 o = mk(0.f, 0.f, 0.f), dir = mk(0.f, 0.f, 0.f);
   float ix = 0.f, iy = 0.f, iz = 0.f, dlen = 0.f, tnx = 0.f, tny = 0.f, tnz = 0.f;
-  int cx = 0, cy = 0, cz = 0, cell = 0;
+  int cx = 0, cy = 0, cz = 0, cell = 0, osh = 0;
   while (true) {
     const unsigned long long idle = __ballot(!have);
     const int n_idle = __popcll(idle);
@@ -339,6 +352,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         if (i < n) {
           cur = i;
           start(i, o, dir);
+          osh = octant_shift(dir);
           best[lane] = kNoHit;
           pend = false;
           have = walk_begin(S.g, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell);
@@ -359,14 +373,15 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         nref = ci.y;
       }
     }
-    const bool empty = step && ci.y <= 0;  // then ci.x - 1 is the empty radius
-    const bool jump = WALK == 1 || __ballot(empty && ci.x >= 2) != 0ull;
+    const bool empty = step && ci.y <= 0;  // then ci.x holds the empty boxes
+    const int kbox = empty ? empty_box(ci.x, osh) : 0;
+    const bool jump = WALK == 1 || __ballot(kbox >= 2) != 0ull;
     if (empty) {
       const unsigned long long key = best[lane];
       float bd;
       int bf;
       decode_key(key, bd, bf);
-      const bool on = jump ? walk_leave<true>(S.g, o, dir, ix, iy, iz, dlen, ci.x - 1, cx, cy, cz, tnx, tny, tnz,
+      const bool on = jump ? walk_leave<true>(S.g, o, dir, ix, iy, iz, dlen, kbox - 1, cx, cy, cz, tnx, tny, tnz,
                                               cell, bd, bf)
                            : walk_leave<false>(S.g, o, dir, ix, iy, iz, dlen, 0, cx, cy, cz, tnx, tny, tnz, cell,
                                                bd, bf);
@@ -478,6 +493,7 @@ __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, 
   const float dlen = sqrtf(dot(dir, dir));
   const int nxy = nx * ny;
   int cell = cx + nx * cy + nxy * cz;
+  const int osh = octant_shift(dir);
   mailbox mb;
   mb.clear();
   while (true) {
@@ -487,10 +503,10 @@ __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, 
       const uint32_t t = test_ref(S, k, o, dir, h, mb);
       if (STATS) n_tests += t;
     }
-    const int R = ci.y > 0 ? 0 : ci.x - 1;
+    const int R = ci.y > 0 ? 0 : empty_box(ci.x, osh) - 1;
     if (R >= 1) {
-      // Every cell within R of this one is empty: leave the box
-      // [c-R, c+R] (clipped to the grid) in one step.
+      // The box [c, c + s*R] toward the ray's octant (clipped to the grid)
+      // is empty: leave it in one step.
       const int hx = sx > 0 ? min(cx + R, nx - 1) : max(cx - R, 0);
       const int hy = sy > 0 ? min(cy + R, ny - 1) : max(cy - R, 0);
       const int hz = sz > 0 ? min(cz + R, nz - 1) : max(cz - R, 0);
