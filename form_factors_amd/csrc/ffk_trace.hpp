@@ -308,6 +308,83 @@ __device__ inline void decode_key(unsigned long long key, float& d2, int& f) {
   d2 = key == kNoHit ? 3.402823466e38f : __uint_as_float((uint32_t)(key >> 32));
 }
 
+// The walk of pooled_walk, on per-ray coefficients. Along each axis the ray
+// parameter of the far boundary of cell c (the side the ray moves toward) is
+// c*A + B, and the ray's grid coordinate at parameter t is P + t*D, so a
+// boundary or an exit cell costs one fused multiply-add. For a zero direction
+// component A = 0 and B = 3.4e38 (the ray crosses no face of that axis). These
+// values round differently from walk_leave's expressions by a few units in
+// the last place; the walk tolerates that: faces are binned into cells grown
+// by 1e-3 of a cell, and the walk stops only with a 1e-4 relative margin on
+// the exit distance. Returns false when the ray misses the grid box.
+__device__ inline bool coef_begin(const grid_desc& g, v3 o, v3 dir, float& ax, float& ay, float& az, float& bx,
+                                  float& by, float& bz, float& px, float& py, float& pz, float& dx, float& dy,
+                                  float& dz, float& dlen, int& cx, int& cy, int& cz) {
+  float ix, iy, iz, tnx, tny, tnz;
+  int cell;
+  if (!walk_begin(g, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell)) return false;
+  ax = dir.x != 0.f ? g.h[0] * ix : 0.f;
+  ay = dir.y != 0.f ? g.h[1] * iy : 0.f;
+  az = dir.z != 0.f ? g.h[2] * iz : 0.f;
+  bx = dir.x != 0.f ? (g.lo[0] + (dir.x > 0.f ? g.h[0] : 0.f) - o.x) * ix : 3.4e38f;
+  by = dir.y != 0.f ? (g.lo[1] + (dir.y > 0.f ? g.h[1] : 0.f) - o.y) * iy : 3.4e38f;
+  bz = dir.z != 0.f ? (g.lo[2] + (dir.z > 0.f ? g.h[2] : 0.f) - o.z) * iz : 3.4e38f;
+  px = (o.x - g.lo[0]) * g.inv_h[0];
+  py = (o.y - g.lo[1]) * g.inv_h[1];
+  pz = (o.z - g.lo[2]) * g.inv_h[2];
+  dx = dir.x * g.inv_h[0];
+  dy = dir.y * g.inv_h[1];
+  dz = dir.z * g.inv_h[2];
+  return true;
+}
+
+// Leaves the current cell: past the empty box [c, c + s*R] toward the ray's
+// octant (R >= 1) or to the next cell of the DDA (R = 0: the box is the cell
+// itself). The exit cell's other two coordinates come from the exit point,
+// clamped to the box on the ray's side of c. BOX = false assumes R = 0 for
+// the lane and skips the exit point. Returns false when the walk is over
+// (past the grid, or the best hit is nearer than the box's exit).
+template <bool BOX>
+__device__ inline bool coef_leave(const grid_desc& g, float ax, float ay, float az, float bx, float by, float bz,
+                                  float px, float py, float pz, float dx, float dy, float dz, float dlen, int R,
+                                  int& cx, int& cy, int& cz, float best_d2, int best_f) {
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const int sx = ax > 0.f ? 1 : (ax < 0.f ? -1 : 0);
+  const int sy = ay > 0.f ? 1 : (ay < 0.f ? -1 : 0);
+  const int sz = az > 0.f ? 1 : (az < 0.f ? -1 : 0);
+  if (!BOX) R = 0;
+  const int hx = sx > 0 ? min(cx + R, nx - 1) : max(cx - R, 0);
+  const int hy = sy > 0 ? min(cy + R, ny - 1) : max(cy - R, 0);
+  const int hz = sz > 0 ? min(cz + R, nz - 1) : max(cz - R, 0);
+  const float tx = fmaf((float)hx, ax, bx), ty = fmaf((float)hy, ay, by), tz = fmaf((float)hz, az, bz);
+  const bool ex = (tx <= ty) && (tx <= tz);
+  const bool ey = !ex && (ty <= tz);
+  const float tb = ex ? tx : (ey ? ty : tz);
+  const float de = tb * dlen;
+  bool on = tb < 3.4e38f && !(best_f >= 0 && best_d2 < de * de * 0.9999f);
+  int nxc = cx, nyc = cy, nzc = cz;
+  if (BOX) {
+    const int fx = (int)floorf(fmaf(tb, dx, px));
+    const int fy = (int)floorf(fmaf(tb, dy, py));
+    const int fz = (int)floorf(fmaf(tb, dz, pz));
+    // [min(c, h), max(c, h)] is the box's range on the ray's side of c; an
+    // axis the ray does not move along keeps c.
+    nxc = sx != 0 ? min(max(fx, min(cx, hx)), max(cx, hx)) : cx;
+    nyc = sy != 0 ? min(max(fy, min(cy, hy)), max(cy, hy)) : cy;
+    nzc = sz != 0 ? min(max(fz, min(cz, hz)), max(cz, hz)) : cz;
+  }
+  nxc = ex ? hx + sx : nxc;
+  nyc = (!ex && ey) ? hy + sy : nyc;
+  nzc = (!ex && !ey) ? hz + sz : nzc;
+  const int ne = ex ? nxc : (ey ? nyc : nzc);
+  const int nn = ex ? nx : (ey ? ny : nz);
+  on = on && ne >= 0 && ne < nn;
+  cx = on ? nxc : cx;
+  cy = on ? nyc : cy;
+  cz = on ? nzc : cz;
+  return on;
+}
+
 // Casts rays 0..n-1 of a work list with a wavefront's 64 lanes working as a
 // pool. Each lane walks its own ray one grid cell per iteration; a lane that
 // reaches an occupied cell waits, and once waiting lanes x `pool` >= active
@@ -318,9 +395,9 @@ __device__ inline void decode_key(unsigned long long key, float& d2, int& f) {
 // the next rays of the list (one LDS atomic on *next per refill). The hits
 // are those of cast_grid, whatever the order. Lanes in an empty cell leave
 // it by the jump past its empty box or, next to occupied cells, by one DDA
-// step; WALK 2 runs the jump code for all of a wavefront's empty-cell lanes
-// when any of them can jump and the DDA step otherwise (never both), WALK 1
-// always runs the jump code (walk_leave<true>: the same cells).
+// step (coef_leave); WALK 2 runs the jump code for all of a wavefront's
+// empty-cell lanes when any of them can jump and the DDA step otherwise
+// (never both), WALK 1 always runs the jump code (the same cells).
 //   best:  this wavefront's 64 words of LDS (8-byte aligned)
 //   next:  the list's LDS ray counter, zero at the start, shared by the
 //          block's wavefronts
@@ -335,8 +412,10 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
   long long cur = 0;
   int ref0 = 0, nref = 0;
   v3 o = mk(0.f, 0.f, 0.f), dir = mk(0.f, 0.f, 0.f);
-  float ix = 0.f, iy = 0.f, iz = 0.f, dlen = 0.f, tnx = 0.f, tny = 0.f, tnz = 0.f;
-  int cx = 0, cy = 0, cz = 0, cell = 0, osh = 0;
+  float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
+  float px = 0.f, py = 0.f, pz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, dlen = 0.f;
+  int cx = 0, cy = 0, cz = 0, osh = 0;
+  const int nx = S.g.n[0], nxy = S.g.n[0] * S.g.n[1];
   while (true) {
     const unsigned long long idle = __ballot(!have);
     const int n_idle = __popcll(idle);
@@ -355,7 +434,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
           osh = octant_shift(dir);
           best[lane] = kNoHit;
           pend = false;
-          have = walk_begin(S.g, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell);
+          have = coef_begin(S.g, o, dir, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, cx, cy, cz);
           if (!have) finish(i, o, dir, kNoHit);
         }
       }
@@ -365,7 +444,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
     int2 ci = make_int2(0, 0);
     const bool step = have && !pend;
     if (step) {
-      ci = gld2(S.cell_info + cell);
+      ci = gld2(S.cell_info + (cx + nx * cy + nxy * cz));
       if (STATS) n_cells += 1;
       if (ci.y > 0) {
         pend = true;
@@ -381,10 +460,10 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
       float bd;
       int bf;
       decode_key(key, bd, bf);
-      const bool on = jump ? walk_leave<true>(S.g, o, dir, ix, iy, iz, dlen, kbox - 1, cx, cy, cz, tnx, tny, tnz,
-                                              cell, bd, bf)
-                           : walk_leave<false>(S.g, o, dir, ix, iy, iz, dlen, 0, cx, cy, cz, tnx, tny, tnz, cell,
-                                               bd, bf);
+      const bool on = jump ? coef_leave<true>(S.g, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, kbox - 1,
+                                              cx, cy, cz, bd, bf)
+                           : coef_leave<false>(S.g, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, 0, cx, cy,
+                                               cz, bd, bf);
       if (!on) {
         finish(cur, o, dir, key);
         have = false;
@@ -430,7 +509,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         float bd;
         int bf;
         decode_key(key, bd, bf);
-        if (!walk_leave(S.g, o, dir, ix, iy, iz, dlen, 0, cx, cy, cz, tnx, tny, tnz, cell, bd, bf)) {
+        if (!coef_leave<false>(S.g, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, 0, cx, cy, cz, bd, bf)) {
           finish(cur, o, dir, key);
           have = false;
         }
