@@ -66,12 +66,13 @@ __host__ __device__ inline v3 cross(v3 a, v3 b) {
 }
 
 // Per-face intersection terms (64 B, four float4):
-//   q0 = {p0, uu}  q1 = {u, uv}  q2 = {v, vv}  q3 = {1/D, D, degenerate, 0}
-// u = p1-p0, v = p2-p0, uu/uv/vv = dots, D = uv*uv - uu*vv, all with the
-// reference's expressions (math/triangle.cpp:35-38,57-63). The normal is
-// recomputed as u x v at test time (the reference's expression, so the same
-// bits). degenerate = 1 for (double)dot(n,n) < 1e-8: such a face is never
-// hit (math/triangle.cpp:38) and is never put in the grid.
+//   q0 = {p0, uu}  q1 = {u, uv}  q2 = {v, vv}  q3 = {n, 1/D}
+// u = p1-p0, v = p2-p0, n = u x v, uu/uv/vv = dots, D = uv*uv - uu*vv, all
+// with the reference's expressions (math/triangle.cpp:35-38,57-63), so the
+// same bits; D is recomputed from uu, uv, vv where the test needs it. A
+// degenerate face ((double)dot(n,n) < 1e-8, never hit: math/triangle.cpp:38)
+// stores n = 0, which every ray meets as parallel (|dot(n, dir)| = 0), so it
+// is never hit either; such faces are never put in the grid.
 struct tri_terms {
   float4 q0, q1, q2, q3;
 };
@@ -86,7 +87,7 @@ __host__ __device__ inline tri_terms make_terms(const float* f) {
   t.q0 = make_float4(p0.x, p0.y, p0.z, uu);
   t.q1 = make_float4(u.x, u.y, u.z, uv);
   t.q2 = make_float4(v.x, v.y, v.z, vv);
-  t.q3 = make_float4(1.0f / D, D, deg ? 1.f : 0.f, 0.f);
+  t.q3 = deg ? make_float4(0.f, 0.f, 0.f, 0.f) : make_float4(n.x, n.y, n.z, 1.0f / D);
   return t;
 }
 
@@ -94,16 +95,18 @@ __host__ __device__ inline tri_terms make_terms(const float* f) {
 // ray with origin o and direction dir = end - origin. Returns true for a
 // unique hit and writes the point and its squared distance to o.
 //
-// Before the two divisions of the inside test, s' = sn * (1/D) and
-// t' = tn * (1/D) are compared with margins far above their rounding error
-// (<= 1.2e-7 relative): a face rejected there is also rejected by the exact
-// test, and every other face takes the exact test, so the result is the
-// reference's bit for bit.
+// The inside test needs s = sn / D and t = tn / D only for their comparisons
+// with 0 and 1. s' = sn * (1/D) and t' = tn * (1/D) are within 2.4e-7
+// (relative) of the exact quotients, so a face with s' or t' beyond the
+// rejection margins below fails the exact test, a face with s', t', s' + t'
+// inside by 1e-5 passes it, and only the faces in between take the two
+// correctly rounded divisions: the result is the reference's bit for bit,
+// and a wavefront runs the divisions only when one of its lanes is that close
+// to an edge.
 __host__ __device__ inline bool hit_terms(const float4 q0, const float4 q1, const float4 q2,
                                           const float4 q3, v3 o, v3 dir, v3* point, float* d2) {
-  if (q3.z != 0.f) return false;  // degenerate
   const v3 p0 = mk(q0.x, q0.y, q0.z), u = mk(q1.x, q1.y, q1.z), v = mk(q2.x, q2.y, q2.z);
-  const v3 n = cross(u, v);
+  const v3 n = mk(q3.x, q3.y, q3.z);
   const v3 w0 = o - p0;
   const float a = -dot(n, w0);
   const float b = dot(n, dir);
@@ -116,12 +119,15 @@ __host__ __device__ inline bool hit_terms(const float4 q0, const float4 q1, cons
   const float wu = dot(w, u), wv = dot(w, v);
   const float sn = uv * wv - vv * wu;
   const float tn = uv * wu - uu * wv;
-  const float s1 = sn * q3.x, t1 = tn * q3.x;
+  const float s1 = sn * q3.w, t1 = tn * q3.w;
   if (s1 < -1e-30f || s1 > 1.000001f || t1 < -1e-30f || s1 + t1 > 1.00001f) return false;
-  const float s = sn / q3.y;
-  if (s < 0.0f || s > 1.0f) return false;
-  const float t = tn / q3.y;
-  if (t < 0.0f || (s + t) > 1.0f) return false;
+  if (!(s1 > 1e-5f && s1 < 0.99999f && t1 > 1e-5f && s1 + t1 < 0.99999f)) {
+    const float D = uv * uv - uu * vv;
+    const float s = sn / D;
+    if (s < 0.0f || s > 1.0f) return false;
+    const float t = tn / D;
+    if (t < 0.0f || (s + t) > 1.0f) return false;
+  }
   const v3 sd = p - o;
   *point = p;
   *d2 = dot(sd, sd);
