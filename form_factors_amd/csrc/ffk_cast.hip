@@ -60,7 +60,7 @@ __global__ void __launch_bounds__(256) k_cast_pooled(scene_desc S, const float* 
                                                      long long chunk, int* __restrict__ idx,
                                                      float* __restrict__ pts, unsigned long long* __restrict__ stats,
                                                      int refill, int pool) {
-  __shared__ unsigned long long best[256];
+  __shared__ unsigned long long best[4 * kPoolWords];
   __shared__ int s_next;
   const long long r0 = (long long)blockIdx.x * chunk;
   const long long cnt = min(chunk, n - r0);
@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(256) k_cast_pooled(scene_desc S, const float* 
   __syncthreads();
   uint32_t nt = 0, nc = 0;
   pooled_walk<STATS, WALK>(
-      S, best + 64 * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
+      S, best + kPoolWords * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
       [&](long long i, v3& o, v3& dir) { load_ray(rays, r0 + i, &o, &dir); },
       [&](long long i, v3 o, v3 dir, unsigned long long key) {
         hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};

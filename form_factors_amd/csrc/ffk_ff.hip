@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(1024)
       // cell by the jump code.
       unsigned long long* best =
           reinterpret_cast<unsigned long long*>(hist + ((HIST == 0 ? 0 : (words + 1)) & ~1)) +
-          64 * (threadIdx.x >> 6);
+          kPoolWords * (threadIdx.x >> 6);
       pooled_walk<STATS, SCHED == 3 ? 1 : 2>(
           S, best, &s_next, T.g1 - T.g0, refill, pool,
           [&](long long i, v3& o, v3& dir) {
@@ -335,11 +335,11 @@ static void set_lds_attr(size_t bytes) {
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
-// Dynamic LDS of a launch: the row histogram, then (pooled tests) one
-// 64-bit best-hit word per lane.
+// Dynamic LDS of a launch: the row histogram, then (pooled tests) the
+// wavefronts' kPoolWords 64-bit words each (best hits and owner slots).
 static size_t fused_lds(int hist, size_t hist_lds, int sched, int block) {
   size_t b = hist ? hist_lds : 0;
-  if (sched >= 2) b = ((b + 7) & ~(size_t)7) + (size_t)block * 8;
+  if (sched >= 2) b = ((b + 7) & ~(size_t)7) + (size_t)(block / 64) * kPoolWords * 8;
   return b;
 }
 

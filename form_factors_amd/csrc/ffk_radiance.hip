@@ -68,7 +68,7 @@ __global__ void __launch_bounds__(256)
                       int n_walk, const unsigned long long* __restrict__ q, long long n_rays, long long chunk,
                       uint32_t k0, uint32_t k1, unsigned long long* __restrict__ acc,
                       unsigned long long* __restrict__ stats, int refill, int pool) {
-  __shared__ unsigned long long best[256];
+  __shared__ unsigned long long best[4 * kPoolWords];
   __shared__ int s_next;
   const long long r0 = (long long)blockIdx.x * chunk;
   const long long cnt = min(chunk, n_rays - r0);
@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256)
   uint32_t nt = 0, nc = 0;
   int hint = find_face(S.face_off, S.n_faces, r0, 0), whint = 0;
   pooled_walk<STATS, WALK>(
-      S, best + 64 * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
+      S, best + kPoolWords * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
       [&](long long i, v3& o, v3& dir) {
         const long long g = r0 + i;
         const int f = find_face(S.face_off, S.n_faces, g, hint);

@@ -385,6 +385,36 @@ __device__ inline bool coef_leave(const grid_desc& g, float ax, float ay, float 
   return on;
 }
 
+// LDS words per wavefront of pooled_walk: 64 best-hit words (hit_key), then
+// 64 owner slots (int) of the test chunks.
+constexpr int kPoolWords = 96;
+
+// Inclusive prefix sum and prefix maximum over the wavefront's 64 lanes: DPP
+// row shifts within each row of 16 lanes, then the rows' last lanes by
+// readlane. Lanes shifted in from outside a row read 0, the identity of both
+// (the maximum is taken over non-negative values). Every lane must be active.
+__device__ inline int wave_incl_sum(int x, int lane) {
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+  x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+  const int r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31);
+  const int r2 = __builtin_amdgcn_readlane(x, 47);
+  const int row = lane >> 4;
+  return x + (row == 0 ? 0 : (row == 1 ? r0 : (row == 2 ? r0 + r1 : r0 + r1 + r2)));
+}
+__device__ inline int wave_incl_max(int x, int lane) {
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
+  const int r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31);
+  const int r2 = __builtin_amdgcn_readlane(x, 47);
+  const int m1 = max(r0, r1), m2 = max(m1, r2);
+  const int row = lane >> 4;
+  return max(x, row == 0 ? 0 : (row == 1 ? r0 : (row == 2 ? m1 : m2)));
+}
+
 // Casts rays 0..n-1 of a work list with a wavefront's 64 lanes working as a
 // pool. Each lane walks its own ray one grid cell per iteration; a lane that
 // reaches an occupied cell waits, and once waiting lanes x `pool` >= active
@@ -398,7 +428,7 @@ __device__ inline bool coef_leave(const grid_desc& g, float ax, float ay, float 
 // step (coef_leave); WALK 2 runs the jump code for all of a wavefront's
 // empty-cell lanes when any of them can jump and the DDA step otherwise
 // (never both), WALK 1 always runs the jump code (the same cells).
-//   best:  this wavefront's 64 words of LDS (8-byte aligned)
+//   best:  this wavefront's kPoolWords words of LDS (8-byte aligned)
 //   next:  the list's LDS ray counter, zero at the start, shared by the
 //          block's wavefronts
 //   start(i, o, dir):            sets up ray i
@@ -472,28 +502,29 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
     const unsigned long long waiting = __ballot(have && pend);
     const unsigned long long active = __ballot(have);
     if (waiting != 0ull && __popcll(waiting) * pool >= __popcll(active)) {
+      // The waiting lanes' (ray, reference) pairs, numbered in lane order:
+      // lane w owns positions [excl, incl) of the prefix sum of the counts.
       const int cnt = (have && pend) ? nref : 0;
-      int incl = cnt;  // inclusive prefix sum of the waiting lanes' reference counts
-#pragma unroll
-      for (int d = 1; d < 64; d <<= 1) {
-        const int v = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += v;
-      }
-      const int total = __shfl(incl, 63, 64);
+      const int incl = wave_incl_sum(cnt, lane);
+      const int excl = incl - cnt;
+      const int total = __builtin_amdgcn_readlane(incl, 63);
+      const int base = ref0 - excl;  // reference index of position q of this lane's range: base + q
+      volatile int* slot = reinterpret_cast<volatile int*>(best + 64);
       for (int p0 = 0; p0 < total; p0 += 64) {
+        // Owner of position p0 + lane: each lane whose range meets the chunk
+        // marks the chunk slot where its range starts (slot 0 for the range
+        // that runs into the chunk), and a prefix maximum carries the mark
+        // over the range. LDS operations of a wavefront execute in order.
+        slot[lane] = 0;
+        if (cnt > 0 && excl < p0 + 64 && incl > p0) slot[max(excl - p0, 0)] = lane + 1;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int owner = wave_incl_max(slot[lane], lane) - 1;
         const int p = p0 + lane;
-        int owner = 0;  // first lane whose inclusive count exceeds p
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1) {
-          const int v = __shfl(incl, owner + step - 1, 64);
-          if (v <= p) owner += step;
-        }
-        const int inc_o = __shfl(incl, owner, 64), cnt_o = __shfl(cnt, owner, 64);
-        const int ref_o = __shfl(ref0, owner, 64);
+        const int ref_o = __shfl(base, owner, 64) + p;
         const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
         const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));
         if (p < total) {
-          const int f = gld(S.refs + ref_o + (p - (inc_o - cnt_o)));
+          const int f = gld(S.refs + ref_o);
           const float4* q = reinterpret_cast<const float4*>(S.tri + f);
           v3 pt;
           float d2;
