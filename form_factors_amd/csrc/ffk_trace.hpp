@@ -320,21 +320,51 @@ __device__ inline void decode_key(unsigned long long key, float& d2, int& f) {
 __device__ inline bool coef_begin(const grid_desc& g, v3 o, v3 dir, float& ax, float& ay, float& az, float& bx,
                                   float& by, float& bz, float& px, float& py, float& pz, float& dx, float& dy,
                                   float& dz, float& dlen, int& cx, int& cy, int& cz) {
-  float ix, iy, iz, tnx, tny, tnz;
-  int cell;
-  if (!walk_begin(g, o, dir, ix, iy, iz, dlen, cx, cy, cz, tnx, tny, tnz, cell)) return false;
-  ax = dir.x != 0.f ? g.h[0] * ix : 0.f;
-  ay = dir.y != 0.f ? g.h[1] * iy : 0.f;
-  az = dir.z != 0.f ? g.h[2] * iz : 0.f;
-  bx = dir.x != 0.f ? (g.lo[0] + (dir.x > 0.f ? g.h[0] : 0.f) - o.x) * ix : 3.4e38f;
-  by = dir.y != 0.f ? (g.lo[1] + (dir.y > 0.f ? g.h[1] : 0.f) - o.y) * iy : 3.4e38f;
-  bz = dir.z != 0.f ? (g.lo[2] + (dir.z > 0.f ? g.h[2] : 0.f) - o.z) * iz : 3.4e38f;
+  // Hardware reciprocals and square root (about 1 ulp): these values only
+  // place the walk, which tolerates far larger errors (see above).
+  const float ix = dir.x != 0.f ? __builtin_amdgcn_rcpf(dir.x) : 0.f;
+  const float iy = dir.y != 0.f ? __builtin_amdgcn_rcpf(dir.y) : 0.f;
+  const float iz = dir.z != 0.f ? __builtin_amdgcn_rcpf(dir.z) : 0.f;
+  // Clip to the grid box (slabs); t is in units of dir.
+  float t0 = 0.f, t1 = 3.4e38f;
+  if (dir.x != 0.f) {
+    const float ta = (g.lo[0] - o.x) * ix, tb = (g.hi[0] - o.x) * ix;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.x < g.lo[0] || o.x > g.hi[0]) {
+    return false;
+  }
+  if (dir.y != 0.f) {
+    const float ta = (g.lo[1] - o.y) * iy, tb = (g.hi[1] - o.y) * iy;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.y < g.lo[1] || o.y > g.hi[1]) {
+    return false;
+  }
+  if (dir.z != 0.f) {
+    const float ta = (g.lo[2] - o.z) * iz, tb = (g.hi[2] - o.z) * iz;
+    t0 = fmaxf(t0, fminf(ta, tb));
+    t1 = fminf(t1, fmaxf(ta, tb));
+  } else if (o.z < g.lo[2] || o.z > g.hi[2]) {
+    return false;
+  }
+  if (!(t0 <= t1)) return false;
   px = (o.x - g.lo[0]) * g.inv_h[0];
   py = (o.y - g.lo[1]) * g.inv_h[1];
   pz = (o.z - g.lo[2]) * g.inv_h[2];
   dx = dir.x * g.inv_h[0];
   dy = dir.y * g.inv_h[1];
   dz = dir.z * g.inv_h[2];
+  cx = min(max((int)floorf(fmaf(t0, dx, px)), 0), g.n[0] - 1);
+  cy = min(max((int)floorf(fmaf(t0, dy, py)), 0), g.n[1] - 1);
+  cz = min(max((int)floorf(fmaf(t0, dz, pz)), 0), g.n[2] - 1);
+  ax = g.h[0] * ix;
+  ay = g.h[1] * iy;
+  az = g.h[2] * iz;
+  bx = dir.x != 0.f ? (g.lo[0] + (dir.x > 0.f ? g.h[0] : 0.f) - o.x) * ix : 3.4e38f;
+  by = dir.y != 0.f ? (g.lo[1] + (dir.y > 0.f ? g.h[1] : 0.f) - o.y) * iy : 3.4e38f;
+  bz = dir.z != 0.f ? (g.lo[2] + (dir.z > 0.f ? g.h[2] : 0.f) - o.z) * iz : 3.4e38f;
+  dlen = __builtin_amdgcn_sqrtf(dot(dir, dir));
   return true;
 }
 
@@ -695,6 +725,10 @@ __device__ inline int find_face(const long long* __restrict__ off, int n_faces, 
 // uniforms come from one Philox4x32-10 block, counter (j, f, 0, 0).
 __device__ inline void malley_ray(const scene_desc& S, int f, int j, uint32_t k0, uint32_t k1,
                                   v3* origin_out, v3* end_out) {
+  // The key is wave-uniform. Made opaque here, its ten round keys are
+  // derived with scalar adds at each call instead of being hoisted out of
+  // the caller's loop into registers that spill to vector-register lanes.
+  asm volatile("" : "+s"(k0), "+s"(k1));
   u4 r = philox(u4{(uint32_t)j, (uint32_t)f, 0u, 0u}, k0, k1);
   float a = u01(r.x), b = u01(r.y), rr = u01(r.z);
   float sphi, cphi;
