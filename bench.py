@@ -229,6 +229,7 @@ def confetti_leg(ffk, scenes, steps):
             "n_rays": n_rays, "unit": "Mrays/s",
             "default_grid": {"dims": list(g0), "value": round(n_rays / t_default / 1e6, 2)},
             "autotuned": {"dims": list(tuned["dims"]), "block": tuned["block"], "walk": tuned["walk"],
+                          "refill": tuned["refill"], "pool": tuned["pool"],
                           "value": round(n_rays / t_tuned / 1e6, 2), "tuning_seconds": round(t_tune, 2)},
             "F_bits_identical": same, "sun": sun}
 
@@ -397,7 +398,8 @@ def main():
     g = sc.build_grid((args.grid,) * 3)
     sc.set_tuning(block=args.block, rays_per_tile=args.rays_per_tile, hist=args.hist)
     if args.refill >= 0 or args.pool >= 0:
-        sc.set_refill(16 if args.refill < 0 else args.refill, 4 if args.pool < 0 else args.pool)
+        r0, p0 = sc.get_refill()
+        sc.set_refill(r0 if args.refill < 0 else args.refill, p0 if args.pool < 0 else args.pool)
     if args.walk >= 0:
         sc.set_walk(args.walk)
     N = len(meshes)

@@ -108,10 +108,10 @@ __global__ void k_zero_rows(int N, const int* __restrict__ rows, int n_rows,
 // PERSIST: blocks pull tiles from a queue (batched launch) instead of one
 // tile per block (plain launch).
 // Idle lanes (of 64) that trigger a refill in the fused kernel; 0 = off.
-constexpr int kDefaultRefill = 16;
+constexpr int kDefaultRefill = 32;
 // Pooled triangle tests run once waiting lanes x kDefaultPool >= active
 // lanes; 0 = each lane tests its own cells.
-constexpr int kDefaultPool = 4;
+constexpr int kDefaultPool = 2;
 
 struct fused_args {
   const scene_desc* descs;
@@ -135,8 +135,10 @@ struct fused_args {
 // slowest ray before the next round, which leaves most lanes idle on
 // incoherent rays. Rays are tallied as integers, so the order in which they
 // are cast does not change F.
+// At least 6 wavefronts per SIMD (at most 80 VGPRs): three 512-lane blocks
+// per CU, the most the row histograms of large scenes leave LDS for.
 template <int HIST, bool PERSIST, bool STATS, int SCHED>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
     k_ff_fused(const scene_desc* __restrict__ descs, const tile* __restrict__ tiles, int n_tiles,
                int* __restrict__ queue, const unsigned long long* __restrict__ seeds,
                unsigned long long* const* __restrict__ tallies, float* const* __restrict__ Fs,
@@ -681,6 +683,13 @@ int ffk_set_refill(ffk_scene* s, int lanes, int pool) {
   FFK_REQUIRE(pool >= 0 && pool <= 64, FFK_ERR_ARG, "pool ratio %d outside [0, 64]", pool);
   s->tune_refill = lanes;
   s->tune_pool = pool;
+  return FFK_OK;
+}
+
+int ffk_get_refill(const ffk_scene* s, int* lanes, int* pool) {
+  FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
+  if (lanes) *lanes = s->tune_refill;
+  if (pool) *pool = s->tune_pool;
   return FFK_OK;
 }
 

@@ -344,8 +344,8 @@ struct ffk_scene {
   std::vector<long long> h_face_off, h_mesh_off;
   // tuning (0 = automatic)
   int tune_block = 0, tune_rays_per_tile = 0, tune_hist = -1;
-  int tune_refill = 16;  // idle lanes that trigger a refill in the fused kernel (0 = off)
-  int tune_pool = 4;     // pooled tests once waiting lanes x tune_pool >= active lanes (0 = off)
+  int tune_refill = 32;  // idle lanes that trigger a refill in the fused kernel (0 = off)
+  int tune_pool = 2;     // pooled tests once waiting lanes x tune_pool >= active lanes (0 = off)
   int tune_walk = 0;     // empty cells: 0 = per-wavefront choice of jump or DDA step, 1 = always the jump code
   int tiles_rays_per_tile = 0;  // value the tile list was built with
 

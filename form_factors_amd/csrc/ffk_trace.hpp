@@ -415,14 +415,12 @@ __device__ inline bool coef_leave(const grid_desc& g, float ax, float ay, float 
   return on;
 }
 
-// LDS words per wavefront of pooled_walk: 64 best-hit words (hit_key), then
-// 64 owner slots (int) of the test chunks.
-constexpr int kPoolWords = 96;
+// LDS words per wavefront of pooled_walk: 64 best-hit words (hit_key).
+constexpr int kPoolWords = 64;
 
-// Inclusive prefix sum and prefix maximum over the wavefront's 64 lanes: DPP
-// row shifts within each row of 16 lanes, then the rows' last lanes by
-// readlane. Lanes shifted in from outside a row read 0, the identity of both
-// (the maximum is taken over non-negative values). Every lane must be active.
+// Inclusive prefix sum over the wavefront's 64 lanes: DPP row shifts within
+// each row of 16 lanes, then the rows' last lanes by readlane. Lanes shifted
+// in from outside a row read 0. Every lane must be active.
 __device__ inline int wave_incl_sum(int x, int lane) {
   x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
   x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
@@ -432,17 +430,6 @@ __device__ inline int wave_incl_sum(int x, int lane) {
   const int r2 = __builtin_amdgcn_readlane(x, 47);
   const int row = lane >> 4;
   return x + (row == 0 ? 0 : (row == 1 ? r0 : (row == 2 ? r0 + r1 : r0 + r1 + r2)));
-}
-__device__ inline int wave_incl_max(int x, int lane) {
-  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));
-  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));
-  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));
-  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));
-  const int r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31);
-  const int r2 = __builtin_amdgcn_readlane(x, 47);
-  const int m1 = max(r0, r1), m2 = max(m1, r2);
-  const int row = lane >> 4;
-  return max(x, row == 0 ? 0 : (row == 1 ? r0 : (row == 2 ? m1 : m2)));
 }
 
 // Casts rays 0..n-1 of a work list with a wavefront's 64 lanes working as a
@@ -539,17 +526,14 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
       const int excl = incl - cnt;
       const int total = __builtin_amdgcn_readlane(incl, 63);
       const int base = ref0 - excl;  // reference index of position q of this lane's range: base + q
-      volatile int* slot = reinterpret_cast<volatile int*>(best + 64);
       for (int p0 = 0; p0 < total; p0 += 64) {
-        // Owner of position p0 + lane: each lane whose range meets the chunk
-        // marks the chunk slot where its range starts (slot 0 for the range
-        // that runs into the chunk), and a prefix maximum carries the mark
-        // over the range. LDS operations of a wavefront execute in order.
-        slot[lane] = 0;
-        if (cnt > 0 && excl < p0 + 64 && incl > p0) slot[max(excl - p0, 0)] = lane + 1;
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int owner = wave_incl_max(slot[lane], lane) - 1;
         const int p = p0 + lane;
+        int owner = 0;  // first lane whose inclusive count exceeds p (binary lifting)
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+          const int v = __shfl(incl, owner + step - 1, 64);
+          if (v <= p) owner += step;
+        }
         const int ref_o = __shfl(base, owner, 64) + p;
         const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
         const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));

@@ -8,7 +8,8 @@
 // resolutions with the scene's ray scheduling. When a full run is short
 // (estimated from the sample) and F fits, the three fastest resolutions are
 // then timed with the fused kernel itself on the full layout, and the
-// fused kernel's block size and walk mode after that. The choice is cached per scene
+// fused kernel's block size, walk mode and ray scheduling (refill threshold,
+// pooled-test ratio) after that. The choice is cached per scene
 // signature (faces, meshes, ray counts, frame) for the life of the process.
 #include <hip/hip_runtime_api.h>
 
@@ -28,7 +29,14 @@ struct tuned {
   int dims[3];
   int block;
   int walk;
+  int refill;
+  int pool;
 };
+
+// (refill threshold, pooled-test ratio) pairs the tuner times: the default
+// first, then schedules that start rays sooner and pool tests later, which
+// suit scenes where rays meet many occupied cells.
+constexpr int kSchedules[][2] = {{32, 2}, {24, 3}, {16, 4}};
 
 std::mutex g_mu;
 std::map<unsigned long long, tuned> g_cache;
@@ -167,6 +175,7 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
     // then the block size
     t.block = 0;
     t.walk = 0;
+    if ((r = ffk_get_refill(s, &t.refill, &t.pool)) < 0) return r;
     const double est_ms = trials[0].ms * (double)n_rays / (double)std::max(1LL, n_sample);
     const size_t nn = (size_t)nm * nm;
     const bool need_tally = ffk_ff_needs_tally(s) != 0;
@@ -226,8 +235,22 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
             t.walk = walk;
           }
         }
+        float best_s = 3.0e38f;
+        const int refill0 = t.refill, pool0 = t.pool;
+        for (const auto& sch : kSchedules) {
+          if (r != FFK_OK) break;
+          if ((r = ffk_set_walk(s, t.walk)) < 0 || (r = ffk_set_refill(s, sch[0], sch[1])) < 0) break;
+          float ms = 0.f;
+          r = time_ms(st, 2, [&] { return ffk_ff_fused(s, FFK_CAST_GRID, 1, d_tally, d_F, nullptr, stream); }, &ms);
+          if (r == FFK_OK && ms < best_s) {
+            best_s = ms;
+            t.refill = sch[0];
+            t.pool = sch[1];
+          }
+        }
         (void)ffk_set_tuning(s, 0, 0, -1);
         (void)ffk_set_walk(s, 0);
+        (void)ffk_set_refill(s, refill0, pool0);
       }
       (void)hipFree(d_F);
       (void)hipFree(d_tally);
@@ -241,6 +264,7 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
   }
   if ((r = ffk_set_tuning(s, t.block, 0, -1)) < 0) return r;
   if ((r = ffk_set_walk(s, t.walk)) < 0) return r;
+  if ((r = ffk_set_refill(s, t.refill, t.pool)) < 0) return r;
   if (out_dims3) std::copy(t.dims, t.dims + 3, out_dims3);
   if (out_block) *out_block = t.block;
   if (out_walk) *out_walk = t.walk;

@@ -53,6 +53,7 @@ SIGNATURES = {
     "ffk_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ffk_ff_needs_tally": (C.c_int, [_vp]),
     "ffk_set_refill": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "ffk_get_refill": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "ffk_batch_set_refill": (C.c_int, [_vp, C.c_int, C.c_int]),
     "ffk_set_walk": (C.c_int, [_vp, C.c_int]),
     "ffk_batch_set_walk": (C.c_int, [_vp, C.c_int]),
@@ -316,22 +317,31 @@ class Scene:
         return {"dims": tuple(dims), "lo": tuple(lo), "hi": tuple(hi), "n_refs": nr.value}
 
     def autotune(self, sample_rays=1 << 22):
-        """Grid resolution, block size and walk mode from timed trials (cached
-        per scene signature). Returns {"dims", "block", "walk", "cached"}."""
+        """Grid resolution, block size, walk mode and ray scheduling (refill
+        threshold, pool ratio) from timed trials (cached per scene signature).
+        Returns {"dims", "block", "walk", "refill", "pool", "cached"}."""
         dims = (C.c_int * 3)()
         block = C.c_int()
         walk = C.c_int()
         rc = check(lib().ffk_autotune(self.h, int(sample_rays), dims, C.byref(block), C.byref(walk), _stream()),
                    "ffk_autotune")
-        return {"dims": tuple(dims), "block": block.value, "walk": walk.value, "cached": rc == 1}
+        refill, pool = self.get_refill()
+        return {"dims": tuple(dims), "block": block.value, "walk": walk.value, "refill": refill, "pool": pool,
+                "cached": rc == 1}
 
     def set_tuning(self, block=0, rays_per_tile=0, hist=-1):
         check(lib().ffk_set_tuning(self.h, block, rays_per_tile, hist), "ffk_set_tuning")
 
-    def set_refill(self, lanes, pool=4):
+    def set_refill(self, lanes, pool=2):
         """Fused-kernel ray scheduling: refill threshold (idle lanes of 64,
         0 = one ray per lane) and pooled-test ratio (0 = per-lane tests)."""
         check(lib().ffk_set_refill(self.h, int(lanes), int(pool)), "ffk_set_refill")
+
+    def get_refill(self):
+        """(refill threshold, pooled-test ratio) currently set on the scene."""
+        lanes, pool = C.c_int(), C.c_int()
+        check(lib().ffk_get_refill(self.h, C.byref(lanes), C.byref(pool)), "ffk_get_refill")
+        return lanes.value, pool.value
 
     def set_walk(self, mode):
         """Empty-cell step of the pooled walk: 0 = per-wavefront choice, 1 = always the jump."""

@@ -149,13 +149,15 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
 /* Ray scheduling of the fused kernel. lanes: 0 = each lane casts one ray at
  * a time (the wavefront waits for its slowest ray); 1..64 = lanes walk one
  * grid cell per iteration and a wavefront starts new rays on its idle lanes
- * once at least `lanes` of its 64 lanes are idle (default 16). pool (with
+ * once at least `lanes` of its 64 lanes are idle (default 32). pool (with
  * lanes > 0): 0 = each lane tests its own cells' triangles; 1..64 = lanes
  * that reach an occupied cell wait, and once waiting lanes x pool >= active
  * lanes the wavefront spreads all their (ray, triangle) tests over its 64
- * lanes (default 4). Results are identical in every setting (integer
+ * lanes (default 2). Results are identical in every setting (integer
  * tallies, the same nearest-hit rule). */
 int ffk_set_refill(ffk_scene* s, int lanes, int pool);
+/* The scene's current refill threshold and pool ratio (either pointer may be NULL). */
+int ffk_get_refill(const ffk_scene* s, int* lanes, int* pool);
 int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool);
 /* How pooled lanes leave an empty grid cell: past its empty box (the
  * distance-field jump) or, next to occupied cells, by one DDA step. mode 0
@@ -196,11 +198,13 @@ int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags);
  * scaled to about sample_rays; when a full run is estimated under 250 ms and
  * F fits in half the free memory, the best candidates are re-timed with the
  * fused kernel on the full layout), the fused kernel's block size (256, 512
- * or 1024) and the walk mode (ffk_set_walk), then leaves the scene built and
- * tuned with them. Results are cached per scene signature (faces, meshes, ray
- * counts, frame) for the life of the process. Returns 0 when tuned, 1 when
- * taken from the cache. None of the choices changes hit indices. out_dims3 /
- * out_block / out_walk may be NULL (block 0 = the automatic choice was kept). */
+ * or 1024), the walk mode (ffk_set_walk) and, timed the same way, the ray
+ * scheduling (ffk_set_refill: 32/2, 24/3 or 16/4), then leaves the scene built
+ * and tuned with them (ffk_get_refill reads the scheduling back). Results are
+ * cached per scene signature (faces, meshes, ray counts, frame) for the life
+ * of the process. Returns 0 when tuned, 1 when taken from the cache. None of
+ * the choices changes hit indices. out_dims3 / out_block / out_walk may be
+ * NULL (block 0 = the automatic choice was kept). */
 int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_block, int* out_walk,
                  void* stream);
 
