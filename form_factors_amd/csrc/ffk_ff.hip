@@ -439,7 +439,16 @@ int build_tiles(ffk_scene* s, int rays_per_tile, hipStream_t st) {
   if (!indirect.empty())
     FFK_HIP(hipMemcpy(s->d_indirect_rows, indirect.data(), indirect.size() * 4, hipMemcpyHostToDevice));
   (void)st;
+  s->tiles_dirty = false;
   return FFK_OK;
+}
+
+// Rebuilds the tile list after a layout change; the old list may still be
+// read by earlier launches on the stream.
+int ensure_tiles(ffk_scene* s, hipStream_t st) {
+  if (!s->has_emission || !s->tiles_dirty) return FFK_OK;
+  FFK_HIP(hipStreamSynchronize(st));
+  return build_tiles(s, s->tune_rays_per_tile, st);
 }
 
 }  // namespace ffk
@@ -672,7 +681,7 @@ int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mod
   s->tune_hist = hist_mode;
   if (rays_per_tile != s->tune_rays_per_tile) {
     s->tune_rays_per_tile = rays_per_tile;
-    if (s->has_emission) return build_tiles(s, rays_per_tile, nullptr);
+    s->tiles_dirty = true;
   }
   return FFK_OK;
 }
@@ -720,9 +729,19 @@ int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool) {
 
 int ffk_ff_needs_tally(const ffk_scene* s) {
   if (!s || !s->has_emission) return 0;
-  const long long max_tile = std::min<long long>(s->tiles_rays_per_tile, 65535);
+  long long rpt = s->tiles_rays_per_tile;
+  size_t n_indirect = (size_t)s->n_indirect_rows;
+  if (s->tiles_dirty) {  // the tile list the next fused launch builds
+    rpt = std::min(s->tune_rays_per_tile > 0 ? (long long)s->tune_rays_per_tile : auto_rays_per_tile(s->n_rays),
+                   65535LL);
+    std::vector<tile> tiles;
+    std::vector<int> indirect;
+    scene_tiles(s, 0, rpt, tiles, indirect);
+    n_indirect = indirect.size();
+  }
+  const long long max_tile = std::min<long long>(rpt, 65535);
   const int hist = pick_hist(s->n_meshes, max_tile, s->tune_hist);
-  return (hist == 0 || s->n_indirect_rows > 0) ? 1 : 0;
+  return (hist == 0 || n_indirect > 0) ? 1 : 0;
 }
 
 int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags) {
@@ -738,6 +757,8 @@ int ffk_ff_fused(ffk_scene* s, int method, unsigned long long seed, unsigned lon
   FFK_REQUIRE(method == FFK_CAST_GRID, FFK_ERR_ARG, "ffk_ff_fused: only the grid caster is fused");
   FFK_REQUIRE(d_F, FFK_ERR_ARG, "ffk_ff_fused: NULL F");
   hipStream_t st = as_stream(stream);
+  int rt;
+  if ((rt = ensure_tiles(s, st)) < 0) return rt;
   const int N = s->n_meshes;
   long long max_tile = std::min<long long>(s->tiles_rays_per_tile, 65535);
   int hist = pick_hist(N, max_tile, s->tune_hist);

@@ -348,11 +348,19 @@ struct ffk_scene {
   int tune_pool = 2;     // pooled tests once waiting lanes x tune_pool >= active lanes (0 = off)
   int tune_walk = 0;     // empty cells: 0 = per-wavefront choice of jump or DDA step, 1 = always the jump code
   int tiles_rays_per_tile = 0;  // value the tile list was built with
+  bool tiles_dirty = false;     // the emission layout changed since the tile list was built
+  int rot_frame = -1;           // frame of d_rot (-1: not computed)
+  // face areas in float (math::triangle_area), computed once with the faces
+  std::vector<float> h_area;
+  // host work arrays of the radiance step
+  std::vector<float> rad_w, rad_p;
+  std::vector<int> rad_front, rad_rear;
 
   ffk::scene_desc desc() const;
 };
 
 namespace ffk {
 int build_tiles(ffk_scene* s, int rays_per_tile, hipStream_t st);
+int ensure_tiles(ffk_scene* s, hipStream_t st);
 int device_cu_count();
 }  // namespace ffk

@@ -137,6 +137,57 @@ bool meshes_valid(const int* h_meshes, int n_meshes, int n_faces) {
   return true;
 }
 
+// Area of face f: from the scene's cache when given (computed there with the
+// same expression), else from the face.
+inline float area_of(const float* h_area, const float* h_faces, int f) {
+  return h_area ? h_area[f] : face_area(h_faces + 9 * (size_t)f);
+}
+
+// radiance_equation::calculate_weights (thermal_equation/radiance_cpu.cpp:96-130):
+// T * eps * area per face side, and their float total in mesh order.
+float weights_impl(const float* h_faces, const float* h_area, int n_faces, const int* h_meshes, int n_meshes,
+                   const float* h_eps2, const float* h_T, float* h_weights) {
+  for (int i = 0; i < 2 * n_faces; ++i) h_weights[i] = 0.f;  // faces outside every mesh
+  float t = 0;
+  for (int m = 0; m < n_meshes; ++m) {
+    const float density = h_T[m];
+    const float front_density = density * h_eps2[2 * m];
+    const float rear_density = density * h_eps2[2 * m + 1];
+    for (int f = h_meshes[3 * m]; f < h_meshes[3 * m] + h_meshes[3 * m + 1]; ++f) {
+      const float a = area_of(h_area, h_faces, f);
+      const float front = front_density * a;
+      const float rear = rear_density * a;
+      t += front + rear;
+      h_weights[2 * f] = front;
+      h_weights[2 * f + 1] = rear;
+    }
+  }
+  return t;
+}
+
+// thermal_equation/radiance_cpu.cpp:148-178: the power of one front / rear
+// ray of each face and the emission added to each mesh.
+void powers_impl(const float* h_faces, const float* h_area, int n_faces, const int* h_meshes, int n_meshes,
+                 const float* h_eps2, const float* h_T, const int* h_front, const int* h_rear, float* h_ray_power,
+                 float* h_emission) {
+  const float sigma = 5.670400e-8f;  // thermal_equation/radiance_cpu.cpp:29
+  for (int i = 0; i < 2 * n_faces; ++i) h_ray_power[i] = 0.f;
+  for (int m = 0; m < n_meshes; ++m) {
+    const float T = h_T[m];
+    const float power_density = sigma * (T * T * T * T);
+    const float front_density = power_density * h_eps2[2 * m];
+    const float rear_density = power_density * h_eps2[2 * m + 1];
+    for (int f = h_meshes[3 * m]; f < h_meshes[3 * m] + h_meshes[3 * m + 1]; ++f) {
+      const float a = area_of(h_area, h_faces, f);
+      const float front_emission = front_density * a;
+      const float rear_emission = rear_density * a;
+      h_emission[m] += front_emission + rear_emission;
+      h_ray_power[2 * f] = h_front[f] > 0 ? front_emission / h_front[f] : 0.f;
+      h_ray_power[2 * f + 1] = h_rear[f] > 0 ? rear_emission / h_rear[f] : 0.f;
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -146,22 +197,7 @@ int ffk_radiance_weights(const float* h_faces, int n_faces, const int* h_meshes,
   FFK_REQUIRE(h_faces && h_meshes && h_eps2 && h_T && h_weights && total && n_faces >= 0 && n_meshes >= 0,
               FFK_ERR_ARG, "ffk_radiance_weights: bad args");
   FFK_REQUIRE(meshes_valid(h_meshes, n_meshes, n_faces), FFK_ERR_ARG, "ffk_radiance_weights: mesh out of range");
-  for (int i = 0; i < 2 * n_faces; ++i) h_weights[i] = 0.f;  // faces outside every mesh
-  float t = 0;
-  for (int m = 0; m < n_meshes; ++m) {
-    const float density = h_T[m];
-    const float front_density = density * h_eps2[2 * m];
-    const float rear_density = density * h_eps2[2 * m + 1];
-    for (int f = h_meshes[3 * m]; f < h_meshes[3 * m] + h_meshes[3 * m + 1]; ++f) {
-      const float a = face_area(h_faces + 9 * (size_t)f);
-      const float front = front_density * a;
-      const float rear = rear_density * a;
-      t += front + rear;
-      h_weights[2 * f] = front;
-      h_weights[2 * f + 1] = rear;
-    }
-  }
-  *total = t;
+  *total = weights_impl(h_faces, nullptr, n_faces, h_meshes, n_meshes, h_eps2, h_T, h_weights);
   return FFK_OK;
 }
 
@@ -172,22 +208,7 @@ int ffk_radiance_powers(const float* h_faces, int n_faces, const int* h_meshes, 
                   n_faces >= 0 && n_meshes >= 0,
               FFK_ERR_ARG, "ffk_radiance_powers: bad args");
   FFK_REQUIRE(meshes_valid(h_meshes, n_meshes, n_faces), FFK_ERR_ARG, "ffk_radiance_powers: mesh out of range");
-  const float sigma = 5.670400e-8f;  // thermal_equation/radiance_cpu.cpp:29
-  for (int i = 0; i < 2 * n_faces; ++i) h_ray_power[i] = 0.f;
-  for (int m = 0; m < n_meshes; ++m) {
-    const float T = h_T[m];
-    const float power_density = sigma * (T * T * T * T);
-    const float front_density = power_density * h_eps2[2 * m];
-    const float rear_density = power_density * h_eps2[2 * m + 1];
-    for (int f = h_meshes[3 * m]; f < h_meshes[3 * m] + h_meshes[3 * m + 1]; ++f) {
-      const float a = face_area(h_faces + 9 * (size_t)f);
-      const float front_emission = front_density * a;
-      const float rear_emission = rear_density * a;
-      h_emission[m] += front_emission + rear_emission;
-      h_ray_power[2 * f] = h_front[f] > 0 ? front_emission / h_front[f] : 0.f;
-      h_ray_power[2 * f + 1] = h_rear[f] > 0 ? rear_emission / h_rear[f] : 0.f;
-    }
-  }
+  powers_impl(h_faces, nullptr, n_faces, h_meshes, n_meshes, h_eps2, h_T, h_front, h_rear, h_ray_power, h_emission);
   return FFK_OK;
 }
 
@@ -285,6 +306,31 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   FFK_HIP(hipStreamSynchronize(st));
   for (int m = 0; m < nm; ++m) h_absorption[m] = (double)acc[m] / scale;
   return FFK_OK;
+}
+
+int ffk_radiance_step(ffk_scene* s, const float* h_eps2, const float* h_T, int n_rays, int frame,
+                      unsigned long long seed, float* h_emission, double* h_absorption,
+                      unsigned long long* d_stats, void* stream) {
+  FFK_REQUIRE(s && s->has_faces && s->has_meshes && h_eps2 && h_T && h_emission && h_absorption && n_rays >= 0,
+              FFK_ERR_ARG, "ffk_radiance_step: bad args");
+  FFK_REQUIRE(s->has_grid, FFK_ERR_ARG, "ffk_radiance_step: the grid must be built");
+  const int nf = s->n_faces, nm = s->n_meshes;
+  s->rad_w.resize(2 * (size_t)nf);
+  const float total =
+      weights_impl(s->h_faces.data(), s->h_area.data(), nf, s->h_meshes.data(), nm, h_eps2, h_T, s->rad_w.data());
+  s->rad_front.resize(nf);
+  s->rad_rear.resize(nf);
+  int r;
+  if ((r = ffk_emitted_counts(s->rad_w.data(), nf, n_rays, total, s->rad_front.data(), s->rad_rear.data())) < 0)
+    return r;
+  if (!s->has_emission || frame != s->frame || s->rad_front != s->h_front || s->rad_rear != s->h_rear) {
+    if ((r = ffk_scene_set_emission(s, s->rad_front.data(), s->rad_rear.data(), frame, nullptr, stream)) < 0)
+      return r;
+  }
+  s->rad_p.resize(2 * (size_t)nf);
+  powers_impl(s->h_faces.data(), s->h_area.data(), nf, s->h_meshes.data(), nm, h_eps2, h_T, s->h_front.data(),
+              s->h_rear.data(), s->rad_p.data(), h_emission);
+  return ffk_radiance(s, seed, s->rad_p.data(), h_absorption, d_stats, stream);
 }
 
 }  // extern "C"

@@ -328,6 +328,8 @@ static void free_emission(ffk_scene* s) {
   s->d_tiles = nullptr;
   s->d_indirect_rows = nullptr;
   s->n_tiles = 0;
+  s->tiles_dirty = false;
+  s->rot_frame = -1;
   s->has_emission = false;
 }
 
@@ -391,6 +393,16 @@ int ffk_scene_set_faces(ffk_scene* s, const float* h_faces, int n_faces, void* s
   s->has_meshes = false;
   s->n_faces = n_faces;
   s->h_faces.assign(h_faces, h_faces + 9 * (size_t)n_faces);
+  // math::triangle_area (math/triangle.cpp:86-92) in float, as the radiance
+  // weights and powers use it every step
+  s->h_area.resize(n_faces);
+  for (int f = 0; f < n_faces; ++f) {
+    const float* p = h_faces + 9 * (size_t)f;
+    const v3 u = mk(p[3], p[4], p[5]) - mk(p[0], p[1], p[2]);
+    const v3 w = mk(p[6], p[7], p[8]) - mk(p[0], p[1], p[2]);
+    const v3 n = cross(u, w);
+    s->h_area[f] = sqrtf(dot(n, n)) / 2;
+  }
   int r;
   if ((r = dalloc(&s->d_faces, 9 * (size_t)n_faces)) < 0) return r;
   if ((r = dalloc(&s->d_tri, (size_t)n_faces)) < 0) return r;
@@ -432,8 +444,10 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
   FFK_REQUIRE(frame == FFK_FRAME_REFERENCE || frame == FFK_FRAME_ORTHONORMAL, FFK_ERR_ARG,
               "unknown frame %d", frame);
   hipStream_t st = as_stream(stream);
+  // The layout buffers are reused (their sizes follow the faces and meshes,
+  // whose setters free them): wait for the work that may still read them.
   FFK_HIP(hipStreamSynchronize(st));
-  free_emission(s);
+  s->has_emission = false;
   const int nf = s->n_faces, nm = s->n_meshes;
   s->h_front.assign(h_front, h_front + nf);
   s->h_rear.assign(h_rear, h_rear + nf);
@@ -459,18 +473,22 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
   s->n_rays = total;
   s->frame = frame;
   int r;
-  if ((r = dalloc(&s->d_rot, 9 * (size_t)nf)) < 0) return r;
-  if ((r = dalloc(&s->d_face_off, (size_t)nf + 1)) < 0) return r;
-  if ((r = dalloc(&s->d_front, (size_t)nf)) < 0) return r;
-  if ((r = dalloc(&s->d_mesh_off, (size_t)nm + 1)) < 0) return r;
+  if (!s->d_rot && (r = dalloc(&s->d_rot, 9 * (size_t)nf)) < 0) return r;
+  if (!s->d_face_off && (r = dalloc(&s->d_face_off, (size_t)nf + 1)) < 0) return r;
+  if (!s->d_front && (r = dalloc(&s->d_front, (size_t)nf)) < 0) return r;
+  if (!s->d_mesh_off && (r = dalloc(&s->d_mesh_off, (size_t)nm + 1)) < 0) return r;
   FFK_HIP(hipMemcpy(s->d_face_off, s->h_face_off.data(), 8 * ((size_t)nf + 1), hipMemcpyHostToDevice));
   FFK_HIP(hipMemcpy(s->d_front, h_front, 4 * (size_t)nf, hipMemcpyHostToDevice));
   FFK_HIP(hipMemcpy(s->d_mesh_off, s->h_mesh_off.data(), 8 * ((size_t)nm + 1), hipMemcpyHostToDevice));
-  k_face_rotation<<<(nf + 255) / 256, 256, 0, st>>>(s->d_faces, nf, frame, s->d_rot);
-  FFK_HIP(hipGetLastError());
+  if (s->rot_frame != frame) {  // the rotations depend on the faces and the frame only
+    k_face_rotation<<<(nf + 255) / 256, 256, 0, st>>>(s->d_faces, nf, frame, s->d_rot);
+    FFK_HIP(hipGetLastError());
+    s->rot_frame = frame;
+  }
   s->has_emission = true;
+  s->tiles_dirty = true;  // the fused kernel's tile list is built when a fused launch needs it
   if (n_rays_out) *n_rays_out = total;
-  return ffk::build_tiles(s, s->tune_rays_per_tile, st);
+  return FFK_OK;
 }
 
 int ffk_scene_layout(const ffk_scene* s, float* h_faces, int* h_meshes, int* h_front, int* h_rear, int* frame) {

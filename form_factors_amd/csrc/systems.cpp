@@ -733,8 +733,7 @@ struct state {
   ffk_scene* s = nullptr;
   hipStream_t st = nullptr;
   bool grid = false;
-  int frame = -1;
-  std::vector<int> face2mesh, front, rear;
+  std::vector<int> face2mesh;
   std::vector<float> eps2, weights, power;
   std::vector<double> absorption;
 };
@@ -783,7 +782,6 @@ int set_scene(system* sys, subject::scene_t* scene) {
     s->eps2[2 * m + 1] = mat.rear.emissivity;
   }
   s->grid = false;
-  s->front.clear();
   if (nf == 0 || nm == 0) return THERMAL_EQUATION_OK;
   if (!s->s && ffk_scene_create(&s->s) < 0) return -THERMAL_EQUATION_ERROR;
   if (ffk_scene_set_faces(s->s, (const float*)scene->faces, nf, s->st) < 0) return -THERMAL_EQUATION_ERROR;
@@ -801,39 +799,34 @@ int calculate(system* sys, thermal_equation::task_t* task) {
   const int nf = scene->n_faces, nm = scene->n_meshes;
   if (nf == 0 || nm == 0) return THERMAL_EQUATION_OK;
   const int n_rays = s->params.n_rays;
-  float total = 0;
-  s->weights.assign(2 * (size_t)nf, 0.f);
-  if (ffk_radiance_weights((const float*)scene->faces, nf, (const int*)scene->meshes, nm, s->eps2.data(),
-                           task->temperatures, s->weights.data(), &total) < 0)
-    return -THERMAL_EQUATION_ERROR;
-  std::vector<int> fr(nf), rr(nf);
-  ffk_emitted_counts(s->weights.data(), nf, n_rays, total, fr.data(), rr.data());
-  s->power.assign(2 * (size_t)nf, 0.f);
-  if (ffk_radiance_powers((const float*)scene->faces, nf, (const int*)scene->meshes, nm, s->eps2.data(),
-                          task->temperatures, fr.data(), rr.data(), s->power.data(), task->emission) < 0)
-    return -THERMAL_EQUATION_ERROR;
-  s->absorption.assign(nm, 0.0);
   emission::system_t* emitter = s->params.emitter;
   hip_malley_emission::state* em =
       hip_malley_emission::is_hip(emitter) ? hip_malley_emission::state_of(emitter) : nullptr;
   int r;
+  s->absorption.assign(nm, 0.0);
   if (em && hip_ray_caster::is_hip(em->caster) && hip_ray_caster::method_of(em->caster) == FFK_CAST_GRID) {
-    // Fused on the device: emission, grid cast and deposit in one pass.
-    if (fr != s->front || rr != s->rear || em->frame != s->frame) {
-      if (ffk_scene_set_emission(s->s, fr.data(), rr.data(), em->frame, nullptr, s->st) < 0)
-        return -THERMAL_EQUATION_ERROR;
-      s->front = fr;
-      s->rear = rr;
-      s->frame = em->frame;
-    }
+    // Fused on the device: weights, layout and powers on the host with the
+    // scene's cached areas, then emission, grid cast and deposit in one pass.
     if (!s->grid) {
       if (ffk_scene_build_grid(s->s, 0, 0, 0, s->st) < 0) return -THERMAL_EQUATION_ERROR;
       s->grid = true;
     }
     const unsigned long long key = hip_malley_emission::next_key(em);
-    if (ffk_radiance(s->s, key, s->power.data(), s->absorption.data(), nullptr, s->st) < 0)
+    if (ffk_radiance_step(s->s, s->eps2.data(), task->temperatures, n_rays, em->frame, key, task->emission,
+                          s->absorption.data(), nullptr, s->st) < 0)
       return -THERMAL_EQUATION_ERROR;
   } else {
+    float total = 0;
+    s->weights.assign(2 * (size_t)nf, 0.f);
+    if (ffk_radiance_weights((const float*)scene->faces, nf, (const int*)scene->meshes, nm, s->eps2.data(),
+                             task->temperatures, s->weights.data(), &total) < 0)
+      return -THERMAL_EQUATION_ERROR;
+    std::vector<int> fr(nf), rr(nf);
+    ffk_emitted_counts(s->weights.data(), nf, n_rays, total, fr.data(), rr.data());
+    s->power.assign(2 * (size_t)nf, 0.f);
+    if (ffk_radiance_powers((const float*)scene->faces, nf, (const int*)scene->meshes, nm, s->eps2.data(),
+                            task->temperatures, fr.data(), rr.data(), s->power.data(), task->emission) < 0)
+      return -THERMAL_EQUATION_ERROR;
     // Any other emitter emits and casts; the walk runs here, in double.
     malley_emission::task_t et(n_rays, total, s->weights.data());
     if ((r = emitter->methods->calculate(emitter, &et)) < 0) {

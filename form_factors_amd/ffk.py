@@ -63,6 +63,7 @@ SIGNATURES = {
     "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
     "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "ffk_radiance": (C.c_int, [_vp, _ull, _f, C.POINTER(C.c_double), _vp, _vp]),
+    "ffk_radiance_step": (C.c_int, [_vp, _f, _f, C.c_int, C.c_int, _ull, _f, C.POINTER(C.c_double), _vp, _vp]),
     "ffk_radiance_weights": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _f, _f]),
     "ffk_radiance_powers": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, _f, _f]),
     "ffk_bsphere": (C.c_int, [_f, C.c_int, _f, _f]),
@@ -253,6 +254,18 @@ class Scene:
         self.frame = frame
         return self.n_rays
 
+    def _sync_layout(self):
+        """Reads back the emission layout a library call set (front and rear
+        counts, total rays, frame)."""
+        fr = np.zeros(self.n_faces, np.int32)
+        rr = np.zeros(self.n_faces, np.int32)
+        frame = C.c_int()
+        check(lib().ffk_scene_layout(self.h, None, None, fr.ctypes.data_as(_i), rr.ctypes.data_as(_i),
+                                     C.byref(frame)), "ffk_scene_layout")
+        n = C.c_longlong()
+        check(lib().ffk_scene_counts(self.h, None, None, C.byref(n)), "ffk_scene_counts")
+        self.front, self.rear, self.n_rays, self.frame = fr, rr, n.value, frame.value
+
     def sun_counts(self, direction, center, distance, radius, n_rays, seed, aperture=0, stats=None):
         """Hits per mesh side of n_rays parallel rays: uint64[n_meshes, 2] (front, rear)."""
         counts = np.zeros(2 * self.n_meshes, np.uint64)
@@ -277,13 +290,19 @@ class Scene:
         return out
 
     def radiance_step(self, eps2, T, n_rays, seed, frame=FRAME_REFERENCE, stats=None):
-        """One radiance_equation::calculate: layout from the radiance weights,
-        then the device pass. Returns (emission float32, absorption float64)."""
-        w, total = radiance_weights(self.faces, self.meshes, eps2, T)
-        front, rear = emitted_counts(w, n_rays, total)
-        self.set_emission(front=front, rear=rear, frame=frame)
-        p, em = radiance_powers(self.faces, self.meshes, eps2, T, front, rear)
-        return em, self.radiance(seed, p, stats=stats)
+        """One radiance_equation::calculate (ffk_radiance_step): layout from
+        the radiance weights, then the device pass. Returns (emission float32,
+        absorption float64)."""
+        e2 = _np(eps2, np.float32).reshape(-1)
+        t = _np(T, np.float32).reshape(-1)
+        assert len(e2) == 2 * self.n_meshes and len(t) == self.n_meshes
+        em = np.zeros(self.n_meshes, np.float32)
+        ab = np.zeros(self.n_meshes, np.float64)
+        check(lib().ffk_radiance_step(self.h, e2.ctypes.data_as(_f), t.ctypes.data_as(_f), int(n_rays), int(frame),
+                                      seed, em.ctypes.data_as(_f), ab.ctypes.data_as(C.POINTER(C.c_double)),
+                                      _ptr(stats), _stream()), "ffk_radiance_step")
+        self._sync_layout()
+        return em, ab
 
     def radiance(self, seed, ray_power, stats=None):
         """Absorbed power per mesh (float64) for the current emission layout;

@@ -236,6 +236,17 @@ int ffk_radiance_weights(const float* h_faces, int n_faces, const int* h_meshes,
 int ffk_radiance_powers(const float* h_faces, int n_faces, const int* h_meshes, int n_meshes,
                         const float* h_eps2, const float* h_T, const int* h_front, const int* h_rear,
                         float* h_ray_power, float* h_emission);
+/* One whole radiance_equation::calculate step (radiance_cpu.cpp:132-190) on
+ * the scene: the weights and their total, the emitted ray counts
+ * (ffk_emitted_counts with n_rays), the emission layout in `frame` (set again
+ * only when the counts or the frame change), the ray powers and emission, and
+ * the device pass of ffk_radiance. The same arithmetic as the calls above, with
+ * the face areas computed once per scene. h_emission (float[n_meshes]) is
+ * added to, h_absorption (double[n_meshes]) overwritten. The scene's grid must
+ * be built. */
+int ffk_radiance_step(ffk_scene* s, const float* h_eps2, const float* h_T, int n_rays, int frame,
+                      unsigned long long seed, float* h_emission, double* h_absorption,
+                      unsigned long long* d_stats, void* stream);
 
 /* ---- distant source (sun) absorption ------------------------------------------
  * parallel_rays_cpu::calculate (thermal_equation/parallel_rays_cpu.cpp:92-131).

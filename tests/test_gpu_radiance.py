@@ -108,3 +108,49 @@ def test_radiance_errors(gpu):
         sc.radiance(1, bad)
     zero = sc.radiance(1, np.zeros(2 * len(faces), np.float32))
     assert (zero == 0).all()
+
+
+def test_radiance_step_equals_explicit_calls(gpu):
+    """ffk_radiance_step (cached areas, layout set again only when the counts
+    change) equals the explicit weights -> counts -> layout -> powers ->
+    device pass sequence bit for bit over several steps, including a repeated
+    temperature (layout kept) and a frame change; a fused form-factor launch
+    afterwards (its tile list rebuilt for the last layout) equals one on a
+    fresh scene with that layout."""
+    import torch
+
+    from form_factors_amd import ffk, scenes
+
+    faces, meshes = scenes.confetti_scene(300, seed=7)
+    rng = np.random.default_rng(3)
+    eps = rng.uniform(0.1, 1, (len(meshes), 2)).astype(np.float32)
+    Ts = [rng.uniform(250, 400, len(meshes)).astype(np.float32) for _ in range(3)]
+    steps = [(Ts[0], 0), (Ts[1], 0), (Ts[1], 0), (Ts[2], 1), (Ts[0], 1)]
+    a = gpu.Scene(faces, meshes)
+    a.build_grid()
+    b = gpu.Scene(faces, meshes)
+    b.build_grid()
+    for k, (T, frame) in enumerate(steps):
+        print(f"radiance step {k} frame {frame}", flush=True)
+        em_a, ab_a = a.radiance_step(eps, T, 120_000, 100 + k, frame=frame)
+        w, total = ffk.radiance_weights(faces, meshes, eps, T)
+        front, rear = ffk.emitted_counts(w, 120_000, total)
+        b.set_emission(front=front, rear=rear, frame=frame)
+        p, em_b = ffk.radiance_powers(faces, meshes, eps, T, front, rear)
+        ab_b = b.radiance(100 + k, p)
+        assert np.array_equal(bits(em_a), bits(em_b)), k
+        assert np.array_equal(ab_a.view(np.uint64), ab_b.view(np.uint64)), k
+        assert np.array_equal(a.front, front) and np.array_equal(a.rear, rear) and a.frame == frame
+        assert a.n_rays == int(front.sum() + rear.sum())
+    N = len(meshes)
+    Fa = torch.empty((N, N), dtype=torch.float32, device="cuda")
+    Fb = torch.empty((N, N), dtype=torch.float32, device="cuda")
+    a.ff_fused(9, F=Fa)
+    c = gpu.Scene(faces, meshes)
+    c.set_emission(front=a.front, rear=a.rear, frame=a.frame)
+    c.build_grid()
+    c.ff_fused(9, F=Fb)
+    torch.cuda.synchronize()
+    assert torch.equal(Fa.view(torch.int32), Fb.view(torch.int32))
+    for s in (a, b, c):
+        s.close()
