@@ -332,7 +332,11 @@ struct ffk_scene {
   // one block per scene so that launches of different scenes on different
   // streams do not share it
   unsigned long long* d_misc = nullptr;
-  // radiance work buffers, grown on demand
+  // radiance work buffers, grown on demand: the walk tables (start, q, face)
+  // and the per-mesh sums share one device block and one pinned host block
+  // of the same layout, so that each pass makes one upload and one download
+  void* d_walk_block = nullptr;
+  void* h_walk_block = nullptr;  // pinned
   long long* d_walk_start = nullptr;
   int* d_walk_face = nullptr;
   unsigned long long* d_walk_q = nullptr;

@@ -219,11 +219,38 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   hipStream_t st = as_stream(stream);
   const int nf = s->n_faces, nm = s->n_meshes;
   const long long total = s->n_rays;
+  // Staging for W walk entries and nm meshes, in this order: start[W + 1]
+  // (int64), q[2W] (uint64), acc[nm] (uint64), face[W + 1] (int32). Every walk
+  // entry is a face, so W = n_faces bounds them.
+  const size_t W = (size_t)nf;
+  const size_t off_q = 8 * (W + 1), off_acc = off_q + 16 * W, off_face = off_acc + 8 * (size_t)nm;
+  const size_t bytes = off_face + 4 * (W + 1);
+  if ((int)W + 1 > s->rad_cap_walk || nm > s->rad_cap_meshes) {
+    FFK_HIP(hipStreamSynchronize(st));
+    dfree(s->d_walk_block);
+    s->d_walk_block = nullptr;
+    if (s->h_walk_block) (void)hipHostFree(s->h_walk_block);
+    s->h_walk_block = nullptr;
+    s->rad_cap_walk = s->rad_cap_meshes = 0;
+    int r;
+    if ((r = dmalloc(&s->d_walk_block, bytes)) < 0) return r;
+    FFK_HIP(hipHostMalloc(&s->h_walk_block, bytes, hipHostMallocDefault));
+    s->rad_cap_walk = (int)W + 1;
+    s->rad_cap_meshes = nm;
+  }
+  char* hb = static_cast<char*>(s->h_walk_block);
+  char* db = static_cast<char*>(s->d_walk_block);
+  long long* start = reinterpret_cast<long long*>(hb);
+  unsigned long long* q = reinterpret_cast<unsigned long long*>(hb + off_q);
+  unsigned long long* acc = reinterpret_cast<unsigned long long*>(hb + off_acc);
+  int* face = reinterpret_cast<int*>(hb + off_face);
+  s->d_walk_start = reinterpret_cast<long long*>(db);
+  s->d_walk_q = reinterpret_cast<unsigned long long*>(db + off_q);
+  s->d_rad_acc = reinterpret_cast<unsigned long long*>(db + off_acc);
+  s->d_walk_face = reinterpret_cast<int*>(db + off_face);
   // Walk entries: faces in mesh order with their first walk position; the
   // walk stops at the last emitted ray (radiance_cpu.cpp:170).
-  std::vector<long long> start;
-  std::vector<int> face;
-  std::vector<double> pw;
+  int n_walk = 0;
   long long pos = 0;
   double upper = 0.0;
   for (int m = 0; m < nm && pos < total; ++m) {
@@ -232,59 +259,44 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
       const float pf = h_ray_power[2 * f], pr = h_ray_power[2 * f + 1];
       FFK_REQUIRE(pf >= 0.f && pr >= 0.f && std::isfinite(pf) && std::isfinite(pr), FFK_ERR_ARG,
                   "ffk_radiance: ray power of face %d is negative or not finite", f);
-      start.push_back(pos);
-      face.push_back(f);
-      pw.push_back(pf);
-      pw.push_back(pr);
+      start[n_walk] = pos;
+      face[n_walk] = f;
+      ++n_walk;
       upper += (double)s->h_front[f] * pf + (double)s->h_rear[f] * pr;
       pos += (long long)s->h_front[f] + s->h_rear[f];
     }
   }
   const long long walk_total = std::min(pos, total);
   for (int m = 0; m < nm; ++m) h_absorption[m] = 0.0;
-  if (start.empty() || upper <= 0.0) return FFK_OK;
-  start.push_back(walk_total);
+  if (n_walk == 0 || upper <= 0.0) return FFK_OK;
+  start[n_walk] = walk_total;
   // Fixed point: the largest possible sum stays below 2^62.
   int e2 = 0;
   std::frexp(upper, &e2);  // upper < 2^e2
   const double scale = std::ldexp(1.0, 62 - e2);
-  std::vector<unsigned long long> q(pw.size());
-  for (size_t i = 0; i < pw.size(); ++i) q[i] = (unsigned long long)std::llrint(pw[i] * scale);
-  const int n_walk = (int)face.size();
-  int r;
-  if (n_walk + 1 > s->rad_cap_walk) {  // the three walk arrays share one capacity
-    FFK_HIP(hipStreamSynchronize(st));
-    dfree(s->d_walk_start);
-    dfree(s->d_walk_face);
-    dfree(s->d_walk_q);
-    s->d_walk_start = nullptr;
-    s->d_walk_face = nullptr;
-    s->d_walk_q = nullptr;
-    s->rad_cap_walk = 0;
-    if ((r = dalloc(&s->d_walk_start, (size_t)n_walk + 1)) < 0 || (r = dalloc(&s->d_walk_face, (size_t)n_walk + 1)) < 0 ||
-        (r = dalloc(&s->d_walk_q, 2 * ((size_t)n_walk + 1))) < 0)
-      return r;
-    s->rad_cap_walk = n_walk + 1;
+  for (int k = 0; k < n_walk; ++k) {
+    const int f = face[k];
+    q[2 * k] = (unsigned long long)std::llrint((double)h_ray_power[2 * f] * scale);
+    q[2 * k + 1] = (unsigned long long)std::llrint((double)h_ray_power[2 * f + 1] * scale);
   }
-  if (nm > s->rad_cap_meshes) {
-    FFK_HIP(hipStreamSynchronize(st));
-    dfree(s->d_rad_acc);
-    s->d_rad_acc = nullptr;
-    s->rad_cap_meshes = 0;
-    if ((r = dalloc(&s->d_rad_acc, (size_t)nm)) < 0) return r;
-    s->rad_cap_meshes = nm;
-  }
-  FFK_HIP(hipMemcpyAsync(s->d_walk_start, start.data(), 8 * start.size(), hipMemcpyHostToDevice, st));
-  FFK_HIP(hipMemcpyAsync(s->d_walk_face, face.data(), 4 * face.size(), hipMemcpyHostToDevice, st));
-  FFK_HIP(hipMemcpyAsync(s->d_walk_q, q.data(), 8 * q.size(), hipMemcpyHostToDevice, st));
+  // start, q and (past acc) face in one copy; acc is zeroed on the device
+  FFK_HIP(hipMemcpyAsync(db, hb, off_acc, hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(db + off_face, hb + off_face, 4 * ((size_t)n_walk + 1), hipMemcpyHostToDevice, st));
   k_fill_acc<<<(nm + 255) / 256, 256, 0, st>>>(s->d_rad_acc, nm);
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   if (walk_total > 0 && s->tune_refill > 0 && s->tune_pool > 0) {
-    // About eight blocks per CU, at least a ray per lane and at most 64.
-    const long long target = 8LL * device_cu_count();
-    const long long chunk = std::min<long long>(16384, std::max<long long>(256, (walk_total + target - 1) / target));
-    const unsigned blocks = (unsigned)((walk_total + chunk - 1) / chunk);
+    // Whole rounds of the blocks the device holds at once (the last round is
+    // not left part empty), at most 16384 rays and at least a ray per lane
+    // per block.
     auto go = [&](auto kernel) {
+      static int per_cu = 0;  // per kernel instantiation (the lambda is instantiated per kernel type)
+      if (per_cu < 1 &&
+          (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1))
+        per_cu = 1;
+      const long long resident = (long long)per_cu * device_cu_count();
+      const long long rounds = std::max<long long>(1, (walk_total + resident * 16384 - 1) / (resident * 16384));
+      const long long chunk = std::max<long long>(256, (walk_total + resident * rounds - 1) / (resident * rounds));
+      const unsigned blocks = (unsigned)((walk_total + chunk - 1) / chunk);
       kernel<<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q, walk_total,
                                      chunk, k0, k1, s->d_rad_acc, d_stats, s->tune_refill, s->tune_pool);
     };
@@ -301,8 +313,7 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
                                                 walk_total, k0, k1, s->d_rad_acc, nullptr);
   }
   FFK_HIP(hipGetLastError());
-  std::vector<unsigned long long> acc(nm);
-  FFK_HIP(hipMemcpyAsync(acc.data(), s->d_rad_acc, 8 * (size_t)nm, hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipMemcpyAsync(acc, s->d_rad_acc, 8 * (size_t)nm, hipMemcpyDeviceToHost, st));
   FFK_HIP(hipStreamSynchronize(st));
   for (int m = 0; m < nm; ++m) h_absorption[m] = (double)acc[m] / scale;
   return FFK_OK;

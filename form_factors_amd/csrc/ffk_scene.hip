@@ -370,10 +370,8 @@ void ffk_scene_destroy(ffk_scene* s) {
   dfree(s->d_face2mesh);
   dfree(s->d_desc);
   dfree(s->d_misc);
-  dfree(s->d_walk_start);
-  dfree(s->d_walk_face);
-  dfree(s->d_walk_q);
-  dfree(s->d_rad_acc);
+  dfree(s->d_walk_block);
+  if (s->h_walk_block) (void)hipHostFree(s->h_walk_block);
   delete s;
 }
 
