@@ -289,9 +289,8 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
     // not left part empty), at most 16384 rays and at least a ray per lane
     // per block.
     auto go = [&](auto kernel) {
-      static int per_cu = 0;  // per kernel instantiation (the lambda is instantiated per kernel type)
-      if (per_cu < 1 &&
-          (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1))
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1)
         per_cu = 1;
       const long long resident = (long long)per_cu * device_cu_count();
       const long long rounds = std::max<long long>(1, (walk_total + resident * 16384 - 1) / (resident * 16384));
