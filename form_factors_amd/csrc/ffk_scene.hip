@@ -547,9 +547,12 @@ int ffk_scene_build_grid(ffk_scene* s, int nx, int ny, int nz, void* stream) {
   }
   int dims[3] = {nx, ny, nz};
   if (nx == 0 || ny == 0 || nz == 0) {
-    // About two cells per face, cells as close to cubes as the box allows.
+    // Cells as close to cubes as the box allows: about 12 per face (empty
+    // cells cost little with the octant jumps, smaller cells mean fewer
+    // tests), but at most 2^18 cells (2 MB of cell words, beside the face
+    // terms in L2) unless that leaves fewer than 2 per face.
     double vol = (double)ext[0] * ext[1] * ext[2];
-    double target = 2.0 * nf;
+    double target = std::min(12.0 * nf, std::max(2.0 * nf, 262144.0));
     double cell = std::cbrt(vol / target);
     for (int a = 0; a < 3; ++a)
       dims[a] = (int)std::min(512.0, std::max(1.0, std::ceil(ext[a] / cell)));

@@ -455,7 +455,7 @@ class Batch:
     def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0):
         check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks), "ffk_batch_set_tuning")
 
-    def set_refill(self, lanes, pool=4):
+    def set_refill(self, lanes, pool=2):
         check(lib().ffk_batch_set_refill(self.h, int(lanes), int(pool)), "ffk_batch_set_refill")
 
     def set_walk(self, mode):
