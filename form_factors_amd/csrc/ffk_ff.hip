@@ -484,6 +484,13 @@ struct ffk_batch {
   int refill = kDefaultRefill;
   int pool = kDefaultPool;
   int walk = 0;
+  // One stream per LDS class (default priority): the class launches are
+  // forked from the launch stream after the tallies are cleared and joined
+  // before the normalisation, so that a class's blocks start while the
+  // previous class drains (in a captured graph: parallel branches).
+  std::vector<hipStream_t> fork;
+  hipEvent_t ev_fork = nullptr;
+  std::vector<hipEvent_t> ev_join;
   // graph state
   hipStream_t cap = nullptr;  // capture stream (default priority)
   hipGraph_t graph = nullptr;
@@ -850,6 +857,9 @@ void ffk_batch_destroy(ffk_batch* b) {
   (void)hipDeviceSynchronize();
   free_graph(b);
   if (b->cap) (void)hipStreamDestroy(b->cap);
+  for (hipStream_t f : b->fork) (void)hipStreamDestroy(f);
+  for (hipEvent_t e : b->ev_join) (void)hipEventDestroy(e);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
   dfree(b->d_tiles);
   dfree(b->d_descs);
   dfree(b->d_rows);
@@ -925,9 +935,26 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
     dim3 grid((unsigned)b->n_rows);
     k_zero_rows_batch<<<grid, 256, 0, st>>>(b->d_rows, b->n_rows, b->d_descs, b->d_tallies);
   }
+  if (n_classes > 1) {
+    while ((int)b->fork.size() < n_classes) {
+      hipStream_t f = nullptr;
+      hipEvent_t e = nullptr;
+      FFK_HIP(hipStreamCreateWithFlags(&f, hipStreamNonBlocking));
+      b->fork.push_back(f);
+      FFK_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      b->ev_join.push_back(e);
+    }
+    if (!b->ev_fork) FFK_HIP(hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    FFK_HIP(hipEventRecord(b->ev_fork, st));
+  }
   for (int c = 0; c < n_classes; ++c) {
     const ffk_batch::klass& k = b->classes[c];
     const int block = b->block > 0 ? b->block : default_block(k.hist, k.lds);
+    hipStream_t cs = st;
+    if (n_classes > 1) {
+      cs = b->fork[c];
+      FFK_HIP(hipStreamWaitEvent(cs, b->ev_fork, 0));
+    }
     fused_args A;
     A.descs = b->d_descs;
     A.tiles = b->d_tiles + k.first;
@@ -941,7 +968,11 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
     A.pool = b->pool;
     A.walk = b->walk;
     const int n_blocks = batch_blocks(b, k.hist, k.lds, block, k.n);
-    FFK_HIP(launch_fused(k.hist, true, stats, dim3((unsigned)n_blocks), block, k.lds, st, A));
+    FFK_HIP(launch_fused(k.hist, true, stats, dim3((unsigned)n_blocks), block, k.lds, cs, A));
+    if (n_classes > 1) {
+      FFK_HIP(hipEventRecord(b->ev_join[c], cs));
+      FFK_HIP(hipStreamWaitEvent(st, b->ev_join[c], 0));
+    }
   }
   for (size_t i = 0; i < b->scenes.size(); ++i) {
     if (!b->scene_global[i]) continue;
