@@ -159,11 +159,12 @@ int ffk_set_refill(ffk_scene* s, int lanes, int pool);
 /* The scene's current refill threshold and pool ratio (either pointer may be NULL). */
 int ffk_get_refill(const ffk_scene* s, int* lanes, int* pool);
 int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool);
-/* How pooled lanes leave an empty grid cell: past its empty box (the
- * distance-field jump) or, next to occupied cells, by one DDA step. mode 0
- * (default): a wavefront runs the jump code for all its empty-cell lanes when
- * any of them can jump, the DDA step otherwise; mode 1: always the jump code
- * (fastest when rays cross large empty regions). Same cells, same hits. */
+/* How pooled lanes leave an empty grid cell: past its empty box toward the
+ * ray's direction octant (the jump) or, when that box is the cell itself, by
+ * one DDA step. mode 0 (default): a wavefront runs the jump code for all its
+ * empty-cell lanes when any of them can jump, the DDA step otherwise; mode 1:
+ * always the jump code (fastest when rays cross large empty regions). Same
+ * cells, same hits. */
 int ffk_set_walk(ffk_scene* s, int mode);
 int ffk_batch_set_walk(ffk_batch* b, int mode);
 
