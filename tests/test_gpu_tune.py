@@ -20,6 +20,8 @@ def test_autotune_keeps_bits_and_caches(gpu, oracle):
     t = sc.autotune(sample_rays=50_000)
     print("tuned", t, flush=True)
     assert not t["cached"] and min(t["dims"]) >= 1 and t["block"] in (0, 256, 512, 1024) and t["walk"] in (0, 1)
+    assert (t["refill"], t["pool"]) in ((32, 2), (24, 3), (16, 4))
+    assert sc.get_refill() == (t["refill"], t["pool"])
     assert sc.grid_info()["dims"] == t["dims"]
     after = sc.ff_fused(21).cpu().numpy()
     assert np.array_equal(bits(after), bits(before))
@@ -31,4 +33,5 @@ def test_autotune_keeps_bits_and_caches(gpu, oracle):
     sc2.set_emission(n_rays=400 * len(faces), frame=gpu.FRAME_ORTHONORMAL)
     t2 = sc2.autotune(sample_rays=50_000)
     assert t2["cached"] and t2["dims"] == t["dims"] and t2["block"] == t["block"] and t2["walk"] == t["walk"]
+    assert (t2["refill"], t2["pool"]) == (t["refill"], t["pool"]) and sc2.get_refill() == (t["refill"], t["pool"])
     assert np.array_equal(bits(sc2.ff_fused(21).cpu().numpy()), bits(before))
