@@ -382,6 +382,56 @@ def test_radiance_equation_plugin(P, oracle, caster_type):
     assert not L.ffp_thermal_equation_create(P.THERMAL_EQUATION_RADIANCE_HIP, None, 1000, 0)
 
 
+
+def test_radiance_equation_plugin_scene_switch(P, oracle):
+    """One radiance equation (fused device pass) given a scene, then a smaller
+    one, then the first again: each call matches the oracle on its own scene
+    (the device scene, its layout buffers and its grid are rebuilt when the
+    scene changes size)."""
+    from form_factors_amd import scenes
+
+    L = P.lib()
+    g = golden("radiance_multi.npz")
+    rng = np.random.default_rng(8)
+    box_faces, _ = scenes.ff_box()
+    box_meshes = scenes.per_face_meshes(12)
+    cases = [(g["faces"], g["meshes"], g["eps"], g["T"]),
+             (box_faces, box_meshes, rng.uniform(0.1, 1, (12, 2)).astype(np.float32),
+              rng.uniform(250, 400, 12).astype(np.float32))]
+    rc = caster(P, 4)
+    em = L.ffp_emission_create(P.EMISSION_MALLEY_HIP, rc)
+    seed = 91
+    assert L.ffp_emission_set_seed(em, seed, 0) == 0
+    eq = L.ffp_thermal_equation_create(P.THERMAL_EQUATION_RADIANCE_HIP, em, 40_000, 0)
+    assert eq
+    for call, which in enumerate((0, 1, 0)):
+        faces, meshes, eps, T = cases[which]
+        nm = len(meshes)
+        mats = []
+        for m in range(nm):
+            mat = P.black_body()
+            mat.front[4], mat.rear[4] = float(eps[m, 0]), float(eps[m, 1])
+            mats.append(mat)
+        smeshes = np.ascontiguousarray(meshes, np.int32).copy()
+        smeshes[:, 2] = np.arange(nm)
+        sc, keep = subject_scene(P, faces, smeshes, mats)
+        print(f"radiance plugin scene switch: call {call} scene {which} ({len(faces)} faces)", flush=True)
+        assert L.ffp_thermal_equation_set_scene(eq, C.byref(sc)) == 0
+        Tc = (C.c_float * nm)(*[float(x) for x in T])
+        emi = (C.c_float * nm)()
+        ab = (C.c_float * nm)()
+        task = P.TeTask(Tc, emi, ab)
+        assert L.ffp_thermal_equation_calculate(eq, C.byref(task)) == 0
+        key = (seed + 0x9E3779B97F4A7C15 * call) % (1 << 64)
+        em_o, ab_o, ab64 = oracle.radiance(faces, meshes, eps, T, 40_000, seed=key, frame=0, n_threads=8)
+        assert np.array_equal(bits(np.frombuffer(emi, np.float32)), bits(em_o))
+        got = np.frombuffer(ab, np.float32).astype(np.float64)
+        assert ab64.sum() > 0
+        assert np.abs(got - ab64).max() <= 1e-6 * ab64.max()
+    L.ffp_thermal_equation_free(eq)
+    L.ffp_emission_free(em)
+    L.ffp_ray_caster_free(rc)
+
 # ------------------ thermal_equation/{conductive,heat_source,parallel_rays}_cpu --
 
 def _materials(P, n, **fields):
