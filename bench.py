@@ -81,6 +81,20 @@ def traffic_from_profiles(name):
         return None
 
 
+def ff_roofline(n_rays, n_tests, n_cells, seconds, f_bytes):
+    """Roofline of a form-factor pass from its device-counted work: algorithmic
+    FP32 FLOPs (emission, reference tests, grid steps) against the FP32 peak,
+    and the algorithmic HBM bytes (the F rows, written once) against the HBM
+    peak."""
+    flops = FLOP_EMIT * n_rays + FLOP_TEST * n_tests + FLOP_STEP * n_cells
+    tf = flops / seconds / 1e12
+    gbs = f_bytes / seconds / 1e9
+    return {"bound": "mfma", "engine": "FP32 VALU", "achieved": round(tf, 3), "peak": PEAK_FP32, "unit": "TFLOP/s",
+            "frac": round(tf / PEAK_FP32, 4), "frac_unpacked_fp32": round(tf / PEAK_FP32_UNPACKED, 4),
+            "hbm_GB_per_s": round(gbs, 1), "hbm_frac": round(gbs / PEAK_HBM, 5),
+            "tests_per_ray": round(n_tests / n_rays, 3), "cells_per_ray": round(n_cells / n_rays, 3)}
+
+
 def cpu_baseline(faces, rays_host, gpu_idx, n_threads):
     """Time the reference zgrid CPU caster (compiled from the reference
     sources when available, else the oracle's restatement) on the same rays,
@@ -157,6 +171,12 @@ def batch_set_leg(ffk, scenes, n_scenes):
     wall_b, ev_b = timed(lambda: b.run(seeds, Fs, tallies=tallies, use_graph=True))
     same = checksums() == sums_plain
     log(f"scene set batched: {n_rays / wall_b / 1e6:.1f} Mrays/s ({wall_b:.3f} s), F identical: {same}")
+    # device-counted work of the set (one counting run, untimed)
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    b.run(seeds, Fs, tallies=tallies, use_graph=False, stats=stats)
+    torch.cuda.synchronize()
+    n_tests, n_cells = (int(x) for x in stats.cpu())
+    roof = ff_roofline(n_rays, n_tests, n_cells, ev_b, sum(sc.n_meshes ** 2 for sc in scs) * 4)
     b.close()
     del Fs, tallies, plain_tallies
     torch.cuda.empty_cache()
@@ -168,7 +188,8 @@ def batch_set_leg(ffk, scenes, n_scenes):
             "batched": {"value": round(n_rays / wall_b / 1e6, 2), "seconds": round(wall_b, 4),
                         "events_seconds": round(ev_b, 4),
                         "path": "persistent tile-queue launches per LDS class, one hipGraph"},
-            "speedup": round(wall_p / wall_b, 4), "F_bits_identical": same, "unit": "Mrays/s"}
+            "speedup": round(wall_p / wall_b, 4), "F_bits_identical": same, "unit": "Mrays/s",
+            "roofline_batched": roof}
 
 
 def confetti_leg(ffk, scenes, steps):
@@ -207,6 +228,11 @@ def confetti_leg(ffk, scenes, steps):
     t_tune = time.perf_counter() - t0
     t_tuned = run(steps)
     same = int(F.view(torch.int32).sum(dtype=torch.int64)) == sum_default
+    stats = torch.zeros(2, dtype=torch.int64, device="cuda")
+    sc.ff_fused(0xC3, F=F, stats=stats)  # device-counted work (untimed)
+    torch.cuda.synchronize()
+    n_tests, n_cells = (int(x) for x in stats.cpu())
+    roof = ff_roofline(n_rays, n_tests, n_cells, t_tuned, N * N * 4)
     log(f"confetti: default grid {g0} {n_rays / t_default / 1e6:.1f} Mrays/s, tuned {tuned} "
         f"{n_rays / t_tuned / 1e6:.1f} Mrays/s (tuning {t_tune:.1f} s)")
     c, r0 = ffk.bsphere(faces)
@@ -230,7 +256,8 @@ def confetti_leg(ffk, scenes, steps):
             "default_grid": {"dims": list(g0), "value": round(n_rays / t_default / 1e6, 2)},
             "autotuned": {"dims": list(tuned["dims"]), "block": tuned["block"], "walk": tuned["walk"],
                           "refill": tuned["refill"], "pool": tuned["pool"],
-                          "value": round(n_rays / t_tuned / 1e6, 2), "tuning_seconds": round(t_tune, 2)},
+                          "value": round(n_rays / t_tuned / 1e6, 2), "tuning_seconds": round(t_tune, 2),
+                          "roofline": roof},
             "F_bits_identical": same, "sun": sun}
 
 
