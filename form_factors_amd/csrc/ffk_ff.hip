@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
     int hint = find_face(S.face_off, S.n_faces, T.g0, 0);
     auto count_hit = [&](const hit_rec& h) {
       if (h.f >= 0) {
-        const int m = gld(S.face2mesh + h.f);
+        const int m = gld(gidx(S.face2mesh, (uint32_t)h.f));
         if (m >= 0) {
           if (HIST == 0) atomicAdd(&tally[(size_t)T.mesh * N + m], 1ull);
           else if (HIST == 1) atomicAdd(&hist[m], 1u);
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
             const int f = find_face(S.face_off, S.n_faces, g, hint);
             hint = f;
             v3 e;
-            malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
+            malley_ray(S, f, (int)(g - gld(gidx(S.face_off, (uint32_t)f))), k0, k1, &o, &e);
             dir = e - o;
           },
           [&](long long, v3, v3, unsigned long long key) {

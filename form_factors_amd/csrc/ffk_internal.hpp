@@ -190,6 +190,13 @@ __device__ inline float4 gld4(const float4* p) {
   const f4v v = *(const FFK_GMEM f4v*)p;
   return make_float4(v.x, v.y, v.z, v.w);
 }
+// Element i of a table as a 32-bit unsigned byte offset from its base: with
+// a wave-uniform base the load takes the scalar-base form (no 64-bit address
+// arithmetic per lane). The tables here stay far below 4 GiB.
+template <class T>
+__device__ inline const T* gidx(const T* base, uint32_t i) {
+  return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + (size_t)(i * (uint32_t)sizeof(T)));
+}
 #else
 template <class T>
 __host__ __device__ inline T gld(const T* p) {
@@ -197,6 +204,10 @@ __host__ __device__ inline T gld(const T* p) {
 }
 __host__ __device__ inline int2 gld2(const int2* p) { return *p; }
 __host__ __device__ inline float4 gld4(const float4* p) { return *p; }
+template <class T>
+__host__ __device__ inline const T* gidx(const T* base, uint32_t i) {
+  return base + i;
+}
 #endif
 
 // mat33 * vec3 with the reference's column access (math/mat.h:30-37).

@@ -491,7 +491,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
     int2 ci = make_int2(0, 0);
     const bool step = have && !pend;
     if (step) {
-      ci = gld2(S.cell_info + (cx + nx * cy + nxy * cz));
+      ci = gld2(gidx(S.cell_info, (uint32_t)(cx + nx * cy + nxy * cz)));
       if (STATS) n_cells += 1;
       if (ci.y > 0) {
         pend = true;
@@ -538,8 +538,8 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
         const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));
         if (p < total) {
-          const int f = gld(S.refs + ref_o);
-          const float4* q = reinterpret_cast<const float4*>(S.tri + f);
+          const int f = gld(gidx(S.refs, (uint32_t)ref_o));
+          const float4* q = reinterpret_cast<const float4*>(gidx(S.tri, (uint32_t)f));
           v3 pt;
           float d2;
           if (hit_terms(gld4(q), gld4(q + 1), gld4(q + 2), gld4(q + 3), oo, dd, &pt, &d2) && d2 < 3.402823466e38f)
@@ -691,13 +691,13 @@ __device__ inline void cast_grid(const scene_desc& S, v3 o, v3 dir, hit_rec& h, 
 
 // First face f with face_off[f+1] > g, starting the search at `hint`.
 __device__ inline int find_face(const long long* __restrict__ off, int n_faces, long long g, int hint) {
-  const long long oh = gld(off + hint);
-  if (oh <= g && g < gld(off + hint + 1)) return hint;
+  const long long oh = gld(gidx(off, (uint32_t)hint));
+  if (oh <= g && g < gld(gidx(off, (uint32_t)hint + 1u))) return hint;
   int lo = (oh <= g) ? hint + 1 : 0;
   int hi = (oh <= g) ? n_faces - 1 : hint;
   while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    if (gld(off + mid + 1) > g) hi = mid;
+    if (gld(gidx(off, (uint32_t)mid + 1u)) > g) hi = mid;
     else lo = mid + 1;
   }
   return lo;
@@ -721,16 +721,16 @@ __device__ inline void malley_ray(const scene_desc& S, int f, int j, uint32_t k0
     a = 1.f - a;
     b = 1.f - b;
   }
-  const float4* q = reinterpret_cast<const float4*>(S.tri + f);
+  const float4* q = reinterpret_cast<const float4*>(gidx(S.tri, (uint32_t)f));
   float4 q0 = gld4(q), q1 = gld4(q + 1), q2 = gld4(q + 2);
   v3 p0 = mk(q0.x, q0.y, q0.z), v0 = mk(q1.x, q1.y, q1.z), v1 = mk(q2.x, q2.y, q2.z);
   v3 origin = (p0 + v0 * a) + v1 * b;
   float rad = sqrtf(rr);
   v3 m = mk(rad * cphi, rad * sphi, sqrtf(1 - rr));
-  if (j >= gld(S.front + f)) m.z = -m.z;
+  if (j >= gld(gidx(S.front, (uint32_t)f))) m.z = -m.z;
   float R[9];
 #pragma unroll
-  for (int i = 0; i < 9; ++i) R[i] = gld(S.rot + 9 * (size_t)f + i);
+  for (int i = 0; i < 9; ++i) R[i] = gld(gidx(S.rot, 9u * (uint32_t)f + (uint32_t)i));
   v3 d = mat_vec(R, m);
   *origin_out = origin + d * 0.0001f;
   *end_out = origin + d;
