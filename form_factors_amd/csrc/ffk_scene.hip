@@ -656,6 +656,14 @@ int ffk_scene_grid_info(const ffk_scene* s, int* dims3, float* lo3, float* hi3, 
   return FFK_OK;
 }
 
+int ffk_scene_grid_cells(const ffk_scene* s, int* h_cells) {
+  FFK_REQUIRE(s && s->has_grid && h_cells, FFK_ERR_ARG, "ffk_scene_grid_cells: grid not built or NULL output");
+  const size_t nc = (size_t)s->grid.n[0] * s->grid.n[1] * s->grid.n[2];
+  FFK_HIP(hipDeviceSynchronize());
+  FFK_HIP(hipMemcpy(h_cells, s->d_cell_info, 8 * nc, hipMemcpyDeviceToHost));
+  return FFK_OK;
+}
+
 int ffk_area_weights(const float* h_faces, int n_faces, float* h_weights, float* total) {
   FFK_REQUIRE(h_faces && h_weights && total && n_faces >= 0, FFK_ERR_ARG, "ffk_area_weights: bad args");
   float t = 0;

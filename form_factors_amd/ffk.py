@@ -40,6 +40,7 @@ SIGNATURES = {
     "ffk_scene_grid_info": (C.c_int, [_vp, _i, _f, _f, C.POINTER(_ll)]),
     "ffk_scene_counts": (C.c_int, [_vp, _i, _i, C.POINTER(_ll)]),
     "ffk_scene_layout": (C.c_int, [_vp, _f, _i, _i, _i, _i]),
+    "ffk_scene_grid_cells": (C.c_int, [_vp, _i]),
     "ffk_autotune": (C.c_int, [_vp, _ll, _i, _i, _i, _vp]),
     "ffk_cast": (C.c_int, [_vp, C.c_int, _vp, _ll, _vp, _vp, _vp, _vp]),
     "ffk_emitted_counts": (C.c_int, [_f, C.c_int, C.c_int, C.c_float, _i, _i]),
@@ -326,6 +327,13 @@ class Scene:
         check(lib().ffk_scene_build_grid(self.h, int(dims[0]), int(dims[1]), int(dims[2]), _stream()),
               "ffk_scene_build_grid")
         return self.grid_info()
+
+    def grid_cells(self):
+        """The grid's cell words as int32[nz, ny, nx, 2] (ffk_scene_grid_cells)."""
+        dims = self.grid_info()["dims"]
+        out = np.zeros((dims[2], dims[1], dims[0], 2), np.int32)
+        check(lib().ffk_scene_grid_cells(self.h, out.ctypes.data_as(_i)), "ffk_scene_grid_cells")
+        return out
 
     def grid_info(self):
         dims = (C.c_int * 3)()

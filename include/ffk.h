@@ -94,6 +94,14 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
 int ffk_scene_build_grid(ffk_scene* s, int nx, int ny, int nz, void* stream);
 int ffk_scene_grid_info(const ffk_scene* s, int* dims3, float* lo3, float* hi3,
                         long long* n_refs);
+/* Host copy of the grid's cell words, two int32 per cell, cell
+ * c = x + nx*(y + ny*z): (first reference, count > 0) for an occupied cell;
+ * (boxes, 0) for an empty one, where bits 4o..4o+3 of boxes hold the size k
+ * (1..15) of the empty box [c, c + s*(k-1)] toward direction octant o (bit 0
+ * of o: x decreasing, bit 1: y, bit 2: z), clipped to the grid, cells outside
+ * counting as empty. Synchronises the device. An inspection call with no
+ * counterpart in the reference (its zgrid keeps no empty-space data). */
+int ffk_scene_grid_cells(const ffk_scene* s, int* h_cells);
 int ffk_scene_counts(const ffk_scene* s, int* n_faces, int* n_meshes, long long* n_rays);
 /* Host copies of the scene's faces (9 per face), meshes (3 per mesh), emitted
  * front/rear ray counts and emission frame; any pointer may be NULL. */
