@@ -150,7 +150,8 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
   // atomics, which need natural alignment.
   extern __shared__ __align__(16) uint32_t hist[];
   __shared__ int s_tile;
-  __shared__ int s_next;  // next ray of the tile (REFILL)
+  __shared__ int s_next;      // next ray of the tile (REFILL)
+  __shared__ int s_faces[2];  // faces of the tile's first and last rays
   uint32_t nt = 0, nc = 0;
   int t_idx = blockIdx.x;
   while (true) {
@@ -168,12 +169,16 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
     const int words = HIST == 1 ? N : (N + 1) >> 1;
     if (HIST != 0)
       for (int w = threadIdx.x; w < words; w += blockDim.x) hist[w] = 0u;
-    if (SCHED != 0 && threadIdx.x == 0) s_next = 0;
+    if (SCHED != 0 && threadIdx.x == 0) {
+      s_next = 0;
+      s_faces[0] = T.f_lo;
+      s_faces[1] = T.f_hi;
+    }
     if (HIST != 0 || SCHED != 0) __syncthreads();
     const unsigned long long seed = seeds[sc];
     const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
     unsigned long long* tally = tallies ? tallies[sc] : nullptr;
-    int hint = find_face(S.face_off, S.n_faces, T.g0, 0);
+    int hint = T.f_lo;
     auto count_hit = [&](const hit_rec& h) {
       if (h.f >= 0) {
         const int m = gld(gidx(S.face2mesh, (uint32_t)h.f));
@@ -195,7 +200,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
           S, best, &s_next, T.g1 - T.g0, refill, pool,
           [&](long long i, v3& o, v3& dir) {
             const long long g = T.g0 + i;
-            const int f = find_face(S.face_off, S.n_faces, g, hint);
+            const int f = find_face_in(S.face_off, s_faces, g, hint);
             hint = f;
             v3 e;
             malley_ray(S, f, (int)(g - gld(gidx(S.face_off, (uint32_t)f))), k0, k1, &o, &e);
@@ -231,7 +236,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
             const long long off = (long long)base + rank;
             if (off < n) {
               const long long g = T.g0 + off;
-              const int f = find_face(S.face_off, S.n_faces, g, hint);
+              const int f = find_face_in(S.face_off, s_faces, g, hint);
               hint = f;
               v3 e;
               malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
@@ -251,7 +256,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
       }
     } else {
       for (long long g = T.g0 + threadIdx.x; g < T.g1; g += blockDim.x) {
-        const int f = find_face(S.face_off, S.n_faces, g, hint);
+        const int f = find_face(S.face_off, S.n_faces, g, hint);  // s_faces is not filled here
         hint = f;
         v3 o, e;
         malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
@@ -398,8 +403,17 @@ static long long auto_rays_per_tile(long long total) {
 
 // Tiles of one scene: each mesh range split into pieces of at most
 // rays_per_tile; a mesh covered by one tile writes its F row directly.
+// The face emitting ray g (g < total rays): the first f with face_off[f + 1] > g.
+static int host_face_of(const ffk_scene* s, long long g) {
+  const auto& off = s->h_face_off;
+  return (int)(std::upper_bound(off.begin(), off.end(), g) - off.begin()) - 1;
+}
+
 static void scene_tiles(const ffk_scene* s, int scene_id, long long rpt, std::vector<tile>& out,
                         std::vector<int>& indirect) {
+  auto add = [&](long long g0, long long g1, int m, int direct) {
+    out.push_back(tile{g0, g1, m, scene_id, direct, host_face_of(s, g0), host_face_of(s, g1 - 1), 0});
+  };
   for (int m = 0; m < s->n_meshes; ++m) {
     long long a = s->h_mesh_off[m], b = s->h_mesh_off[m + 1];
     if (b <= a) {  // no rays: 0 * (1/0) in the reference, produced by the tally path
@@ -407,11 +421,11 @@ static void scene_tiles(const ffk_scene* s, int scene_id, long long rpt, std::ve
       continue;
     }
     if (b - a <= rpt) {
-      out.push_back(tile{a, b, m, scene_id, 1, 0});
+      add(a, b, m, 1);
       continue;
     }
     indirect.push_back(m);
-    for (long long g = a; g < b; g += rpt) out.push_back(tile{g, std::min(b, g + rpt), m, scene_id, 0, 0});
+    for (long long g = a; g < b; g += rpt) add(g, std::min(b, g + rpt), m, 0);
   }
 }
 

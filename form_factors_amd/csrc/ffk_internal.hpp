@@ -303,12 +303,15 @@ struct scene_desc {
 
 // One unit of fused work: rays [g0, g1) tallied for mesh `mesh` of scene
 // `scene`. `direct` = the tile covers the whole mesh range, so the row of F
-// is written from the tile's histogram without the global tally.
+// is written from the tile's histogram without the global tally. The tile's
+// rays are emitted by faces f_lo..f_hi (face order), which bounds the face
+// lookup of each ray.
 struct tile {
   long long g0, g1;
   int mesh;
   int scene;
   int direct;
+  int f_lo, f_hi;
   int pad;
 };
 

@@ -63,24 +63,33 @@ __global__ void __launch_bounds__(256) k_radiance(scene_desc S, const long long*
 // The same deposit with lane refill and pooled triangle tests
 // (pooled_walk): block b casts walk positions [b * chunk, (b + 1) * chunk).
 template <bool STATS, int WALK>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
     k_radiance_pooled(scene_desc S, const long long* __restrict__ walk_start, const int* __restrict__ walk_face,
                       int n_walk, const unsigned long long* __restrict__ q, long long n_rays, long long chunk,
                       uint32_t k0, uint32_t k1, unsigned long long* __restrict__ acc,
                       unsigned long long* __restrict__ stats, int refill, int pool) {
   __shared__ unsigned long long best[4 * kPoolWords];
   __shared__ int s_next;
+  // The faces and the walk entries of the block's first and last rays bound
+  // their lookups.
+  __shared__ int s_faces[2], s_walk[2];
   const long long r0 = (long long)blockIdx.x * chunk;
   const long long cnt = min(chunk, n_rays - r0);
-  if (threadIdx.x == 0) s_next = 0;
+  if (threadIdx.x == 0) {
+    s_next = 0;
+    s_faces[0] = find_face(S.face_off, S.n_faces, r0, 0);
+    s_faces[1] = find_face(S.face_off, S.n_faces, r0 + cnt - 1, s_faces[0]);
+    s_walk[0] = find_face(walk_start, n_walk, r0, 0);
+    s_walk[1] = find_face(walk_start, n_walk, r0 + cnt - 1, s_walk[0]);
+  }
   __syncthreads();
   uint32_t nt = 0, nc = 0;
-  int hint = find_face(S.face_off, S.n_faces, r0, 0), whint = 0;
+  int hint = s_faces[0], whint = s_walk[0];
   pooled_walk<STATS, WALK>(
       S, best + kPoolWords * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
       [&](long long i, v3& o, v3& dir) {
         const long long g = r0 + i;
-        const int f = find_face(S.face_off, S.n_faces, g, hint);
+        const int f = find_face_in(S.face_off, s_faces, g, hint);
         hint = f;
         v3 e;
         malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
@@ -94,7 +103,7 @@ __global__ void __launch_bounds__(256)
         const int m = gld(S.face2mesh + hf);
         if (m < 0) return;
         const long long g = r0 + i;
-        const int k = find_face(walk_start, n_walk, g, whint);
+        const int k = find_face_in(walk_start, s_walk, g, whint);
         whint = k;
         const int wf = walk_face[k];
         const unsigned long long v = q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)];

@@ -703,6 +703,30 @@ __device__ inline int find_face(const long long* __restrict__ off, int n_faces, 
   return lo;
 }
 
+// find_face over the faces bounds[0]..bounds[1] only, which hold ray g
+// (off[bounds[0]] <= g < off[bounds[1] + 1]); hint lies between them. Returns
+// the same face as find_face: the first f with off[f + 1] > g. A block whose
+// rays span a few faces searches those instead of the scene's whole table
+// when the hint misses. The bounds are read (from LDS) only on a miss, so
+// they hold no registers through the caller's ray loop.
+__device__ inline int find_face_in(const long long* __restrict__ off, const int* bounds, long long g, int hint) {
+  int lo, hi;
+  if (gld(gidx(off, (uint32_t)hint)) <= g) {
+    if (g < gld(gidx(off, (uint32_t)hint + 1u))) return hint;
+    lo = hint + 1;
+    hi = bounds[1];
+  } else {
+    lo = bounds[0];
+    hi = hint - 1;
+  }
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (gld(gidx(off, (uint32_t)mid + 1u)) > g) hi = mid;
+    else lo = mid + 1;
+  }
+  return lo;
+}
+
 // Malley ray j of face f (emission/malley_cpu.cpp:109-186): uniform point on
 // the face, cosine-weighted direction about the face frame, rear rays with
 // the z component negated, origin pushed 1e-4 along the direction. The four
