@@ -196,6 +196,34 @@ def test_fused_ff_tiling_and_histograms(gpu, oracle, hist, rays_per_tile):
     assert np.array_equal(bits(F), bits(want))
 
 
+@pytest.mark.parametrize("refill,pool,walk", [(16, 0, 0), (16, 4, 0), (32, 2, 1)])
+@pytest.mark.parametrize("rays_per_tile", [1000, 4096])
+@pytest.mark.parametrize("order", ["face_order", "reversed"])
+def test_fused_ff_split_multi_face_meshes(gpu, oracle, refill, pool, walk, rays_per_tile, order):
+    """Meshes of several faces split over several tiles: a tile starts inside
+    a mesh, its lanes' face hints miss as refill hands them rays a block
+    apart, and the lookup searches only the tile's own faces. With the mesh
+    list reversed the tally walk attributes rays to meshes in list order
+    (cpu_system.cpp:165-191), so tiles cover faces of other meshes. F is the
+    oracle's bit for bit."""
+    from form_factors_amd import scenes
+
+    faces, _ = scenes.parallel_plates(6)  # 2 x 72 faces
+    first = np.arange(0, len(faces), 4, dtype=np.int32)
+    meshes = np.stack([first, np.full_like(first, 4), np.zeros_like(first)], 1)  # 36 meshes of 4 faces
+    if order == "reversed":
+        meshes = np.ascontiguousarray(meshes[::-1])
+    print(f"split meshes {order} rpt={rays_per_tile} sched={refill}/{pool}/{walk}", flush=True)
+    sc = make_scene(gpu, faces, meshes, n_rays=150_000)
+    sc.set_tuning(block=256, rays_per_tile=rays_per_tile, hist=2)
+    sc.set_refill(refill, pool)
+    sc.set_walk(walk)
+    F = sc.ff_fused(seed=23).cpu().numpy()
+    r, want = oracle.ff_calculate(faces, meshes, 150_000, seed=23)
+    assert r == 0
+    assert np.array_equal(bits(F), bits(want))
+
+
 @pytest.mark.parametrize("refill,pool,walk", [(1, 0, 0), (16, 0, 0), (48, 0, 0), (64, 0, 0), (1, 1, 0), (16, 4, 0),
                                               (16, 1, 0), (64, 64, 0), (8, 2, 0), (16, 4, 1), (1, 1, 1), (64, 8, 1)])
 @pytest.mark.parametrize("hist,rays_per_tile", [(0, 0), (2, 1000), (1, 0)])
