@@ -75,12 +75,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
   __shared__ int s_faces[2], s_walk[2];
   const long long r0 = (long long)blockIdx.x * chunk;
   const long long cnt = min(chunk, n_rays - r0);
-  if (threadIdx.x == 0) {
-    s_next = 0;
-    s_faces[0] = find_face(S.face_off, S.n_faces, r0, 0);
-    s_faces[1] = find_face(S.face_off, S.n_faces, r0 + cnt - 1, s_faces[0]);
-    s_walk[0] = find_face(walk_start, n_walk, r0, 0);
-    s_walk[1] = find_face(walk_start, n_walk, r0 + cnt - 1, s_walk[0]);
+  if (threadIdx.x < 4) {  // the four searches side by side on lanes 0-3
+    const bool walk = threadIdx.x >= 2;
+    const long long g = (threadIdx.x & 1) ? r0 + cnt - 1 : r0;
+    const int v = find_face(walk ? walk_start : S.face_off, walk ? n_walk : S.n_faces, g, 0);
+    (walk ? s_walk : s_faces)[threadIdx.x & 1] = v;
+    if (threadIdx.x == 0) s_next = 0;
   }
   __syncthreads();
   uint32_t nt = 0, nc = 0;
