@@ -35,11 +35,16 @@ def _cases():
     faces, meshes = scenes.confetti_scene(300, seed=5)
     out.append(("confetti", faces, meshes, rng.uniform(0.1, 1, (len(meshes), 2)).astype(np.float32),
                 rng.uniform(250, 400, len(meshes)).astype(np.float32), 200_000))
+    # about 86 rays per face with the mesh list reversed: a block's rays span
+    # several faces, and their walk entries (mesh order) run against face order
+    meshes = np.ascontiguousarray(meshes[::-1])
+    out.append(("confetti_reversed", faces, meshes, rng.uniform(0.1, 1, (len(meshes), 2)).astype(np.float32),
+                rng.uniform(250, 400, len(meshes)).astype(np.float32), 200_000))
     return out
 
 
 @pytest.mark.parametrize("frame", [0, 1])
-@pytest.mark.parametrize("case", range(4))
+@pytest.mark.parametrize("case", range(5))
 def test_radiance_matches_oracle(gpu, oracle, case, frame):
     name, faces, meshes, eps, T, n_rays = _cases()[case]
     print(f"radiance case {name} frame {frame}", flush=True)
@@ -75,7 +80,7 @@ def test_radiance_homogeneous_and_repeatable(gpu):
     assert abs(ab1.sum() / em1.astype(np.float64).sum() - 0.5) < 1e-3
 
 
-@pytest.mark.parametrize("case", [0, 3])
+@pytest.mark.parametrize("case", [0, 3, 4])
 def test_radiance_schedules_agree(gpu, case):
     """Per-ray casting, lane refill and pooled tests deposit the same fixed
     point sums: identical absorption bits in every setting."""
