@@ -78,8 +78,11 @@ __global__ void __launch_bounds__(256) k_tally(scene_desc S, const int* __restri
   atomicAdd(&tally[(size_t)src * S.n_meshes + dst], 1ull);
 }
 
-// F[m, j] = (float)count * (1.f / (float)rays_of_m): the float count is
-// exact below 2^24 and then equals the reference's float accumulation.
+// F[m, j] = (float)count * (1.f / (float)rays_of_m). While a cell's count is
+// below 2^24 this equals the reference's float accumulation `F += 1.f`
+// (cpu_system.cpp:181-191) bit for bit. Above it the two part on purpose: the
+// reference's sum stops at 2^24 (adding 1 no longer changes it), while here
+// the exact integer count is rounded once to the nearest float.
 __device__ inline float normalise_one(unsigned long long cnt, float ratio) { return (float)cnt * ratio; }
 
 __global__ void __launch_bounds__(256) k_normalise_rows(const long long* __restrict__ mesh_off, int N,
@@ -473,6 +476,7 @@ using namespace ffk;
 
 struct ffk_batch {
   std::vector<ffk_scene*> scenes;
+  std::vector<unsigned long long> generations;  // each scene's generation when the batch was created
   tile* d_tiles = nullptr;
   int n_tiles = 0;
   scene_desc* d_descs = nullptr;
@@ -512,6 +516,13 @@ struct ffk_batch {
   std::vector<unsigned long long*> g_tallies;
   std::vector<float*> g_Fs;
   std::vector<unsigned long long> g_seeds;
+  // Seeds reach d_seeds through a pinned ring: a slot is rewritten only after
+  // the copy that last read it has completed (its event), so a run that only
+  // changes the seeds neither re-captures the graph nor waits for the device.
+  static constexpr int kSeedSlots = 4;
+  unsigned long long* h_seed_ring = nullptr;  // kSeedSlots * scenes.size(), pinned
+  hipEvent_t seed_ev[kSeedSlots] = {};
+  int seed_slot = 0;
 };
 
 namespace ffk {
@@ -842,6 +853,7 @@ int ffk_batch_create(ffk_batch** out, ffk_scene* const* scenes, int n_scenes, vo
                 "ffk_batch_create: scene %d lacks emission layout or grid", i);
   ffk_batch* b = new ffk_batch();
   b->scenes.assign(scenes, scenes + n_scenes);
+  for (int i = 0; i < n_scenes; ++i) b->generations.push_back(scenes[i]->generation);
   int r;
   if ((r = dalloc(&b->d_descs, (size_t)n_scenes)) < 0 ||
       (r = dalloc(&b->d_seeds, (size_t)n_scenes)) < 0 || (r = dalloc(&b->d_tallies, (size_t)n_scenes)) < 0 ||
@@ -874,6 +886,9 @@ void ffk_batch_destroy(ffk_batch* b) {
   for (hipStream_t f : b->fork) (void)hipStreamDestroy(f);
   for (hipEvent_t e : b->ev_join) (void)hipEventDestroy(e);
   if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  for (hipEvent_t e : b->seed_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->h_seed_ring) (void)hipHostFree(b->h_seed_ring);
   dfree(b->d_tiles);
   dfree(b->d_descs);
   dfree(b->d_rows);
@@ -884,8 +899,22 @@ void ffk_batch_destroy(ffk_batch* b) {
   delete b;
 }
 
+// The batch holds each scene's device pointers and ray layout as they were at
+// ffk_batch_create; a scene whose grid, faces, meshes or emission layout has
+// been set again since would be read through freed or stale buffers.
+static int batch_scenes_current(const ffk_batch* b, const char* fn) {
+  for (size_t i = 0; i < b->scenes.size(); ++i)
+    FFK_REQUIRE(b->scenes[i]->generation == b->generations[i], FFK_ERR_ARG,
+                "%s: scene %zu was rebuilt (grid, faces, meshes or emission set again) after "
+                "ffk_batch_create; create the batch again",
+                fn, i);
+  return FFK_OK;
+}
+
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks) {
   FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  int rc;
+  if ((rc = batch_scenes_current(b, "ffk_batch_set_tuning")) < 0) return rc;
   FFK_REQUIRE(block_size == 0 || (block_size % 64 == 0 && block_size >= 64 && block_size <= 1024),
               FFK_ERR_ARG, "block size %d", block_size);
   FFK_REQUIRE(rays_per_tile >= 0 && rays_per_tile <= 65535, FFK_ERR_LIMIT, "rays per tile %d", rays_per_tile);
@@ -1006,6 +1035,8 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
 int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream) {
   FFK_REQUIRE(b && seeds && d_Fs, FFK_ERR_ARG, "ffk_batch_run: bad args");
+  int rc;
+  if ((rc = batch_scenes_current(b, "ffk_batch_run")) < 0) return rc;
   const size_t n = b->scenes.size();
   for (size_t i = 0; i < n; ++i) {
     FFK_REQUIRE(d_Fs[i], FFK_ERR_ARG, "ffk_batch_run: NULL F for scene %zu", i);
@@ -1017,17 +1048,32 @@ int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long l
   std::vector<float*> fs(d_Fs, d_Fs + n);
   if (d_tallies) tal.assign(d_tallies, d_tallies + n);
   std::vector<unsigned long long> sd(seeds, seeds + n);
-  const bool same = b->exec && tal == b->g_tallies && fs == b->g_Fs && sd == b->g_seeds;
-  if (!same) {
+  // The kernels read seeds, tally and F pointers from device arrays, so a
+  // captured graph stays valid when only the seeds change: they are copied
+  // in stream order ahead of the launch (after the previous run's kernels
+  // have read the old ones). New tally or F pointers re-plan the graph.
+  const bool same_bufs = b->exec && tal == b->g_tallies && fs == b->g_Fs;
+  if (!same_bufs) {
     free_graph(b);
-    FFK_HIP(hipMemcpyAsync(b->d_seeds, sd.data(), n * 8, hipMemcpyHostToDevice, st));
     FFK_HIP(hipMemcpyAsync(b->d_tallies, tal.data(), n * 8, hipMemcpyHostToDevice, st));
     FFK_HIP(hipMemcpyAsync(b->d_Fs, fs.data(), n * 8, hipMemcpyHostToDevice, st));
-    FFK_HIP(hipStreamSynchronize(st));
     b->g_tallies = tal;
     b->g_Fs = fs;
+  }
+  if (!same_bufs || sd != b->g_seeds) {
+    if (!b->h_seed_ring)
+      FFK_HIP(hipHostMalloc((void**)&b->h_seed_ring, ffk_batch::kSeedSlots * n * 8, hipHostMallocDefault));
+    const int slot = b->seed_slot;
+    if (b->seed_ev[slot]) FFK_HIP(hipEventSynchronize(b->seed_ev[slot]));
+    else FFK_HIP(hipEventCreateWithFlags(&b->seed_ev[slot], hipEventDisableTiming));
+    unsigned long long* src = b->h_seed_ring + (size_t)slot * n;
+    std::copy(sd.begin(), sd.end(), src);
+    FFK_HIP(hipMemcpyAsync(b->d_seeds, src, n * 8, hipMemcpyHostToDevice, st));
+    FFK_HIP(hipEventRecord(b->seed_ev[slot], st));
+    b->seed_slot = (slot + 1) % ffk_batch::kSeedSlots;
     b->g_seeds = sd;
   }
+  if (!same_bufs) FFK_HIP(hipStreamSynchronize(st));
   if (d_stats || !use_graph) return batch_enqueue(b, d_stats != nullptr, d_stats, st);
   if (!b->exec) {
     // Record on the batch's own stream (the caller's may be the legacy null

@@ -322,6 +322,10 @@ struct ffk_scene {
   long long n_rays = 0;
   int frame = FFK_FRAME_REFERENCE;
   bool has_faces = false, has_meshes = false, has_emission = false, has_grid = false;
+  // Bumped whenever a device buffer named by desc() is freed or the emission
+  // layout changes (grid rebuilt, faces/meshes/emission set again): a batch
+  // records it at creation and refuses to run on a scene that has moved on.
+  unsigned long long generation = 0;
   // device buffers
   float* d_faces = nullptr;
   ffk::tri_terms* d_tri = nullptr;

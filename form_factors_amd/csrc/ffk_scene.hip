@@ -304,6 +304,7 @@ ffk::scene_desc ffk_scene::desc() const {
 }
 
 static void free_grid(ffk_scene* s) {
+  ++s->generation;
   dfree(s->d_cell_start);
   dfree(s->d_refs);
   dfree(s->d_cell_info);
@@ -315,6 +316,7 @@ static void free_grid(ffk_scene* s) {
 }
 
 static void free_emission(ffk_scene* s) {
+  ++s->generation;
   dfree(s->d_rot);
   dfree(s->d_face_off);
   dfree(s->d_front);
@@ -446,6 +448,7 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
   // whose setters free them): wait for the work that may still read them.
   FFK_HIP(hipStreamSynchronize(st));
   s->has_emission = false;
+  ++s->generation;  // ray counts, offsets and tiles change in place
   const int nf = s->n_faces, nm = s->n_meshes;
   s->h_front.assign(h_front, h_front + nf);
   s->h_rear.assign(h_rear, h_rear + nf);

@@ -189,12 +189,18 @@ int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mod
 /* ---- batched form factors ---------------------------------------------------- */
 /* Many scenes of uneven size through one persistent launch over a queue of
  * (scene, mesh, ray-range) tiles. Scenes must have faces, meshes, emission
- * and grid set. */
+ * and grid set, and must outlive the batch. The batch holds the scenes'
+ * device tables and ray layouts as they are at creation: once a scene's grid
+ * is rebuilt or its faces, meshes or emission are set again (directly, by
+ * ffk_autotune or by a radiance step), ffk_batch_run and ffk_batch_set_tuning
+ * return FFK_ERR_ARG before launching anything; create the batch again. */
 int ffk_batch_create(ffk_batch** out, ffk_scene* const* scenes, int n_scenes, void* stream);
 void ffk_batch_destroy(ffk_batch* b);
 /* seeds[i] keys scene i; d_tallies[i] and d_Fs[i] are that scene's buffers.
  * use_graph != 0 records the launch sequence into a hipGraph on first use and
- * replays it afterwards (same buffers required). */
+ * replays it afterwards while the tally and F buffers stay the same; new
+ * seeds are copied ahead of the replay in stream order, so successive runs of
+ * one batch go on one stream (or the caller orders them). */
 int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream);
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);

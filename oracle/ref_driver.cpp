@@ -28,6 +28,7 @@
 #include "ray_caster/zgrid_cpu.h"
 #include "subject/system.h"
 #include "thermal_equation/conductive_cpu.h"
+#include "thermal_equation/form_factors.h"
 #include "thermal_equation/heat_source_cpu.h"
 #include "thermal_equation/parallel_rays_cpu.h"
 #include "thermal_equation/radiance_cpu.h"
@@ -428,6 +429,50 @@ EXPORT int ffref_radiance(const float* faces, int n_faces, const int* meshes, in
   thermal_equation::system_free(eq);
   emission::system_free(em);
   free_caster(rc);
+  return r;
+}
+
+/* form_factors_equation (thermal_equation/form_factors.cpp:77-128) over
+ * FORM_FACTORS_CPU, MALLEY_CPU and the naive caster: set_scene computes F
+ * with n_rays, then one calculate() adds the per-mesh emission and the
+ * absorption q = F^T E. F is computed a second time by a fresh calculator with
+ * the same seeds (identical rays, so identical bits) and returned in F_out;
+ * areas_out receives subject::build_meshes_areas. Materials carry front/rear
+ * emissivity per mesh. */
+EXPORT int ffref_sbff(const float* faces, int n_faces, const int* meshes, int n_meshes, const float* eps2,
+                      const float* T, int n_rays, float* F_out, float* areas_out, float* emission,
+                      float* absorption) {
+  std::vector<subject::material_t> mats(n_meshes);
+  std::vector<subject::mesh_t> ms(n_meshes);
+  for (int m = 0; m < n_meshes; ++m) {
+    mats[m] = subject::black_body();
+    mats[m].front.emissivity = eps2[2 * m];
+    mats[m].rear.emissivity = eps2[2 * m + 1];
+    ms[m] = subject::make_mesh(meshes[3 * m], meshes[3 * m + 1], m);
+  }
+  subject::scene_t scene = {n_faces, (subject::face_t*)faces, n_meshes, ms.data(), n_meshes,
+                            mats.data()};
+  float* areas = 0;
+  subject::build_meshes_areas(&scene, &areas);
+  memcpy(areas_out, areas, n_meshes * sizeof(float));
+  free(areas);
+
+  ray_caster::system_t* rc = make_caster(0);
+  emission::system_t* em = emission::system_create(EMISSION_MALLEY_CPU, rc);
+  form_factors::system_t* ff = form_factors::system_create(FORM_FACTORS_CPU, em);
+  form_factors_equation::params_t params = {ff, n_rays};
+  thermal_equation::system_t* eq = form_factors_equation::system_create();
+  thermal_equation::system_init(eq, &params);
+  int r = thermal_equation::system_set_scene(eq, &scene);
+  thermal_equation::task_t task = {T, emission, absorption};
+  memset(emission, 0, n_meshes * sizeof(float));
+  memset(absorption, 0, n_meshes * sizeof(float));
+  if (r == 0) r = thermal_equation::system_calculate(eq, &task);
+  thermal_equation::system_free(eq);
+  form_factors::system_free(ff);
+  emission::system_free(em);
+  free_caster(rc);
+  if (r == 0) r = ffref_ff_calculate(faces, n_faces, meshes, n_meshes, n_rays, 0, F_out);
   return r;
 }
 

@@ -376,9 +376,61 @@ def gen_conduction(rng):
     save("conduction.npz", **out)
 
 
-def main():
+def gen_sbff():
+    """form_factors_equation (thermal_equation/form_factors.cpp:77-128): the
+    reference computes F in set_scene, then one calculate() adds emission and
+    the absorption q = F^T E. Meshes of unequal area (so F is not symmetric)
+    with unequal emissivities and temperatures; own random stream."""
+    R = O.ref()
+    R.ffref_sbff.restype = C.c_int
+    R.ffref_sbff.argtypes = [_f, C.c_int, _i, C.c_int, _f, _f, C.c_int, _f, _f, _f, _f]
+    r2 = np.random.default_rng(11)
+    big = scenes.plane_grid_faces(1, 1, 3, 3)
+    small = scenes.translate(scenes.plane_grid_faces(0.6, 0.6, 2, 2), (0.15, 0.3, 0.7))
+    plates = np.concatenate([big, small]).astype(np.float32)
+    plate_meshes = np.concatenate([scenes.plane_grid_meshes(3, 3),
+                                   scenes.plane_grid_meshes(2, 2) + np.array([18, 0, 0], np.int32)])
+    sph = scenes.icosphere(1, inward=True)
+    counts = [1, 3, 2, 4, 2, 1, 3, 4] * 4  # 80 faces in 32 meshes of 1-4 faces, listed out of face order
+    firsts = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    order = r2.permutation(len(counts))
+    sph_meshes = np.stack([firsts[order], np.array(counts)[order], np.zeros(len(counts), int)], 1)
+    out = {}
+    for name, faces, meshes, n_rays in (("plates", plates, plate_meshes, 40_000),
+                                        ("sphere", sph, sph_meshes, 30_000)):
+        faces = np.ascontiguousarray(faces, np.float32)
+        meshes = np.ascontiguousarray(meshes, np.int32)
+        nm = len(meshes)
+        eps = r2.uniform(0.1, 1.0, (nm, 2)).astype(np.float32)
+        T = r2.uniform(200, 450, nm).astype(np.float32)
+        F = np.zeros((nm, nm), np.float32)
+        A = np.zeros(nm, np.float32)
+        em = np.zeros(nm, np.float32)
+        ab = np.zeros(nm, np.float32)
+        rc = R.ffref_sbff(P(faces), len(faces), P(meshes, _i), nm, P(eps), P(T), n_rays, P(F), P(A), P(em), P(ab))
+        assert rc == 0, (name, rc)
+        # F_out is the F the equation used: its float accumulation in the
+        # equation's own order reproduces the reference's absorption bits
+        sigma = np.float32(5.670400e-8)
+        acc = np.zeros(nm, np.float32)
+        for m in range(nm):
+            power = np.float32(np.float32(sigma * np.float32(T[m] * T[m] * T[m] * T[m])) * A[m])
+            e = np.float32(np.float32(eps[m, 0] * power) + np.float32(eps[m, 1] * power))
+            acc = (acc + (e * F[m]).astype(np.float32)).astype(np.float32)
+        assert np.array_equal(acc.view(np.uint32), ab.view(np.uint32)), name
+        out.update({f"{name}_faces": faces, f"{name}_meshes": meshes, f"{name}_eps": eps, f"{name}_T": T,
+                    f"{name}_n_rays": np.int32(n_rays), f"{name}_F": F, f"{name}_area": A,
+                    f"{name}_emission": em, f"{name}_absorption": ab})
+    save("sbff.npz", **out)
+
+
+def main(only=()):
     if not O.ref_available():
         O.build()
+    if only:  # python tests/golden/make_golden.py sbff ...: generators with their own streams
+        for name in only:
+            globals()["gen_" + name]()
+        return
     rng = np.random.default_rng(20261203)
     gen_intersect(rng)
     gen_rotate(rng)
@@ -391,7 +443,8 @@ def main():
     gen_parallel(rng)
     gen_conduction(rng)
     gen_sun()
+    gen_sbff()
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

@@ -122,3 +122,62 @@ def test_batch_scene_set_batched_equals_plain(gpu):
         for k, (F, want) in enumerate(zip(Fs, plain)):
             assert np.array_equal(bits(F.cpu().numpy()), bits(want)), (rep, k)
     b.close()
+
+
+def test_graph_replay_with_new_seeds_each_run(gpu):
+    """A captured batch replayed with different seeds on every run (the seeds
+    are copied ahead of each replay; the graph is kept) gives each run the
+    bits of plain launches with that run's seeds, with eager work between."""
+    import torch
+
+    scs = build_scenes(gpu, [150, 400, 90], [300, 100, 500], seed0=33)
+    Fs = [torch.empty((sc.n_meshes,) * 2, device="cuda") for sc in scs]
+    tallies = [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda") for sc in scs]
+    b = gpu.Batch(scs)
+    for rep in range(6):
+        seeds = [1000 * rep + 17 * k + 1 for k in range(len(scs))]
+        print(f"batch replay {rep} seeds {seeds}", flush=True)
+        b.run(seeds, Fs, tallies=tallies, use_graph=True)
+        got = [F.cpu().numpy() for F in Fs]  # synchronises
+        junk = torch.randn(4096, device="cuda").sum().item()
+        assert junk == junk
+        for sc, sd, F in zip(scs, seeds, got):
+            assert np.array_equal(bits(F), bits(sc.ff_fused(sd).cpu().numpy())), (rep, sd)
+    # several replays queued back to back before one synchronisation: the last one's seeds win
+    for rep in range(5):
+        b.run([7 + rep, 8 + rep, 9 + rep], Fs, tallies=tallies, use_graph=True)
+    torch.cuda.synchronize()
+    for sc, sd, F in zip(scs, [11, 12, 13], Fs):
+        assert np.array_equal(bits(F.cpu().numpy()), bits(sc.ff_fused(sd).cpu().numpy()))
+    b.close()
+
+
+def test_batch_refuses_a_rebuilt_scene(gpu):
+    """A batch holds its scenes' device tables as they were at creation: after
+    a scene's grid is rebuilt or its emission set again, run and set_tuning
+    raise before launching anything; a new batch runs."""
+    import torch
+
+    scs = build_scenes(gpu, [120, 200], [200, 200], seed0=5)
+    Fs = [torch.empty((sc.n_meshes,) * 2, device="cuda") for sc in scs]
+    tallies = [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda") for sc in scs]
+    for rebuild in ("grid", "emission"):
+        b = gpu.Batch(scs)
+        b.run([1, 2], Fs, tallies=tallies, use_graph=True)
+        torch.cuda.synchronize()
+        if rebuild == "grid":
+            scs[1].build_grid((5, 5, 5))
+        else:
+            scs[0].set_emission(n_rays=150 * len(scs[0].faces), frame=gpu.FRAME_ORTHONORMAL)
+        for call in (lambda: b.run([1, 2], Fs, tallies=tallies, use_graph=True),
+                     lambda: b.run([1, 2], Fs, tallies=tallies, use_graph=False),
+                     lambda: b.set_tuning(512, 0, 0)):
+            with pytest.raises(RuntimeError, match="rebuilt"):
+                call()
+        b.close()
+        b = gpu.Batch(scs)
+        b.run([1, 2], Fs, tallies=tallies, use_graph=False)
+        torch.cuda.synchronize()
+        for sc, sd, F in zip(scs, [1, 2], Fs):
+            assert np.array_equal(bits(F.cpu().numpy()), bits(sc.ff_fused(sd).cpu().numpy()))
+        b.close()

@@ -292,3 +292,31 @@ def test_ff_analytic_parallel_plates_c1(gpu):
     assert abs(plate - 0.19982) < 3e-3
     # symmetry of the two plates
     assert abs(2 * F[200:, :200].sum() / 200 - plate) < 3e-3
+
+
+def test_tally_above_2_24_is_exact_not_saturated(gpu, oracle):
+    """A cell count above 2^24: the reference's float `F += 1.f` stops at 2^24
+    (the oracle's or_ff_tally restates it), while the device tallies integers
+    and rounds the exact count once -- a deliberate deviation."""
+    import torch
+
+    from form_factors_amd import scenes
+
+    faces, _ = scenes.parallel_plates(cells=1, gap=1.0)  # 4 faces
+    meshes = np.array([[0, 2, 0], [2, 2, 0]], np.int32)
+    n0 = (1 << 24) + 5
+    front = np.array([n0, 0, 1, 0], np.int32)
+    rear = np.zeros(4, np.int32)
+    sc = gpu.Scene(faces, meshes)
+    n = sc.set_emission(front=front, rear=rear)
+    assert n == n0 + 1
+    hit = torch.full((n,), 2, dtype=torch.int32, device="cuda")  # every ray of mesh 0 hits face 2 (mesh 1)
+    hit[n0:] = -1
+    print("tally of 2^24 + 5 hits in one cell", flush=True)
+    F = sc.normalise(sc.tally(hit)).cpu().numpy()
+    ratio = np.float32(1) / np.float32(n0)
+    assert F[0, 1] == np.float32(np.float32(n0) * ratio)  # exact count, rounded once
+    F_ref = oracle.ff_tally(meshes, 4, front, rear, hit.cpu().numpy())
+    assert F_ref[0, 1] == np.float32(np.float32(1 << 24) * ratio)  # the reference's saturated sum
+    assert F[0, 1] != F_ref[0, 1] and F[1].sum() == 0 and F_ref[1].sum() == 0
+    sc.close()

@@ -148,6 +148,40 @@ def test_pack_exchange_rows(gpu):
     assert np.abs(got - ref_rt).max() <= 2e-6 * np.abs(ref).max()
 
 
+@pytest.mark.parametrize("name", ["plates", "sphere"])
+def test_sbff_matches_reference_equation(gpu, name):
+    """ffk_sbff on the reference's own F against the emission and absorption
+    of its form_factors_equation (thermal_equation/form_factors.cpp:109-128):
+    FP32 and FP16 storage hold these count rows exactly (counts <= 2048), BF16
+    rounds counts above 256 to 8 significant bits."""
+    import torch
+
+    from tests.conftest import golden
+
+    g = golden("sbff.npz")
+    F = np.ascontiguousarray(g[f"{name}_F"])
+    n = len(F)
+    em_ref = g[f"{name}_emission"].astype(np.float64)
+    ab_ref = g[f"{name}_absorption"].astype(np.float64)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a, np.float64)).cuda()  # noqa: E731
+    T, A = dev(g[f"{name}_T"]), dev(g[f"{name}_area"])
+    ef, er = dev(g[f"{name}_eps"][:, 0]), dev(g[f"{name}_eps"][:, 1])
+    Fd = torch.from_numpy(F).cuda()
+    p = gpu._ptr
+    for dtype, tol in ((gpu.F32, 2e-6), (gpu.F16, 2e-6), (gpu.BF16, 2.0 ** -8)):
+        print(f"sbff {name} dtype {dtype}", flush=True)
+        S, scale = gpu.pack_exchange(Fd, dtype)
+        em = torch.zeros(n, dtype=torch.float64, device="cuda")
+        ab = torch.zeros(n, dtype=torch.float64, device="cuda")
+        work = torch.empty(n, dtype=torch.float32, device="cuda")
+        gpu.check(gpu.lib().ffk_sbff(n, p(S), dtype, p(scale), p(T), p(A), p(ef), p(er), p(em), p(ab), p(work),
+                                     gpu._stream()), "ffk_sbff")
+        torch.cuda.synchronize()
+        em, ab = em.cpu().numpy(), ab.cpu().numpy()
+        assert np.all(np.abs(em - em_ref) <= 1e-6 * em_ref), dtype
+        assert np.all(np.abs(ab - ab_ref) <= tol * ab_ref), (dtype, np.abs(ab / ab_ref - 1).max())
+
+
 def test_plain_bf16_rounding_is_coherent_and_packing_removes_it(gpu):
     """Rounding count-valued F entries to BF16 gives every cell of equal count
     the same relative error; the packed storage has none."""
