@@ -133,6 +133,7 @@ def ref():
         _sig(R, "ffref_bsphere", None, [_f, C.c_int, _f, _f])
         _sig(R, "ffref_sun", C.c_int, [_f, C.c_int, _i, C.c_int, _f, C.c_float, _f, C.c_int, _f])
         _sig(R, "ffref_heat_source", C.c_int, [_i, _f, C.c_int, _f, _f])
+        _sig(R, "ffref_ff_timed", C.c_int, [_f, C.c_int, _i, C.c_int, C.c_int, C.c_int, _f, _d])
         _ref = R
     return _ref
 
@@ -374,3 +375,18 @@ def ref_cast(kind, faces, rays, n_threads=1):
     if rc < 0:
         raise RuntimeError(f"reference caster returned {rc}")
     return idx, pts
+
+
+def ref_ff_timed(faces, meshes, n_rays, caster=1):
+    """One reference FORM_FACTORS_CPU calculate() (single-threaded, mt19937
+    Malley emission, caster 0 = naive_cpu, 1 = zgrid_cpu): F and its time
+    split (ray generation, casting, tally + normalisation) in seconds."""
+    faces, meshes = f32(faces).reshape(-1, 9), i32(meshes).reshape(-1, 3)
+    nm = len(meshes)
+    F = np.zeros((nm, nm), np.float32)
+    t = np.zeros(3, np.float64)
+    rc = ref().ffref_ff_timed(_p(faces, _f), len(faces), _p(meshes, _i), nm, int(n_rays), caster, _p(F, _f),
+                              _p(t, _d))
+    if rc < 0:
+        raise RuntimeError(f"reference form-factor calculator returned {rc}")
+    return F, t

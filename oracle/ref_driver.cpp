@@ -36,6 +36,7 @@
 #include "thermal_solution/cpu_adams.h"
 #include "thermal_solution/system.h"
 
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <thread>
@@ -292,6 +293,94 @@ EXPORT int ffref_ff_calculate(const float* faces, int n_faces, const int* meshes
   form_factors::system_free(ff);
   emission::system_free(em);
   free_caster(rc);
+  return r;
+}
+
+namespace {
+/* Timing plugins: a caster and an emitter that forward every method to the
+ * reference's own system and add the wall time of cast() / calculate(). The
+ * form-factor calculator calls the emitter, which calls the caster, so the
+ * time of one FORM_FACTORS_CPU calculate() splits into ray generation
+ * (emitter minus caster), casting, and the tally and normalisation
+ * (calculator minus emitter). */
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+struct timed_caster_t : ray_caster::system_t {
+  ray_caster::system_t* inner;
+  double seconds;
+};
+int tc_init(timed_caster_t*) { return 0; }
+int tc_shutdown(timed_caster_t*) { return 0; }
+int tc_set_scene(timed_caster_t* s, ray_caster::scene_t* sc) { return ray_caster::system_set_scene(s->inner, sc); }
+int tc_prepare(timed_caster_t* s) { return ray_caster::system_prepare(s->inner); }
+int tc_cast(timed_caster_t* s, ray_caster::task_t* t) {
+  const double t0 = now_s();
+  int r = ray_caster::system_cast(s->inner, t);
+  s->seconds += now_s() - t0;
+  return r;
+}
+const ray_caster::system_methods_t tc_methods = {
+    (int (*)(ray_caster::system_t*))&tc_init,
+    (int (*)(ray_caster::system_t*))&tc_shutdown,
+    (int (*)(ray_caster::system_t*, ray_caster::scene_t*))&tc_set_scene,
+    (int (*)(ray_caster::system_t*))&tc_prepare,
+    (int (*)(ray_caster::system_t*, ray_caster::task_t*))&tc_cast,
+};
+struct timed_emitter_t : emission::system_t {
+  emission::system_t* inner;
+  double seconds;
+};
+int te_init(timed_emitter_t*, ray_caster::system_t*) { return 0; }
+int te_shutdown(timed_emitter_t*) { return 0; }
+int te_set_scene(timed_emitter_t* s, ray_caster::scene_t* sc) { return emission::system_set_scene(s->inner, sc); }
+int te_calculate(timed_emitter_t* s, emission::task_t* t) {
+  const double t0 = now_s();
+  int r = emission::system_calculate(s->inner, t);
+  s->seconds += now_s() - t0;
+  return r;
+}
+const emission::system_methods_t te_methods = {
+    (int (*)(emission::system_t*, ray_caster::system_t*))&te_init,
+    (int (*)(emission::system_t*))&te_shutdown,
+    (int (*)(emission::system_t*, ray_caster::scene_t*))&te_set_scene,
+    (int (*)(emission::system_t*, emission::task_t*))&te_calculate,
+};
+}  // namespace
+
+/* One FORM_FACTORS_CPU calculate() (cpuFactorsCalculator/cpu_system.cpp:149-201)
+ * over MALLEY_CPU over the chosen CPU caster, single-threaded as the
+ * reference runs it, with its time split into ray generation, casting and
+ * tally + normalisation (seconds3). */
+EXPORT int ffref_ff_timed(const float* faces, int n_faces, const int* meshes, int n_meshes, int n_rays,
+                          int caster, float* F, double* seconds3) {
+  timed_caster_t tc;
+  tc.methods = &tc_methods;
+  tc.inner = make_caster(caster);
+  tc.seconds = 0;
+  emission::system_t* em_inner = emission::system_create(EMISSION_MALLEY_CPU, &tc);
+  timed_emitter_t te;
+  te.methods = &te_methods;
+  te.inner = em_inner;
+  te.seconds = 0;
+  form_factors::system_t* ff = form_factors::system_create(FORM_FACTORS_CPU, &te);
+  form_factors::scene_t scene = {n_faces, (form_factors::face_t*)faces, n_meshes,
+                                 (form_factors::mesh_t*)meshes};
+  int r = form_factors::system_set_scene(ff, &scene);
+  if (r == 0) r = form_factors::system_prepare(ff);
+  double total = 0;
+  if (r == 0) {
+    form_factors::task_t task = {n_rays, F};
+    const double t0 = now_s();
+    r = form_factors::system_calculate(ff, &task);
+    total = now_s() - t0;
+  }
+  seconds3[0] = te.seconds - tc.seconds;
+  seconds3[1] = tc.seconds;
+  seconds3[2] = total - te.seconds;
+  form_factors::system_free(ff);
+  emission::system_free(em_inner);
+  free_caster(tc.inner);
   return r;
 }
 

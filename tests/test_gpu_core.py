@@ -278,14 +278,21 @@ def test_meshes_out_of_face_order_follow_the_reference_walk(gpu, oracle):
     assert np.array_equal(bits(F), bits(want))
 
 
-def test_ff_analytic_parallel_plates_c1(gpu):
-    # Config C1: 400 triangles (one mesh each), 10k rays per triangle.
+def test_ff_analytic_parallel_plates_c1(gpu, oracle):
+    # Config C1 at full size: 400 triangles (one mesh each), 10k rays per
+    # triangle (4e6 rays), grid 16^3; F bit for bit against the oracle fed
+    # the identical rays, and the analytic plate-to-plate value.
     from form_factors_amd import scenes
 
     faces, meshes = scenes.parallel_plates(10)
     sc = make_scene(gpu, faces, meshes, n_rays=4_000_000, dims=(16, 16, 16))
     assert sc.n_rays == 4_000_000
-    F = sc.ff_fused(seed=1).cpu().numpy().astype(np.float64)
+    F32 = sc.ff_fused(seed=1).cpu().numpy()
+    print("C1 full size: oracle on 4e6 rays", flush=True)
+    r, want = oracle.ff_calculate(faces, meshes, 4_000_000, rng=oracle.RNG_PHILOX, seed=1,
+                                  frame=oracle.FRAME_REFERENCE, caster=0, n_threads=16)
+    assert r == 0 and np.array_equal(bits(F32), bits(want))
+    F = F32.astype(np.float64)
     plate = 2 * F[:200, 200:].sum() / 200
     # analytic plate-to-plate F for unit squares at distance 1 (the
     # reference halves it for two-sided emission, test/form_factors.cpp:192-196)

@@ -1,9 +1,16 @@
-"""The 20,480-triangle sphere at its full size (10,000 Malley rays per
-triangle, 2.05e8 rays, grid 64^3), checked through properties that do not
-depend on the size: identical F bits under every ray schedule and through the
-batched launch, exact row sums of a closed surface, binomial agreement with
-A_j / (2 A_total), and a prefix of the same rays against the oracle's brute
-force."""
+"""Workloads at their full stated sizes, checked through properties that do
+not depend on the size:
+
+* the 20,480-triangle sphere (10,000 Malley rays per triangle, 2.05e8 rays,
+  grid 64^3): identical F bits under every ray schedule and through the
+  batched launch, exact row sums of a closed surface, binomial agreement with
+  A_j / (2 A_total), and a prefix of the same rays against the oracle's brute
+  force;
+* the confetti scene (4028 triangles, 2.01e8 rays): default grid against
+  autotuned F bit for bit, prefix against the oracle, first row against the
+  cast hits;
+* the 64-scene uneven set (6.1e9 rays): batched against plain F, bitwise.
+"""
 import numpy as np
 import pytest
 
@@ -100,4 +107,92 @@ def test_full_size_prefix_and_first_row_against_the_oracle(gpu, oracle, sphere):
     want = (counts.astype(np.float32) * (np.float32(1.0) / np.float32(n0))).astype(np.float32)
     assert np.array_equal(bits(row), bits(want))
     del rays, F0
+    torch.cuda.empty_cache()
+
+
+def test_full_size_confetti_tuned_default_and_oracle_prefix(gpu, oracle):
+    """The confetti workload at its stated size: 2000 confetti triangles
+    (skinny ones below the reference's degeneracy threshold included) inside a
+    2028-triangle cube shell, 50,000 Malley rays per triangle (2.01e8 rays).
+    F on the default grid equals F after the autotuner bit for bit; the first
+    rays of the same set cast on the GPU equal the oracle's brute force; the
+    hits of face 0's rays, counted per target, are F's first row."""
+    import torch
+
+    from form_factors_amd import scenes
+
+    faces, meshes = scenes.confetti_scene(2000, cells=13, seed=1)
+    sc = gpu.Scene(faces, meshes)
+    n = sc.set_emission(n_rays=50_000 * len(faces), frame=gpu.FRAME_ORTHONORMAL)
+    sc.build_grid()
+    assert n > 2e8
+    N = len(meshes)
+    print(f"full-size confetti: {len(faces)} faces, {n} rays, default grid", flush=True)
+    F = torch.empty((N, N), device="cuda")
+    G = torch.empty((N, N), device="cuda")
+    sc.ff_fused(0xC3, F=F)
+    print("full-size confetti: autotune", flush=True)
+    sc.autotune(sample_rays=1 << 22)
+    sc.ff_fused(0xC3, F=G)
+    torch.cuda.synchronize()
+    assert torch.equal(F.view(torch.int32), G.view(torch.int32))
+    del G
+    rays = sc.emit_malley(seed=0xC3)
+    prefix = rays[:1 << 16]
+    idx, pts = sc.cast(prefix, method=gpu.CAST_GRID)
+    want_i, want_p = oracle.naive_cast(faces, prefix.cpu().numpy(), n_threads=16)
+    got_i = idx.cpu().numpy()
+    assert np.array_equal(got_i, want_i)
+    assert np.array_equal(bits(pts.cpu().numpy()[got_i >= 0]), bits(want_p[want_i >= 0]))
+    n0 = int(sc.rays_per_mesh()[0])
+    idx0, _ = sc.cast(rays[:n0], method=gpu.CAST_GRID, points=False)
+    h = idx0.cpu().numpy()
+    counts = np.bincount(h[h >= 0], minlength=len(faces))
+    want = (counts.astype(np.float32) * (np.float32(1.0) / np.float32(n0))).astype(np.float32)
+    assert np.array_equal(bits(F[0].cpu().numpy()), bits(want))
+    del rays, F
+    sc.close()
+    torch.cuda.empty_cache()
+
+
+def test_full_size_batch_set_batched_equals_plain(gpu):
+    """The 64-scene batched workload at its stated size (triangle counts
+    log-uniform in [200, 50000], 1k-20k rays per triangle, 6.1e9 rays, F 54 GB):
+    one graph-replayed persistent launch per LDS class gives every scene the
+    F of its own plain launch, all cells compared bitwise on the device."""
+    import torch
+
+    from form_factors_amd import scenes
+
+    specs = scenes.batch_scene_set(64, seed=0)
+    scs, seeds = [], []
+    for faces, meshes, rpt, sd in specs:
+        sc = gpu.Scene(faces, meshes)
+        sc.set_emission(n_rays=rpt * len(faces), frame=gpu.FRAME_ORTHONORMAL)
+        sc.build_grid()
+        scs.append(sc)
+        seeds.append(sd)
+    print(f"full-size batch set: {len(scs)} scenes, {sum(sc.n_rays for sc in scs)} rays", flush=True)
+    Fs = [torch.empty((sc.n_meshes,) * 2, dtype=torch.float32, device="cuda") for sc in scs]
+    tallies = [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda")
+               if gpu.lib().ffk_ff_needs_tally(sc.h) else None for sc in scs]
+    for sc, sd, F, T in zip(scs, seeds, Fs, tallies):
+        sc.ff_fused(sd, F=F, tally=T, need_tally=T is not None)
+    torch.cuda.synchronize()
+    F_plain = [F.clone() for F in Fs]
+    del tallies
+    torch.cuda.empty_cache()
+    print("full-size batch set: batched launch", flush=True)
+    b = gpu.Batch(scs)
+    bt = b.make_tallies()
+    for F in Fs:
+        F.fill_(-1.0)
+    b.run(seeds, Fs, tallies=bt, use_graph=True)
+    torch.cuda.synchronize()
+    for k, (F, G) in enumerate(zip(Fs, F_plain)):
+        assert torch.equal(F.view(torch.int32), G.view(torch.int32)), k
+    b.close()
+    del Fs, F_plain, bt
+    for sc in scs:
+        sc.close()
     torch.cuda.empty_cache()

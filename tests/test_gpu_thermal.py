@@ -244,3 +244,63 @@ def test_thermal_full_size_prefix_vs_fp64(gpu):
         err = np.abs(T - rec[20]).max() / rec[20].max()
         assert err <= 1e-4, (dtype, err)
         th.close()
+
+
+def test_device_fp64_reference_matches_cpu_fp64(gpu):
+    """The torch-FP64 device restatement used at full size equals the CPU
+    FP64 run (oracle Adams combination) to rounding."""
+    from tests.thermal_ref import fp64_run_device
+
+    prob, F, csr, n = setup(gpu, cells=12, n_rays_per_tri=2000)
+    T0 = np.linspace(290, 310, n)
+    T_cpu, _ = fp64_run(F.cpu().numpy(), prob, *csr, T0, 300, 0.1)
+    T_dev = fp64_run_device(F, prob, *csr, T0, 300, 0.1)
+    assert np.abs(T_dev - T_cpu).max() <= 1e-12 * T_cpu.max()
+
+
+def test_thermal_full_size_10k_steps_vs_fp64(gpu):
+    """The mixed-precision workload at its stated size and length: 20,000
+    meshes (two 100 x 100-cell plates), 10,000 Adams steps of 0.1 s, F stored
+    in BF16 and FP16 (FP32 accumulation, FP64 update) against the all-FP64
+    device run from the same F: max relative T error <= 1e-4."""
+    from tests.thermal_ref import fp64_run_device
+
+    prob, F, csr, n = setup(gpu, cells=100, n_rays_per_tri=500)
+    assert n == 20_000
+    T0 = np.full(n, 300.0)
+    print("C5 full size: FP64 device reference, 10,000 steps", flush=True)
+    T_ref = fp64_run_device(F, prob, *csr, T0, 10_000, 0.1)
+    assert T_ref.max() - T_ref.min() > 1.0  # the heated patch stands out
+    errs = {}
+    for dtype in (gpu.BF16, gpu.F16):
+        print(f"C5 full size: dtype {dtype}, 10,000 steps", flush=True)
+        th = make_run(gpu, prob, F, csr, dtype)
+        T = th.run(10_000, 0.1, T0=T0)
+        errs[dtype] = float(np.abs(T - T_ref).max() / np.abs(T_ref).max())
+        th.close()
+    print("max relative T error vs FP64 at 20,000 meshes x 10,000 steps:", errs)
+    assert all(e <= 1e-4 for e in errs.values()), errs
+
+
+def test_thermal_counts_above_256_vs_fp64(gpu):
+    """Counts above 256 per cell (200 meshes, 1e6 rays per mesh): BF16 keeps
+    8 significant bits and FP16 11, so the stored counts round; the error of
+    a 10,000-step run against FP64 is bounded and reported."""
+    from tests.thermal_ref import fp64_run_device
+
+    prob, F, csr, n = setup(gpu, cells=10, n_rays_per_tri=500_000)
+    counts = (F.double().cpu().numpy() * prob["rays_per_mesh"][:, None])
+    print(f"counts per cell up to {counts.max():.0f}", flush=True)
+    assert counts.max() > 2048
+    T0 = np.full(n, 300.0)
+    T_ref = fp64_run_device(F, prob, *csr, T0, 10_000, 0.1)
+    errs = {}
+    for dtype in (gpu.F32, gpu.BF16, gpu.F16):
+        th = make_run(gpu, prob, F, csr, dtype)
+        T = th.run(10_000, 0.1, T0=T0)
+        errs[dtype] = float(np.abs(T - T_ref).max() / np.abs(T_ref).max())
+        th.close()
+    print("counts > 256: max relative T error vs FP64:", errs, flush=True)
+    assert errs[gpu.F32] <= 1e-7
+    assert errs[gpu.F16] <= 1e-4
+    assert errs[gpu.BF16] <= 1e-3
