@@ -377,6 +377,7 @@ struct ffk_scene {
   // host work arrays of the radiance step
   std::vector<float> rad_w, rad_p;
   std::vector<int> rad_front, rad_rear;
+  std::vector<unsigned char> rad_band;  // power band of each walk entry's front / rear ray
 
   ffk::scene_desc desc() const;
 };

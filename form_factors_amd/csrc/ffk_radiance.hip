@@ -9,12 +9,31 @@
 // their first walk position), and the powers are summed per mesh in 64-bit
 // fixed point: integer additions, so the result does not depend on the order
 // of the atomics and carries no float accumulation error.
+//
+// Ray powers can span many decades (they scale with T^3, and a face whose
+// share of the rays rounds below one still emits one ray carrying its whole
+// power), so one fixed-point scale for all rays would round the smallest
+// ones coarsely. Each ray power is therefore assigned to a band of 2^8 in
+// magnitude below the largest power; every band has its own scale and its
+// own accumulator row, and the band travels in the top 4 bits of the ray's
+// fixed-point word.
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "ffk_trace.hpp"
 
 namespace ffk {
+
+constexpr int kPowerBands = 16;      // accumulator rows (bands of 2^kBandOrders)
+constexpr int kBandOrders = 8;       // binary orders of magnitude per band
+constexpr int kBandShift = 60;       // band index in bits 60..63 of a q word
+constexpr unsigned long long kValueMask = (1ull << kBandShift) - 1;
+
+// Adds the fixed-point power word v to mesh m's row of its band.
+__device__ inline void deposit(unsigned long long* __restrict__ acc, int n_meshes, int m, unsigned long long v) {
+  if (v) atomicAdd(&acc[(size_t)(v >> kBandShift) * n_meshes + m], v & kValueMask);
+}
 
 __global__ void k_fill_acc(unsigned long long* __restrict__ p, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -43,8 +62,7 @@ __global__ void __launch_bounds__(256) k_radiance(scene_desc S, const long long*
     const int k = find_face(walk_start, n_walk, g, whint);
     whint = k;
     const int wf = walk_face[k];
-    const unsigned long long v = q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)];
-    if (v) atomicAdd(&acc[m], v);
+    deposit(acc, S.n_meshes, m, q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)]);
   }
   if (STATS) {
     unsigned long long t = nt, c = nc;
@@ -106,8 +124,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
         const int k = find_face_in(walk_start, s_walk, g, whint);
         whint = k;
         const int wf = walk_face[k];
-        const unsigned long long v = q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)];
-        if (v) atomicAdd(&acc[m], v);
+        deposit(acc, S.n_meshes, m, q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)]);
       },
       nt, nc);
   if (STATS) {
@@ -229,10 +246,10 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   const int nf = s->n_faces, nm = s->n_meshes;
   const long long total = s->n_rays;
   // Staging for W walk entries and nm meshes, in this order: start[W + 1]
-  // (int64), q[2W] (uint64), acc[nm] (uint64), face[W + 1] (int32). Every walk
-  // entry is a face, so W = n_faces bounds them.
+  // (int64), q[2W] (uint64), acc[kPowerBands * nm] (uint64), face[W + 1]
+  // (int32). Every walk entry is a face, so W = n_faces bounds them.
   const size_t W = (size_t)nf;
-  const size_t off_q = 8 * (W + 1), off_acc = off_q + 16 * W, off_face = off_acc + 8 * (size_t)nm;
+  const size_t off_q = 8 * (W + 1), off_acc = off_q + 16 * W, off_face = off_acc + 8 * (size_t)kPowerBands * nm;
   const size_t bytes = off_face + 4 * (W + 1);
   if ((int)W + 1 > s->rad_cap_walk || nm > s->rad_cap_meshes) {
     FFK_HIP(hipStreamSynchronize(st));
@@ -261,7 +278,7 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   // walk stops at the last emitted ray (radiance_cpu.cpp:170).
   int n_walk = 0;
   long long pos = 0;
-  double upper = 0.0;
+  float p_max = 0.f;
   for (int m = 0; m < nm && pos < total; ++m) {
     const int f0 = s->h_meshes[3 * m], n = s->h_meshes[3 * m + 1];
     for (int f = f0; f < f0 + n && pos < total; ++f) {
@@ -271,27 +288,75 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
       start[n_walk] = pos;
       face[n_walk] = f;
       ++n_walk;
-      upper += (double)s->h_front[f] * pf + (double)s->h_rear[f] * pr;
+      if (s->h_front[f] > 0) p_max = std::max(p_max, pf);
+      if (s->h_rear[f] > 0) p_max = std::max(p_max, pr);
       pos += (long long)s->h_front[f] + s->h_rear[f];
     }
   }
   const long long walk_total = std::min(pos, total);
   for (int m = 0; m < nm; ++m) h_absorption[m] = 0.0;
-  if (n_walk == 0 || upper <= 0.0) return FFK_OK;
+  if (n_walk == 0 || p_max <= 0.f) return FFK_OK;
   start[n_walk] = walk_total;
-  // Fixed point: the largest possible sum stays below 2^62.
-  int e2 = 0;
-  std::frexp(upper, &e2);  // upper < 2^e2
-  const double scale = std::ldexp(1.0, 62 - e2);
+  // Band of a ray power: its binary order below the largest power, in steps
+  // of kBandOrders (the last band takes everything smaller). Per band, the
+  // largest possible sum (every ray of the band absorbed by one mesh) and the
+  // largest single power fix the scale: sums stay below 2^63 and a single
+  // ray's value below 2^60, clear of the band bits. A ray of power p in band
+  // b is then rounded by at most 0.5 / scale_b, i.e. relatively by at most
+  // 2^-55 times the rays in the band, or 2^-52 (2^kBandOrders / 2^60).
+  // binary exponent of a positive float from its bits (ilogb; subnormals rare)
+  auto exponent = [](float p) {
+    uint32_t u;
+    std::memcpy(&u, &p, 4);
+    const int e = (int)((u >> 23) & 0xff);
+    return e ? e - 127 : std::ilogb(p);
+  };
+  const int e_max = exponent(p_max);
+  double upper[kPowerBands] = {}, pmax_b[kPowerBands] = {};
+  int n_bands = 1;
+  s->rad_band.resize(2 * (size_t)n_walk);
+  unsigned char* band = s->rad_band.data();
   for (int k = 0; k < n_walk; ++k) {
     const int f = face[k];
-    q[2 * k] = (unsigned long long)std::llrint((double)h_ray_power[2 * f] * scale);
-    q[2 * k + 1] = (unsigned long long)std::llrint((double)h_ray_power[2 * f + 1] * scale);
+    const int cnt[2] = {s->h_front[f], s->h_rear[f]};
+    for (int side = 0; side < 2; ++side) {
+      const float p = h_ray_power[2 * f + side];
+      const int b = p > 0.f ? std::min(kPowerBands - 1, (e_max - exponent(p)) / kBandOrders) : 0;
+      band[2 * k + side] = (unsigned char)b;
+      if (p <= 0.f || cnt[side] <= 0) continue;
+      upper[b] += (double)cnt[side] * p;
+      pmax_b[b] = std::max(pmax_b[b], (double)p);
+      n_bands = std::max(n_bands, b + 1);
+    }
   }
+  double scale[kPowerBands];
+  for (int b = 0; b < kPowerBands; ++b) {
+    int e_sum = 0, e_one = 0;
+    std::frexp(upper[b], &e_sum);   // upper < 2^e_sum
+    std::frexp(pmax_b[b], &e_one);  // pmax < 2^e_one
+    scale[b] = upper[b] > 0.0 ? std::ldexp(1.0, std::min(63 - e_sum, kBandShift - e_one)) : 1.0;
+  }
+  // round half to even of a non-negative x < 2^60 (what llrint does in the
+  // default rounding mode; x - trunc(x) is exact, so the comparison is too)
+  auto round_even = [](double x) {
+    unsigned long long v = (unsigned long long)x;
+    const double frac = x - (double)v;
+    return v + (frac > 0.5 || (frac == 0.5 && (v & 1)) ? 1ull : 0ull);
+  };
+  for (int k = 0; k < n_walk; ++k) {
+    const int f = face[k];
+    for (int side = 0; side < 2; ++side) {
+      const float p = h_ray_power[2 * f + side];
+      const int b = band[2 * k + side];
+      q[2 * k + side] =
+          p > 0.f ? ((unsigned long long)b << kBandShift) | round_even((double)p * scale[b]) : 0ull;
+    }
+  }
+  const size_t n_acc = (size_t)n_bands * nm;
   // start, q and (past acc) face in one copy; acc is zeroed on the device
   FFK_HIP(hipMemcpyAsync(db, hb, off_acc, hipMemcpyHostToDevice, st));
   FFK_HIP(hipMemcpyAsync(db + off_face, hb + off_face, 4 * ((size_t)n_walk + 1), hipMemcpyHostToDevice, st));
-  k_fill_acc<<<(nm + 255) / 256, 256, 0, st>>>(s->d_rad_acc, nm);
+  k_fill_acc<<<(unsigned)((n_acc + 255) / 256), 256, 0, st>>>(s->d_rad_acc, (int)n_acc);
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   if (walk_total > 0 && s->tune_refill > 0 && s->tune_pool > 0) {
     // Whole rounds of the blocks the device holds at once (the last round is
@@ -321,9 +386,13 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
                                                 walk_total, k0, k1, s->d_rad_acc, nullptr);
   }
   FFK_HIP(hipGetLastError());
-  FFK_HIP(hipMemcpyAsync(acc, s->d_rad_acc, 8 * (size_t)nm, hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipMemcpyAsync(acc, s->d_rad_acc, 8 * n_acc, hipMemcpyDeviceToHost, st));
   FFK_HIP(hipStreamSynchronize(st));
-  for (int m = 0; m < nm; ++m) h_absorption[m] = (double)acc[m] / scale;
+  for (int m = 0; m < nm; ++m) {
+    double a = 0.0;
+    for (int b = 0; b < n_bands; ++b) a += (double)acc[(size_t)b * nm + m] / scale[b];
+    h_absorption[m] = a;
+  }
   return FFK_OK;
 }
 

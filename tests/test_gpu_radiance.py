@@ -49,12 +49,12 @@ def _cases():
     return out
 
 
-# Per-mesh precision of the fixed-point sums: every ray power is rounded to a
-# multiple of 2^-scale_bits of the largest possible total `upper`, so a mesh
-# whose absorbed rays have power >= p_min is off by at most
-# upper / (2^61 p_min) relative. Cases with T in [250, 400] K stay below 1e-12;
-# the 20 K / 2000 K case below 1e-7.
-REL_TOL = {"wide_T_20K_2000K": 1e-7}
+# Per-mesh precision of the fixed-point sums: ray powers are grouped in bands
+# of 2^8 below the largest one, each band with its own scale, so every ray is
+# rounded by at most max(rays * 2^-55, 2^-52) of its power and so is each
+# mesh's sum, however widely the powers spread (200,000 rays: 6e-12).
+def rel_tol(n_rays):
+    return n_rays * 2.0 ** -55 + 2.0 ** -50
 
 
 @pytest.mark.parametrize("frame", [0, 1])
@@ -72,7 +72,8 @@ def test_radiance_matches_oracle(gpu, oracle, case, frame):
     assert ab64.sum() > 0
     assert np.abs(ab - ab64).max() <= 1e-12 * ab64.max()
     # per mesh, relative to that mesh's own absorption (zero stays zero)
-    assert np.all(np.abs(ab - ab64) <= REL_TOL.get(name, 1e-12) * ab64), np.max(np.abs(ab / np.where(ab64 > 0, ab64, 1) - 1))
+    rel = np.abs(ab - ab64) / np.where(ab64 > 0, ab64, 1)
+    assert np.all(np.abs(ab - ab64) <= rel_tol(n_rays) * ab64), rel.max()
     # the reference's float accumulation in ray order differs only by rounding
     assert np.allclose(ab, ab_o, rtol=2e-4, atol=1e-6 * ab64.max())
 

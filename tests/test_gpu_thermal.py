@@ -151,9 +151,11 @@ def test_pack_exchange_rows(gpu):
 @pytest.mark.parametrize("name", ["plates", "sphere"])
 def test_sbff_matches_reference_equation(gpu, name):
     """ffk_sbff on the reference's own F against the emission and absorption
-    of its form_factors_equation (thermal_equation/form_factors.cpp:109-128):
-    FP32 and FP16 storage hold these count rows exactly (counts <= 2048), BF16
-    rounds counts above 256 to 8 significant bits."""
+    of its form_factors_equation (thermal_equation/form_factors.cpp:109-128).
+    With the rays per mesh as row multipliers the storage holds the hit counts:
+    exact in FP32 and FP16 (counts <= 2048) and in BF16 up to 256. Without
+    them, rows whose counts are not all multiples of the smallest one are
+    stored as rounded F: 2^-11 (FP16) and 2^-8 (BF16) relative."""
     import torch
 
     from tests.conftest import golden
@@ -168,9 +170,19 @@ def test_sbff_matches_reference_equation(gpu, name):
     ef, er = dev(g[f"{name}_eps"][:, 0]), dev(g[f"{name}_eps"][:, 1])
     Fd = torch.from_numpy(F).cuda()
     p = gpu._ptr
-    for dtype, tol in ((gpu.F32, 2e-6), (gpu.F16, 2e-6), (gpu.BF16, 2.0 ** -8)):
-        print(f"sbff {name} dtype {dtype}", flush=True)
-        S, scale = gpu.pack_exchange(Fd, dtype)
+    faces, meshes = g[f"{name}_faces"], g[f"{name}_meshes"]
+    w, total = gpu.area_weights(faces)
+    fr, rr = gpu.emitted_counts(w, int(g[f"{name}_n_rays"]), total)
+    per_face = fr.astype(np.int64) + rr
+    rays = np.array([per_face[a:a + c].sum() for a, c, _ in meshes], np.float64)
+    counts = F.astype(np.float64) * rays[:, None]
+    assert np.abs(counts - np.round(counts)).max() < 1e-3  # F rows are counts / rays of the mesh
+    bf16_exact = counts.max() <= 256
+    cases = [(gpu.F32, rays, 2e-6), (gpu.F16, rays, 2e-6), (gpu.BF16, rays, 2e-6 if bf16_exact else 2.0 ** -8),
+             (gpu.F32, None, 2e-6), (gpu.F16, None, 2.0 ** -11), (gpu.BF16, None, 2.0 ** -8)]
+    for dtype, row_mul, tol in cases:
+        print(f"sbff {name} dtype {dtype} row_mul {row_mul is not None}", flush=True)
+        S, scale = gpu.pack_exchange(Fd, dtype, row_mul=row_mul)
         em = torch.zeros(n, dtype=torch.float64, device="cuda")
         ab = torch.zeros(n, dtype=torch.float64, device="cuda")
         work = torch.empty(n, dtype=torch.float32, device="cuda")
@@ -179,7 +191,7 @@ def test_sbff_matches_reference_equation(gpu, name):
         torch.cuda.synchronize()
         em, ab = em.cpu().numpy(), ab.cpu().numpy()
         assert np.all(np.abs(em - em_ref) <= 1e-6 * em_ref), dtype
-        assert np.all(np.abs(ab - ab_ref) <= tol * ab_ref), (dtype, np.abs(ab / ab_ref - 1).max())
+        assert np.all(np.abs(ab - ab_ref) <= tol * ab_ref), (dtype, row_mul is not None, np.abs(ab / ab_ref - 1).max())
 
 
 def test_plain_bf16_rounding_is_coherent_and_packing_removes_it(gpu):
