@@ -14,6 +14,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "ffk_internal.hpp"
 
@@ -657,6 +658,58 @@ int ffk_sbff(int n, const void* d_S, int dtype, const double* d_row_scale, const
   else launch_rows<FFK_F32>(n, d_S, E32, d_absorption, st);
   FFK_HIP(hipGetLastError());
   return FFK_OK;
+}
+
+}  // extern "C"
+
+// ---- allocation-free SB_FF ------------------------------------------------------
+// ffk_sbff_noalloc's scratch: the scaled FP32 emission vector, in a static
+// device array instead of caller or library allocations. Each device holds
+// its own copy for the life of the process; host code reaches it only
+// through the address hipGetSymbolAddress returns on that device (the array's
+// name in host code is a stand-in, not a device address).
+static __device__ __align__(256) float scratch[FFK_NOALLOC_MAX_MESHES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call made on that device and kept.
+int scratch_on_current_device(float** out) {
+  int dev = 0;
+  FFK_HIP(hipGetDevice(&dev));
+  FFK_REQUIRE(dev >= 0 && dev < kMaxDevices, FFK_ERR_LIMIT, "device index %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    FFK_REQUIRE(e == hipSuccess && p, FFK_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", dev,
+                hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return FFK_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int ffk_sbff_noalloc(int n, const void* d_S, int dtype, const double* d_row_scale, const double* d_T,
+                     const double* d_area, const double* d_eps_front, const double* d_eps_rear, double* d_emission,
+                     double* d_absorption, void* stream) {
+  FFK_REQUIRE(n > 0, FFK_ERR_ARG, "ffk_sbff_noalloc: n = %d", n);
+  FFK_REQUIRE(n <= FFK_NOALLOC_MAX_MESHES, FFK_ERR_LIMIT,
+              "ffk_sbff_noalloc: %d meshes exceed the static scratch of %d meshes (use ffk_sbff)", n,
+              FFK_NOALLOC_MAX_MESHES);
+  float* work = nullptr;
+  int r;
+  if ((r = scratch_on_current_device(&work)) < 0) return r;
+  // the same kernels as ffk_sbff; the scratch is written in full (n values)
+  // before it is read, so nothing left by an earlier call is used
+  return ffk_sbff(n, d_S, dtype, d_row_scale, d_T, d_area, d_eps_front, d_eps_rear, d_emission, d_absorption, work,
+                  stream);
 }
 
 int ffk_conduction(int n, const int* d_row_ptr, const int* d_cols, const double* d_G, const double* d_T,

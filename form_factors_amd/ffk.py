@@ -73,6 +73,7 @@ SIGNATURES = {
     "ffk_convert_f": (C.c_int, [_vp, _ll, C.c_int, _vp, _vp]),
     "ffk_pack_exchange": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "ffk_sbff": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_sbff_noalloc": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_adams_step": (C.c_int, [C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction_graph": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, C.POINTER(C.c_double), C.c_int,

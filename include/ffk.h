@@ -309,6 +309,20 @@ int ffk_pack_exchange(const float* d_F, int n, int dtype, const double* d_row_mu
 int ffk_sbff(int n, const void* d_S, int dtype, const double* d_row_scale, const double* d_T,
              const double* d_area, const double* d_eps_front, const double* d_eps_rear,
              double* d_emission, double* d_absorption, float* d_work, void* stream);
+/* ffk_sbff without d_work, allocating nothing. No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The
+ * scaled-emission scratch is a static device array of FFK_NOALLOC_MAX_MESHES
+ * floats (4 MiB), held by each device for the life of the process and shared
+ * by every call on that device: calls that overlap on one device race, so
+ * order them (one stream). n above the limit returns FFK_ERR_LIMIT before
+ * any launch. The address of the array is looked up on the first call made
+ * on a device, with that device current; call it with the stream's device
+ * current. Results equal ffk_sbff's bit for bit. */
+#define FFK_NOALLOC_MAX_MESHES (1 << 20)
+int ffk_sbff_noalloc(int n, const void* d_S, int dtype, const double* d_row_scale, const double* d_T,
+                     const double* d_area, const double* d_eps_front, const double* d_eps_rear,
+                     double* d_emission, double* d_absorption, void* stream);
 /* Conduction (thermal_equation/conductive_cpu.cpp:83-134) as a CSR operator
  * over mesh adjacency. For each neighbour k of mesh m, f = G_mk (T_k - T_m):
  * f > 0 is added to absorption[m], else -f to emission[m] (the reference's
