@@ -53,6 +53,9 @@ def parse():
     ap.add_argument("--no-radiance", action="store_true")
     ap.add_argument("--no-controller", action="store_true")
     ap.add_argument("--no-plates", action="store_true")
+    ap.add_argument("--only", choices=["plates", "batch_set", "confetti", "radiance", "controller", "thermal"],
+                    help="run this one workload alone (no headline line): for profiling")
+    ap.add_argument("--leg-steps", type=int, default=3, help="timed launches of the confetti workload")
     ap.add_argument("--batch-scenes", type=int, default=64)
     ap.add_argument("--thermal-steps", type=int, default=10_000)
     ap.add_argument("--block", type=int, default=0)
@@ -173,6 +176,72 @@ def cpu_baseline(faces, meshes, rays_host, gpu_idx, n_threads, c1):
             "emission_s": round(float(t1[0]), 3), "cast_s": round(float(t1[1]), 3),
             "tally_normalise_s": round(float(t1[2]), 3)}
     return out, mism
+
+
+def perftest_crosscheck(ffk, n_threads):
+    """perfTest's caster comparison (perfTest/perfTest.cpp:155-255) on its own
+    workload: 38,000 confetti triangles in a ball of radius 20 and 3.8e6 rays
+    collapsing toward the centre (from the reference's spherical generator
+    when the compiled reference is present). The HIP grid caster (the ZGRID
+    slot: scene set, grid built, rays cast, timed as perfTest times it), the
+    HIP brute force (the NAIVE slot) and the reference's zgrid_cpu caster on
+    n_threads threads cast the same rays; CalculateError's report (mismatched
+    hit faces; average, max and total hit-point distance) compares each with
+    the HIP grid caster, as perfTest compares with its CUDA zgrid caster."""
+    import torch
+
+    from form_factors_amd import crosscheck
+    from oracle import oracle as O
+
+    if O.ref_available():
+        faces, rays = O.ref_perftest_scene(38000, 20.0)
+        gen = "reference spherical generator (subject/spherial_generator.cpp)"
+    else:
+        faces, rays = crosscheck.perftest_scene(38000, 20.0)
+        gen = "restated generator (form_factors_amd/crosscheck.py)"
+    d_rays = torch.from_numpy(rays).cuda()
+    torch.cuda.synchronize()
+
+    def gpu_cast(method):
+        t0 = time.perf_counter()
+        sc = ffk.Scene(faces)
+        if method == ffk.CAST_GRID:  # the brute force needs no grid
+            sc.build_grid()
+        idx, pts = sc.cast(d_rays, method=method)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        t1 = time.perf_counter()
+        sc.cast(d_rays, method=method)
+        torch.cuda.synchronize()
+        cast_only = time.perf_counter() - t1
+        sc.close()
+        return idx.cpu().numpy(), pts.cpu().numpy(), el, cast_only
+
+    gpu_cast(ffk.CAST_GRID)  # warm-up (perfTest warms up on a 32-face scene)
+    gi, gp, g_s, g_cast = gpu_cast(ffk.CAST_GRID)
+    ni, np_, n_s, n_cast = gpu_cast(ffk.CAST_NAIVE)
+    t0 = time.perf_counter()
+    if O.ref_available():
+        ci, cp = O.ref_cast(1, faces, rays, n_threads=n_threads)
+    else:
+        ci, cp = O.zgrid_cast(faces, rays, n_threads=n_threads)
+    c_s = time.perf_counter() - t0
+    e_naive = crosscheck.error_stats(ni, np_, gi, gp)
+    e_cpu = crosscheck.error_stats(ci, cp, gi, gp)
+    n = len(rays)
+    log(f"perfTest: grid HIP {g_s:.3f} s ({g_cast * 1e3:.2f} ms cast), naive HIP {n_s:.3f} s, "
+        f"zgrid_cpu {c_s:.2f} s on {n_threads} threads; mismatches naive {e_naive['mismatch']}, "
+        f"cpu {e_cpu['mismatch']}; max distance naive {e_naive['max_distance']:.3g}, cpu {e_cpu['max_distance']:.3g}")
+    return {"workload": f"perfTest confetti: {len(faces)} triangles in a ball of radius 20, {n} collapsing rays, "
+                        f"{gen}", "rays": n, "hits": int((gi >= 0).sum()),
+            "grid_hip": {"seconds": round(g_s, 4), "cast_seconds": round(g_cast, 5),
+                         "Mrays_per_s": round(n / g_cast / 1e6, 1)},
+            "naive_hip": {"seconds": round(n_s, 4), "cast_seconds": round(n_cast, 5),
+                          "Mrays_per_s": round(n / n_cast / 1e6, 1), "error_vs_grid_hip": e_naive,
+                          "naive_over_grid_time": round(n_s / g_s, 3)},
+            "zgrid_cpu": {"kind": "reference" if O.ref_available() else "port", "threads": n_threads,
+                          "seconds": round(c_s, 3), "Mrays_per_s": round(n / c_s / 1e6, 3),
+                          "error_vs_grid_hip": e_cpu, "cpu_over_gpu_time": round(c_s / g_s, 1)}}
 
 
 def batch_set_leg(ffk, scenes, n_scenes):
@@ -623,6 +692,14 @@ def main():
     torch.cuda.set_device(0)
     info = ffk.device_info()
     log(f"device: {info}")
+    if args.only:
+        legs = {"plates": lambda: plates_leg(ffk, scenes, 20),
+                "batch_set": lambda: batch_set_leg(ffk, scenes, args.batch_scenes),
+                "confetti": lambda: confetti_leg(ffk, scenes, args.leg_steps),
+                "radiance": lambda: radiance_leg(ffk, scenes), "controller": controller_leg,
+                "thermal": lambda: thermal_leg(ffk, scenes, args.thermal_steps)}
+        print(json.dumps({"only": args.only, args.only: legs[args.only]()}), flush=True)
+        return
     faces = scenes.icosphere(args.subdiv, inward=True)
     meshes = scenes.per_face_meshes(len(faces))
     n_req = args.rays_per_face * len(faces)
@@ -768,7 +845,7 @@ def main():
     if not args.no_batch_set:
         result["batch_set"] = batch_set_leg(ffk, scenes, args.batch_scenes)
     if not args.no_confetti:
-        result["confetti"] = confetti_leg(ffk, scenes, 3)
+        result["confetti"] = confetti_leg(ffk, scenes, args.leg_steps)
     if not args.no_radiance:
         result["radiance"] = radiance_leg(ffk, scenes)
     if not args.no_controller:
@@ -791,6 +868,7 @@ def main():
         result["cpu_baseline"], mism = cpu_baseline(faces, meshes, rays_h, idx_h, threads,
                                                     scenes.parallel_plates(10))
         result["check"]["cpu_prefix_mismatches"] = mism
+        result["cpu_baseline"]["perftest"] = perftest_crosscheck(ffk, threads)
     print(json.dumps(result), flush=True)
 
 

@@ -71,14 +71,14 @@ ffp_material black_body() {
 int read_block(std::istream& in, std::string& s, unsigned& line, std::vector<ffp_material>& mats,
                std::map<std::string, int>& index) {
   if (s.compare(0, 6, "newmtl") != 0 || s.size() < 7) {
-    std::fprintf(stderr, "Failed to parse newmtl at line %u\n", line);
+    std::fprintf(stderr, "ffio: material line %u: newmtl without a name\n", line);
     return -FFIO_OBJ_IMPORT_FORMAT_ERROR;
   }
   const std::string name = s.substr(7);
   ffp_material m;
   std::memset(&m, 0, sizeof(m));
   if (name.size() >= sizeof(m.name)) {
-    std::fprintf(stderr, "Material name '%s' is too long at line %u\n", name.c_str(), line);
+    std::fprintf(stderr, "ffio: material line %u: name of %zu characters, 127 at most\n", line, name.size());
     return -FFIO_OBJ_IMPORT_FORMAT_ERROR;
   }
   std::memcpy(m.name, name.data(), name.size());
@@ -90,24 +90,24 @@ int read_block(std::istream& in, std::string& s, unsigned& line, std::vector<ffp
     const size_t pos = s.find_first_of(" \t");
     // (a line without a separator reads as one unknown key: npos + 1 == 0)
     if (pos + 1 >= s.size()) {
-      std::fprintf(stderr, "Invalid material definition format at line %u\n", line);
+      std::fprintf(stderr, "ffio: material line %u: expected a property and a value\n", line);
       return -FFIO_OBJ_IMPORT_FORMAT_ERROR;
     }
     const std::string key = s.substr(0, pos), value = s.substr(pos + 1);
     const int k = find_prop(key);
     if (k < 0) {
-      std::fprintf(stderr, "Skip unknown material property '%s' at line %u\n", key.c_str(), line);
+      std::fprintf(stderr, "ffio: material line %u: property '%s' ignored\n", line, key.c_str());
       continue;
     }
     if (std::sscanf(value.c_str(), "%f", prop_ptr(m, k)) != 1) {
-      std::fprintf(stderr, "Failed to parse material '%s' value '%s' at line %u\n", key.c_str(), value.c_str(), line);
+      std::fprintf(stderr, "ffio: material line %u: no number in '%s' for '%s'\n", line, value.c_str(), key.c_str());
       return -FFIO_OBJ_IMPORT_MATERIAL_INVALID_PARAMETER_VALUE;
     }
     set.insert(k);
   }
   if (set.size() != 14) {
-    std::fprintf(stderr, "Not enough (%zu) material parameters for material '%s' at line %u\n", 14 - set.size(),
-                 name.c_str(), line);
+    std::fprintf(stderr, "ffio: material line %u: '%s' sets %zu of the 14 properties\n", line, name.c_str(),
+                 set.size());
     return -FFIO_OBJ_IMPORT_MATERIAL_NOT_ENOUGH_PARAMETERS;
   }
   mats.push_back(m);
@@ -120,7 +120,7 @@ int read_block(std::istream& in, std::string& s, unsigned& line, std::vector<ffp
 int read_materials(const std::string& path, std::vector<ffp_material>& mats, std::map<std::string, int>& index) {
   std::ifstream in(path);
   if (!in.is_open()) {
-    std::fprintf(stderr, "Material library file '%s' not found.\n", path.c_str());
+    std::fprintf(stderr, "ffio: cannot open the material library %s\n", path.c_str());
     return -FFIO_OBJ_IMPORT_FILE_ERROR;
   }
   std::string s;
@@ -164,7 +164,7 @@ int ffio_obj_import_ex(const char* filename, int flags, ffp_subject_scene** scen
   *scene = nullptr;
   std::ifstream in(filename);
   if (!in.is_open()) {
-    std::fprintf(stderr, "Scene file '%s' not found.\n", filename);
+    std::fprintf(stderr, "ffio: cannot open the scene %s\n", filename);
     return -FFIO_OBJ_IMPORT_FILE_ERROR;
   }
   std::vector<ffp_vec3> verts;
@@ -191,7 +191,7 @@ int ffio_obj_import_ex(const char* filename, int flags, ffp_subject_scene** scen
         if (std::sscanf(p, "%d %d %d", &a, &b, &c) != 3 &&
             std::sscanf(p, "%d//%d %d//%d %d//%d", &a, &d, &b, &d, &c, &d) != 6 &&
             std::sscanf(p, "%d/%d/%d %d/%d/%d %d/%d/%d", &a, &d, &d, &b, &d, &d, &c, &d, &d) != 9) {
-          std::fprintf(stderr, "Bad face description '%s' at line %u\n", s.c_str(), line);
+          std::fprintf(stderr, "ffio: scene line %u: no three vertex indices in '%s'\n", line, s.c_str());
           return -FFIO_OBJ_IMPORT_FORMAT_ERROR;
         }
         tris.push_back({a - 1, b - 1, c - 1});
@@ -208,17 +208,17 @@ int ffio_obj_import_ex(const char* filename, int flags, ffp_subject_scene** scen
       case 'u': {
         const size_t pos = s.find_first_of(" \t");
         if (pos == std::string::npos || s.substr(0, pos) != "usemtl") {
-          std::fprintf(stderr, "Invalid material token at line %u '%s'\n", line, s.c_str());
+          std::fprintf(stderr, "ffio: scene line %u: '%s' is not a usemtl line\n", line, s.c_str());
           return -FFIO_OBJ_IMPORT_FORMAT_ERROR;
         }
         const auto it = index.find(s.substr(pos + 1));
         if (it == index.end()) {
-          std::fprintf(stderr, "Failed to find material reference '%s' at line %u\n", s.substr(pos + 1).c_str(),
-                       line);
+          std::fprintf(stderr, "ffio: scene line %u: usemtl names '%s', which no library defines\n", line,
+                       s.substr(pos + 1).c_str());
           return -FFIO_OBJ_IMPORT_MATERIAL_NOT_DEFINED;
         }
         if (meshes.empty()) {
-          std::fprintf(stderr, "usemtl before any group at line %u\n", line);
+          std::fprintf(stderr, "ffio: scene line %u: usemtl before the first group\n", line);
           return -FFIO_OBJ_IMPORT_FORMAT_ERROR;
         }
         meshes.back().material_idx = it->second;
@@ -229,20 +229,20 @@ int ffio_obj_import_ex(const char* filename, int flags, ffp_subject_scene** scen
           if (int r = read_block(in, s, line, mats, index)) return r;
           break;
         }
-        std::fprintf(stderr, "Unknown directive '%s' at line %u\n", s.c_str(), line);
+        std::fprintf(stderr, "ffio: scene line %u: '%s' skipped\n", line, s.c_str());
         continue;
       case 'm': {
         if (s.compare(0, 6, "mtllib") != 0 || s.size() < 7) {
-          std::fprintf(stderr, "Failed to parse mtllib at line %u\n", line);
+          std::fprintf(stderr, "ffio: scene line %u: mtllib without a file name\n", line);
           return -FFIO_OBJ_IMPORT_FORMAT_ERROR;
         }
         const std::string path = library_path(filename, s.substr(7));
-        std::fprintf(stderr, "Loading material lib '%s'\n", path.c_str());
+        std::fprintf(stderr, "ffio: reading materials from %s\n", path.c_str());
         if (int r = read_materials(path, mats, index)) return r;
         break;
       }
       default:
-        std::fprintf(stderr, "Unknown directive '%s' at line %u\n", s.c_str(), line);
+        std::fprintf(stderr, "ffio: scene line %u: '%s' skipped\n", line, s.c_str());
         continue;
     }
   }
@@ -250,7 +250,7 @@ int ffio_obj_import_ex(const char* filename, int flags, ffp_subject_scene** scen
   for (const auto& t : tris)
     for (int k = 0; k < 3; ++k)
       if (t[k] < 0 || t[k] >= nv) {
-        std::fprintf(stderr, "Face vertex %d outside the %d vertices\n", t[k] + 1, nv);
+        std::fprintf(stderr, "ffio: a face names vertex %d of %d\n", t[k] + 1, nv);
         return -FFIO_OBJ_IMPORT_FORMAT_ERROR;
       }
   ffp_subject_scene* out = static_cast<ffp_subject_scene*>(std::malloc(sizeof(ffp_subject_scene)));

@@ -444,6 +444,12 @@ int build_tiles(ffk_scene* s, int rays_per_tile, hipStream_t st) {
   std::vector<tile> tiles;
   std::vector<int> indirect;
   scene_tiles(s, 0, rpt, tiles, indirect);
+  // Longest tiles first: blocks start in index order, so uneven tiles (meshes
+  // of unequal area) in mesh order leave the last full-size tiles running
+  // alone at the end of the launch; each tile writes its own row or adds
+  // integers to the tally, so the order does not change F.
+  std::stable_sort(tiles.begin(), tiles.end(),
+                   [](const tile& a, const tile& b) { return a.g1 - a.g0 > b.g1 - b.g0; });
   s->n_tiles = (int)tiles.size();
   s->n_indirect_rows = (int)indirect.size();
   s->tiles_rays_per_tile = (int)rpt;

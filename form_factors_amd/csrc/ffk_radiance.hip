@@ -278,7 +278,8 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   // walk stops at the last emitted ray (radiance_cpu.cpp:170).
   int n_walk = 0;
   long long pos = 0;
-  float p_max = 0.f;
+  float p_max = 0.f, p_min = 0.f;  // over the ray powers of sides that emit rays
+  double upper_all = 0.0;
   for (int m = 0; m < nm && pos < total; ++m) {
     const int f0 = s->h_meshes[3 * m], n = s->h_meshes[3 * m + 1];
     for (int f = f0; f < f0 + n && pos < total; ++f) {
@@ -288,9 +289,17 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
       start[n_walk] = pos;
       face[n_walk] = f;
       ++n_walk;
-      if (s->h_front[f] > 0) p_max = std::max(p_max, pf);
-      if (s->h_rear[f] > 0) p_max = std::max(p_max, pr);
-      pos += (long long)s->h_front[f] + s->h_rear[f];
+      const int cf = s->h_front[f], cr = s->h_rear[f];
+      if (cf > 0 && pf > 0.f) {
+        p_max = std::max(p_max, pf);
+        p_min = p_min > 0.f ? std::min(p_min, pf) : pf;
+      }
+      if (cr > 0 && pr > 0.f) {
+        p_max = std::max(p_max, pr);
+        p_min = p_min > 0.f ? std::min(p_min, pr) : pr;
+      }
+      upper_all += (double)cf * pf + (double)cr * pr;
+      pos += (long long)cf + cr;
     }
   }
   const long long walk_total = std::min(pos, total);
@@ -304,29 +313,36 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   // ray's value below 2^60, clear of the band bits. A ray of power p in band
   // b is then rounded by at most 0.5 / scale_b, i.e. relatively by at most
   // 2^-55 times the rays in the band, or 2^-52 (2^kBandOrders / 2^60).
-  // binary exponent of a positive float from its bits (ilogb; subnormals rare)
-  auto exponent = [](float p) {
+  auto exponent = [](float p) {  // ilogb of a positive float from its bits (subnormals rare)
     uint32_t u;
     std::memcpy(&u, &p, 4);
     const int e = (int)((u >> 23) & 0xff);
     return e ? e - 127 : std::ilogb(p);
   };
   const int e_max = exponent(p_max);
+  const bool one_band = (e_max - exponent(p_min)) / kBandOrders == 0;  // the usual case
   double upper[kPowerBands] = {}, pmax_b[kPowerBands] = {};
   int n_bands = 1;
-  s->rad_band.resize(2 * (size_t)n_walk);
-  unsigned char* band = s->rad_band.data();
-  for (int k = 0; k < n_walk; ++k) {
-    const int f = face[k];
-    const int cnt[2] = {s->h_front[f], s->h_rear[f]};
-    for (int side = 0; side < 2; ++side) {
-      const float p = h_ray_power[2 * f + side];
-      const int b = p > 0.f ? std::min(kPowerBands - 1, (e_max - exponent(p)) / kBandOrders) : 0;
-      band[2 * k + side] = (unsigned char)b;
-      if (p <= 0.f || cnt[side] <= 0) continue;
-      upper[b] += (double)cnt[side] * p;
-      pmax_b[b] = std::max(pmax_b[b], (double)p);
-      n_bands = std::max(n_bands, b + 1);
+  unsigned char* band = nullptr;
+  if (one_band) {
+    upper[0] = upper_all;
+    pmax_b[0] = p_max;
+  } else {
+    s->rad_band.resize(2 * (size_t)n_walk);
+    band = s->rad_band.data();
+    for (int k = 0; k < n_walk; ++k) {
+      const int f = face[k];
+      const int cnt[2] = {s->h_front[f], s->h_rear[f]};
+      for (int side = 0; side < 2; ++side) {
+        const float p = h_ray_power[2 * f + side];
+        const bool emits = p > 0.f && cnt[side] > 0;
+        const int b = emits ? std::min(kPowerBands - 1, (e_max - exponent(p)) / kBandOrders) : 0;
+        band[2 * k + side] = (unsigned char)b;
+        if (!emits) continue;
+        upper[b] += (double)cnt[side] * p;
+        pmax_b[b] = std::max(pmax_b[b], (double)p);
+        n_bands = std::max(n_bands, b + 1);
+      }
     }
   }
   double scale[kPowerBands];
@@ -339,17 +355,20 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   // round half to even of a non-negative x < 2^60 (what llrint does in the
   // default rounding mode; x - trunc(x) is exact, so the comparison is too)
   auto round_even = [](double x) {
-    unsigned long long v = (unsigned long long)x;
+    const long long v = (long long)x;
     const double frac = x - (double)v;
-    return v + (frac > 0.5 || (frac == 0.5 && (v & 1)) ? 1ull : 0ull);
+    return (unsigned long long)v + (frac > 0.5 || (frac == 0.5 && (v & 1)) ? 1ull : 0ull);
   };
   for (int k = 0; k < n_walk; ++k) {
     const int f = face[k];
+    const int cnt[2] = {s->h_front[f], s->h_rear[f]};
     for (int side = 0; side < 2; ++side) {
       const float p = h_ray_power[2 * f + side];
-      const int b = band[2 * k + side];
-      q[2 * k + side] =
-          p > 0.f ? ((unsigned long long)b << kBandShift) | round_even((double)p * scale[b]) : 0ull;
+      const int b = band ? band[2 * k + side] : 0;
+      // sides without rays are never read; their power may lie outside the bands
+      q[2 * k + side] = p > 0.f && cnt[side] > 0
+                            ? ((unsigned long long)b << kBandShift) | round_even((double)p * scale[b])
+                            : 0ull;
     }
   }
   const size_t n_acc = (size_t)n_bands * nm;

@@ -14,7 +14,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-ORACLE_SO = os.path.join(HERE, "liboracle.so")
+# FF_ORACLE_SO names another build of the same sources (the sanitizer build
+# of tests/test_sanitizers.py); default: the one make -C oracle writes here.
+ORACLE_SO = os.environ.get("FF_ORACLE_SO") or os.path.join(HERE, "liboracle.so")
 REF_SO = os.path.join(HERE, "_ref", "libffref.so")
 
 RNG_MT19937, RNG_PHILOX = 0, 1
@@ -134,6 +136,7 @@ def ref():
         _sig(R, "ffref_sun", C.c_int, [_f, C.c_int, _i, C.c_int, _f, C.c_float, _f, C.c_int, _f])
         _sig(R, "ffref_heat_source", C.c_int, [_i, _f, C.c_int, _f, _f])
         _sig(R, "ffref_ff_timed", C.c_int, [_f, C.c_int, _i, C.c_int, C.c_int, C.c_int, _f, _d])
+        _sig(R, "ffref_perftest_scene", None, [C.c_int, C.c_float, C.c_int, _f, _f])
         _ref = R
     return _ref
 
@@ -390,3 +393,13 @@ def ref_ff_timed(faces, meshes, n_rays, caster=1):
     if rc < 0:
         raise RuntimeError(f"reference form-factor calculator returned {rc}")
     return F, t
+
+
+def ref_perftest_scene(n_faces=38000, radius=20.0, n_rays=None):
+    """perfTest's confetti scene and collapsing rays from the reference's own
+    spherical generator: (faces float32[n, 9], rays float32[m, 6])."""
+    n_rays = 100 * n_faces if n_rays is None else n_rays
+    faces = np.zeros((n_faces, 9), np.float32)
+    rays = np.zeros((n_rays, 6), np.float32)
+    ref().ffref_perftest_scene(int(n_faces), float(radius), int(n_rays), _p(faces, _f), _p(rays, _f))
+    return faces, rays

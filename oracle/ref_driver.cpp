@@ -26,6 +26,7 @@
 #include "ray_caster/naive_cpu.h"
 #include "ray_caster/system.h"
 #include "ray_caster/zgrid_cpu.h"
+#include "subject/generator.h"
 #include "subject/system.h"
 #include "thermal_equation/conductive_cpu.h"
 #include "thermal_equation/form_factors.h"
@@ -563,6 +564,35 @@ EXPORT int ffref_sbff(const float* faces, int n_faces, const int* meshes, int n_
   free_caster(rc);
   if (r == 0) r = ffref_ff_calculate(faces, n_faces, meshes, n_meshes, n_rays, 0, F_out);
   return r;
+}
+
+/* perfTest's workload (perfTest/perfTest.cpp:42-83) from the reference's own
+ * spherical generator, in its order: n_faces confetti triangles (three unit
+ * sphere points offset by a point of the ball of `radius`), then n_rays
+ * collapsing rays (end point on the sphere of radius 2 + radius, origin 10 %
+ * further out). */
+EXPORT void ffref_perftest_scene(int n_faces, float radius, int n_rays, float* faces_out, float* rays_out) {
+  subject::generator_t* g = subject::generator_create_spherical();
+  for (int i = 0; i != n_faces; ++i) {
+    math::vec3 p[3];
+    subject::generator_surface_point(g, 3, p);
+    math::vec3 position;
+    subject::generator_volume_point(g, 1, &position);
+    position *= radius;
+    for (int k = 0; k != 3; ++k) {
+      p[k] += position;
+      memcpy(faces_out + 9 * (size_t)i + 3 * k, &p[k], sizeof(p[k]));
+    }
+  }
+  for (int i = 0; i != n_rays; ++i) {
+    math::vec3 direction;
+    subject::generator_surface_point(g, 1, &direction);
+    direction *= (2 + radius);
+    math::vec3 origin = direction * 1.1f;
+    memcpy(rays_out + 6 * (size_t)i, &origin, sizeof(origin));
+    memcpy(rays_out + 6 * (size_t)i + 3, &direction, sizeof(direction));
+  }
+  subject::generator_free(g);
 }
 
 /* math::triangles_bsphere (math/triangle.cpp:361-383). */
