@@ -322,13 +322,20 @@ static int max_lds_bytes() {
 }
 
 // Histogram mode for a row of N meshes and tiles of at most max_tile rays.
+// Above this many bytes of 16-bit row histogram a block would hold more than
+// a third of the CU's LDS, and fewer than three blocks of 512 fit: the global
+// uint64 tally (N^2 words zeroed and normalised, one L2 atomic per hit) then
+// runs faster than the LDS histogram (measured on the 64-scene set: scenes of
+// 24,577-81,920 meshes 12-17 % faster, below that slower).
+constexpr long long kMaxDefaultHist = 49152;
+
 static int pick_hist(int N, long long max_tile, int requested) {
   const int lds = max_lds_bytes();
   if (requested == 0) return 0;
   if (requested == 1 && 4LL * N <= lds) return 1;
   if (requested == 2 && 2LL * ((N + 1) / 2) * 2 <= lds && max_tile <= 65535) return 2;
   if (4LL * N <= std::min(65536, lds)) return 1;
-  if (4LL * ((N + 1) / 2) <= lds && max_tile <= 65535) return 2;
+  if (4LL * ((N + 1) / 2) <= std::min<long long>(kMaxDefaultHist, lds) && max_tile <= 65535) return 2;
   return 0;
 }
 
@@ -492,11 +499,20 @@ struct ffk_batch {
   // launch per class, so small scenes are not run at the occupancy of the
   // largest one. Tiles of class c are d_tiles[first, first + n).
   struct klass {
+    int id;  // 0..4: 16-bit histogram of at most 8, 24, 48, 96, 160 KB; 5: global tally
     int hist;
     size_t lds;
     int first, n;
+    long long rays;
   };
   std::vector<klass> classes;
+  std::vector<long long> tile_rays;  // rays of each queued tile, in queue order
+  // Per-class launch settings (-1 = the batch-wide value), kept across
+  // re-plans; set by ffk_batch_set_class_tuning / ffk_batch_autotune.
+  struct class_tuning {
+    int block = -1, refill = -1, pool = -1, walk = -1;
+  };
+  class_tuning ctune[6];
   std::vector<char> scene_global;  // scene tallied entirely through the global tally (hist 0)
   std::vector<char> scene_tally;   // scene needs a tally buffer (global, or rows over several tiles)
   int* d_queue = nullptr;          // one counter per class
@@ -505,6 +521,9 @@ struct ffk_batch {
   float** d_Fs = nullptr;
   long long total_rays = 0;
   int block = 0, rpt = 0, n_blocks = 0;
+  size_t lds_limit = kMaxDefaultHist;  // scenes whose 16-bit row histogram is larger use the global tally
+  bool concurrent = false;             // class launches on forked streams (else back to back)
+  int order = 1;  // tile queue: 0 longest first, 1 by scene (most rays first), longest first within
   int refill = kDefaultRefill;
   int pool = kDefaultPool;
   int walk = 0;
@@ -566,9 +585,10 @@ static void free_graph(ffk_batch* b) {
 // Histogram class of a scene with N meshes (tiles of at most 65535 rays):
 // 0..4 = packed 16-bit LDS histogram of at most 8, 24, 48, 96 or 160 KB,
 // 5 = global uint64 tally.
-static int hist_class(int N) {
+static int hist_class(int N, size_t lds_limit) {
   static const size_t limits[5] = {8192, 24576, 49152, 98304, 163840};
   const size_t need = hist_bytes(2, N);
+  if (need > lds_limit) return 5;
   for (int c = 0; c < 5; ++c)
     if (need <= limits[c] && (int)need <= max_lds_bytes()) return c;
   return 5;
@@ -592,7 +612,7 @@ static int batch_plan(ffk_batch* b) {
   b->scene_global.assign(b->scenes.size(), 0);
   b->scene_tally.assign(b->scenes.size(), 0);
   for (size_t i = 0; i < b->scenes.size(); ++i) {
-    const int c = hist_class(b->scenes[i]->n_meshes);
+    const int c = hist_class(b->scenes[i]->n_meshes, b->lds_limit);
     std::vector<int> ind;
     scene_tiles(b->scenes[i], (int)i, rpt, per_class[c], ind);
     b->scene_tally[i] = c == 5 || !ind.empty();
@@ -608,18 +628,33 @@ static int batch_plan(ffk_batch* b) {
   // end of the queue.
   std::vector<tile> tiles;
   b->classes.clear();
+  std::vector<long long> scene_rays(b->scenes.size());
+  for (size_t i = 0; i < b->scenes.size(); ++i) scene_rays[i] = b->scenes[i]->n_rays;
   for (int c = 5; c >= 0; --c) {
     if (per_class[c].empty()) continue;
-    std::stable_sort(per_class[c].begin(), per_class[c].end(),
-                     [](const tile& x, const tile& y) { return (x.g1 - x.g0) > (y.g1 - y.g0); });
+    if (b->order == 1)  // one scene's tables stay hot in L2 while its tiles run
+      std::stable_sort(per_class[c].begin(), per_class[c].end(), [&](const tile& x, const tile& y) {
+        if (x.scene != y.scene)
+          return scene_rays[x.scene] != scene_rays[y.scene] ? scene_rays[x.scene] > scene_rays[y.scene]
+                                                              : x.scene < y.scene;
+        return (x.g1 - x.g0) > (y.g1 - y.g0);
+      });
+    else
+      std::stable_sort(per_class[c].begin(), per_class[c].end(),
+                       [](const tile& x, const tile& y) { return (x.g1 - x.g0) > (y.g1 - y.g0); });
     ffk_batch::klass k;
+    k.id = c;
     k.hist = c == 5 ? 0 : 2;
     k.lds = c == 5 ? 0 : class_lds[c];
     k.first = (int)tiles.size();
     k.n = (int)per_class[c].size();
+    k.rays = 0;
+    for (const tile& t : per_class[c]) k.rays += t.g1 - t.g0;
     tiles.insert(tiles.end(), per_class[c].begin(), per_class[c].end());
     b->classes.push_back(k);
   }
+  b->tile_rays.resize(tiles.size());
+  for (size_t i = 0; i < tiles.size(); ++i) b->tile_rays[i] = tiles[i].g1 - tiles[i].g0;
   b->n_tiles = (int)tiles.size();
   b->n_rows = (int)rows.size();
   b->total_rays = total;
@@ -917,6 +952,34 @@ static int batch_scenes_current(const ffk_batch* b, const char* fn) {
   return FFK_OK;
 }
 
+int ffk_batch_set_order(ffk_batch* b, int order) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  FFK_REQUIRE(order == 0 || order == 1, FFK_ERR_ARG, "tile order %d is not 0 or 1", order);
+  int rc;
+  if ((rc = batch_scenes_current(b, "ffk_batch_set_order")) < 0) return rc;
+  if (order == b->order) return FFK_OK;
+  b->order = order;
+  return batch_plan(b);
+}
+
+int ffk_batch_set_concurrent(ffk_batch* b, int on) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  b->concurrent = on != 0;
+  free_graph(b);
+  return FFK_OK;
+}
+
+int ffk_batch_set_lds_limit(ffk_batch* b, int bytes) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  FFK_REQUIRE(bytes >= 0, FFK_ERR_ARG, "LDS limit %d", bytes);
+  int rc;
+  if ((rc = batch_scenes_current(b, "ffk_batch_set_lds_limit")) < 0) return rc;
+  const size_t lim = bytes == 0 ? (size_t)kMaxDefaultHist : (size_t)bytes;
+  if (lim == b->lds_limit) return FFK_OK;
+  b->lds_limit = lim;
+  return batch_plan(b);
+}
+
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks) {
   FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
   int rc;
@@ -953,15 +1016,17 @@ static hipError_t persistent_occupancy(int hist, int block, size_t lds, int* per
   }
 }
 
-static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int n_tiles);
+static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int n_tiles, int refill, int pool,
+                        int walk);
 
 extern "C" {
 
-static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int n_tiles) {
+static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int n_tiles, int refill, int pool,
+                        int walk) {
   int n_blocks = b->n_blocks;
   if (n_blocks <= 0) {
     int per_cu = 0;
-    const int sched = sched_of(b->refill, b->pool, b->walk, hist, lds, block);
+    const int sched = sched_of(refill, pool, walk, hist, lds, block);
     const hipError_t e = sched == 3   ? persistent_occupancy<3>(hist, block, lds, &per_cu)
                          : sched == 2 ? persistent_occupancy<2>(hist, block, lds, &per_cu)
                          : sched == 1 ? persistent_occupancy<1>(hist, block, lds, &per_cu)
@@ -970,6 +1035,29 @@ static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int
     n_blocks = per_cu * device_cu_count();
   }
   return std::max(1, std::min(n_blocks, n_tiles));
+}
+
+// The persistent launch of class c over the first n_tiles of its queue
+// (its queue counter must be zero).
+static int launch_class(ffk_batch* b, int c, int n_tiles, bool stats, unsigned long long* d_stats, hipStream_t st) {
+  const ffk_batch::klass& k = b->classes[c];
+  const ffk_batch::class_tuning& ct = b->ctune[k.id];
+  const int block = ct.block > 0 ? ct.block : b->block > 0 ? b->block : default_block(k.hist, k.lds);
+  fused_args A;
+  A.descs = b->d_descs;
+  A.tiles = b->d_tiles + k.first;
+  A.n_tiles = n_tiles;
+  A.queue = b->d_queue + c;
+  A.seeds = b->d_seeds;
+  A.tallies = b->d_tallies;
+  A.Fs = b->d_Fs;
+  A.stats = d_stats;
+  A.refill = ct.refill >= 0 ? ct.refill : b->refill;
+  A.pool = ct.pool >= 0 ? ct.pool : b->pool;
+  A.walk = ct.walk >= 0 ? ct.walk : b->walk;
+  const int n_blocks = batch_blocks(b, k.hist, k.lds, block, n_tiles, A.refill, A.pool, A.walk);
+  FFK_HIP(launch_fused(k.hist, true, stats, dim3((unsigned)n_blocks), block, k.lds, st, A));
+  return FFK_OK;
 }
 
 static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, hipStream_t st) {
@@ -984,7 +1072,8 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
     dim3 grid((unsigned)b->n_rows);
     k_zero_rows_batch<<<grid, 256, 0, st>>>(b->d_rows, b->n_rows, b->d_descs, b->d_tallies);
   }
-  if (n_classes > 1) {
+  const bool fork = n_classes > 1 && b->concurrent;
+  if (fork) {
     while ((int)b->fork.size() < n_classes) {
       hipStream_t f = nullptr;
       hipEvent_t e = nullptr;
@@ -997,28 +1086,14 @@ static int batch_enqueue(ffk_batch* b, bool stats, unsigned long long* d_stats, 
     FFK_HIP(hipEventRecord(b->ev_fork, st));
   }
   for (int c = 0; c < n_classes; ++c) {
-    const ffk_batch::klass& k = b->classes[c];
-    const int block = b->block > 0 ? b->block : default_block(k.hist, k.lds);
     hipStream_t cs = st;
-    if (n_classes > 1) {
+    if (fork) {
       cs = b->fork[c];
       FFK_HIP(hipStreamWaitEvent(cs, b->ev_fork, 0));
     }
-    fused_args A;
-    A.descs = b->d_descs;
-    A.tiles = b->d_tiles + k.first;
-    A.n_tiles = k.n;
-    A.queue = b->d_queue + c;
-    A.seeds = b->d_seeds;
-    A.tallies = b->d_tallies;
-    A.Fs = b->d_Fs;
-    A.stats = d_stats;
-    A.refill = b->refill;
-    A.pool = b->pool;
-    A.walk = b->walk;
-    const int n_blocks = batch_blocks(b, k.hist, k.lds, block, k.n);
-    FFK_HIP(launch_fused(k.hist, true, stats, dim3((unsigned)n_blocks), block, k.lds, cs, A));
-    if (n_classes > 1) {
+    int rc;
+    if ((rc = launch_class(b, c, b->classes[c].n, stats, d_stats, cs)) < 0) return rc;
+    if (fork) {
       FFK_HIP(hipEventRecord(b->ev_join[c], cs));
       FFK_HIP(hipStreamWaitEvent(st, b->ev_join[c], 0));
     }
@@ -1100,6 +1175,84 @@ int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long l
   }
   FFK_HIP(hipGraphLaunch(b->exec, st));
   return FFK_OK;
+}
+
+int ffk_batch_signature(const ffk_batch* b, unsigned long long* key) {
+  FFK_REQUIRE(b && key, FFK_ERR_ARG, "ffk_batch_signature: bad args");
+  unsigned long long h = 1469598103934665603ull;
+  auto mix = [&h](const void* p, size_t n) {
+    const unsigned char* c = static_cast<const unsigned char*>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+  };
+  for (const ffk_scene* s : b->scenes) {
+    mix(s->h_faces.data(), s->h_faces.size() * 4);
+    mix(s->h_meshes.data(), s->h_meshes.size() * 4);
+    mix(s->h_front.data(), s->h_front.size() * 4);
+    mix(s->h_rear.data(), s->h_rear.size() * 4);
+    mix(&s->frame, sizeof(s->frame));
+    const int dims[3] = {s->grid.n[0], s->grid.n[1], s->grid.n[2]};
+    mix(dims, sizeof(dims));
+  }
+  const long long settings[6] = {(long long)b->lds_limit, b->order, b->concurrent ? 1 : 0, b->rpt, b->block,
+                                 b->n_blocks};
+  mix(settings, sizeof(settings));
+  *key = h;
+  return FFK_OK;
+}
+
+int ffk_batch_classes(const ffk_batch* b, int* ids, long long* rays, int* n_tiles, int capacity) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  const int n = (int)b->classes.size();
+  for (int c = 0; c < n && c < capacity; ++c) {
+    if (ids) ids[c] = b->classes[c].id;
+    if (rays) rays[c] = b->classes[c].rays;
+    if (n_tiles) n_tiles[c] = b->classes[c].n;
+  }
+  return n;
+}
+
+int ffk_batch_set_class_tuning(ffk_batch* b, int class_id, int block_size, int refill, int pool, int walk) {
+  FFK_REQUIRE(b && class_id >= 0 && class_id < 6, FFK_ERR_ARG, "ffk_batch_set_class_tuning: class %d", class_id);
+  FFK_REQUIRE(block_size <= 0 || (block_size % 64 == 0 && block_size <= 1024), FFK_ERR_ARG, "block size %d",
+              block_size);
+  FFK_REQUIRE(refill <= 64 && pool <= 64 && walk <= 1, FFK_ERR_ARG, "refill %d / pool %d / walk %d", refill, pool,
+              walk);
+  ffk_batch::class_tuning& t = b->ctune[class_id];
+  t.block = block_size > 0 ? block_size : -1;
+  t.refill = refill;
+  t.pool = pool;
+  t.walk = walk;
+  free_graph(b);
+  return FFK_OK;
+}
+
+int ffk_batch_get_class_tuning(const ffk_batch* b, int class_id, int* block_size, int* refill, int* pool,
+                               int* walk) {
+  FFK_REQUIRE(b && class_id >= 0 && class_id < 6, FFK_ERR_ARG, "ffk_batch_get_class_tuning: class %d", class_id);
+  const ffk_batch::class_tuning& t = b->ctune[class_id];
+  if (block_size) *block_size = t.block;
+  if (refill) *refill = t.refill;
+  if (pool) *pool = t.pool;
+  if (walk) *walk = t.walk;
+  return FFK_OK;
+}
+
+int ffk_batch_run_class(ffk_batch* b, int class_id, long long max_rays, void* stream) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  int rc;
+  if ((rc = batch_scenes_current(b, "ffk_batch_run_class")) < 0) return rc;
+  FFK_REQUIRE(!b->g_Fs.empty(), FFK_ERR_ARG, "ffk_batch_run_class: run the batch once first (binds its buffers)");
+  int c = -1;
+  for (int i = 0; i < (int)b->classes.size(); ++i)
+    if (b->classes[i].id == class_id) c = i;
+  FFK_REQUIRE(c >= 0, FFK_ERR_ARG, "ffk_batch_run_class: the batch has no class %d", class_id);
+  const ffk_batch::klass& k = b->classes[c];
+  int n = 0;
+  long long sum = 0;
+  while (n < k.n && (n == 0 || max_rays <= 0 || sum < max_rays)) sum += b->tile_rays[k.first + n++];
+  hipStream_t st = as_stream(stream);
+  k_fill_i32<<<1, 64, 0, st>>>(b->d_queue + c, 1, 0);
+  return launch_class(b, c, n, false, nullptr, st);
 }
 
 }  // extern "C"

@@ -10,15 +10,28 @@
 // then timed with the fused kernel itself on the full layout, and the
 // fused kernel's block size, walk mode and ray scheduling (refill threshold,
 // pooled-test ratio) after that. The choice is cached per scene
-// signature (faces, meshes, ray counts, frame) for the life of the process.
+// signature (faces, meshes, ray counts, frame) and device, in memory and in
+// a text file (ffk_tune_cache_path), so that later processes skip the
+// timing.
+//
+// The batch tuner (ffk_batch_autotune) times each LDS class of a batch alone
+// over a sample of its tile queue, greedily over the walk mode, the ray
+// scheduling and the block size, and keeps a change only when it wins by 3 %
+// (repeated timings of one setting spread by 1-2 %); its results are cached
+// the same way under the batch's signature.
 #include <hip/hip_runtime_api.h>
+
+#include <sys/stat.h>
 
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "ffk.h"
@@ -38,13 +51,88 @@ struct tuned {
 // suit scenes where rays meet many occupied cells.
 constexpr int kSchedules[][2] = {{32, 2}, {24, 3}, {16, 4}};
 
-std::mutex g_mu;
-std::map<unsigned long long, tuned> g_cache;
-
 unsigned long long fnv1a(const void* p, size_t n, unsigned long long h) {
   const unsigned char* b = static_cast<const unsigned char*>(p);
   for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
   return h;
+}
+
+// ---- results cache: in memory, backed by a text file ---------------------------
+// One line per result, "<kind> <key in hex> <n> <n integers>", appended as
+// results are found; when the file is read, the last line of a key wins.
+// kind "scene": dims0 dims1 dims2 block walk refill pool; kind "batch": per
+// class id block refill pool walk.
+std::mutex g_mu;
+std::map<std::pair<std::string, unsigned long long>, std::vector<int>> g_results;
+bool g_loaded = false;
+
+std::string cache_file() {
+  const char* env = std::getenv("FFK_TUNE_CACHE");
+  if (env) return (!*env || !std::strcmp(env, "off")) ? std::string() : std::string(env);
+  const char* xdg = std::getenv("XDG_CACHE_HOME");
+  const char* home = std::getenv("HOME");
+  std::string dir = xdg && *xdg ? std::string(xdg) : home && *home ? std::string(home) + "/.cache" : std::string();
+  return dir.empty() ? std::string() : dir + "/form_factors_amd/autotune.txt";
+}
+
+// mkdir -p of the file's directory (best effort: without it there is no file cache)
+void make_dirs(const std::string& file) {
+  for (size_t pos = file.find('/', 1); pos != std::string::npos; pos = file.find('/', pos + 1))
+    (void)mkdir(file.substr(0, pos).c_str(), 0755);
+}
+
+void load_locked() {
+  if (g_loaded) return;
+  g_loaded = true;
+  const std::string path = cache_file();
+  if (path.empty()) return;
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return;
+  char kind[16];
+  unsigned long long key;
+  int n;
+  while (std::fscanf(f, "%15s %llx %d", kind, &key, &n) == 3) {
+    if (n < 0 || n > 4096) break;
+    std::vector<int> v(n);
+    bool ok = true;
+    for (int i = 0; i < n && ok; ++i) ok = std::fscanf(f, "%d", &v[i]) == 1;
+    if (!ok) break;
+    g_results[{kind, key}] = v;
+  }
+  std::fclose(f);
+}
+
+bool cache_get(const char* kind, unsigned long long key, std::vector<int>& v) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  load_locked();
+  auto it = g_results.find({kind, key});
+  if (it == g_results.end()) return false;
+  v = it->second;
+  return true;
+}
+
+void cache_put(const char* kind, unsigned long long key, const std::vector<int>& v) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  load_locked();
+  g_results[{kind, key}] = v;
+  const std::string path = cache_file();
+  if (path.empty()) return;
+  make_dirs(path);
+  FILE* f = std::fopen(path.c_str(), "a");
+  if (!f) return;
+  std::fprintf(f, "%s %016llx %d", kind, key, (int)v.size());
+  for (int x : v) std::fprintf(f, " %d", x);
+  std::fprintf(f, "\n");
+  std::fclose(f);
+}
+
+// The current device's identity in cache keys: results depend on it.
+unsigned long long device_key(unsigned long long h) {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return h;
+  h = fnv1a(prop.gcnArchName, std::strlen(prop.gcnArchName), h);
+  return fnv1a(&prop.multiProcessorCount, sizeof(prop.multiProcessorCount), h);
 }
 
 // ms of fn() on stream st, best of `reps`.
@@ -103,13 +191,13 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
   key = fnv1a(front.data(), front.size() * 4, key);
   key = fnv1a(rear.data(), rear.size() * 4, key);
   key = fnv1a(&frame, sizeof(frame), key);
+  key = device_key(key);
   tuned t;
   bool hit = false;
   {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_cache.find(key);
-    if (it != g_cache.end()) {
-      t = it->second;
+    std::vector<int> v;
+    if (cache_get("scene", key, v) && v.size() == 7) {
+      t = tuned{{v[0], v[1], v[2]}, v[3], v[4], v[5], v[6]};
       hit = true;
     }
   }
@@ -257,8 +345,7 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
       if (r < 0) return r;
     }
     if (!fused_timed && (r = ffk_scene_build_grid(s, t.dims[0], t.dims[1], t.dims[2], stream)) < 0) return r;
-    std::lock_guard<std::mutex> lk(g_mu);
-    g_cache[key] = t;
+    cache_put("scene", key, {t.dims[0], t.dims[1], t.dims[2], t.block, t.walk, t.refill, t.pool});
   } else {
     if ((r = ffk_scene_build_grid(s, t.dims[0], t.dims[1], t.dims[2], stream)) < 0) return r;
   }
@@ -269,6 +356,90 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
   if (out_block) *out_block = t.block;
   if (out_walk) *out_walk = t.walk;
   return hit ? 1 : FFK_OK;
+}
+
+int ffk_tune_cache_path(char* buf, int capacity) {
+  const std::string p = cache_file();
+  if (buf && capacity > 0) {
+    std::strncpy(buf, p.c_str(), (size_t)capacity - 1);
+    buf[capacity - 1] = 0;
+  }
+  return (int)p.size();
+}
+
+void ffk_tune_cache_reset(void) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_results.clear();
+  g_loaded = false;
+}
+
+int ffk_batch_autotune(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
+                       float* const* d_Fs, long long sample_rays, void* stream) {
+  if (!b || !seeds || !d_Fs) return FFK_ERR_ARG;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  int r;
+  // the untuned run: binds the buffers (and gives valid results)
+  if ((r = ffk_batch_run(b, seeds, d_tallies, d_Fs, 0, nullptr, stream)) < 0) return r;
+  const int n_classes = ffk_batch_classes(b, nullptr, nullptr, nullptr, 0);
+  if (n_classes < 0) return n_classes;
+  std::vector<int> ids(n_classes), tiles(n_classes);
+  std::vector<long long> rays(n_classes);
+  if ((r = ffk_batch_classes(b, ids.data(), rays.data(), tiles.data(), n_classes)) < 0) return r;
+  unsigned long long key = 0;
+  if ((r = ffk_batch_signature(b, &key)) < 0) return r;
+  key = device_key(fnv1a(&sample_rays, sizeof(sample_rays), key));
+  std::vector<int> v;
+  if (cache_get("batch", key, v) && v.size() == 5 * (size_t)n_classes) {
+    for (int c = 0; c < n_classes; ++c)
+      if ((r = ffk_batch_set_class_tuning(b, v[5 * c], v[5 * c + 1], v[5 * c + 2], v[5 * c + 3], v[5 * c + 4])) < 0)
+        return r;
+    return 1;
+  }
+  v.clear();
+  for (int c = 0; c < n_classes; ++c) {
+    const int id = ids[c];
+    struct cfg {
+      int block, refill, pool, walk;
+    };
+    // the batch-wide defaults, spelled out: automatic block, 32/2, walk 0
+    cfg best = {-1, 32, 2, 0};
+    float best_ms = 3.0e38f;
+    auto trial = [&](const cfg& x) -> int {
+      int rr = ffk_batch_set_class_tuning(b, id, x.block, x.refill, x.pool, x.walk);
+      if (rr < 0) return rr;
+      float ms = 0.f;
+      rr = ffk_batch_run_class(b, id, sample_rays, stream);  // warm-up of this configuration
+      if (rr == FFK_OK) rr = time_ms(st, 2, [&] { return ffk_batch_run_class(b, id, sample_rays, stream); }, &ms);
+      if (rr == FFK_ERR_HIP) {  // a launch shape the hardware refuses (LDS): not a candidate
+        (void)hipGetLastError();
+        return FFK_OK;
+      }
+      if (rr < 0) return rr;
+      // the first trial (the defaults) sets the bar; a change must win by 3 %
+      // on the sample, more than the spread of repeated timings
+      if (best_ms > 1.0e38f || ms < 0.97f * best_ms) {
+        best_ms = ms;
+        best = x;
+      }
+      return FFK_OK;
+    };
+    if ((r = trial(best)) < 0) return r;
+    const cfg base = best;
+    if ((r = trial({base.block, base.refill, base.pool, 1})) < 0) return r;
+    const cfg after_walk = best;
+    for (const auto& sch : kSchedules) {
+      if (sch[0] == after_walk.refill && sch[1] == after_walk.pool) continue;
+      if ((r = trial({after_walk.block, sch[0], sch[1], after_walk.walk})) < 0) return r;
+    }
+    const cfg after_sched = best;
+    for (int block : {256, 512, 1024})
+      if ((r = trial({block, after_sched.refill, after_sched.pool, after_sched.walk})) < 0) return r;
+    if ((r = ffk_batch_set_class_tuning(b, id, best.block, best.refill, best.pool, best.walk)) < 0) return r;
+    v.insert(v.end(), {id, best.block, best.refill, best.pool, best.walk});
+  }
+  (void)hipStreamSynchronize(st);
+  cache_put("batch", key, v);
+  return FFK_OK;
 }
 
 }  // extern "C"

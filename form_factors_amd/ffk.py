@@ -63,6 +63,17 @@ SIGNATURES = {
     "ffk_batch_destroy": (None, [_vp]),
     "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
     "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ffk_batch_set_lds_limit": (C.c_int, [_vp, C.c_int]),
+    "ffk_batch_set_concurrent": (C.c_int, [_vp, C.c_int]),
+    "ffk_batch_set_order": (C.c_int, [_vp, C.c_int]),
+    "ffk_batch_classes": (C.c_int, [_vp, _i, C.POINTER(_ll), _i, C.c_int]),
+    "ffk_batch_signature": (C.c_int, [_vp, C.POINTER(_ull)]),
+    "ffk_batch_set_class_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "ffk_batch_get_class_tuning": (C.c_int, [_vp, C.c_int, _i, _i, _i, _i]),
+    "ffk_batch_run_class": (C.c_int, [_vp, C.c_int, _ll, _vp]),
+    "ffk_batch_autotune": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), _ll, _vp]),
+    "ffk_tune_cache_path": (C.c_int, [C.c_char_p, C.c_int]),
+    "ffk_tune_cache_reset": (None, []),
     "ffk_radiance": (C.c_int, [_vp, _ull, _f, C.POINTER(C.c_double), _vp, _vp]),
     "ffk_radiance_step": (C.c_int, [_vp, _f, _f, C.c_int, C.c_int, _ull, _f, C.POINTER(C.c_double), _vp, _vp]),
     "ffk_radiance_weights": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _f, _f]),
@@ -467,6 +478,18 @@ class Batch:
     def set_refill(self, lanes, pool=2):
         check(lib().ffk_batch_set_refill(self.h, int(lanes), int(pool)), "ffk_batch_set_refill")
 
+    def set_concurrent(self, on):
+        """Class launches on forked streams (True) or one after another (False)."""
+        check(lib().ffk_batch_set_concurrent(self.h, 1 if on else 0), "ffk_batch_set_concurrent")
+
+    def set_order(self, order):
+        """Tile queue: 0 longest first across scenes, 1 scene by scene."""
+        check(lib().ffk_batch_set_order(self.h, int(order)), "ffk_batch_set_order")
+
+    def set_lds_limit(self, nbytes):
+        """Scenes whose row histogram needs more LDS use the global tally (0 = default)."""
+        check(lib().ffk_batch_set_lds_limit(self.h, int(nbytes)), "ffk_batch_set_lds_limit")
+
     def set_walk(self, mode):
         check(lib().ffk_batch_set_walk(self.h, int(mode)), "ffk_batch_set_walk")
 
@@ -482,6 +505,36 @@ class Batch:
 
         return [torch.empty((sc.n_meshes,) * 2, dtype=torch.int64, device="cuda") if need else None
                 for sc, need in zip(self.scenes, self.tally_flags())]
+
+    def classes(self):
+        """[(class id, rays, tiles)] in launch order."""
+        n = lib().ffk_batch_classes(self.h, None, None, None, 0)
+        ids, tiles = np.zeros(n, np.int32), np.zeros(n, np.int32)
+        rays = np.zeros(n, np.int64)
+        check(lib().ffk_batch_classes(self.h, ids.ctypes.data_as(_i), rays.ctypes.data_as(C.POINTER(_ll)),
+                                      tiles.ctypes.data_as(_i), n), "ffk_batch_classes")
+        return [(int(a), int(b), int(c)) for a, b, c in zip(ids, rays, tiles)]
+
+    def class_tuning(self, class_id):
+        v = [C.c_int() for _ in range(4)]
+        check(lib().ffk_batch_get_class_tuning(self.h, int(class_id), *[C.byref(x) for x in v]),
+              "ffk_batch_get_class_tuning")
+        return dict(zip(("block", "refill", "pool", "walk"), (x.value for x in v)))
+
+    def set_class_tuning(self, class_id, block=-1, refill=-1, pool=-1, walk=-1):
+        check(lib().ffk_batch_set_class_tuning(self.h, int(class_id), int(block), int(refill), int(pool), int(walk)),
+              "ffk_batch_set_class_tuning")
+
+    def autotune(self, seeds, Fs, tallies=None, sample_rays=1 << 28):
+        """Per-class tuning (ffk_batch_autotune); returns {class id: settings}
+        and whether they came from a cache."""
+        n = len(self.scenes)
+        sd = (C.c_ulonglong * n)(*[int(x) for x in seeds])
+        fp = (C.c_void_p * n)(*[F.data_ptr() for F in Fs])
+        tp = None if tallies is None else (C.c_void_p * n)(*[(t.data_ptr() if t is not None else 0)
+                                                            for t in tallies])
+        rc = check(lib().ffk_batch_autotune(self.h, sd, tp, fp, int(sample_rays), _stream()), "ffk_batch_autotune")
+        return {cid: self.class_tuning(cid) for cid, _, _ in self.classes()}, rc == 1
 
     def run(self, seeds, Fs, tallies=None, use_graph=True, stats=None):
         n = len(self.scenes)
@@ -611,3 +664,10 @@ class Thermal:
             self.close()
         except Exception:
             pass
+
+
+def tune_cache_path():
+    """The autotune cache file ('' when disabled)."""
+    buf = C.create_string_buffer(4096)
+    lib().ffk_tune_cache_path(buf, 4096)
+    return buf.value.decode()

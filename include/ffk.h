@@ -177,13 +177,15 @@ int ffk_set_walk(ffk_scene* s, int mode);
 int ffk_batch_set_walk(ffk_batch* b, int mode);
 
 /* 1 when ffk_ff_fused needs d_tally for this scene and tuning (the rays of a
- * mesh span several tiles, or the row histogram does not fit in LDS), else 0. */
+ * mesh span several tiles, or the row histogram is tallied globally: by
+ * default when a 16-bit histogram would take more than 48 KB of LDS, i.e.
+ * more than 24,576 meshes), else 0. */
 int ffk_ff_needs_tally(const ffk_scene* s);
 
-/* Launch tuning of the fused kernel (0 = automatic; hist_mode -1 = automatic):
- * threads per block, rays per tile (<= 65535), row histogram (0 global
- * atomics, 1 uint32 LDS, 2 uint16 LDS) and the number of waiting lanes
- * (1..64) at which a wavefront fetches new rays. */
+/* Launch tuning of the fused kernel (0 = automatic; hist_mode -1 = automatic:
+ * uint32 LDS up to 16,384 meshes, uint16 LDS up to 24,576, global atomics
+ * above): threads per block, rays per tile (<= 65535) and the row histogram
+ * (0 global atomics, 1 uint32 LDS, 2 uint16 LDS, whenever it fits). */
 int ffk_set_tuning(ffk_scene* s, int block_size, int rays_per_tile, int hist_mode);
 
 /* ---- batched form factors ---------------------------------------------------- */
@@ -204,6 +206,49 @@ void ffk_batch_destroy(ffk_batch* b);
 int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream);
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
+/* Scenes whose 16-bit row histogram would exceed `bytes` of LDS are tallied
+ * in the global uint64 tally instead (more resident wavefronts per CU, at
+ * the cost of N^2 tally traffic); 0 restores the default (48 KB, the rule
+ * of the single-scene launch). Changes which scenes need a tally buffer
+ * (ffk_batch_tally_flags). */
+int ffk_batch_set_lds_limit(ffk_batch* b, int bytes);
+/* on != 0: the launches of the LDS classes go on forked streams and may
+ * overlap; 0 (default): one after another on the launch stream. */
+int ffk_batch_set_concurrent(ffk_batch* b, int on);
+/* Tile queue order inside an LDS class: 0 longest tiles first across all
+ * scenes; 1 (default) scene by scene (most rays first), longest tiles first
+ * within a scene, which keeps one scene's grid and face tables in L2. */
+int ffk_batch_set_order(ffk_batch* b, int order);
+/* The batch's LDS classes (one persistent launch each, run in this order):
+ * ids[c] (0..4: 16-bit row histogram of at most 8, 24, 48, 96, 160 KB; 5:
+ * global tally), their rays and queued tiles; returns the number of classes
+ * (arrays filled up to capacity; any may be NULL). */
+int ffk_batch_classes(const ffk_batch* b, int* ids, long long* rays, int* n_tiles, int capacity);
+/* A 64-bit signature of the batch (every scene's faces, meshes, ray counts,
+ * frame and grid shape, and the batch-wide settings): the key of its tuning
+ * results. */
+int ffk_batch_signature(const ffk_batch* b, unsigned long long* key);
+/* Launch settings of one class (by id), overriding the batch-wide ones; -1 =
+ * the batch-wide value (block_size <= 0 likewise). None changes results. */
+int ffk_batch_set_class_tuning(ffk_batch* b, int class_id, int block_size, int refill, int pool, int walk);
+int ffk_batch_get_class_tuning(const ffk_batch* b, int class_id, int* block_size, int* refill, int* pool,
+                               int* walk);
+/* Tuning probe: the persistent launch of one class over the first tiles of
+ * its queue holding at least max_rays rays (<= 0: all), with the buffers of
+ * the last ffk_batch_run. It leaves partial sums in F and the tallies: the
+ * next ffk_batch_run recomputes them. */
+int ffk_batch_run_class(ffk_batch* b, int class_id, long long max_rays, void* stream);
+/* Per-class autotuning of the batch: one untuned ffk_batch_run with these
+ * buffers (its results are valid), then for each class the walk mode, the
+ * ray scheduling (refill/pool 32/2, 24/3, 16/4) and the block size (256,
+ * 512, 1024 where the histogram allows), timed greedily with
+ * ffk_batch_run_class on about sample_rays of the class; a setting replaces
+ * the defaults only when it is 3 % faster there. The winners are set with
+ * ffk_batch_set_class_tuning. Results are cached per scene set (the
+ * scenes' signatures and the device) in memory and in the tuning cache file
+ * (ffk_tune_cache_path). Returns 0 when tuned, 1 when taken from a cache. */
+int ffk_batch_autotune(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
+                       float* const* d_Fs, long long sample_rays, void* stream);
 /* h_flags[i] = 1 when scene i needs a tally buffer in ffk_batch_run, else 0. */
 int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags);
 
@@ -222,6 +267,16 @@ int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags);
  * NULL (block 0 = the automatic choice was kept). */
 int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_block, int* out_walk,
                  void* stream);
+/* The tuning cache file shared by ffk_autotune and ffk_batch_autotune: the
+ * path in FFK_TUNE_CACHE if set ("" or "off": no file), else
+ * $XDG_CACHE_HOME/form_factors_amd/autotune.txt or
+ * $HOME/.cache/form_factors_amd/autotune.txt. Text, one result per line,
+ * appended as results are found; the last line for a key wins. Writes the
+ * path (empty when there is no file) into buf; returns its length. */
+int ffk_tune_cache_path(char* buf, int capacity);
+/* Forgets the in-memory results, so that the next autotune call reads the
+ * file again. */
+void ffk_tune_cache_reset(void);
 
 /* ---- radiance equation --------------------------------------------------------
  * radiance_equation::calculate (thermal_equation/radiance_cpu.cpp:132-190) on

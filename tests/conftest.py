@@ -12,6 +12,12 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X) and libff_amd.so")
+    # autotune results of this session go to a file of its own, so that no
+    # result of an earlier run is taken from the user's tuning cache
+    if "FFK_TUNE_CACHE" not in os.environ:
+        import tempfile
+
+        os.environ["FFK_TUNE_CACHE"] = os.path.join(tempfile.mkdtemp(prefix="ffk_tune_"), "autotune.txt")
 
 
 def golden(name):

@@ -181,3 +181,49 @@ def test_batch_refuses_a_rebuilt_scene(gpu):
         for sc, sd, F in zip(scs, [1, 2], Fs):
             assert np.array_equal(bits(F.cpu().numpy()), bits(sc.ff_fused(sd).cpu().numpy()))
         b.close()
+
+
+def test_batch_autotune_keeps_bits_and_caches_on_disk(gpu, tmp_path, monkeypatch):
+    """Per-class tuning (walk, ray scheduling, block) changes no F bit; its
+    result is written to the tuning cache file and, once the in-memory
+    results are dropped, read back from it for a new batch of the same
+    scenes. The single-scene autotuner shares the file."""
+    import torch
+
+    cache = tmp_path / "autotune.txt"
+    monkeypatch.setenv("FFK_TUNE_CACHE", str(cache))
+    gpu.lib().ffk_tune_cache_reset()
+    assert gpu.tune_cache_path() == str(cache)
+    scs = build_scenes(gpu, [300, 900, 120, 2500], [200, 60, 400, 30], seed0=71)
+    seeds = [5, 6, 7, 8]
+    plain = [sc.ff_fused(s).cpu().numpy() for sc, s in zip(scs, seeds)]
+    b = gpu.Batch(scs)
+    Fs = [torch.full((sc.n_meshes,) * 2, -1.0, device="cuda") for sc in scs]
+    tallies = b.make_tallies()
+    print("batch autotune", flush=True)
+    tuned, cached = b.autotune(seeds, Fs, tallies, sample_rays=1 << 18)
+    assert not cached and len(tuned) == len(b.classes())
+    torch.cuda.synchronize()
+    for F, want in zip(Fs, plain):  # the autotuner's own (untuned) run
+        assert np.array_equal(bits(F.cpu().numpy()), bits(want))
+    for F in Fs:
+        F.fill_(-1.0)
+    b.run(seeds, Fs, tallies=tallies, use_graph=True)
+    torch.cuda.synchronize()
+    for F, want in zip(Fs, plain):
+        assert np.array_equal(bits(F.cpu().numpy()), bits(want))
+    b.close()
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and lines[0].startswith("batch ")
+    gpu.lib().ffk_tune_cache_reset()
+    b2 = gpu.Batch(scs)
+    tuned2, cached2 = b2.autotune(seeds, Fs, b2.make_tallies(), sample_rays=1 << 18)
+    assert cached2 and tuned2 == tuned
+    b2.close()
+    # the single-scene tuner: timed once, then read back from the file
+    sc = scs[1]
+    t1 = sc.autotune(sample_rays=1 << 16)
+    assert not t1["cached"] and sum(l.startswith("scene ") for l in cache.read_text().splitlines()) == 1
+    gpu.lib().ffk_tune_cache_reset()
+    t2 = sc.autotune(sample_rays=1 << 16)
+    assert t2["cached"] and t2["dims"] == t1["dims"] and t2["block"] == t1["block"]
