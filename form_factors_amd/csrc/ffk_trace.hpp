@@ -499,23 +499,12 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         nref = ci.y;
       }
     }
-    const bool empty = step && ci.y <= 0;  // then ci.x holds the empty boxes
-    const int kbox = empty ? empty_box(ci.x, osh) : 0;
-    const bool jump = WALK == 1 || __ballot(kbox >= 2) != 0ull;
-    if (empty) {
-      const unsigned long long key = best[lane];
-      float bd;
-      int bf;
-      decode_key(key, bd, bf);
-      const bool on = jump ? coef_leave<true>(S.g, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, kbox - 1,
-                                              cx, cy, cz, bd, bf)
-                           : coef_leave<false>(S.g, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, 0, cx, cy,
-                                               cz, bd, bf);
-      if (!on) {
-        finish(cur, o, dir, key);
-        have = false;
-      }
-    }
+    // Lanes in an empty cell leave it below, together with the lanes whose
+    // pooled tests finish in this iteration: one exit computation per
+    // iteration for all of them (R = 0 for a tested cell: the box code then
+    // takes exactly the DDA step).
+    bool leave = step && ci.y <= 0;  // then ci.x holds the empty boxes
+    int R = leave ? empty_box(ci.x, osh) - 1 : 0;
     const unsigned long long waiting = __ballot(have && pend);
     const unsigned long long active = __ballot(have);
     if (waiting != 0ull && __popcll(waiting) * pool >= __popcll(active)) {
@@ -550,14 +539,23 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       if (have && pend) {
         pend = false;
-        const unsigned long long key = best[lane];
-        float bd;
-        int bf;
-        decode_key(key, bd, bf);
-        if (!coef_leave<false>(S.g, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, 0, cx, cy, cz, bd, bf)) {
-          finish(cur, o, dir, key);
-          have = false;
-        }
+        leave = true;
+        R = 0;
+      }
+    }
+    const bool jump = WALK == 1 || __ballot(leave && R >= 1) != 0ull;
+    if (leave) {
+      const unsigned long long key = best[lane];
+      float bd;
+      int bf;
+      decode_key(key, bd, bf);
+      const bool on = jump ? coef_leave<true>(S.g, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, R, cx, cy,
+                                              cz, bd, bf)
+                           : coef_leave<false>(S.g, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, 0, cx, cy,
+                                               cz, bd, bf);
+      if (!on) {
+        finish(cur, o, dir, key);
+        have = false;
       }
     }
   }
