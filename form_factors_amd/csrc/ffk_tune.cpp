@@ -434,6 +434,22 @@ int ffk_batch_autotune(ffk_batch* b, const unsigned long long* seeds, unsigned l
     const cfg after_sched = best;
     for (int block : {256, 512, 1024})
       if ((r = trial({block, after_sched.refill, after_sched.pool, after_sched.walk})) < 0) return r;
+    // The sample is the head of the queue (the class's largest scenes); a
+    // change is kept only if it also wins over the whole class.
+    const bool changed = best.block != base.block || best.refill != base.refill || best.pool != base.pool ||
+                         best.walk != base.walk;
+    if (changed && sample_rays > 0 && sample_rays < rays[c]) {
+      float full[2] = {0.f, 0.f};
+      const cfg pair[2] = {base, best};
+      for (int k = 0; k < 2 && r == FFK_OK; ++k) {
+        if ((r = ffk_batch_set_class_tuning(b, id, pair[k].block, pair[k].refill, pair[k].pool, pair[k].walk)) < 0)
+          return r;
+        if ((r = ffk_batch_run_class(b, id, 0, stream)) < 0) return r;
+        r = time_ms(st, 1, [&] { return ffk_batch_run_class(b, id, 0, stream); }, &full[k]);
+      }
+      if (r < 0) return r;
+      if (!(full[1] < 0.99f * full[0])) best = base;
+    }
     if ((r = ffk_batch_set_class_tuning(b, id, best.block, best.refill, best.pool, best.walk)) < 0) return r;
     v.insert(v.end(), {id, best.block, best.refill, best.pool, best.walk});
   }

@@ -6,6 +6,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 
 import numpy as np
 
@@ -72,7 +73,10 @@ def libc():
         _libc = C.CDLL(None)
         _libc.fopen.restype = C.c_void_p
         _libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        _libc.fdopen.restype = C.c_void_p
+        _libc.fdopen.argtypes = [C.c_int, C.c_char_p]
         _libc.fclose.argtypes = [C.c_void_p]
+        _libc.fflush.argtypes = [C.c_void_p]
         _libc.free.argtypes = [C.c_void_p]
     return _libc
 
@@ -194,6 +198,35 @@ def task_export(path, frames, binary=False):
             r = fn(f, len(T), C.byref(t))
             if r != 0:
                 raise FormatError(r, "ffio_obj_export_task")
+
+
+class FrameWriter:
+    """Frames written one at a time as obj_export::task (or task_binary)
+    writes them, to a file or, for path "-", to standard output (the
+    controller's result argument)."""
+
+    def __init__(self, path, binary=False):
+        self.fn = lib().ffio_obj_export_task_binary if binary else lib().ffio_obj_export_task
+        if path == "-":
+            sys.stdout.flush()
+            self.f = libc().fdopen(os.dup(1), b"w")
+        else:
+            self.f = libc().fopen(os.fsencode(path), b"w")
+        if not self.f:
+            raise FormatError(-FILE_ERROR, f"cannot open {path!r} for the frames")
+
+    def write(self, n_step, dt, temps):
+        T = np.ascontiguousarray(temps, np.float32)
+        t = P.TsTask(int(n_step), float(dt), T.ctypes.data_as(C.POINTER(C.c_float)))
+        r = self.fn(self.f, len(T), C.byref(t))
+        if r != 0:
+            raise FormatError(r, "ffio_obj_export_task")
+        libc().fflush(self.f)
+
+    def close(self):
+        if self.f:
+            libc().fclose(self.f)
+            self.f = None
 
 
 def csv_export(path, F):

@@ -243,8 +243,8 @@ int ffk_batch_run_class(ffk_batch* b, int class_id, long long max_rays, void* st
  * ray scheduling (refill/pool 32/2, 24/3, 16/4) and the block size (256,
  * 512, 1024 where the histogram allows), timed greedily with
  * ffk_batch_run_class on about sample_rays of the class; a setting replaces
- * the defaults only when it is 3 % faster there. The winners are set with
- * ffk_batch_set_class_tuning. Results are cached per scene set (the
+ * the defaults only when it is 3 % faster there and then 1 % faster over the
+ * whole class. The winners are set with ffk_batch_set_class_tuning. Results are cached per scene set (the
  * scenes' signatures and the device) in memory and in the tuning cache file
  * (ffk_tune_cache_path). Returns 0 when tuned, 1 when taken from a cache. */
 int ffk_batch_autotune(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
