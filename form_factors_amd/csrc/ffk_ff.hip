@@ -86,6 +86,43 @@ __global__ void __launch_bounds__(256) k_tally(scene_desc S, const int* __restri
 // the exact integer count is rounded once to the nearest float.
 __device__ inline float normalise_one(unsigned long long cnt, float ratio) { return (float)cnt * ratio; }
 
+template <int HIST>
+__device__ inline uint32_t hist_count(const uint32_t* hist, int j) {
+  return HIST == 1 ? hist[j] : ((hist[j >> 1] >> ((j & 1) * 16)) & 0xffffu);
+}
+
+// Writes a tile's F row from its LDS histogram: four cells per lane in 16-byte
+// non-temporal stores where the row is 16-byte aligned (the row is written
+// once and not read back by this kernel, so it should not displace the scene
+// tables from L2), the rest one cell per lane.
+template <int HIST>
+__device__ inline void write_row(float* F, const uint32_t* hist, int N, float ratio) {
+  typedef float f32x4 __attribute__((ext_vector_type(4)));
+  int j0 = 0;
+  if ((reinterpret_cast<uintptr_t>(F) & 15u) == 0) {
+    const int n4 = N >> 2;
+    f32x4* F4 = reinterpret_cast<f32x4*>(F);
+    for (int k = threadIdx.x; k < n4; k += blockDim.x) {
+      uint32_t c0, c1, c2, c3;
+      if (HIST == 1) {
+        const uint4 h = reinterpret_cast<const uint4*>(hist)[k];
+        c0 = h.x, c1 = h.y, c2 = h.z, c3 = h.w;
+      } else {
+        const uint2 h = reinterpret_cast<const uint2*>(hist)[k];
+        c0 = h.x & 0xffffu, c1 = h.x >> 16, c2 = h.y & 0xffffu, c3 = h.y >> 16;
+      }
+      f32x4 v;
+      v.x = normalise_one(c0, ratio);
+      v.y = normalise_one(c1, ratio);
+      v.z = normalise_one(c2, ratio);
+      v.w = normalise_one(c3, ratio);
+      __builtin_nontemporal_store(v, F4 + k);
+    }
+    j0 = n4 << 2;
+  }
+  for (int j = j0 + threadIdx.x; j < N; j += blockDim.x) F[j] = normalise_one(hist_count<HIST>(hist, j), ratio);
+}
+
 __global__ void __launch_bounds__(256) k_normalise_rows(const long long* __restrict__ mesh_off, int N,
                                                         const int* __restrict__ rows, int n_rows,
                                                         const unsigned long long* __restrict__ tally,
@@ -272,16 +309,11 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
     if (HIST != 0) {
       __syncthreads();
       if (T.direct) {
-        float* F = Fs[sc] + (size_t)T.mesh * N;
-        const float ratio = 1.f / (float)(int)(T.g1 - T.g0);
-        for (int j = threadIdx.x; j < N; j += blockDim.x) {
-          uint32_t c = HIST == 1 ? hist[j] : ((hist[j >> 1] >> ((j & 1) * 16)) & 0xffffu);
-          F[j] = normalise_one(c, ratio);
-        }
+        write_row<HIST>(Fs[sc] + (size_t)T.mesh * N, hist, N, 1.f / (float)(int)(T.g1 - T.g0));
       } else {
         unsigned long long* row = tally + (size_t)T.mesh * N;
         for (int j = threadIdx.x; j < N; j += blockDim.x) {
-          uint32_t c = HIST == 1 ? hist[j] : ((hist[j >> 1] >> ((j & 1) * 16)) & 0xffffu);
+          const uint32_t c = hist_count<HIST>(hist, j);
           if (c) atomicAdd(&row[j], (unsigned long long)c);
         }
       }

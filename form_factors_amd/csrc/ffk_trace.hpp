@@ -419,17 +419,31 @@ __device__ inline bool coef_leave(const grid_desc& g, float ax, float ay, float 
 constexpr int kPoolWords = 64;
 
 // Inclusive prefix sum over the wavefront's 64 lanes: DPP row shifts within
-// each row of 16 lanes, then the rows' last lanes by readlane. Lanes shifted
-// in from outside a row read 0. Every lane must be active.
-__device__ inline int wave_incl_sum(int x, int lane) {
+// each row of 16 lanes (lanes shifted in from outside a row read 0), then
+// row_bcast:15 adds lane 15 of rows 0 and 2 to rows 1 and 3, and
+// row_bcast:31 adds lane 31 to rows 2 and 3. Every lane must be active.
+__device__ inline int wave_incl_sum(int x) {
   x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);  // row_shr:1
   x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);  // row_shr:2
   x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);  // row_shr:4
   x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);  // row_shr:8
-  const int r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31);
-  const int r2 = __builtin_amdgcn_readlane(x, 47);
-  const int row = lane >> 4;
-  return x + (row == 0 ? 0 : (row == 1 ? r0 : (row == 2 ? r0 + r1 : r0 + r1 + r2)));
+  x += __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+  x += __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+  return x;
+}
+
+// Inclusive prefix maximum over the wavefront's 64 lanes of non-negative
+// values: DPP row shifts within each row of 16 lanes, then row_bcast:15
+// (lane 15 of each row into the next row, rows 1 and 3) and row_bcast:31
+// (lane 31 into rows 2 and 3). Every lane must be active.
+__device__ inline int wave_incl_max(int x) {
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return x;
 }
 
 // Casts rays 0..n-1 of a work list with a wavefront's 64 lanes working as a
@@ -511,18 +525,24 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
       // The waiting lanes' (ray, reference) pairs, numbered in lane order:
       // lane w owns positions [excl, incl) of the prefix sum of the counts.
       const int cnt = (have && pend) ? nref : 0;
-      const int incl = wave_incl_sum(cnt, lane);
+      const int incl = wave_incl_sum(cnt);
       const int excl = incl - cnt;
       const int total = __builtin_amdgcn_readlane(incl, 63);
       const int base = ref0 - excl;  // reference index of position q of this lane's range: base + q
       for (int p0 = 0; p0 < total; p0 += 64) {
         const int p = p0 + lane;
-        int owner = 0;  // first lane whose inclusive count exceeds p (binary lifting)
-#pragma unroll
-        for (int step = 32; step >= 1; step >>= 1) {
-          const int v = __shfl(incl, owner + step - 1, 64);
-          if (v <= p) owner += step;
-        }
+        // Owner of position p: the waiting lane whose range [excl, incl)
+        // holds it. Each lane whose range starts inside this chunk sends
+        // lane + 1 to the chunk position of its start (ds_permute: a forward
+        // permute, where the highest sending lane wins and a position no lane
+        // sends to reads 0); the lane holding p0 and every other lane send
+        // the same value, that lane + 1, to position 0. A prefix maximum then
+        // carries each start's lane over its range. Lanes past the last range
+        // get its lane and test nothing (p >= total).
+        const int first = __builtin_ctzll(__ballot(incl > p0));  // wave-uniform
+        const bool starts = cnt > 0 && excl > p0 && excl < p0 + 64;
+        const int sent = __builtin_amdgcn_ds_permute((starts ? excl - p0 : 0) << 2, starts ? lane + 1 : first + 1);
+        const int owner = wave_incl_max(sent) - 1;
         const int ref_o = __shfl(base, owner, 64) + p;
         const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
         const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));
