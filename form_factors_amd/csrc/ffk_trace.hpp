@@ -308,18 +308,24 @@ __device__ inline void decode_key(unsigned long long key, float& d2, int& f) {
   d2 = key == kNoHit ? 3.402823466e38f : __uint_as_float((uint32_t)(key >> 32));
 }
 
-// The walk of pooled_walk, on per-ray coefficients. Along each axis the ray
-// parameter of the far boundary of cell c (the side the ray moves toward) is
-// c*A + B, and the ray's grid coordinate at parameter t is P + t*D, so a
-// boundary or an exit cell costs one fused multiply-add. For a zero direction
-// component A = 0 and B = 3.4e38 (the ray crosses no face of that axis). These
-// values round differently from walk_leave's expressions by a few units in
-// the last place; the walk tolerates that: faces are binned into cells grown
-// by 1e-3 of a cell, and the walk stops only with a 1e-4 relative margin on
-// the exit distance. Returns false when the ray misses the grid box.
+// The walk of pooled_walk, on per-ray coefficients and in mirrored cell
+// coordinates: along an axis where the ray moves toward lower cells, cell c
+// is walked as c' = n - 1 - c, so on every axis the ray moves toward higher
+// c'. The ray parameter of the far boundary of cell c' is c'*A + B (A >= 0),
+// and the ray's mirrored grid coordinate at parameter t is P + t*D (D >= 0),
+// so a boundary or an exit cell costs one fused multiply-add, and the exit's
+// clamps and range checks need no direction signs. The grid cell of (c'x,
+// c'y, c'z) is c0 + gx*c'x + gy*c'y + gz*c'z (signed strides per ray). For a
+// zero direction component A = 0, B = 3.4e38 (the ray crosses no face of that
+// axis) and P is the middle of the ray's cell. These values round
+// differently from walk_leave's expressions by a few units in the last place;
+// the walk tolerates that: faces are binned into cells grown by 1e-3 of a
+// cell, and the walk stops only with a 1e-4 relative margin on the exit
+// distance. Returns false when the ray misses the grid box.
 __device__ inline bool coef_begin(const grid_desc& g, v3 o, v3 dir, float& ax, float& ay, float& az, float& bx,
                                   float& by, float& bz, float& px, float& py, float& pz, float& dx, float& dy,
-                                  float& dz, float& dlen, int& cx, int& cy, int& cz) {
+                                  float& dz, float& dlen, int& cx, int& cy, int& cz, int& c0, int& gx, int& gy,
+                                  int& gz) {
   // Hardware reciprocals and square root (about 1 ulp): these values only
   // place the walk, which tolerates far larger errors (see above).
   const float ix = dir.x != 0.f ? __builtin_amdgcn_rcpf(dir.x) : 0.f;
@@ -349,43 +355,51 @@ __device__ inline bool coef_begin(const grid_desc& g, v3 o, v3 dir, float& ax, f
     return false;
   }
   if (!(t0 <= t1)) return false;
-  px = (o.x - g.lo[0]) * g.inv_h[0];
-  py = (o.y - g.lo[1]) * g.inv_h[1];
-  pz = (o.z - g.lo[2]) * g.inv_h[2];
-  dx = dir.x * g.inv_h[0];
-  dy = dir.y * g.inv_h[1];
-  dz = dir.z * g.inv_h[2];
-  cx = min(max((int)floorf(fmaf(t0, dx, px)), 0), g.n[0] - 1);
-  cy = min(max((int)floorf(fmaf(t0, dy, py)), 0), g.n[1] - 1);
-  cz = min(max((int)floorf(fmaf(t0, dz, pz)), 0), g.n[2] - 1);
-  ax = g.h[0] * ix;
-  ay = g.h[1] * iy;
-  az = g.h[2] * iz;
-  bx = dir.x != 0.f ? (g.lo[0] + (dir.x > 0.f ? g.h[0] : 0.f) - o.x) * ix : 3.4e38f;
-  by = dir.y != 0.f ? (g.lo[1] + (dir.y > 0.f ? g.h[1] : 0.f) - o.y) * iy : 3.4e38f;
-  bz = dir.z != 0.f ? (g.lo[2] + (dir.z > 0.f ? g.h[2] : 0.f) - o.z) * iz : 3.4e38f;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const bool fx = dir.x < 0.f, fy = dir.y < 0.f, fz = dir.z < 0.f;
+  // Mirrored grid coordinates of the origin, measured from the face the ray
+  // moves away from, and the entry cell.
+  px = (fx ? g.hi[0] - o.x : o.x - g.lo[0]) * g.inv_h[0];
+  py = (fy ? g.hi[1] - o.y : o.y - g.lo[1]) * g.inv_h[1];
+  pz = (fz ? g.hi[2] - o.z : o.z - g.lo[2]) * g.inv_h[2];
+  dx = fabsf(dir.x) * g.inv_h[0];
+  dy = fabsf(dir.y) * g.inv_h[1];
+  dz = fabsf(dir.z) * g.inv_h[2];
+  cx = min(max((int)floorf(fmaf(t0, dx, px)), 0), nx - 1);
+  cy = min(max((int)floorf(fmaf(t0, dy, py)), 0), ny - 1);
+  cz = min(max((int)floorf(fmaf(t0, dz, pz)), 0), nz - 1);
+  gx = fx ? -1 : 1;
+  gy = fy ? -nx : nx;
+  gz = fz ? -nx * ny : nx * ny;
+  c0 = (fx ? nx - 1 : 0) + (fy ? (ny - 1) * nx : 0) + (fz ? (nz - 1) * nx * ny : 0);
+  // A cell is one unit of the grid coordinate: the far boundary of cell c'
+  // is reached at t = (c' + 1 - P) * A.
+  ax = fabsf(g.h[0] * ix);
+  ay = fabsf(g.h[1] * iy);
+  az = fabsf(g.h[2] * iz);
+  bx = dir.x != 0.f ? fmaf(-px, ax, ax) : 3.4e38f;
+  by = dir.y != 0.f ? fmaf(-py, ay, ay) : 3.4e38f;
+  bz = dir.z != 0.f ? fmaf(-pz, az, az) : 3.4e38f;
+  if (dir.x == 0.f) px = (float)cx + 0.5f;
+  if (dir.y == 0.f) py = (float)cy + 0.5f;
+  if (dir.z == 0.f) pz = (float)cz + 0.5f;
   dlen = __builtin_amdgcn_sqrtf(dot(dir, dir));
   return true;
 }
 
-// Leaves the current cell: past the empty box [c, c + s*R] toward the ray's
-// octant (R >= 1) or to the next cell of the DDA (R = 0: the box is the cell
-// itself). The exit cell's other two coordinates come from the exit point,
-// clamped to the box on the ray's side of c. BOX = false assumes R = 0 for
-// the lane and skips the exit point. Returns false when the walk is over
-// (past the grid, or the best hit is nearer than the box's exit).
+// Leaves the current cell: past the empty box [c', c' + R] (mirrored
+// coordinates; R >= 1) or to the next cell of the DDA (R = 0: the box is the
+// cell itself). The exit cell's other two coordinates come from the exit
+// point, clamped to the box. BOX = false assumes R = 0 for the lane and skips
+// the exit point. Returns false when the walk is over (past the grid, or the
+// best hit is nearer than the box's exit).
 template <bool BOX>
 __device__ inline bool coef_leave(const grid_desc& g, float ax, float ay, float az, float bx, float by, float bz,
                                   float px, float py, float pz, float dx, float dy, float dz, float dlen, int R,
                                   int& cx, int& cy, int& cz, float best_d2, int best_f) {
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-  const int sx = ax > 0.f ? 1 : (ax < 0.f ? -1 : 0);
-  const int sy = ay > 0.f ? 1 : (ay < 0.f ? -1 : 0);
-  const int sz = az > 0.f ? 1 : (az < 0.f ? -1 : 0);
   if (!BOX) R = 0;
-  const int hx = sx > 0 ? min(cx + R, nx - 1) : max(cx - R, 0);
-  const int hy = sy > 0 ? min(cy + R, ny - 1) : max(cy - R, 0);
-  const int hz = sz > 0 ? min(cz + R, nz - 1) : max(cz - R, 0);
+  const int hx = min(cx + R, nx - 1), hy = min(cy + R, ny - 1), hz = min(cz + R, nz - 1);
   const float tx = fmaf((float)hx, ax, bx), ty = fmaf((float)hy, ay, by), tz = fmaf((float)hz, az, bz);
   const bool ex = (tx <= ty) && (tx <= tz);
   const bool ey = !ex && (ty <= tz);
@@ -394,21 +408,17 @@ __device__ inline bool coef_leave(const grid_desc& g, float ax, float ay, float 
   bool on = tb < 3.4e38f && !(best_f >= 0 && best_d2 < de * de * 0.9999f);
   int nxc = cx, nyc = cy, nzc = cz;
   if (BOX) {
-    const int fx = (int)floorf(fmaf(tb, dx, px));
-    const int fy = (int)floorf(fmaf(tb, dy, py));
-    const int fz = (int)floorf(fmaf(tb, dz, pz));
-    // [min(c, h), max(c, h)] is the box's range on the ray's side of c; an
-    // axis the ray does not move along keeps c.
-    nxc = sx != 0 ? min(max(fx, min(cx, hx)), max(cx, hx)) : cx;
-    nyc = sy != 0 ? min(max(fy, min(cy, hy)), max(cy, hy)) : cy;
-    nzc = sz != 0 ? min(max(fz, min(cz, hz)), max(cz, hz)) : cz;
+    // the exit point's cell, clamped to [c', h'] on each axis
+    nxc = min(max((int)floorf(fmaf(tb, dx, px)), cx), hx);
+    nyc = min(max((int)floorf(fmaf(tb, dy, py)), cy), hy);
+    nzc = min(max((int)floorf(fmaf(tb, dz, pz)), cz), hz);
   }
-  nxc = ex ? hx + sx : nxc;
-  nyc = (!ex && ey) ? hy + sy : nyc;
-  nzc = (!ex && !ey) ? hz + sz : nzc;
-  const int ne = ex ? nxc : (ey ? nyc : nzc);
-  const int nn = ex ? nx : (ey ? ny : nz);
-  on = on && ne >= 0 && ne < nn;
+  const int he = ex ? hx : (ey ? hy : hz);
+  const int ne = ex ? nx : (ey ? ny : nz);
+  nxc = ex ? hx + 1 : nxc;
+  nyc = (!ex && ey) ? hy + 1 : nyc;
+  nzc = (!ex && !ey) ? hz + 1 : nzc;
+  on = on && he + 1 < ne;
   cx = on ? nxc : cx;
   cy = on ? nyc : cy;
   cz = on ? nzc : cz;
@@ -475,8 +485,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
   v3 o = mk(0.f, 0.f, 0.f), dir = mk(0.f, 0.f, 0.f);
   float ax = 0.f, ay = 0.f, az = 0.f, bx = 0.f, by = 0.f, bz = 0.f;
   float px = 0.f, py = 0.f, pz = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, dlen = 0.f;
-  int cx = 0, cy = 0, cz = 0, osh = 0;
-  const int nx = S.g.n[0], nxy = S.g.n[0] * S.g.n[1];
+  int cx = 0, cy = 0, cz = 0, c0 = 0, gx = 0, gy = 0, gz = 0, osh = 0;
   while (true) {
     const unsigned long long idle = __ballot(!have);
     const int n_idle = __popcll(idle);
@@ -495,7 +504,8 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
           osh = octant_shift(dir);
           best[lane] = kNoHit;
           pend = false;
-          have = coef_begin(S.g, o, dir, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, cx, cy, cz);
+          have = coef_begin(S.g, o, dir, ax, ay, az, bx, by, bz, px, py, pz, dx, dy, dz, dlen, cx, cy, cz, c0, gx,
+                            gy, gz);
           if (!have) finish(i, o, dir, kNoHit);
         }
       }
@@ -505,7 +515,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
     int2 ci = make_int2(0, 0);
     const bool step = have && !pend;
     if (step) {
-      ci = gld2(gidx(S.cell_info, (uint32_t)(cx + nx * cy + nxy * cz)));
+      ci = gld2(gidx(S.cell_info, (uint32_t)(c0 + gx * cx + gy * cy + gz * cz)));
       if (STATS) n_cells += 1;
       if (ci.y > 0) {
         pend = true;
