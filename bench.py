@@ -63,7 +63,9 @@ def parse():
     ap.add_argument("--hist", type=int, default=-1)
     ap.add_argument("--refill", type=int, default=-1, help="fused-kernel refill threshold (lanes; -1 = default)")
     ap.add_argument("--pool", type=int, default=-1, help="pooled-test ratio (0 = per-lane tests; -1 = default)")
-    ap.add_argument("--walk", type=int, default=-1, help="empty-cell walk mode (0 per wavefront, 1 jump; -1 = default)")
+    ap.add_argument("--walk", type=int, default=-1,
+                    help="walk mode 0..3 (bit 0: jump for every empty cell, bit 1: whole chunks of pooled tests); "
+                         "-1 = one probe launch of each, the fastest kept")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one untimed device copy of this size (HBM counter calibration under rocprofv3)")
     return ap.parse_args()
@@ -748,7 +750,7 @@ def main():
     # without changing the grid this configuration fixes.
     walk_times = {}
     if args.walk < 0:
-        for m in (0, 1):
+        for m in (0, 1, 2, 3):
             sc.set_walk(m)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
@@ -758,7 +760,7 @@ def main():
             walk_times[m] = e0.elapsed_time(e1)
         walk = min(walk_times, key=walk_times.get)
         sc.set_walk(walk)
-        log(f"walk mode {walk} (one launch each: {walk_times[0]:.2f} / {walk_times[1]:.2f} ms)")
+        log(f"walk mode {walk} (one launch each: " + " / ".join(f"{walk_times[m]:.2f}" for m in sorted(walk_times)) + " ms)")
     else:
         walk = args.walk
     for _ in range(args.warmup):

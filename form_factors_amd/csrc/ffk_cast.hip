@@ -162,10 +162,11 @@ extern "C" int ffk_cast(ffk_scene* s, int method, const float* d_rays, long long
       const unsigned blocks = (unsigned)((n_rays + chunk - 1) / chunk);
       auto go = [&](auto kernel) {
         kernel<<<blocks, 256, 0, st>>>(s->desc(), d_rays, n_rays, chunk, d_hit_idx, d_hit_pts, d_stats,
-                                       s->tune_refill, s->tune_pool);
+                                       s->tune_refill, pool_arg(s->tune_pool, s->tune_walk));
       };
-      if (d_stats) s->tune_walk == 1 ? go(k_cast_pooled<true, 1>) : go(k_cast_pooled<true, 2>);
-      else s->tune_walk == 1 ? go(k_cast_pooled<false, 1>) : go(k_cast_pooled<false, 2>);
+      const bool jump = (s->tune_walk & kWalkJump) != 0;
+      if (d_stats) jump ? go(k_cast_pooled<true, 1>) : go(k_cast_pooled<true, 2>);
+      else jump ? go(k_cast_pooled<false, 1>) : go(k_cast_pooled<false, 2>);
     } else {
       const unsigned blocks = (unsigned)((n_rays + 255) / 256);
       if (d_stats)

@@ -166,7 +166,8 @@ struct fused_args {
   unsigned long long* stats;
   int refill;  // 0: one ray per lane at a time; else the idle-lane threshold of the refill loop
   int pool;    // 0: per-lane tests; else pool the tests once waiting lanes x pool >= active lanes
-  int walk;    // 0: per-wavefront choice of the empty-cell step; 1: always the jump code
+  int walk;    // bit 0 (kWalkJump): always the jump code, else a per-wavefront choice of the
+               // empty-cell step; bit 1 (kWalkWhole): pooled tests dealt in whole chunks
 };
 
 // REFILL: lanes walk one grid cell per iteration and, once at least
@@ -398,7 +399,7 @@ static void launch_one(dim3 grid, int block, size_t lds, hipStream_t st, const f
   const size_t bytes = fused_lds(H, lds, R, block);
   set_lds_attr<H, P, S, R>(bytes);
   k_ff_fused<H, P, S, R><<<grid, block, bytes, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies,
-                                                     A.Fs, A.stats, A.refill, A.pool);
+                                                     A.Fs, A.stats, A.refill, pool_arg(A.pool, A.walk));
 }
 
 template <bool P, bool S, int R>
@@ -413,12 +414,12 @@ static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStr
 
 // Scheduling: 0 = one ray per lane, 1 = lane refill, 2 = lane refill with
 // pooled tests (when the best-hit words fit beside the histogram), 3 = as 2
-// with the jump code for every empty cell (walk 1). All cast the same rays
+// with the jump code for every empty cell (walk bit 0). All cast the same rays
 // and tally the same hits.
 static int sched_of(int refill, int pool, int walk, int hist, size_t hist_lds, int block) {
   if (refill <= 0) return 0;
   if (pool <= 0 || fused_lds(hist, hist_lds, 2, block) > (size_t)max_lds_bytes()) return 1;
-  return walk == 1 ? 3 : 2;
+  return (walk & kWalkJump) ? 3 : 2;
 }
 
 template <bool P, bool S>
@@ -566,7 +567,7 @@ struct ffk_batch {
   int order = 1;  // tile queue: 0 longest first, 1 by scene (most rays first), longest first within
   int refill = kDefaultRefill;
   int pool = kDefaultPool;
-  int walk = 0;
+  int walk = kDefaultWalk;
   // One stream per LDS class (default priority): the class launches are
   // forked from the launch stream after the tallies are cleared and joined
   // before the normalisation, so that a class's blocks start while the
@@ -816,14 +817,14 @@ int ffk_get_refill(const ffk_scene* s, int* lanes, int* pool) {
 
 int ffk_set_walk(ffk_scene* s, int mode) {
   FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
-  FFK_REQUIRE(mode == 0 || mode == 1, FFK_ERR_ARG, "walk mode %d is not 0 or 1", mode);
+  FFK_REQUIRE(mode >= 0 && mode <= 3, FFK_ERR_ARG, "walk mode %d is not in 0..3", mode);
   s->tune_walk = mode;
   return FFK_OK;
 }
 
 int ffk_batch_set_walk(ffk_batch* b, int mode) {
   FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
-  FFK_REQUIRE(mode == 0 || mode == 1, FFK_ERR_ARG, "walk mode %d is not 0 or 1", mode);
+  FFK_REQUIRE(mode >= 0 && mode <= 3, FFK_ERR_ARG, "walk mode %d is not in 0..3", mode);
   b->walk = mode;
   free_graph(b);
   return FFK_OK;
@@ -1254,7 +1255,7 @@ int ffk_batch_set_class_tuning(ffk_batch* b, int class_id, int block_size, int r
   FFK_REQUIRE(b && class_id >= 0 && class_id < 6, FFK_ERR_ARG, "ffk_batch_set_class_tuning: class %d", class_id);
   FFK_REQUIRE(block_size <= 0 || (block_size % 64 == 0 && block_size <= 1024), FFK_ERR_ARG, "block size %d",
               block_size);
-  FFK_REQUIRE(refill <= 64 && pool <= 64 && walk <= 1, FFK_ERR_ARG, "refill %d / pool %d / walk %d", refill, pool,
+  FFK_REQUIRE(refill <= 64 && pool <= 64 && walk <= 3, FFK_ERR_ARG, "refill %d / pool %d / walk %d", refill, pool,
               walk);
   ffk_batch::class_tuning& t = b->ctune[class_id];
   t.block = block_size > 0 ? block_size : -1;

@@ -368,7 +368,7 @@ struct ffk_scene {
   int tune_block = 0, tune_rays_per_tile = 0, tune_hist = -1;
   int tune_refill = 32;  // idle lanes that trigger a refill in the fused kernel (0 = off)
   int tune_pool = 2;     // pooled tests once waiting lanes x tune_pool >= active lanes (0 = off)
-  int tune_walk = 0;     // empty cells: 0 = per-wavefront choice of jump or DDA step, 1 = always the jump code
+  int tune_walk = FFK_WALK_DEFAULT;  // walk mode: bit 0 = always the jump code for empty cells, bit 1 = whole chunks of pooled tests
   int tiles_rays_per_tile = 0;  // value the tile list was built with
   bool tiles_dirty = false;     // the emission layout changed since the tile list was built
   int rot_frame = -1;           // frame of d_rot (-1: not computed)

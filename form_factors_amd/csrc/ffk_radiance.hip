@@ -390,10 +390,12 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
       const long long chunk = std::max<long long>(256, (walk_total + resident * rounds - 1) / (resident * rounds));
       const unsigned blocks = (unsigned)((walk_total + chunk - 1) / chunk);
       kernel<<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q, walk_total,
-                                     chunk, k0, k1, s->d_rad_acc, d_stats, s->tune_refill, s->tune_pool);
+                                     chunk, k0, k1, s->d_rad_acc, d_stats, s->tune_refill,
+                                     pool_arg(s->tune_pool, s->tune_walk));
     };
-    if (d_stats) s->tune_walk == 1 ? go(k_radiance_pooled<true, 1>) : go(k_radiance_pooled<true, 2>);
-    else s->tune_walk == 1 ? go(k_radiance_pooled<false, 1>) : go(k_radiance_pooled<false, 2>);
+    const bool jump = (s->tune_walk & kWalkJump) != 0;
+    if (d_stats) jump ? go(k_radiance_pooled<true, 1>) : go(k_radiance_pooled<true, 2>);
+    else jump ? go(k_radiance_pooled<false, 1>) : go(k_radiance_pooled<false, 2>);
   } else if (walk_total > 0) {
     const long long blocks_needed = (walk_total + 255) / 256;
     const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));

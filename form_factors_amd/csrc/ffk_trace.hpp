@@ -428,6 +428,13 @@ __device__ inline bool coef_leave(const grid_desc& g, float ax, float ay, float 
 // LDS words per wavefront of pooled_walk: 64 best-hit words (hit_key).
 constexpr int kPoolWords = 64;
 
+// Walk mode bits (ffk_set_walk): kWalkJump = the jump code for every empty
+// cell (template WALK 1, else WALK 2); kWalkWhole = pooled tests dealt in
+// whole chunks, passed to pooled_walk as kPoolWhole in its pool argument.
+constexpr int kWalkJump = FFK_WALK_JUMP, kWalkWhole = FFK_WALK_WHOLE, kPoolWhole = 0x100;
+constexpr int kDefaultWalk = FFK_WALK_DEFAULT;
+__host__ __device__ inline int pool_arg(int pool, int walk) { return pool | ((walk & kWalkWhole) ? kPoolWhole : 0); }
+
 // Inclusive prefix sum over the wavefront's 64 lanes: DPP row shifts within
 // each row of 16 lanes (lanes shifted in from outside a row read 0), then
 // row_bcast:15 adds lane 15 of rows 0 and 2 to rows 1 and 3, and
@@ -472,6 +479,10 @@ __device__ inline int wave_incl_max(int x) {
 //   best:  this wavefront's kPoolWords words of LDS (8-byte aligned)
 //   next:  the list's LDS ray counter, zero at the start, shared by the
 //          block's wavefronts
+//   pool:  the pooling ratio in the low byte; with kPoolWhole set, a round of
+//          at least 64 pairs deals whole chunks of 64 only, and the lanes
+//          whose references lie past the last whole chunk stay waiting with
+//          the rest for the next round (their best hit so far stays in LDS)
 //   start(i, o, dir):            sets up ray i
 //   finish(i, o, dir, key):      ray i is done; key = hit_key of its hit or kNoHit
 // Every lane of the wavefront must call this together.
@@ -479,6 +490,8 @@ template <bool STATS, int WALK, class Start, class Finish>
 __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best, int* next, long long n, int refill,
                                    int pool, Start start, Finish finish, uint32_t& n_tests, uint32_t& n_cells) {
   const int lane = threadIdx.x & 63;
+  const bool whole = (pool & kPoolWhole) != 0;
+  pool &= 0xff;
   bool have = false, more = true, pend = false;  // `more` is wave-uniform
   long long cur = 0;
   int ref0 = 0, nref = 0;
@@ -539,7 +552,8 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
       const int excl = incl - cnt;
       const int total = __builtin_amdgcn_readlane(incl, 63);
       const int base = ref0 - excl;  // reference index of position q of this lane's range: base + q
-      for (int p0 = 0; p0 < total; p0 += 64) {
+      const int deal = whole && total >= 64 ? (total & ~63) : total;  // pairs tested in this round
+      for (int p0 = 0; p0 < deal; p0 += 64) {
         const int p = p0 + lane;
         // Owner of position p: the waiting lane whose range [excl, incl)
         // holds it. Each lane whose range starts inside this chunk sends
@@ -548,7 +562,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         // sends to reads 0); the lane holding p0 and every other lane send
         // the same value, that lane + 1, to position 0. A prefix maximum then
         // carries each start's lane over its range. Lanes past the last range
-        // get its lane and test nothing (p >= total).
+        // get its lane and test nothing (p >= deal).
         const int first = __builtin_ctzll(__ballot(incl > p0));  // wave-uniform
         const bool starts = cnt > 0 && excl > p0 && excl < p0 + 64;
         const int sent = __builtin_amdgcn_ds_permute((starts ? excl - p0 : 0) << 2, starts ? lane + 1 : first + 1);
@@ -556,7 +570,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         const int ref_o = __shfl(base, owner, 64) + p;
         const v3 oo = mk(__shfl(o.x, owner, 64), __shfl(o.y, owner, 64), __shfl(o.z, owner, 64));
         const v3 dd = mk(__shfl(dir.x, owner, 64), __shfl(dir.y, owner, 64), __shfl(dir.z, owner, 64));
-        if (p < total) {
+        if (p < deal) {
           const int f = gld(gidx(S.refs, (uint32_t)ref_o));
           const float4* q = reinterpret_cast<const float4*>(gidx(S.tri, (uint32_t)f));
           v3 pt;
@@ -568,9 +582,14 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       if (have && pend) {
-        pend = false;
-        leave = true;
-        R = 0;
+        if (incl <= deal) {  // every reference of the cell tested
+          pend = false;
+          leave = true;
+          R = 0;
+        } else if (excl < deal) {  // the rest waits for the next round
+          ref0 += deal - excl;
+          nref -= deal - excl;
+        }
       }
     }
     const bool jump = WALK == 1 || __ballot(leave && R >= 1) != 0ull;

@@ -383,7 +383,9 @@ class Scene:
         return lanes.value, pool.value
 
     def set_walk(self, mode):
-        """Empty-cell step of the pooled walk: 0 = per-wavefront choice, 1 = always the jump."""
+        """Walk mode of the pooled schedule (0..3, default 3): bit 0 = always the
+        jump code for empty cells (else a per-wavefront choice), bit 1 = pooled
+        tests dealt in whole chunks of 64."""
         check(lib().ffk_set_walk(self.h, int(mode)), "ffk_set_walk")
 
     # -- device calls (torch tensors in, torch tensors out) --

@@ -167,12 +167,20 @@ int ffk_set_refill(ffk_scene* s, int lanes, int pool);
 /* The scene's current refill threshold and pool ratio (either pointer may be NULL). */
 int ffk_get_refill(const ffk_scene* s, int* lanes, int* pool);
 int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool);
-/* How pooled lanes leave an empty grid cell: past its empty box toward the
- * ray's direction octant (the jump) or, when that box is the cell itself, by
- * one DDA step. mode 0 (default): a wavefront runs the jump code for all its
- * empty-cell lanes when any of them can jump, the DDA step otherwise; mode 1:
- * always the jump code (fastest when rays cross large empty regions). Same
- * cells, same hits. */
+/* Walk mode of the pooled schedule, two bits. Bit 0, how lanes leave an
+ * empty grid cell: past its empty box toward the ray's direction octant (the
+ * jump) or, when that box is the cell itself, by one DDA step; clear
+ * (default): a wavefront runs the jump code for all its empty-cell lanes when
+ * any of them can jump, the DDA step otherwise; set: always the jump code
+ * (fastest when rays cross large empty regions). Bit 1, how pooled tests are
+ * dealt: clear (default): every waiting lane's tests in each round; set: a
+ * round of at least 64 tests deals whole chunks of 64 only, and the tests
+ * past the last whole chunk wait for the next round (fuller chunks, later
+ * exits: faster on scenes whose rounds are large). Modes 0..3; same cells,
+ * same hits in every mode. Default FFK_WALK_DEFAULT (both bits). */
+#define FFK_WALK_JUMP 1
+#define FFK_WALK_WHOLE 2
+#define FFK_WALK_DEFAULT (FFK_WALK_JUMP | FFK_WALK_WHOLE)
 int ffk_set_walk(ffk_scene* s, int mode);
 int ffk_batch_set_walk(ffk_batch* b, int mode);
 

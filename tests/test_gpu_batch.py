@@ -83,7 +83,7 @@ def test_batched_tuning_does_not_change_bits(gpu):
     for block, rays_per_tile, n_blocks, refill, pool, walk in [
             (256, 1024, 0, 0, 0, 0), (512, 4096, 64, 16, 0, 0), (1024, 0, 7, 0, 0, 0), (512, 0, 0, 32, 0, 0),
             (512, 0, 0, 16, 4, 0), (256, 4096, 9, 8, 1, 0), (1024, 0, 0, 16, 8, 0), (512, 0, 0, 16, 4, 1),
-            (256, 4096, 9, 8, 1, 1)]:
+            (256, 4096, 9, 8, 1, 1), (512, 0, 0, 32, 2, 3), (256, 4096, 9, 8, 1, 2)]:
         b.set_tuning(block, rays_per_tile, n_blocks)
         b.set_refill(refill, pool)
         b.set_walk(walk)

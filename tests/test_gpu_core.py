@@ -78,7 +78,8 @@ def test_grid_cast_pooled_equals_per_ray(gpu, oracle, n):
     if n <= 5000:
         want_i, _ = oracle.naive_cast(faces, rays, n_threads=8)
         assert np.array_equal(ref_i, want_i)
-    for refill, pool, walk in [(16, 4, 0), (1, 1, 0), (64, 64, 0), (8, 2, 0), (32, 16, 0), (16, 4, 1), (1, 1, 1)]:
+    for refill, pool, walk in [(16, 4, 0), (1, 1, 0), (64, 64, 0), (8, 2, 0), (32, 16, 0), (16, 4, 1), (1, 1, 1),
+                              (16, 4, 2), (1, 1, 3), (64, 64, 2), (32, 2, 3)]:
         print(f"pooled cast n={n} refill={refill} pool={pool} walk={walk}", flush=True)
         sc.set_refill(refill, pool)
         sc.set_walk(walk)
@@ -196,7 +197,7 @@ def test_fused_ff_tiling_and_histograms(gpu, oracle, hist, rays_per_tile):
     assert np.array_equal(bits(F), bits(want))
 
 
-@pytest.mark.parametrize("refill,pool,walk", [(16, 0, 0), (16, 4, 0), (32, 2, 1)])
+@pytest.mark.parametrize("refill,pool,walk", [(16, 0, 0), (16, 4, 0), (32, 2, 1), (32, 2, 3)])
 @pytest.mark.parametrize("rays_per_tile", [1000, 4096])
 @pytest.mark.parametrize("order", ["face_order", "reversed"])
 def test_fused_ff_split_multi_face_meshes(gpu, oracle, refill, pool, walk, rays_per_tile, order):
@@ -225,7 +226,8 @@ def test_fused_ff_split_multi_face_meshes(gpu, oracle, refill, pool, walk, rays_
 
 
 @pytest.mark.parametrize("refill,pool,walk", [(1, 0, 0), (16, 0, 0), (48, 0, 0), (64, 0, 0), (1, 1, 0), (16, 4, 0),
-                                              (16, 1, 0), (64, 64, 0), (8, 2, 0), (16, 4, 1), (1, 1, 1), (64, 8, 1)])
+                                              (16, 1, 0), (64, 64, 0), (8, 2, 0), (16, 4, 1), (1, 1, 1), (64, 8, 1),
+                                              (16, 4, 2), (1, 1, 3), (32, 2, 3)])
 @pytest.mark.parametrize("hist,rays_per_tile", [(0, 0), (2, 1000), (1, 0)])
 def test_fused_ff_refill_bit_exact(gpu, oracle, refill, pool, walk, hist, rays_per_tile):
     """Lanes that take new rays as soon as enough of them are idle, with the

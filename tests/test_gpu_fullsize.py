@@ -44,7 +44,7 @@ def test_full_size_f_is_schedule_and_launch_independent(gpu, sphere):
     G = torch.empty((N, N), device="cuda")
     sc.set_refill(16, 4)
     sc.ff_fused(SEED, F=F)
-    for refill, pool, walk in [(0, 0, 0), (16, 0, 0), (16, 4, 1)]:
+    for refill, pool, walk in [(0, 0, 0), (16, 0, 0), (16, 4, 1), (32, 2, 3)]:
         print(f"full size refill={refill} pool={pool} walk={walk}", flush=True)
         sc.set_refill(refill, pool)
         sc.set_walk(walk)

@@ -105,7 +105,8 @@ def test_radiance_schedules_agree(gpu, case):
     sc = gpu.Scene(faces, meshes)
     sc.build_grid()
     out = []
-    for refill, pool, walk in [(0, 0, 0), (16, 0, 0), (16, 4, 0), (1, 1, 0), (64, 64, 0), (16, 4, 1)]:
+    for refill, pool, walk in [(0, 0, 0), (16, 0, 0), (16, 4, 0), (1, 1, 0), (64, 64, 0), (16, 4, 1), (16, 4, 2),
+                              (32, 2, 3)]:
         print(f"radiance {name} refill={refill} pool={pool} walk={walk}", flush=True)
         sc.set_refill(refill, pool)
         sc.set_walk(walk)
