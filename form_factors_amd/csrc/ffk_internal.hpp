@@ -350,16 +350,32 @@ struct ffk_scene {
   // one block per scene so that launches of different scenes on different
   // streams do not share it
   unsigned long long* d_misc = nullptr;
-  // radiance work buffers, grown on demand: the walk tables (start, q, face)
-  // and the per-mesh sums share one device block and one pinned host block
-  // of the same layout, so that each pass makes one upload and one download
-  void* d_walk_block = nullptr;
-  void* h_walk_block = nullptr;  // pinned
-  long long* d_walk_start = nullptr;
-  int* d_walk_face = nullptr;
-  unsigned long long* d_walk_q = nullptr;
-  unsigned long long* d_rad_acc = nullptr;
-  int rad_cap_walk = 0, rad_cap_meshes = 0;
+  // radiance work (ffk_radiance.hip): one device block and one pinned host
+  // block, laid out for the current faces and meshes on the first pass and
+  // released when either is set again (rad_W < 0: not built)
+  int rad_W = -1;         // walk entries: the faces of each mesh, mesh by mesh
+  int rad_parts = 0;      // blocks of the per-entry reductions
+  size_t rad_scan_bytes = 0;
+  void* d_rad_block = nullptr;
+  void* h_rad_pin = nullptr;
+  int* d_walk_face = nullptr;               // W: face of each walk entry
+  long long* d_walk_cnt = nullptr;          // W + 1: rays of each entry's face (then 0)
+  long long* d_walk_start = nullptr;        // W + 1: first walk position of each entry
+  unsigned long long* d_walk_q = nullptr;   // 2W: fixed-point front / rear ray power
+  unsigned char* d_walk_band = nullptr;     // 2W: power band of each side
+  unsigned long long* d_rad_acc = nullptr;  // kPowerBands x n_meshes fixed-point sums
+  double* d_rad_abs = nullptr;              // n_meshes: absorbed power
+  double* d_rad_scale = nullptr;            // kPowerBands fixed-point scales
+  float* d_rad_p = nullptr;                 // 2 n_faces: ray power per face side
+  float* d_rad_te = nullptr;                // T (n_meshes), eps (2 n_meshes)
+  float* d_rad_area = nullptr;              // n_faces: h_area
+  unsigned* d_rad_part_u = nullptr;         // reduction partials (see ffk_radiance.hip)
+  double* d_rad_part_d = nullptr;
+  int* d_rad_info = nullptr;                // e_max, n_bands, status, first bad face
+  void* d_rad_scan = nullptr;               // hipCUB scan storage
+  // layout upload staging (pinned): face offsets, front counts, mesh offsets
+  void* h_layout_pin = nullptr;
+  size_t layout_pin_bytes = 0;
   // host copies
   std::vector<float> h_faces;
   std::vector<int> h_meshes, h_front, h_rear;
@@ -377,12 +393,14 @@ struct ffk_scene {
   // host work arrays of the radiance step
   std::vector<float> rad_w, rad_p;
   std::vector<int> rad_front, rad_rear;
-  std::vector<unsigned char> rad_band;  // power band of each walk entry's front / rear ray
 
   ffk::scene_desc desc() const;
 };
 
 namespace ffk {
+// Releases the radiance work buffers (ffk_radiance.hip); they are laid out
+// again for the current faces and meshes on the next radiance pass.
+void free_radiance(ffk_scene* s);
 int build_tiles(ffk_scene* s, int rays_per_tile, hipStream_t st);
 int ensure_tiles(ffk_scene* s, hipStream_t st);
 int device_cu_count();

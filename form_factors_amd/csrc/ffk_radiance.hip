@@ -17,9 +17,12 @@
 // magnitude below the largest power; every band has its own scale and its
 // own accumulator row, and the band travels in the top 4 bits of the ray's
 // fixed-point word.
+#include <hipcub/hipcub.hpp>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "ffk_trace.hpp"
 
@@ -141,6 +144,414 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
   }
 }
 
+constexpr float kSigma = 5.670400e-8f;  // thermal_equation/radiance_cpu.cpp:29
+constexpr int kRadThreads = 256;
+
+// Binary exponent of a positive float (ilogb), from its bits.
+__host__ __device__ inline int pow_exponent(float p) {
+#ifdef __HIP_DEVICE_COMPILE__
+  const uint32_t u = __float_as_uint(p);
+#else
+  uint32_t u;
+  std::memcpy(&u, &p, 4);
+#endif
+  const int e = (int)((u >> 23) & 0xff);
+  return e ? e - 127 : ilogbf(p);
+}
+
+// Round half to even of a non-negative x < 2^60 (what llrint does in the
+// default rounding mode; x - trunc(x) is exact, so the comparison is too).
+__host__ __device__ inline unsigned long long round_even(double x) {
+  const long long v = (long long)x;
+  const double frac = x - (double)v;
+  return (unsigned long long)v + (frac > 0.5 || (frac == 0.5 && (v & 1)) ? 1ull : 0ull);
+}
+
+// Deterministic block reduction (fixed shuffle tree, then the wavefronts in
+// order); the result is valid in every thread. sh: blockDim.x / 64 slots.
+template <class T, class Op>
+__device__ inline T block_reduce(T v, Op op, T* sh) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_down(v, off, 64));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  T r = sh[0];
+  for (int i = 1; i < (int)(blockDim.x >> 6); ++i) r = op(r, sh[i]);
+  __syncthreads();
+  return r;
+}
+
+// Ray power of each face side (radiance_cpu.cpp:148-178): the side's
+// emission over its ray count, in the float expressions of the host
+// powers_impl, so the values are its bits. Faces in no mesh carry 0; a face of
+// several meshes takes the last one's (as powers_impl, which overwrites).
+// te: T[n_meshes] then eps[2 * n_meshes].
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_powers(int nf, const int* __restrict__ face2mesh, const float* __restrict__ te, int nm,
+                 const float* __restrict__ area, const long long* __restrict__ face_off,
+                 const int* __restrict__ front, float* __restrict__ p) {
+  const int f = blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nf) return;
+  const int m = face2mesh[f];
+  float pf = 0.f, pr = 0.f;
+  if (m >= 0) {
+    const float T = te[m];
+    const float power_density = kSigma * (T * T * T * T);
+    const float front_density = power_density * te[nm + 2 * m];
+    const float rear_density = power_density * te[nm + 2 * m + 1];
+    const float a = area[f];
+    const float front_emission = front_density * a;
+    const float rear_emission = rear_density * a;
+    const int cf = front[f], cr = (int)(face_off[f + 1] - face_off[f]) - cf;
+    pf = cf > 0 ? front_emission / cf : 0.f;
+    pr = cr > 0 ? rear_emission / cr : 0.f;
+  }
+  p[2 * f] = pf;
+  p[2 * f + 1] = pr;
+}
+
+// Rays of each walk entry's face; entry W (the end) has none.
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_counts(int W, const int* __restrict__ walk_face, const long long* __restrict__ face_off,
+                 long long* __restrict__ cnt) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > W) return;
+  long long c = 0;
+  if (k < W) {
+    const int f = walk_face[k];
+    c = face_off[f + 1] - face_off[f];
+  }
+  cnt[k] = c;
+}
+
+// Entries the walk reaches (first position < total; radiance_cpu.cpp:170):
+// the power check (negative or not finite: status bit, lowest such face) and
+// per block the largest power of a side that emits rays (as bits: positive
+// floats order as their bits).
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_entries(int W, long long total, const long long* __restrict__ start, const int* __restrict__ walk_face,
+                  const long long* __restrict__ face_off, const int* __restrict__ front, const float* __restrict__ p,
+                  unsigned* __restrict__ part_max, int* __restrict__ info) {
+  __shared__ unsigned sh[kRadThreads / 64];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned vmax = 0u;
+  if (k < W && start[k] < total) {
+    const int f = walk_face[k];
+    const float pf = p[2 * f], pr = p[2 * f + 1];
+    if (!(pf >= 0.f && pr >= 0.f && isfinite(pf) && isfinite(pr))) {
+      atomicOr(&info[2], 1);
+      atomicMin(&info[3], f);
+    }
+    const int cf = front[f], cr = (int)(face_off[f + 1] - face_off[f]) - cf;
+    if (cf > 0 && pf > 0.f) vmax = max(vmax, __float_as_uint(pf));
+    if (cr > 0 && pr > 0.f) vmax = max(vmax, __float_as_uint(pr));
+  }
+  const unsigned r = block_reduce(vmax, [](unsigned a, unsigned b) { return max(a, b); }, sh);
+  if (threadIdx.x == 0) part_max[blockIdx.x] = r;
+}
+
+// One block: the largest power over the blocks' maxima; info[0] = its exponent.
+__global__ void __launch_bounds__(kRadThreads) k_rad_emax(int n_parts, const unsigned* __restrict__ part_max,
+                                                          int* __restrict__ info) {
+  __shared__ unsigned sh[kRadThreads / 64];
+  unsigned v = 0u;
+  for (int i = threadIdx.x; i < n_parts; i += blockDim.x) v = max(v, part_max[i]);
+  v = block_reduce(v, [](unsigned a, unsigned b) { return max(a, b); }, sh);
+  if (threadIdx.x == 0) info[0] = v ? pow_exponent(__uint_as_float(v)) : 0;
+}
+
+// Band of each emitting side: its binary order below the largest power in
+// steps of kBandOrders (the last band takes everything smaller); per block and
+// band the sum of count * power and the largest power.
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_bands(int W, long long total, const long long* __restrict__ start, const int* __restrict__ walk_face,
+                const long long* __restrict__ face_off, const int* __restrict__ front, const float* __restrict__ p,
+                const int* __restrict__ info, unsigned char* __restrict__ band, double* __restrict__ part_sum,
+                unsigned* __restrict__ part_max) {
+  __shared__ double shd[kRadThreads / 64];
+  __shared__ unsigned shu[kRadThreads / 64];
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int e_max = info[0];
+  int b2[2] = {-1, -1};
+  double v2[2] = {0.0, 0.0};
+  unsigned u2[2] = {0u, 0u};
+  if (k < W && start[k] < total) {
+    const int f = walk_face[k];
+    const int cnt[2] = {front[f], (int)(face_off[f + 1] - face_off[f]) - front[f]};
+    for (int side = 0; side < 2; ++side) {
+      const float ps = p[2 * f + side];
+      const bool emits = ps > 0.f && cnt[side] > 0;
+      const int b = emits ? min(kPowerBands - 1, (e_max - pow_exponent(ps)) / kBandOrders) : 0;
+      band[2 * k + side] = (unsigned char)b;
+      if (!emits) continue;
+      b2[side] = b;
+      v2[side] = (double)cnt[side] * ps;
+      u2[side] = __float_as_uint(ps);
+    }
+  }
+  for (int b = 0; b < kPowerBands; ++b) {
+    const double v = (b2[0] == b ? v2[0] : 0.0) + (b2[1] == b ? v2[1] : 0.0);
+    const unsigned u = max(b2[0] == b ? u2[0] : 0u, b2[1] == b ? u2[1] : 0u);
+    const double rs = block_reduce(v, [](double x, double y) { return x + y; }, shd);
+    const unsigned ru = block_reduce(u, [](unsigned x, unsigned y) { return max(x, y); }, shu);
+    if (threadIdx.x == 0) {
+      part_sum[(size_t)blockIdx.x * kPowerBands + b] = rs;
+      part_max[(size_t)blockIdx.x * kPowerBands + b] = ru;
+    }
+  }
+}
+
+// One block: per band the largest possible sum (every ray of the band
+// absorbed by one mesh) and the largest single power fix the scale: sums stay
+// below 2^63 and a single ray's value below 2^60, clear of the band bits. A
+// ray of power p in band b is then rounded by at most 0.5 / scale_b, i.e.
+// relatively by at most 2^-55 times the rays in the band, or 2^-52
+// (2^kBandOrders / 2^60). info[1] = bands in use (0: no side emits).
+__global__ void k_rad_scales(int n_parts, const double* __restrict__ part_sum, const unsigned* __restrict__ part_max,
+                             double* __restrict__ scale, int* __restrict__ info) {
+  __shared__ int used[kPowerBands];
+  const int b = threadIdx.x;
+  if (b < kPowerBands) {
+    double up = 0.0;
+    unsigned um = 0u;
+    for (int i = 0; i < n_parts; ++i) {  // in block order: the same sum every run
+      up += part_sum[(size_t)i * kPowerBands + b];
+      um = max(um, part_max[(size_t)i * kPowerBands + b]);
+    }
+    int e_sum = 0, e_one = 0;
+    (void)frexp(up, &e_sum);                             // up < 2^e_sum
+    (void)frexp((double)__uint_as_float(um), &e_one);   // the largest power < 2^e_one
+    scale[b] = up > 0.0 ? ldexp(1.0, min(63 - e_sum, kBandShift - e_one)) : 1.0;
+    used[b] = up > 0.0 ? 1 : 0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int n = 0;
+    for (int i = 0; i < kPowerBands; ++i)
+      if (used[i]) n = i + 1;
+    info[1] = n;
+  }
+}
+
+// Fixed-point word of each walked side that emits (band in the top bits);
+// the walk positions capped at the walk's end (the entries past it are never
+// reached: no ray position lies at or past their start).
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_q(int W, long long total, long long walk_total, long long* __restrict__ start,
+            const int* __restrict__ walk_face, const long long* __restrict__ face_off, const int* __restrict__ front,
+            const float* __restrict__ p, const unsigned char* __restrict__ band, const double* __restrict__ scale,
+            unsigned long long* __restrict__ q) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > W) return;
+  const long long s0 = start[k];
+  if (k < W) {
+    const int f = walk_face[k];
+    const int cnt[2] = {front[f], (int)(face_off[f + 1] - face_off[f]) - front[f]};
+    for (int side = 0; side < 2; ++side) {
+      const float ps = p[2 * f + side];
+      const int b = band[2 * k + side];
+      // sides without rays are never read; their power may lie outside the bands
+      q[2 * k + side] = s0 < total && ps > 0.f && cnt[side] > 0
+                            ? ((unsigned long long)b << kBandShift) | round_even((double)ps * scale[b])
+                            : 0ull;
+    }
+  }
+  start[k] = min(s0, walk_total);
+}
+
+// Zeroes the accumulator rows of the bands in use.
+__global__ void k_rad_zero(unsigned long long* __restrict__ acc, int nm, const int* __restrict__ info) {
+  const size_t n = (size_t)info[1] * nm;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    acc[i] = 0ull;
+}
+
+// Absorbed power per mesh: the band rows' fixed-point sums over their scales.
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_absorb(int nm, const unsigned long long* __restrict__ acc, const double* __restrict__ scale,
+                 const int* __restrict__ info, double* __restrict__ out) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const int n_bands = info[1];
+  double a = 0.0;
+  for (int b = 0; b < n_bands; ++b) a += (double)acc[(size_t)b * nm + m] / scale[b];
+  out[m] = a;
+}
+
+// Lays out the radiance work buffers for the scene's faces and meshes (once
+// per faces / meshes): the walk entries (faces of each mesh, mesh by mesh)
+// and their tables, the accumulators, the reduction partials, the hipCUB scan
+// storage, and the pinned staging of the per-step uploads and downloads.
+int rad_prepare(ffk_scene* s, hipStream_t st) {
+  if (s->rad_W >= 0) return FFK_OK;
+  const int nf = s->n_faces, nm = s->n_meshes;
+  long long W64 = 0;
+  for (int m = 0; m < nm; ++m) W64 += s->h_meshes[3 * m + 1];
+  FFK_REQUIRE(W64 < (1LL << 30), FFK_ERR_LIMIT, "radiance: %lld mesh faces", W64);
+  const int W = (int)W64;
+  const int n_parts = (W + 1 + kRadThreads - 1) / kRadThreads;
+  size_t scan_bytes = 0;
+  FFK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_bytes, (long long*)nullptr, (long long*)nullptr, W + 1,
+                                           st));
+  auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+  size_t off = 0;
+  auto take = [&](size_t b) {
+    const size_t o = off;
+    off += up(b);
+    return o;
+  };
+  const size_t o_face = take(4 * (size_t)W + 4), o_cnt = take(8 * ((size_t)W + 1)),
+               o_start = take(8 * ((size_t)W + 1)), o_q = take(16 * (size_t)W + 16), o_band = take(2 * (size_t)W + 2),
+               o_acc = take(8 * (size_t)kPowerBands * nm), o_abs = take(8 * (size_t)nm),
+               o_scale = take(8 * kPowerBands), o_p = take(8 * (size_t)nf), o_te = take(12 * (size_t)nm),
+               o_area = take(4 * (size_t)nf), o_pu = take(4 * (size_t)n_parts * (kPowerBands + 1)),
+               o_pd = take(8 * (size_t)n_parts * kPowerBands), o_info = take(16), o_scan = take(scan_bytes);
+  free_radiance(s);
+  int r;
+  if ((r = dmalloc(&s->d_rad_block, off)) < 0) return r;
+  char* d = static_cast<char*>(s->d_rad_block);
+  s->d_walk_face = reinterpret_cast<int*>(d + o_face);
+  s->d_walk_cnt = reinterpret_cast<long long*>(d + o_cnt);
+  s->d_walk_start = reinterpret_cast<long long*>(d + o_start);
+  s->d_walk_q = reinterpret_cast<unsigned long long*>(d + o_q);
+  s->d_walk_band = reinterpret_cast<unsigned char*>(d + o_band);
+  s->d_rad_acc = reinterpret_cast<unsigned long long*>(d + o_acc);
+  s->d_rad_abs = reinterpret_cast<double*>(d + o_abs);
+  s->d_rad_scale = reinterpret_cast<double*>(d + o_scale);
+  s->d_rad_p = reinterpret_cast<float*>(d + o_p);
+  s->d_rad_te = reinterpret_cast<float*>(d + o_te);
+  s->d_rad_area = reinterpret_cast<float*>(d + o_area);
+  s->d_rad_part_u = reinterpret_cast<unsigned*>(d + o_pu);
+  s->d_rad_part_d = reinterpret_cast<double*>(d + o_pd);
+  s->d_rad_info = reinterpret_cast<int*>(d + o_info);
+  s->d_rad_scan = d + o_scan;
+  s->rad_scan_bytes = scan_bytes;
+  s->rad_parts = n_parts;
+  // pinned staging: T and eps (3 nm floats), ray powers (2 nf floats), then
+  // the absorption (nm doubles) and the status words (4 ints)
+  const size_t pin = up(12 * (size_t)nm) + up(8 * (size_t)nf) + up(8 * (size_t)nm) + 16;
+  FFK_HIP(hipHostMalloc(&s->h_rad_pin, pin, hipHostMallocDefault));
+  // the walk entries' faces and the face areas, once
+  std::vector<int> wf((size_t)W + 1, 0);
+  int k = 0;
+  for (int m = 0; m < nm; ++m)
+    for (int f = s->h_meshes[3 * m]; f < s->h_meshes[3 * m] + s->h_meshes[3 * m + 1]; ++f) wf[k++] = f;
+  FFK_HIP(hipMemcpyAsync(s->d_walk_face, wf.data(), 4 * ((size_t)W + 1), hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(s->d_rad_area, s->h_area.data(), 4 * (size_t)nf, hipMemcpyHostToDevice, st));
+  FFK_HIP(hipStreamSynchronize(st));  // the vector above goes out of scope
+  s->rad_W = W;
+  return FFK_OK;
+}
+
+void free_radiance(ffk_scene* s) {
+  dfree(s->d_rad_block);
+  if (s->h_rad_pin) (void)hipHostFree(s->h_rad_pin);
+  s->d_rad_block = nullptr;
+  s->h_rad_pin = nullptr;
+  s->rad_W = -1;
+  s->rad_parts = 0;
+  s->d_walk_face = nullptr;
+  s->d_walk_cnt = nullptr;
+  s->d_walk_start = nullptr;
+  s->d_walk_q = nullptr;
+  s->d_walk_band = nullptr;
+  s->d_rad_acc = nullptr;
+  s->d_rad_abs = nullptr;
+  s->d_rad_scale = nullptr;
+  s->d_rad_p = nullptr;
+  s->d_rad_te = nullptr;
+  s->d_rad_area = nullptr;
+  s->d_rad_part_u = nullptr;
+  s->d_rad_part_d = nullptr;
+  s->d_rad_info = nullptr;
+  s->d_rad_scan = nullptr;
+}
+
+// Pinned staging regions of rad_prepare's layout.
+inline float* pin_te(ffk_scene* s) { return static_cast<float*>(s->h_rad_pin); }
+inline float* pin_p(ffk_scene* s) {
+  return reinterpret_cast<float*>(static_cast<char*>(s->h_rad_pin) + ((12 * (size_t)s->n_meshes + 255) & ~(size_t)255));
+}
+inline double* pin_abs(ffk_scene* s) {
+  const size_t o = ((12 * (size_t)s->n_meshes + 255) & ~(size_t)255) + ((8 * (size_t)s->n_faces + 255) & ~(size_t)255);
+  return reinterpret_cast<double*>(static_cast<char*>(s->h_rad_pin) + o);
+}
+inline int* pin_info(ffk_scene* s) {
+  return reinterpret_cast<int*>(reinterpret_cast<char*>(pin_abs(s)) + ((8 * (size_t)s->n_meshes + 255) & ~(size_t)255));
+}
+
+// The device pass over the ray powers in d_rad_p: walk table, bands, scales
+// and fixed-point words, the emit-cast-deposit kernel, the per-mesh sums;
+// copies the absorption to h_absorption (left untouched when a walked ray
+// power is negative or not finite: FFK_ERR_ARG).
+int rad_pass(ffk_scene* s, unsigned long long seed, double* h_absorption, unsigned long long* d_stats,
+             hipStream_t st) {
+  const int nm = s->n_meshes, W = s->rad_W, nb = (W + kRadThreads) / kRadThreads;
+  const long long total = s->n_rays;
+  const long long walk_total = s->h_mesh_off[nm];  // the walk's end: min(rays of the meshes' faces, total)
+  int* info = pin_info(s);
+  info[0] = 0;
+  info[1] = 0;
+  info[2] = 0;
+  info[3] = 0x7fffffff;
+  FFK_HIP(hipMemcpyAsync(s->d_rad_info, info, 16, hipMemcpyHostToDevice, st));
+  k_rad_counts<<<nb, kRadThreads, 0, st>>>(W, s->d_walk_face, s->d_face_off, s->d_walk_cnt);
+  size_t scan_bytes = s->rad_scan_bytes;
+  FFK_HIP(hipcub::DeviceScan::ExclusiveSum(s->d_rad_scan, scan_bytes, s->d_walk_cnt, s->d_walk_start, W + 1, st));
+  k_rad_entries<<<nb, kRadThreads, 0, st>>>(W, total, s->d_walk_start, s->d_walk_face, s->d_face_off, s->d_front,
+                                            s->d_rad_p, s->d_rad_part_u, s->d_rad_info);
+  k_rad_emax<<<1, kRadThreads, 0, st>>>(nb, s->d_rad_part_u, s->d_rad_info);
+  unsigned* part_bmax = s->d_rad_part_u + s->rad_parts;
+  k_rad_bands<<<nb, kRadThreads, 0, st>>>(W, total, s->d_walk_start, s->d_walk_face, s->d_face_off, s->d_front,
+                                          s->d_rad_p, s->d_rad_info, s->d_walk_band, s->d_rad_part_d, part_bmax);
+  k_rad_scales<<<1, 64, 0, st>>>(nb, s->d_rad_part_d, part_bmax, s->d_rad_scale, s->d_rad_info);
+  k_rad_q<<<nb, kRadThreads, 0, st>>>(W, total, walk_total, s->d_walk_start, s->d_walk_face, s->d_face_off,
+                                      s->d_front, s->d_rad_p, s->d_walk_band, s->d_rad_scale, s->d_walk_q);
+  k_rad_zero<<<(unsigned)std::min<long long>(1024, ((long long)kPowerBands * nm + 255) / 256), 256, 0, st>>>(
+      s->d_rad_acc, nm, s->d_rad_info);
+  FFK_HIP(hipGetLastError());
+  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
+  if (walk_total > 0 && s->tune_refill > 0 && s->tune_pool > 0) {
+    // Whole rounds of the blocks the device holds at once (the last round is
+    // not left part empty), at most 16384 rays and at least a ray per lane
+    // per block.
+    auto go = [&](auto kernel) {
+      int per_cu = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+      const long long resident = (long long)per_cu * device_cu_count();
+      const long long rounds = std::max<long long>(1, (walk_total + resident * 16384 - 1) / (resident * 16384));
+      const long long chunk = std::max<long long>(256, (walk_total + resident * rounds - 1) / (resident * rounds));
+      const unsigned blocks = (unsigned)((walk_total + chunk - 1) / chunk);
+      kernel<<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, W, s->d_walk_q, walk_total, chunk,
+                                     k0, k1, s->d_rad_acc, d_stats, s->tune_refill,
+                                     pool_arg(s->tune_pool, s->tune_walk));
+    };
+    const bool jump = (s->tune_walk & kWalkJump) != 0;
+    if (d_stats) jump ? go(k_radiance_pooled<true, 1>) : go(k_radiance_pooled<true, 2>);
+    else jump ? go(k_radiance_pooled<false, 1>) : go(k_radiance_pooled<false, 2>);
+  } else if (walk_total > 0) {
+    const long long blocks_needed = (walk_total + 255) / 256;
+    const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));
+    if (d_stats)
+      k_radiance<true><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, W, s->d_walk_q,
+                                               walk_total, k0, k1, s->d_rad_acc, d_stats);
+    else
+      k_radiance<false><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, W, s->d_walk_q,
+                                                walk_total, k0, k1, s->d_rad_acc, nullptr);
+  }
+  k_rad_absorb<<<(nm + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(nm, s->d_rad_acc, s->d_rad_scale,
+                                                                           s->d_rad_info, s->d_rad_abs);
+  FFK_HIP(hipGetLastError());
+  double* ab = pin_abs(s);
+  FFK_HIP(hipMemcpyAsync(ab, s->d_rad_abs, 8 * (size_t)nm, hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipMemcpyAsync(info, s->d_rad_info, 16, hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipStreamSynchronize(st));
+  FFK_REQUIRE(info[2] == 0, FFK_ERR_ARG, "ffk_radiance: ray power of face %d is negative or not finite", info[3]);
+  std::memcpy(h_absorption, ab, 8 * (size_t)nm);
+  return FFK_OK;
+}
+
 }  // namespace ffk
 
 using namespace ffk;
@@ -191,16 +602,17 @@ float weights_impl(const float* h_faces, const float* h_area, int n_faces, const
   return t;
 }
 
-// thermal_equation/radiance_cpu.cpp:148-178: the power of one front / rear
-// ray of each face and the emission added to each mesh.
+// thermal_equation/radiance_cpu.cpp:148-178: the emission added to each mesh
+// (float, face by face), and, when h_ray_power is given, the power of one
+// front / rear ray of each face (k_rad_powers computes the same on the device).
 void powers_impl(const float* h_faces, const float* h_area, int n_faces, const int* h_meshes, int n_meshes,
                  const float* h_eps2, const float* h_T, const int* h_front, const int* h_rear, float* h_ray_power,
                  float* h_emission) {
-  const float sigma = 5.670400e-8f;  // thermal_equation/radiance_cpu.cpp:29
-  for (int i = 0; i < 2 * n_faces; ++i) h_ray_power[i] = 0.f;
+  if (h_ray_power)
+    for (int i = 0; i < 2 * n_faces; ++i) h_ray_power[i] = 0.f;
   for (int m = 0; m < n_meshes; ++m) {
     const float T = h_T[m];
-    const float power_density = sigma * (T * T * T * T);
+    const float power_density = kSigma * (T * T * T * T);
     const float front_density = power_density * h_eps2[2 * m];
     const float rear_density = power_density * h_eps2[2 * m + 1];
     for (int f = h_meshes[3 * m]; f < h_meshes[3 * m] + h_meshes[3 * m + 1]; ++f) {
@@ -208,6 +620,7 @@ void powers_impl(const float* h_faces, const float* h_area, int n_faces, const i
       const float front_emission = front_density * a;
       const float rear_emission = rear_density * a;
       h_emission[m] += front_emission + rear_emission;
+      if (!h_ray_power) continue;
       h_ray_power[2 * f] = h_front[f] > 0 ? front_emission / h_front[f] : 0.f;
       h_ray_power[2 * f + 1] = h_rear[f] > 0 ? rear_emission / h_rear[f] : 0.f;
     }
@@ -243,178 +656,12 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   FFK_REQUIRE(s && h_ray_power && h_absorption, FFK_ERR_ARG, "ffk_radiance: bad args");
   FFK_REQUIRE(s->has_emission && s->has_grid, FFK_ERR_ARG, "ffk_radiance: emission layout and grid must be set");
   hipStream_t st = as_stream(stream);
-  const int nf = s->n_faces, nm = s->n_meshes;
-  const long long total = s->n_rays;
-  // Staging for W walk entries and nm meshes, in this order: start[W + 1]
-  // (int64), q[2W] (uint64), acc[kPowerBands * nm] (uint64), face[W + 1]
-  // (int32). Every walk entry is a face, so W = n_faces bounds them.
-  const size_t W = (size_t)nf;
-  const size_t off_q = 8 * (W + 1), off_acc = off_q + 16 * W, off_face = off_acc + 8 * (size_t)kPowerBands * nm;
-  const size_t bytes = off_face + 4 * (W + 1);
-  if ((int)W + 1 > s->rad_cap_walk || nm > s->rad_cap_meshes) {
-    FFK_HIP(hipStreamSynchronize(st));
-    dfree(s->d_walk_block);
-    s->d_walk_block = nullptr;
-    if (s->h_walk_block) (void)hipHostFree(s->h_walk_block);
-    s->h_walk_block = nullptr;
-    s->rad_cap_walk = s->rad_cap_meshes = 0;
-    int r;
-    if ((r = dmalloc(&s->d_walk_block, bytes)) < 0) return r;
-    FFK_HIP(hipHostMalloc(&s->h_walk_block, bytes, hipHostMallocDefault));
-    s->rad_cap_walk = (int)W + 1;
-    s->rad_cap_meshes = nm;
-  }
-  char* hb = static_cast<char*>(s->h_walk_block);
-  char* db = static_cast<char*>(s->d_walk_block);
-  long long* start = reinterpret_cast<long long*>(hb);
-  unsigned long long* q = reinterpret_cast<unsigned long long*>(hb + off_q);
-  unsigned long long* acc = reinterpret_cast<unsigned long long*>(hb + off_acc);
-  int* face = reinterpret_cast<int*>(hb + off_face);
-  s->d_walk_start = reinterpret_cast<long long*>(db);
-  s->d_walk_q = reinterpret_cast<unsigned long long*>(db + off_q);
-  s->d_rad_acc = reinterpret_cast<unsigned long long*>(db + off_acc);
-  s->d_walk_face = reinterpret_cast<int*>(db + off_face);
-  // Walk entries: faces in mesh order with their first walk position; the
-  // walk stops at the last emitted ray (radiance_cpu.cpp:170).
-  int n_walk = 0;
-  long long pos = 0;
-  float p_max = 0.f, p_min = 0.f;  // over the ray powers of sides that emit rays
-  double upper_all = 0.0;
-  for (int m = 0; m < nm && pos < total; ++m) {
-    const int f0 = s->h_meshes[3 * m], n = s->h_meshes[3 * m + 1];
-    for (int f = f0; f < f0 + n && pos < total; ++f) {
-      const float pf = h_ray_power[2 * f], pr = h_ray_power[2 * f + 1];
-      FFK_REQUIRE(pf >= 0.f && pr >= 0.f && std::isfinite(pf) && std::isfinite(pr), FFK_ERR_ARG,
-                  "ffk_radiance: ray power of face %d is negative or not finite", f);
-      start[n_walk] = pos;
-      face[n_walk] = f;
-      ++n_walk;
-      const int cf = s->h_front[f], cr = s->h_rear[f];
-      if (cf > 0 && pf > 0.f) {
-        p_max = std::max(p_max, pf);
-        p_min = p_min > 0.f ? std::min(p_min, pf) : pf;
-      }
-      if (cr > 0 && pr > 0.f) {
-        p_max = std::max(p_max, pr);
-        p_min = p_min > 0.f ? std::min(p_min, pr) : pr;
-      }
-      upper_all += (double)cf * pf + (double)cr * pr;
-      pos += (long long)cf + cr;
-    }
-  }
-  const long long walk_total = std::min(pos, total);
-  for (int m = 0; m < nm; ++m) h_absorption[m] = 0.0;
-  if (n_walk == 0 || p_max <= 0.f) return FFK_OK;
-  start[n_walk] = walk_total;
-  // Band of a ray power: its binary order below the largest power, in steps
-  // of kBandOrders (the last band takes everything smaller). Per band, the
-  // largest possible sum (every ray of the band absorbed by one mesh) and the
-  // largest single power fix the scale: sums stay below 2^63 and a single
-  // ray's value below 2^60, clear of the band bits. A ray of power p in band
-  // b is then rounded by at most 0.5 / scale_b, i.e. relatively by at most
-  // 2^-55 times the rays in the band, or 2^-52 (2^kBandOrders / 2^60).
-  auto exponent = [](float p) {  // ilogb of a positive float from its bits (subnormals rare)
-    uint32_t u;
-    std::memcpy(&u, &p, 4);
-    const int e = (int)((u >> 23) & 0xff);
-    return e ? e - 127 : std::ilogb(p);
-  };
-  const int e_max = exponent(p_max);
-  const bool one_band = (e_max - exponent(p_min)) / kBandOrders == 0;  // the usual case
-  double upper[kPowerBands] = {}, pmax_b[kPowerBands] = {};
-  int n_bands = 1;
-  unsigned char* band = nullptr;
-  if (one_band) {
-    upper[0] = upper_all;
-    pmax_b[0] = p_max;
-  } else {
-    s->rad_band.resize(2 * (size_t)n_walk);
-    band = s->rad_band.data();
-    for (int k = 0; k < n_walk; ++k) {
-      const int f = face[k];
-      const int cnt[2] = {s->h_front[f], s->h_rear[f]};
-      for (int side = 0; side < 2; ++side) {
-        const float p = h_ray_power[2 * f + side];
-        const bool emits = p > 0.f && cnt[side] > 0;
-        const int b = emits ? std::min(kPowerBands - 1, (e_max - exponent(p)) / kBandOrders) : 0;
-        band[2 * k + side] = (unsigned char)b;
-        if (!emits) continue;
-        upper[b] += (double)cnt[side] * p;
-        pmax_b[b] = std::max(pmax_b[b], (double)p);
-        n_bands = std::max(n_bands, b + 1);
-      }
-    }
-  }
-  double scale[kPowerBands];
-  for (int b = 0; b < kPowerBands; ++b) {
-    int e_sum = 0, e_one = 0;
-    std::frexp(upper[b], &e_sum);   // upper < 2^e_sum
-    std::frexp(pmax_b[b], &e_one);  // pmax < 2^e_one
-    scale[b] = upper[b] > 0.0 ? std::ldexp(1.0, std::min(63 - e_sum, kBandShift - e_one)) : 1.0;
-  }
-  // round half to even of a non-negative x < 2^60 (what llrint does in the
-  // default rounding mode; x - trunc(x) is exact, so the comparison is too)
-  auto round_even = [](double x) {
-    const long long v = (long long)x;
-    const double frac = x - (double)v;
-    return (unsigned long long)v + (frac > 0.5 || (frac == 0.5 && (v & 1)) ? 1ull : 0ull);
-  };
-  for (int k = 0; k < n_walk; ++k) {
-    const int f = face[k];
-    const int cnt[2] = {s->h_front[f], s->h_rear[f]};
-    for (int side = 0; side < 2; ++side) {
-      const float p = h_ray_power[2 * f + side];
-      const int b = band ? band[2 * k + side] : 0;
-      // sides without rays are never read; their power may lie outside the bands
-      q[2 * k + side] = p > 0.f && cnt[side] > 0
-                            ? ((unsigned long long)b << kBandShift) | round_even((double)p * scale[b])
-                            : 0ull;
-    }
-  }
-  const size_t n_acc = (size_t)n_bands * nm;
-  // start, q and (past acc) face in one copy; acc is zeroed on the device
-  FFK_HIP(hipMemcpyAsync(db, hb, off_acc, hipMemcpyHostToDevice, st));
-  FFK_HIP(hipMemcpyAsync(db + off_face, hb + off_face, 4 * ((size_t)n_walk + 1), hipMemcpyHostToDevice, st));
-  k_fill_acc<<<(unsigned)((n_acc + 255) / 256), 256, 0, st>>>(s->d_rad_acc, (int)n_acc);
-  const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
-  if (walk_total > 0 && s->tune_refill > 0 && s->tune_pool > 0) {
-    // Whole rounds of the blocks the device holds at once (the last round is
-    // not left part empty), at most 16384 rays and at least a ray per lane
-    // per block.
-    auto go = [&](auto kernel) {
-      int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
-      const long long resident = (long long)per_cu * device_cu_count();
-      const long long rounds = std::max<long long>(1, (walk_total + resident * 16384 - 1) / (resident * 16384));
-      const long long chunk = std::max<long long>(256, (walk_total + resident * rounds - 1) / (resident * rounds));
-      const unsigned blocks = (unsigned)((walk_total + chunk - 1) / chunk);
-      kernel<<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q, walk_total,
-                                     chunk, k0, k1, s->d_rad_acc, d_stats, s->tune_refill,
-                                     pool_arg(s->tune_pool, s->tune_walk));
-    };
-    const bool jump = (s->tune_walk & kWalkJump) != 0;
-    if (d_stats) jump ? go(k_radiance_pooled<true, 1>) : go(k_radiance_pooled<true, 2>);
-    else jump ? go(k_radiance_pooled<false, 1>) : go(k_radiance_pooled<false, 2>);
-  } else if (walk_total > 0) {
-    const long long blocks_needed = (walk_total + 255) / 256;
-    const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));
-    if (d_stats)
-      k_radiance<true><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q,
-                                               walk_total, k0, k1, s->d_rad_acc, d_stats);
-    else
-      k_radiance<false><<<blocks, 256, 0, st>>>(s->desc(), s->d_walk_start, s->d_walk_face, n_walk, s->d_walk_q,
-                                                walk_total, k0, k1, s->d_rad_acc, nullptr);
-  }
-  FFK_HIP(hipGetLastError());
-  FFK_HIP(hipMemcpyAsync(acc, s->d_rad_acc, 8 * n_acc, hipMemcpyDeviceToHost, st));
-  FFK_HIP(hipStreamSynchronize(st));
-  for (int m = 0; m < nm; ++m) {
-    double a = 0.0;
-    for (int b = 0; b < n_bands; ++b) a += (double)acc[(size_t)b * nm + m] / scale[b];
-    h_absorption[m] = a;
-  }
-  return FFK_OK;
+  int r;
+  if ((r = rad_prepare(s, st)) < 0) return r;
+  float* pp = pin_p(s);
+  std::memcpy(pp, h_ray_power, 8 * (size_t)s->n_faces);
+  FFK_HIP(hipMemcpyAsync(s->d_rad_p, pp, 8 * (size_t)s->n_faces, hipMemcpyHostToDevice, st));
+  return rad_pass(s, seed, h_absorption, d_stats, st);
 }
 
 int ffk_radiance_step(ffk_scene* s, const float* h_eps2, const float* h_T, int n_rays, int frame,
@@ -423,6 +670,7 @@ int ffk_radiance_step(ffk_scene* s, const float* h_eps2, const float* h_T, int n
   FFK_REQUIRE(s && s->has_faces && s->has_meshes && h_eps2 && h_T && h_emission && h_absorption && n_rays >= 0,
               FFK_ERR_ARG, "ffk_radiance_step: bad args");
   FFK_REQUIRE(s->has_grid, FFK_ERR_ARG, "ffk_radiance_step: the grid must be built");
+  hipStream_t st = as_stream(stream);
   const int nf = s->n_faces, nm = s->n_meshes;
   s->rad_w.resize(2 * (size_t)nf);
   const float total =
@@ -436,10 +684,20 @@ int ffk_radiance_step(ffk_scene* s, const float* h_eps2, const float* h_T, int n
     if ((r = ffk_scene_set_emission(s, s->rad_front.data(), s->rad_rear.data(), frame, nullptr, stream)) < 0)
       return r;
   }
-  s->rad_p.resize(2 * (size_t)nf);
-  powers_impl(s->h_faces.data(), s->h_area.data(), nf, s->h_meshes.data(), nm, h_eps2, h_T, s->h_front.data(),
-              s->h_rear.data(), s->rad_p.data(), h_emission);
-  return ffk_radiance(s, seed, s->rad_p.data(), h_absorption, d_stats, stream);
+  if ((r = rad_prepare(s, st)) < 0) return r;
+  // T and eps to the device, where the ray powers are computed (their bits
+  // are powers_impl's); meanwhile the per-mesh emission is added to
+  // h_emission here (radiance_cpu.cpp adds to the task's emission)
+  float* te = pin_te(s);
+  std::memcpy(te, h_T, 4 * (size_t)nm);
+  std::memcpy(te + nm, h_eps2, 8 * (size_t)nm);
+  FFK_HIP(hipMemcpyAsync(s->d_rad_te, te, 12 * (size_t)nm, hipMemcpyHostToDevice, st));
+  k_rad_powers<<<(nf + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(
+      nf, s->d_face2mesh, s->d_rad_te, nm, s->d_rad_area, s->d_face_off, s->d_front, s->d_rad_p);
+  FFK_HIP(hipGetLastError());
+  powers_impl(s->h_faces.data(), s->h_area.data(), nf, s->h_meshes.data(), nm, h_eps2, h_T, nullptr, nullptr,
+              nullptr, h_emission);
+  return rad_pass(s, seed, h_absorption, d_stats, st);
 }
 
 }  // extern "C"

@@ -372,8 +372,8 @@ void ffk_scene_destroy(ffk_scene* s) {
   dfree(s->d_face2mesh);
   dfree(s->d_desc);
   dfree(s->d_misc);
-  dfree(s->d_walk_block);
-  if (s->h_walk_block) (void)hipHostFree(s->h_walk_block);
+  free_radiance(s);
+  if (s->h_layout_pin) (void)hipHostFree(s->h_layout_pin);
   delete s;
 }
 
@@ -384,6 +384,7 @@ int ffk_scene_set_faces(ffk_scene* s, const float* h_faces, int n_faces, void* s
   FFK_HIP(hipStreamSynchronize(st));
   free_grid(s);
   free_emission(s);
+  free_radiance(s);
   dfree(s->d_faces);
   dfree(s->d_tri);
   dfree(s->d_face2mesh);
@@ -426,6 +427,7 @@ int ffk_scene_set_meshes(ffk_scene* s, const int* h_meshes, int n_meshes, void* 
   }
   FFK_HIP(hipStreamSynchronize(st));
   free_emission(s);
+  free_radiance(s);
   dfree(s->d_face2mesh);
   s->d_face2mesh = nullptr;
   int r;
@@ -478,9 +480,25 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
   if (!s->d_face_off && (r = dalloc(&s->d_face_off, (size_t)nf + 1)) < 0) return r;
   if (!s->d_front && (r = dalloc(&s->d_front, (size_t)nf)) < 0) return r;
   if (!s->d_mesh_off && (r = dalloc(&s->d_mesh_off, (size_t)nm + 1)) < 0) return r;
-  FFK_HIP(hipMemcpy(s->d_face_off, s->h_face_off.data(), 8 * ((size_t)nf + 1), hipMemcpyHostToDevice));
-  FFK_HIP(hipMemcpy(s->d_front, h_front, 4 * (size_t)nf, hipMemcpyHostToDevice));
-  FFK_HIP(hipMemcpy(s->d_mesh_off, s->h_mesh_off.data(), 8 * ((size_t)nm + 1), hipMemcpyHostToDevice));
+  // Uploads through pinned staging (the stream was drained above, so the
+  // staging is free to be rewritten).
+  const size_t b_face = 8 * ((size_t)nf + 1), b_front = 4 * (size_t)nf, b_mesh = 8 * ((size_t)nm + 1);
+  if (s->layout_pin_bytes < b_face + b_front + b_mesh) {
+    if (s->h_layout_pin) (void)hipHostFree(s->h_layout_pin);
+    s->h_layout_pin = nullptr;
+    s->layout_pin_bytes = 0;
+    FFK_HIP(hipHostMalloc(&s->h_layout_pin, b_face + b_front + b_mesh, hipHostMallocDefault));
+    s->layout_pin_bytes = b_face + b_front + b_mesh;
+  }
+  char* pin = static_cast<char*>(s->h_layout_pin);
+  std::memcpy(pin, s->h_face_off.data(), b_face);
+  std::memcpy(pin + b_face, h_front, b_front);
+  std::memcpy(pin + b_face + b_front, s->h_mesh_off.data(), b_mesh);
+  FFK_HIP(hipMemcpyAsync(s->d_face_off, pin, b_face, hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(s->d_front, pin + b_face, b_front, hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(s->d_mesh_off, pin + b_face + b_front, b_mesh, hipMemcpyHostToDevice, st));
+  // one wait for the three copies: callers may go on with another stream
+  FFK_HIP(hipStreamSynchronize(st));
   if (s->rot_frame != frame) {  // the rotations depend on the faces and the frame only
     k_face_rotation<<<(nf + 255) / 256, 256, 0, st>>>(s->d_faces, nf, frame, s->d_rot);
     FFK_HIP(hipGetLastError());

@@ -239,8 +239,30 @@ class Scene:
                                         _stream()), "ffk_scene_set_faces")
         check(lib().ffk_scene_set_meshes(self.h, self.meshes.ctypes.data_as(_i), len(self.meshes),
                                          _stream()), "ffk_scene_set_meshes")
-        self.n_rays = 0
-        self.front = self.rear = None
+        self._front = self._rear = None
+        self._n_rays, self._frame = 0, FRAME_REFERENCE
+        self._stale = False  # the library set the layout (radiance_step): read it back on access
+
+    def _layout(self):
+        if self._stale:
+            self._sync_layout()
+        return self
+
+    @property
+    def front(self):
+        return self._layout()._front
+
+    @property
+    def rear(self):
+        return self._layout()._rear
+
+    @property
+    def n_rays(self):
+        return self._layout()._n_rays
+
+    @property
+    def frame(self):
+        return self._layout()._frame
 
     @property
     def n_faces(self):
@@ -258,14 +280,12 @@ class Scene:
             if weights is None:
                 weights, total = area_weights(self.faces)
             front, rear = emitted_counts(weights, n_rays, total)
-        self.front, self.rear = _np(front, np.int32), _np(rear, np.int32)
+        fr, rr = _np(front, np.int32), _np(rear, np.int32)
         n = C.c_longlong()
-        check(lib().ffk_scene_set_emission(self.h, self.front.ctypes.data_as(_i),
-                                           self.rear.ctypes.data_as(_i), frame, C.byref(n), _stream()),
-              "ffk_scene_set_emission")
-        self.n_rays = n.value
-        self.frame = frame
-        return self.n_rays
+        check(lib().ffk_scene_set_emission(self.h, fr.ctypes.data_as(_i), rr.ctypes.data_as(_i), frame, C.byref(n),
+                                           _stream()), "ffk_scene_set_emission")
+        self._front, self._rear, self._n_rays, self._frame, self._stale = fr, rr, n.value, frame, False
+        return self._n_rays
 
     def _sync_layout(self):
         """Reads back the emission layout a library call set (front and rear
@@ -277,7 +297,7 @@ class Scene:
                                      C.byref(frame)), "ffk_scene_layout")
         n = C.c_longlong()
         check(lib().ffk_scene_counts(self.h, None, None, C.byref(n)), "ffk_scene_counts")
-        self.front, self.rear, self.n_rays, self.frame = fr, rr, n.value, frame.value
+        self._front, self._rear, self._n_rays, self._frame, self._stale = fr, rr, n.value, frame.value, False
 
     def sun_counts(self, direction, center, distance, radius, n_rays, seed, aperture=0, stats=None):
         """Hits per mesh side of n_rays parallel rays: uint64[n_meshes, 2] (front, rear)."""
@@ -314,7 +334,7 @@ class Scene:
         check(lib().ffk_radiance_step(self.h, e2.ctypes.data_as(_f), t.ctypes.data_as(_f), int(n_rays), int(frame),
                                       seed, em.ctypes.data_as(_f), ab.ctypes.data_as(C.POINTER(C.c_double)),
                                       _ptr(stats), _stream()), "ffk_radiance_step")
-        self._sync_layout()
+        self._stale = True
         return em, ab
 
     def radiance(self, seed, ray_power, stats=None):
