@@ -308,7 +308,8 @@ def batch_set_leg(ffk, scenes, n_scenes):
     t0 = time.perf_counter()
     tuned, cached = b.autotune(seeds, Fs, tallies=tallies, sample_rays=1 << 28)
     t_tune = time.perf_counter() - t0
-    log(f"batch autotune ({t_tune:.1f} s{', cached' if cached else ''}): {tuned}")
+    rpt_tuned = b.tuning()[1]
+    log(f"batch autotune ({t_tune:.1f} s{', cached' if cached else ''}): {tuned}, rays per tile {rpt_tuned or 'auto'}")
     b.run(seeds, Fs, tallies=tallies, use_graph=True)  # re-capture with the tuned settings
     wall_b, ev_b = timed(lambda: b.run(timed_seeds, Fs, tallies=tallies, use_graph=True))
     same = all(torch.equal(F.view(torch.int32), G.view(torch.int32)) for F, G in zip(Fs, F_plain))
@@ -334,6 +335,7 @@ def batch_set_leg(ffk, scenes, n_scenes):
                                 "than 24,576 meshes in the global tally), one hipGraph, autotuned per class",
                         "default_settings_value": round(n_rays / wall_u / 1e6, 2),
                         "class_tuning": {str(k): v for k, v in tuned.items()},
+                        "rays_per_tile": rpt_tuned or "auto (at most 65535)",
                         "tuning_seconds": round(t_tune, 2), "tuning_cached": cached},
             "speedup": round(wall_p / wall_b, 4), "F_bits_identical": same,
             "F_compare": "every scene's F, all N^2 cells, bitwise on the device", "unit": "Mrays/s",

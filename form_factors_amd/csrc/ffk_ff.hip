@@ -1038,6 +1038,19 @@ int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_
   return FFK_OK;
 }
 
+int ffk_batch_n_scenes(const ffk_batch* b) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  return (int)b->scenes.size();
+}
+
+int ffk_batch_get_tuning(const ffk_batch* b, int* block_size, int* rays_per_tile, int* n_blocks) {
+  FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
+  if (block_size) *block_size = b->block;
+  if (rays_per_tile) *rays_per_tile = b->rpt;
+  if (n_blocks) *n_blocks = b->n_blocks;
+  return FFK_OK;
+}
+
 }  // extern "C"
 
 template <int R>

@@ -390,8 +390,12 @@ int ffk_batch_autotune(ffk_batch* b, const unsigned long long* seeds, unsigned l
   unsigned long long key = 0;
   if ((r = ffk_batch_signature(b, &key)) < 0) return r;
   key = device_key(fnv1a(&sample_rays, sizeof(sample_rays), key));
+  int block0 = 0, rpt0 = 0, nb0 = 0;
+  if ((r = ffk_batch_get_tuning(b, &block0, &rpt0, &nb0)) < 0) return r;
   std::vector<int> v;
-  if (cache_get("batch", key, v) && v.size() == 5 * (size_t)n_classes) {
+  // cached: five words per class, then the batch-wide tile size
+  if (cache_get("batch", key, v) && v.size() == 5 * (size_t)n_classes + 1) {
+    if (v.back() != rpt0 && (r = ffk_batch_set_tuning(b, block0, v.back(), nb0)) < 0) return r;
     for (int c = 0; c < n_classes; ++c)
       if ((r = ffk_batch_set_class_tuning(b, v[5 * c], v[5 * c + 1], v[5 * c + 2], v[5 * c + 3], v[5 * c + 4])) < 0)
         return r;
@@ -456,6 +460,36 @@ int ffk_batch_autotune(ffk_batch* b, const unsigned long long* seeds, unsigned l
     if ((r = ffk_batch_set_class_tuning(b, id, best.block, best.refill, best.pool, best.walk)) < 0) return r;
     v.insert(v.end(), {id, best.block, best.refill, best.pool, best.walk});
   }
+  // Batch-wide tile size, on whole runs (the per-class settings stay: a new
+  // tile size plans the same classes).
+  const int n_sc = ffk_batch_n_scenes(b);
+  if (n_sc < 0) return n_sc;
+  std::vector<int> flags0(n_sc), flags(n_sc);
+  if ((r = ffk_batch_tally_flags(b, flags0.data())) < 0) return r;
+  auto whole = [&](float* ms) {
+    int rr = ffk_batch_run(b, seeds, d_tallies, d_Fs, 0, nullptr, stream);  // warm-up (re-plans the graph)
+    if (rr == FFK_OK) rr = time_ms(st, 2, [&] { return ffk_batch_run(b, seeds, d_tallies, d_Fs, 0, nullptr, stream); }, ms);
+    return rr;
+  };
+  float best_ms = 0.f;
+  if ((r = whole(&best_ms)) < 0) return r;
+  int best_rpt = rpt0;
+  for (int rpt : {32768, 16384, 8192}) {
+    if (rpt == rpt0) continue;
+    if ((r = ffk_batch_set_tuning(b, block0, rpt, nb0)) < 0) return r;
+    if ((r = ffk_batch_tally_flags(b, flags.data())) < 0) return r;
+    if (flags != flags0) continue;  // would need tally buffers the caller did not pass
+    float ms = 0.f;
+    if ((r = whole(&ms)) < 0) return r;
+    if (ms < 0.99f * best_ms) {
+      best_ms = ms;
+      best_rpt = rpt;
+    }
+  }
+  if ((r = ffk_batch_set_tuning(b, block0, best_rpt, nb0)) < 0) return r;
+  v.push_back(best_rpt);
+  // valid results for the chosen settings
+  if ((r = ffk_batch_run(b, seeds, d_tallies, d_Fs, 0, nullptr, stream)) < 0) return r;
   (void)hipStreamSynchronize(st);
   cache_put("batch", key, v);
   return FFK_OK;

@@ -63,6 +63,8 @@ SIGNATURES = {
     "ffk_batch_destroy": (None, [_vp]),
     "ffk_batch_run": (C.c_int, [_vp, C.POINTER(_ull), C.POINTER(_vp), C.POINTER(_vp), C.c_int, _vp, _vp]),
     "ffk_batch_set_tuning": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "ffk_batch_get_tuning": (C.c_int, [_vp, _i, _i, _i]),
+    "ffk_batch_n_scenes": (C.c_int, [_vp]),
     "ffk_batch_set_lds_limit": (C.c_int, [_vp, C.c_int]),
     "ffk_batch_set_concurrent": (C.c_int, [_vp, C.c_int]),
     "ffk_batch_set_order": (C.c_int, [_vp, C.c_int]),
@@ -496,6 +498,12 @@ class Batch:
 
     def set_tuning(self, block=0, rays_per_tile=0, n_blocks=0):
         check(lib().ffk_batch_set_tuning(self.h, block, rays_per_tile, n_blocks), "ffk_batch_set_tuning")
+
+    def tuning(self):
+        """(block, rays_per_tile, n_blocks) of the batch (0 = automatic)."""
+        v = [C.c_int() for _ in range(3)]
+        check(lib().ffk_batch_get_tuning(self.h, *(C.byref(x) for x in v)), "ffk_batch_get_tuning")
+        return tuple(x.value for x in v)
 
     def set_refill(self, lanes, pool=2):
         check(lib().ffk_batch_set_refill(self.h, int(lanes), int(pool)), "ffk_batch_set_refill")

@@ -214,6 +214,10 @@ void ffk_batch_destroy(ffk_batch* b);
 int ffk_batch_run(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
                   float* const* d_Fs, int use_graph, unsigned long long* d_stats, void* stream);
 int ffk_batch_set_tuning(ffk_batch* b, int block_size, int rays_per_tile, int n_blocks);
+/* Number of scenes in the batch. */
+int ffk_batch_n_scenes(const ffk_batch* b);
+/* The batch-wide settings of ffk_batch_set_tuning (0 = automatic; any pointer may be NULL). */
+int ffk_batch_get_tuning(const ffk_batch* b, int* block_size, int* rays_per_tile, int* n_blocks);
 /* Scenes whose 16-bit row histogram would exceed `bytes` of LDS are tallied
  * in the global uint64 tally instead (more resident wavefronts per CU, at
  * the cost of N^2 tally traffic); 0 restores the default (48 KB, the rule
@@ -252,9 +256,15 @@ int ffk_batch_run_class(ffk_batch* b, int class_id, long long max_rays, void* st
  * 512, 1024 where the histogram allows), timed greedily with
  * ffk_batch_run_class on about sample_rays of the class; a setting replaces
  * the defaults only when it is 3 % faster there and then 1 % faster over the
- * whole class. The winners are set with ffk_batch_set_class_tuning. Results are cached per scene set (the
- * scenes' signatures and the device) in memory and in the tuning cache file
- * (ffk_tune_cache_path). Returns 0 when tuned, 1 when taken from a cache. */
+ * whole class. The winners are set with ffk_batch_set_class_tuning. Then,
+ * batch-wide, the tile size (rays_per_tile 32768, 16384, 8192 against the
+ * current one), timed on whole runs and kept when 1 % faster; only sizes that
+ * leave every scene's tally flag as it is are tried, so the caller's buffers
+ * stay valid. (The persistent grid is not tuned: it is the blocks the device
+ * holds at once; more only find the tile queue empty, fewer leave CUs idle.)
+ * Results are cached per scene set (the scenes' signatures and the device)
+ * in memory and in the tuning cache file (ffk_tune_cache_path). Returns 0
+ * when tuned, 1 when taken from a cache. */
 int ffk_batch_autotune(ffk_batch* b, const unsigned long long* seeds, unsigned long long* const* d_tallies,
                        float* const* d_Fs, long long sample_rays, void* stream);
 /* h_flags[i] = 1 when scene i needs a tally buffer in ffk_batch_run, else 0. */

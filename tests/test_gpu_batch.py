@@ -184,10 +184,11 @@ def test_batch_refuses_a_rebuilt_scene(gpu):
 
 
 def test_batch_autotune_keeps_bits_and_caches_on_disk(gpu, tmp_path, monkeypatch):
-    """Per-class tuning (walk, ray scheduling, block) changes no F bit; its
-    result is written to the tuning cache file and, once the in-memory
-    results are dropped, read back from it for a new batch of the same
-    scenes. The single-scene autotuner shares the file."""
+    """Per-class tuning (walk, ray scheduling, block) and the batch-wide tile
+    size change no F bit; the result is written to the tuning cache file and,
+    once the in-memory results are dropped, read back from it for a new batch
+    of the same scenes (the same class settings and tile size). The
+    single-scene autotuner shares the file."""
     import torch
 
     cache = tmp_path / "autotune.txt"
@@ -204,7 +205,7 @@ def test_batch_autotune_keeps_bits_and_caches_on_disk(gpu, tmp_path, monkeypatch
     tuned, cached = b.autotune(seeds, Fs, tallies, sample_rays=1 << 18)
     assert not cached and len(tuned) == len(b.classes())
     torch.cuda.synchronize()
-    for F, want in zip(Fs, plain):  # the autotuner's own (untuned) run
+    for F, want in zip(Fs, plain):  # the autotuner's own last run
         assert np.array_equal(bits(F.cpu().numpy()), bits(want))
     for F in Fs:
         F.fill_(-1.0)
@@ -212,13 +213,14 @@ def test_batch_autotune_keeps_bits_and_caches_on_disk(gpu, tmp_path, monkeypatch
     torch.cuda.synchronize()
     for F, want in zip(Fs, plain):
         assert np.array_equal(bits(F.cpu().numpy()), bits(want))
+    tiling = b.tuning()
     b.close()
     lines = cache.read_text().splitlines()
     assert len(lines) == 1 and lines[0].startswith("batch ")
     gpu.lib().ffk_tune_cache_reset()
     b2 = gpu.Batch(scs)
     tuned2, cached2 = b2.autotune(seeds, Fs, b2.make_tallies(), sample_rays=1 << 18)
-    assert cached2 and tuned2 == tuned
+    assert cached2 and tuned2 == tuned and b2.tuning() == tiling
     b2.close()
     # the single-scene tuner: timed once, then read back from the file
     sc = scs[1]
