@@ -765,6 +765,25 @@ def main():
         log(f"walk mode {walk} (one launch each: " + " / ".join(f"{walk_times[m]:.2f}" for m in sorted(walk_times)) + " ms)")
     else:
         walk = args.walk
+    # Ray schedule (same hits in every setting): unless given, two untimed
+    # launches of each of the autotuner's candidates, the fastest kept.
+    sched_times = {}
+    if args.refill < 0 and args.pool < 0:
+        for r, p in ((32, 2), (24, 3), (16, 4), (40, 3), (40, 4)):
+            sc.set_refill(r, p)
+            best = 3e38
+            for _ in range(2):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                sc.ff_fused(seed, F=F, need_tally=False)
+                e1.record(stream)
+                torch.cuda.synchronize()
+                best = min(best, e0.elapsed_time(e1))
+            sched_times[f"{r}/{p}"] = best
+        pick = min(sched_times, key=sched_times.get)
+        sc.set_refill(*(int(x) for x in pick.split("/")))
+        log(f"ray schedule {pick} (" + " / ".join(f"{k} {v:.2f}" for k, v in sched_times.items()) + " ms)")
+    schedule = "%d/%d" % sc.get_refill()
     for _ in range(args.warmup):
         sc.ff_fused(seed, F=F, need_tally=False)
     torch.cuda.synchronize()
@@ -817,7 +836,8 @@ def main():
                                f"triangle, {n_rays} Malley rays (~{args.rays_per_face}/triangle), "
                                f"grid {args.grid}^3, plain fused emit-cast-tally kernel, F {N}x{N} fp32",
                    "n_faces": len(faces), "n_rays": n_rays, "grid": list(g["dims"]), "parallelism": "single-gpu",
-                   "walk_mode": walk, "walk_probe_ms": {str(k): round(v, 3) for k, v in walk_times.items()}},
+                   "walk_mode": walk, "walk_probe_ms": {str(k): round(v, 3) for k, v in walk_times.items()},
+                   "ray_schedule": schedule, "schedule_probe_ms": {k: round(v, 3) for k, v in sched_times.items()}},
         "roofline": {"bound": "valu", "achieved": round(flops / (kern_ms * 1e-3) / 1e12, 3), "peak": PEAK_FP32,
                      "unit": "TFLOP/s", "frac": round(flops / (kern_ms * 1e-3) / 1e12 / PEAK_FP32, 4),
                      "traffic": traffic_from_profiles("k_ff_fused_icosphere20k_traffic.json"),
@@ -838,6 +858,7 @@ def main():
         try:
             b = ffk.Batch([sc])
             b.set_walk(walk)
+            b.set_refill(*sc.get_refill())
             b.run([seed], [F], use_graph=True)
             torch.cuda.synchronize()
             reps = max(2, args.steps // 2)

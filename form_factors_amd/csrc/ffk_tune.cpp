@@ -49,7 +49,7 @@ struct tuned {
 // (refill threshold, pooled-test ratio) pairs the tuner times: the default
 // first, then schedules that start rays sooner and pool tests later, which
 // suit scenes where rays meet many occupied cells.
-constexpr int kSchedules[][2] = {{32, 2}, {24, 3}, {16, 4}};
+constexpr int kSchedules[][2] = {{32, 2}, {24, 3}, {16, 4}, {40, 3}, {40, 4}};
 
 unsigned long long fnv1a(const void* p, size_t n, unsigned long long h) {
   const unsigned char* b = static_cast<const unsigned char*>(p);

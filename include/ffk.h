@@ -252,7 +252,7 @@ int ffk_batch_get_class_tuning(const ffk_batch* b, int class_id, int* block_size
 int ffk_batch_run_class(ffk_batch* b, int class_id, long long max_rays, void* stream);
 /* Per-class autotuning of the batch: one untuned ffk_batch_run with these
  * buffers (its results are valid), then for each class the walk mode, the
- * ray scheduling (refill/pool 32/2, 24/3, 16/4) and the block size (256,
+ * ray scheduling (refill/pool 32/2, 24/3, 16/4, 40/3, 40/4) and the block size (256,
  * 512, 1024 where the histogram allows), timed greedily with
  * ffk_batch_run_class on about sample_rays of the class; a setting replaces
  * the defaults only when it is 3 % faster there and then 1 % faster over the
@@ -277,7 +277,7 @@ int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags);
  * F fits in half the free memory, the best candidates are re-timed with the
  * fused kernel on the full layout), the fused kernel's block size (256, 512
  * or 1024), the walk mode (ffk_set_walk) and, timed the same way, the ray
- * scheduling (ffk_set_refill: 32/2, 24/3 or 16/4), then leaves the scene built
+ * scheduling (ffk_set_refill: 32/2, 24/3, 16/4, 40/3 or 40/4), then leaves the scene built
  * and tuned with them (ffk_get_refill reads the scheduling back). Results are
  * cached per scene signature (faces, meshes, ray counts, frame) for the life
  * of the process. Returns 0 when tuned, 1 when taken from the cache. None of
