@@ -359,6 +359,7 @@ struct ffk_scene {
   void* d_rad_block = nullptr;
   void* h_rad_pin = nullptr;
   int* d_walk_face = nullptr;               // W: face of each walk entry
+  int* d_walk_mesh = nullptr;               // W: mesh of each walk entry
   long long* d_walk_cnt = nullptr;          // W + 1: rays of each entry's face (then 0)
   long long* d_walk_start = nullptr;        // W + 1: first walk position of each entry
   unsigned long long* d_walk_q = nullptr;   // 2W: fixed-point front / rear ray power
@@ -366,7 +367,8 @@ struct ffk_scene {
   unsigned long long* d_rad_acc = nullptr;  // kPowerBands x n_meshes fixed-point sums
   double* d_rad_abs = nullptr;              // n_meshes: absorbed power
   double* d_rad_scale = nullptr;            // kPowerBands fixed-point scales
-  float* d_rad_p = nullptr;                 // 2 n_faces: ray power per face side
+  float* d_rad_p = nullptr;                 // 2W: ray power per walk entry side
+  float* d_rad_pface = nullptr;             // 2 n_faces: caller-given power per face side
   float* d_rad_te = nullptr;                // T (n_meshes), eps (2 n_meshes)
   float* d_rad_area = nullptr;              // n_faces: h_area
   unsigned* d_rad_part_u = nullptr;         // reduction partials (see ffk_radiance.hip)

@@ -181,20 +181,21 @@ __device__ inline T block_reduce(T v, Op op, T* sh) {
   return r;
 }
 
-// Ray power of each face side (radiance_cpu.cpp:148-178): the side's
-// emission over its ray count, in the float expressions of the host
-// powers_impl, so the values are its bits. Faces in no mesh carry 0; a face of
-// several meshes takes the last one's (as powers_impl, which overwrites).
+// Ray power of each walk entry's sides (radiance_cpu.cpp:148-176): the
+// emission of the entry's face under the entry's mesh over the face's ray
+// count, in the float expressions of the host powers_impl, so the values are
+// its bits. (A face in several meshes has an entry in each, each with its
+// mesh's temperature and emissivities, as in the reference's walk.)
 // te: T[n_meshes] then eps[2 * n_meshes].
 __global__ void __launch_bounds__(kRadThreads)
-    k_rad_powers(int nf, const int* __restrict__ face2mesh, const float* __restrict__ te, int nm,
-                 const float* __restrict__ area, const long long* __restrict__ face_off,
-                 const int* __restrict__ front, float* __restrict__ p) {
-  const int f = blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= nf) return;
-  const int m = face2mesh[f];
+    k_rad_powers(int W, const int* __restrict__ walk_face, const int* __restrict__ walk_mesh,
+                 const float* __restrict__ te, int nm, const float* __restrict__ area,
+                 const long long* __restrict__ face_off, const int* __restrict__ front, float* __restrict__ p) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= W) return;
+  const int f = walk_face[k], m = walk_mesh[k];
   float pf = 0.f, pr = 0.f;
-  if (m >= 0) {
+  {
     const float T = te[m];
     const float power_density = kSigma * (T * T * T * T);
     const float front_density = power_density * te[nm + 2 * m];
@@ -206,8 +207,18 @@ __global__ void __launch_bounds__(kRadThreads)
     pf = cf > 0 ? front_emission / cf : 0.f;
     pr = cr > 0 ? rear_emission / cr : 0.f;
   }
-  p[2 * f] = pf;
-  p[2 * f + 1] = pr;
+  p[2 * k] = pf;
+  p[2 * k + 1] = pr;
+}
+
+// Caller-given powers per face side, copied to the walk entries.
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_expand(int W, const int* __restrict__ walk_face, const float* __restrict__ pface, float* __restrict__ p) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= W) return;
+  const int f = walk_face[k];
+  p[2 * k] = pface[2 * f];
+  p[2 * k + 1] = pface[2 * f + 1];
 }
 
 // Rays of each walk entry's face; entry W (the end) has none.
@@ -237,7 +248,7 @@ __global__ void __launch_bounds__(kRadThreads)
   unsigned vmax = 0u;
   if (k < W && start[k] < total) {
     const int f = walk_face[k];
-    const float pf = p[2 * f], pr = p[2 * f + 1];
+    const float pf = p[2 * k], pr = p[2 * k + 1];
     if (!(pf >= 0.f && pr >= 0.f && isfinite(pf) && isfinite(pr))) {
       atomicOr(&info[2], 1);
       atomicMin(&info[3], f);
@@ -279,7 +290,7 @@ __global__ void __launch_bounds__(kRadThreads)
     const int f = walk_face[k];
     const int cnt[2] = {front[f], (int)(face_off[f + 1] - face_off[f]) - front[f]};
     for (int side = 0; side < 2; ++side) {
-      const float ps = p[2 * f + side];
+      const float ps = p[2 * k + side];
       const bool emits = ps > 0.f && cnt[side] > 0;
       const int b = emits ? min(kPowerBands - 1, (e_max - pow_exponent(ps)) / kBandOrders) : 0;
       band[2 * k + side] = (unsigned char)b;
@@ -348,7 +359,7 @@ __global__ void __launch_bounds__(kRadThreads)
     const int f = walk_face[k];
     const int cnt[2] = {front[f], (int)(face_off[f + 1] - face_off[f]) - front[f]};
     for (int side = 0; side < 2; ++side) {
-      const float ps = p[2 * f + side];
+      const float ps = p[2 * k + side];
       const int b = band[2 * k + side];
       // sides without rays are never read; their power may lie outside the bands
       q[2 * k + side] = s0 < total && ps > 0.f && cnt[side] > 0
@@ -400,10 +411,11 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
     off += up(b);
     return o;
   };
-  const size_t o_face = take(4 * (size_t)W + 4), o_cnt = take(8 * ((size_t)W + 1)),
+  const size_t o_face = take(4 * (size_t)W + 4), o_mesh = take(4 * (size_t)W + 4), o_cnt = take(8 * ((size_t)W + 1)),
                o_start = take(8 * ((size_t)W + 1)), o_q = take(16 * (size_t)W + 16), o_band = take(2 * (size_t)W + 2),
                o_acc = take(8 * (size_t)kPowerBands * nm), o_abs = take(8 * (size_t)nm),
-               o_scale = take(8 * kPowerBands), o_p = take(8 * (size_t)nf), o_te = take(12 * (size_t)nm),
+               o_scale = take(8 * kPowerBands), o_p = take(8 * (size_t)W + 8), o_pface = take(8 * (size_t)nf),
+               o_te = take(12 * (size_t)nm),
                o_area = take(4 * (size_t)nf), o_pu = take(4 * (size_t)n_parts * (kPowerBands + 1)),
                o_pd = take(8 * (size_t)n_parts * kPowerBands), o_info = take(16), o_scan = take(scan_bytes);
   free_radiance(s);
@@ -411,6 +423,7 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
   if ((r = dmalloc(&s->d_rad_block, off)) < 0) return r;
   char* d = static_cast<char*>(s->d_rad_block);
   s->d_walk_face = reinterpret_cast<int*>(d + o_face);
+  s->d_walk_mesh = reinterpret_cast<int*>(d + o_mesh);
   s->d_walk_cnt = reinterpret_cast<long long*>(d + o_cnt);
   s->d_walk_start = reinterpret_cast<long long*>(d + o_start);
   s->d_walk_q = reinterpret_cast<unsigned long long*>(d + o_q);
@@ -419,6 +432,7 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
   s->d_rad_abs = reinterpret_cast<double*>(d + o_abs);
   s->d_rad_scale = reinterpret_cast<double*>(d + o_scale);
   s->d_rad_p = reinterpret_cast<float*>(d + o_p);
+  s->d_rad_pface = reinterpret_cast<float*>(d + o_pface);
   s->d_rad_te = reinterpret_cast<float*>(d + o_te);
   s->d_rad_area = reinterpret_cast<float*>(d + o_area);
   s->d_rad_part_u = reinterpret_cast<unsigned*>(d + o_pu);
@@ -431,12 +445,16 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
   // the absorption (nm doubles) and the status words (4 ints)
   const size_t pin = up(12 * (size_t)nm) + up(8 * (size_t)nf) + up(8 * (size_t)nm) + 16;
   FFK_HIP(hipHostMalloc(&s->h_rad_pin, pin, hipHostMallocDefault));
-  // the walk entries' faces and the face areas, once
-  std::vector<int> wf((size_t)W + 1, 0);
+  // the walk entries' faces and meshes and the face areas, once
+  std::vector<int> wf((size_t)W + 1, 0), wm((size_t)W + 1, 0);
   int k = 0;
   for (int m = 0; m < nm; ++m)
-    for (int f = s->h_meshes[3 * m]; f < s->h_meshes[3 * m] + s->h_meshes[3 * m + 1]; ++f) wf[k++] = f;
+    for (int f = s->h_meshes[3 * m]; f < s->h_meshes[3 * m] + s->h_meshes[3 * m + 1]; ++f) {
+      wf[k] = f;
+      wm[k++] = m;
+    }
   FFK_HIP(hipMemcpyAsync(s->d_walk_face, wf.data(), 4 * ((size_t)W + 1), hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(s->d_walk_mesh, wm.data(), 4 * ((size_t)W + 1), hipMemcpyHostToDevice, st));
   FFK_HIP(hipMemcpyAsync(s->d_rad_area, s->h_area.data(), 4 * (size_t)nf, hipMemcpyHostToDevice, st));
   FFK_HIP(hipStreamSynchronize(st));  // the vector above goes out of scope
   s->rad_W = W;
@@ -451,6 +469,7 @@ void free_radiance(ffk_scene* s) {
   s->rad_W = -1;
   s->rad_parts = 0;
   s->d_walk_face = nullptr;
+  s->d_walk_mesh = nullptr;
   s->d_walk_cnt = nullptr;
   s->d_walk_start = nullptr;
   s->d_walk_q = nullptr;
@@ -459,6 +478,7 @@ void free_radiance(ffk_scene* s) {
   s->d_rad_abs = nullptr;
   s->d_rad_scale = nullptr;
   s->d_rad_p = nullptr;
+  s->d_rad_pface = nullptr;
   s->d_rad_te = nullptr;
   s->d_rad_area = nullptr;
   s->d_rad_part_u = nullptr;
@@ -480,7 +500,7 @@ inline int* pin_info(ffk_scene* s) {
   return reinterpret_cast<int*>(reinterpret_cast<char*>(pin_abs(s)) + ((8 * (size_t)s->n_meshes + 255) & ~(size_t)255));
 }
 
-// The device pass over the ray powers in d_rad_p: walk table, bands, scales
+// The device pass over the walk entries' ray powers in d_rad_p: walk table, bands, scales
 // and fixed-point words, the emit-cast-deposit kernel, the per-mesh sums;
 // copies the absorption to h_absorption (left untouched when a walked ray
 // power is negative or not finite: FFK_ERR_ARG).
@@ -660,7 +680,11 @@ int ffk_radiance(ffk_scene* s, unsigned long long seed, const float* h_ray_power
   if ((r = rad_prepare(s, st)) < 0) return r;
   float* pp = pin_p(s);
   std::memcpy(pp, h_ray_power, 8 * (size_t)s->n_faces);
-  FFK_HIP(hipMemcpyAsync(s->d_rad_p, pp, 8 * (size_t)s->n_faces, hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(s->d_rad_pface, pp, 8 * (size_t)s->n_faces, hipMemcpyHostToDevice, st));
+  if (s->rad_W > 0)
+    k_rad_expand<<<(s->rad_W + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(s->rad_W, s->d_walk_face,
+                                                                                    s->d_rad_pface, s->d_rad_p);
+  FFK_HIP(hipGetLastError());
   return rad_pass(s, seed, h_absorption, d_stats, st);
 }
 
@@ -692,8 +716,10 @@ int ffk_radiance_step(ffk_scene* s, const float* h_eps2, const float* h_T, int n
   std::memcpy(te, h_T, 4 * (size_t)nm);
   std::memcpy(te + nm, h_eps2, 8 * (size_t)nm);
   FFK_HIP(hipMemcpyAsync(s->d_rad_te, te, 12 * (size_t)nm, hipMemcpyHostToDevice, st));
-  k_rad_powers<<<(nf + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(
-      nf, s->d_face2mesh, s->d_rad_te, nm, s->d_rad_area, s->d_face_off, s->d_front, s->d_rad_p);
+  if (s->rad_W > 0)
+    k_rad_powers<<<(s->rad_W + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(
+        s->rad_W, s->d_walk_face, s->d_walk_mesh, s->d_rad_te, nm, s->d_rad_area, s->d_face_off, s->d_front,
+        s->d_rad_p);
   FFK_HIP(hipGetLastError());
   powers_impl(s->h_faces.data(), s->h_area.data(), nf, s->h_meshes.data(), nm, h_eps2, h_T, nullptr, nullptr,
               nullptr, h_emission);

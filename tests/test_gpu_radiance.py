@@ -46,6 +46,14 @@ def _cases():
     T = np.where(rng.random(len(meshes)) < 0.5, 20.0, 2000.0).astype(np.float32)
     out.append(("wide_T_20K_2000K", faces, meshes, rng.uniform(0.1, 1, (len(meshes), 2)).astype(np.float32), T,
                 200_000))
+    # meshes that share faces (the reference walks a shared face once per
+    # mesh, each time with that mesh's temperature and emissivities; a hit on
+    # it counts for the last mesh): more walk entries than faces
+    faces, meshes = scenes.confetti_scene(200, seed=9)
+    extra = np.array([[f0, n, 0] for f0, n in ((0, 40), (30, 60), (100, 7), (5, 200), (150, 90))], np.int32)
+    meshes = np.concatenate([meshes, extra]).astype(np.int32)
+    out.append(("overlapping_meshes", faces, meshes, rng.uniform(0.1, 1, (len(meshes), 2)).astype(np.float32),
+                rng.uniform(250, 400, len(meshes)).astype(np.float32), 150_000))
     return out
 
 
@@ -58,7 +66,7 @@ def rel_tol(n_rays):
 
 
 @pytest.mark.parametrize("frame", [0, 1])
-@pytest.mark.parametrize("case", range(6))
+@pytest.mark.parametrize("case", range(7))
 def test_radiance_matches_oracle(gpu, oracle, case, frame):
     name, faces, meshes, eps, T, n_rays = _cases()[case]
     print(f"radiance case {name} frame {frame}", flush=True)
@@ -97,7 +105,7 @@ def test_radiance_homogeneous_and_repeatable(gpu):
     assert abs(ab1.sum() / em1.astype(np.float64).sum() - 0.5) < 1e-3
 
 
-@pytest.mark.parametrize("case", [0, 3, 4])
+@pytest.mark.parametrize("case", [0, 3, 4, 6])
 def test_radiance_schedules_agree(gpu, case):
     """Per-ray casting, lane refill and pooled tests deposit the same fixed
     point sums: identical absorption bits in every setting."""
