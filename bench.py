@@ -64,7 +64,8 @@ def parse():
     ap.add_argument("--refill", type=int, default=-1, help="fused-kernel refill threshold (lanes; -1 = default)")
     ap.add_argument("--pool", type=int, default=-1, help="pooled-test ratio (0 = per-lane tests; -1 = default)")
     ap.add_argument("--walk", type=int, default=-1,
-                    help="walk mode 0..3 (bit 0: jump for every empty cell, bit 1: whole chunks of pooled tests); "
+                    help="walk mode 0..7 (bit 0: jump for every empty cell, bit 1: whole chunks of pooled tests, "
+                         "bit 2: face terms loaded in two stages); "
                          "-1 = one probe launch of each, the fastest kept")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="run one untimed device copy of this size (HBM counter calibration under rocprofv3)")
@@ -752,7 +753,7 @@ def main():
     # without changing the grid this configuration fixes.
     walk_times = {}
     if args.walk < 0:
-        for m in (0, 1, 2, 3):
+        for m in range(8):
             sc.set_walk(m)
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)

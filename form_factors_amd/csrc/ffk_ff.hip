@@ -817,14 +817,14 @@ int ffk_get_refill(const ffk_scene* s, int* lanes, int* pool) {
 
 int ffk_set_walk(ffk_scene* s, int mode) {
   FFK_REQUIRE(s, FFK_ERR_ARG, "scene is NULL");
-  FFK_REQUIRE(mode >= 0 && mode <= 3, FFK_ERR_ARG, "walk mode %d is not in 0..3", mode);
+  FFK_REQUIRE(mode >= 0 && mode <= 7, FFK_ERR_ARG, "walk mode %d is not in 0..7", mode);
   s->tune_walk = mode;
   return FFK_OK;
 }
 
 int ffk_batch_set_walk(ffk_batch* b, int mode) {
   FFK_REQUIRE(b, FFK_ERR_ARG, "batch is NULL");
-  FFK_REQUIRE(mode >= 0 && mode <= 3, FFK_ERR_ARG, "walk mode %d is not in 0..3", mode);
+  FFK_REQUIRE(mode >= 0 && mode <= 7, FFK_ERR_ARG, "walk mode %d is not in 0..7", mode);
   b->walk = mode;
   free_graph(b);
   return FFK_OK;
@@ -1268,7 +1268,7 @@ int ffk_batch_set_class_tuning(ffk_batch* b, int class_id, int block_size, int r
   FFK_REQUIRE(b && class_id >= 0 && class_id < 6, FFK_ERR_ARG, "ffk_batch_set_class_tuning: class %d", class_id);
   FFK_REQUIRE(block_size <= 0 || (block_size % 64 == 0 && block_size <= 1024), FFK_ERR_ARG, "block size %d",
               block_size);
-  FFK_REQUIRE(refill <= 64 && pool <= 64 && walk <= 3, FFK_ERR_ARG, "refill %d / pool %d / walk %d", refill, pool,
+  FFK_REQUIRE(refill <= 64 && pool <= 64 && walk <= 7, FFK_ERR_ARG, "refill %d / pool %d / walk %d", refill, pool,
               walk);
   ffk_batch::class_tuning& t = b->ctune[class_id];
   t.block = block_size > 0 ? block_size : -1;

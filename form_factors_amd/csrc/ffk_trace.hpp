@@ -431,9 +431,47 @@ constexpr int kPoolWords = 64;
 // Walk mode bits (ffk_set_walk): kWalkJump = the jump code for every empty
 // cell (template WALK 1, else WALK 2); kWalkWhole = pooled tests dealt in
 // whole chunks, passed to pooled_walk as kPoolWhole in its pool argument.
-constexpr int kWalkJump = FFK_WALK_JUMP, kWalkWhole = FFK_WALK_WHOLE, kPoolWhole = 0x100;
+constexpr int kWalkJump = FFK_WALK_JUMP, kWalkWhole = FFK_WALK_WHOLE, kWalkStaged = FFK_WALK_STAGED;
+constexpr int kPoolWhole = 0x100, kPoolStaged = 0x200;
 constexpr int kDefaultWalk = FFK_WALK_DEFAULT;
-__host__ __device__ inline int pool_arg(int pool, int walk) { return pool | ((walk & kWalkWhole) ? kPoolWhole : 0); }
+__host__ __device__ inline int pool_arg(int pool, int walk) {
+  return pool | ((walk & kWalkWhole) ? kPoolWhole : 0) | ((walk & kWalkStaged) ? kPoolStaged : 0);
+}
+
+// hit_terms on a face's terms in memory, loaded in two stages: the plane
+// terms (p0, uu | n, 1/D) first, the edge terms (u, uv | v, vv) only in the
+// lanes whose ray meets the face's plane in front of it. The arithmetic and
+// the outcome are hit_terms' (the hit point itself is not kept).
+__device__ inline bool hit_staged(const float4* q, v3 o, v3 dir, float* d2) {
+  const float4 q0 = gld4(q), q3 = gld4(q + 3);
+  const v3 p0 = mk(q0.x, q0.y, q0.z), n = mk(q3.x, q3.y, q3.z);
+  const v3 w0 = o - p0;
+  const float a = -dot(n, w0);
+  const float b = dot(n, dir);
+  if (fabsf(b) <= kEps8) return false;
+  const float r = a / b;
+  if (r < 0.0f) return false;
+  const float4 q1 = gld4(q + 1), q2 = gld4(q + 2);
+  const v3 u = mk(q1.x, q1.y, q1.z), v = mk(q2.x, q2.y, q2.z);
+  const v3 p = o + dir * r;
+  const v3 w = p - p0;
+  const float uu = q0.w, uv = q1.w, vv = q2.w;
+  const float wu = dot(w, u), wv = dot(w, v);
+  const float sn = uv * wv - vv * wu;
+  const float tn = uv * wu - uu * wv;
+  const float s1 = sn * q3.w, t1 = tn * q3.w;
+  if (s1 < -1e-30f || s1 > 1.000001f || t1 < -1e-30f || s1 + t1 > 1.00001f) return false;
+  if (!(s1 > 1e-5f && s1 < 0.99999f && t1 > 1e-5f && s1 + t1 < 0.99999f)) {
+    const float D = uv * uv - uu * vv;
+    const float s = sn / D;
+    if (s < 0.0f || s > 1.0f) return false;
+    const float t = tn / D;
+    if (t < 0.0f || (s + t) > 1.0f) return false;
+  }
+  const v3 sd = p - o;
+  *d2 = dot(sd, sd);
+  return true;
+}
 
 // Inclusive prefix sum over the wavefront's 64 lanes: DPP row shifts within
 // each row of 16 lanes (lanes shifted in from outside a row read 0), then
@@ -490,8 +528,9 @@ template <bool STATS, int WALK, class Start, class Finish>
 __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best, int* next, long long n, int refill,
                                    int pool, Start start, Finish finish, uint32_t& n_tests, uint32_t& n_cells) {
   const int lane = threadIdx.x & 63;
-  const bool whole = (pool & kPoolWhole) != 0;
+  const bool whole = (pool & kPoolWhole) != 0, staged = (pool & kPoolStaged) != 0;
   pool &= 0xff;
+  float d2 = 0.f;
   bool have = false, more = true, pend = false;  // `more` is wave-uniform
   long long cur = 0;
   int ref0 = 0, nref = 0;
@@ -573,10 +612,13 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         if (p < deal) {
           const int f = gld(gidx(S.refs, (uint32_t)ref_o));
           const float4* q = reinterpret_cast<const float4*>(gidx(S.tri, (uint32_t)f));
-          v3 pt;
-          float d2;
-          if (hit_terms(gld4(q), gld4(q + 1), gld4(q + 2), gld4(q + 3), oo, dd, &pt, &d2) && d2 < 3.402823466e38f)
-            atomicMin(&best[owner], hit_key(d2, f));
+          if (staged) {
+            if (hit_staged(q, oo, dd, &d2) && d2 < 3.402823466e38f) atomicMin(&best[owner], hit_key(d2, f));
+          } else {
+            v3 pt;
+            if (hit_terms(gld4(q), gld4(q + 1), gld4(q + 2), gld4(q + 3), oo, dd, &pt, &d2) && d2 < 3.402823466e38f)
+              atomicMin(&best[owner], hit_key(d2, f));
+          }
           if (STATS) n_tests += 1;
         }
       }

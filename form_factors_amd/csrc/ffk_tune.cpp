@@ -315,7 +315,7 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
         // the default walk mode first; another must be faster by 1 %, more
         // than the spread of repeated timings
         float best_w = 3.0e38f;
-        for (int walk : {FFK_WALK_DEFAULT, 0, 1, 2}) {
+        for (int walk : {FFK_WALK_DEFAULT, 0, 1, 2, 4, 5, 6, 7}) {
           if (r != FFK_OK) break;
           if ((r = ffk_set_tuning(s, t.block, 0, -1)) < 0 || (r = ffk_set_walk(s, walk)) < 0) break;
           float ms = 0.f;
@@ -431,7 +431,7 @@ int ffk_batch_autotune(ffk_batch* b, const unsigned long long* seeds, unsigned l
     };
     if ((r = trial(best)) < 0) return r;
     const cfg base = best;
-    for (int walk : {0, 1, 2})
+    for (int walk : {0, 1, 2, 7})
       if ((r = trial({base.block, base.refill, base.pool, walk})) < 0) return r;
     const cfg after_walk = best;
     for (const auto& sch : kSchedules) {

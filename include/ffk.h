@@ -176,10 +176,15 @@ int ffk_batch_set_refill(ffk_batch* b, int lanes, int pool);
  * dealt: clear (default): every waiting lane's tests in each round; set: a
  * round of at least 64 tests deals whole chunks of 64 only, and the tests
  * past the last whole chunk wait for the next round (fuller chunks, later
- * exits: faster on scenes whose rounds are large). Modes 0..3; same cells,
- * same hits in every mode. Default FFK_WALK_DEFAULT (both bits). */
+ * exits: faster on scenes whose rounds are large). Bit 2, how a pooled test
+ * reads the face's 64 bytes of terms: clear (default): all four 16-byte parts
+ * at once; set: the plane parts first and the edge parts only in the lanes
+ * whose ray meets the plane in front (fewer L1 accesses, one more load round
+ * trip: faster where many tests fail at the plane). Modes 0..7; same cells,
+ * same hits in every mode. Default FFK_WALK_DEFAULT (bits 0 and 1). */
 #define FFK_WALK_JUMP 1
 #define FFK_WALK_WHOLE 2
+#define FFK_WALK_STAGED 4
 #define FFK_WALK_DEFAULT (FFK_WALK_JUMP | FFK_WALK_WHOLE)
 int ffk_set_walk(ffk_scene* s, int mode);
 int ffk_batch_set_walk(ffk_batch* b, int mode);
@@ -251,8 +256,8 @@ int ffk_batch_get_class_tuning(const ffk_batch* b, int class_id, int* block_size
  * next ffk_batch_run recomputes them. */
 int ffk_batch_run_class(ffk_batch* b, int class_id, long long max_rays, void* stream);
 /* Per-class autotuning of the batch: one untuned ffk_batch_run with these
- * buffers (its results are valid), then for each class the walk mode, the
- * ray scheduling (refill/pool 32/2, 24/3, 16/4, 40/3, 40/4) and the block size (256,
+ * buffers (its results are valid), then for each class the walk mode (3
+ * against 0, 1, 2 and 7), the ray scheduling (refill/pool 32/2, 24/3, 16/4, 40/3, 40/4) and the block size (256,
  * 512, 1024 where the histogram allows), timed greedily with
  * ffk_batch_run_class on about sample_rays of the class; a setting replaces
  * the defaults only when it is 3 % faster there and then 1 % faster over the
@@ -276,7 +281,8 @@ int ffk_batch_tally_flags(const ffk_batch* b, int* h_flags);
  * scaled to about sample_rays; when a full run is estimated under 250 ms and
  * F fits in half the free memory, the best candidates are re-timed with the
  * fused kernel on the full layout), the fused kernel's block size (256, 512
- * or 1024), the walk mode (ffk_set_walk) and, timed the same way, the ray
+ * or 1024), the walk mode (ffk_set_walk, all eight; another than the default
+ * must be 1 % faster) and, timed the same way, the ray
  * scheduling (ffk_set_refill: 32/2, 24/3, 16/4, 40/3 or 40/4), then leaves the scene built
  * and tuned with them (ffk_get_refill reads the scheduling back). Results are
  * cached per scene signature (faces, meshes, ray counts, frame) for the life

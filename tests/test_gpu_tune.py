@@ -19,8 +19,8 @@ def test_autotune_keeps_bits_and_caches(gpu, oracle):
     before = sc.ff_fused(21).cpu().numpy()
     t = sc.autotune(sample_rays=50_000)
     print("tuned", t, flush=True)
-    assert not t["cached"] and min(t["dims"]) >= 1 and t["block"] in (0, 256, 512, 1024) and t["walk"] in (0, 1, 2, 3)
-    assert (t["refill"], t["pool"]) in ((32, 2), (24, 3), (16, 4))
+    assert not t["cached"] and min(t["dims"]) >= 1 and t["block"] in (0, 256, 512, 1024) and t["walk"] in range(8)
+    assert (t["refill"], t["pool"]) in ((32, 2), (24, 3), (16, 4), (40, 3), (40, 4))
     assert sc.get_refill() == (t["refill"], t["pool"])
     assert sc.grid_info()["dims"] == t["dims"]
     after = sc.ff_fused(21).cpu().numpy()
