@@ -221,11 +221,24 @@ __global__ void __launch_bounds__(kRadThreads)
   p[2 * k + 1] = pface[2 * f + 1];
 }
 
-// Rays of each walk entry's face; entry W (the end) has none.
+// Status words of a pass (d_rad_info): [0] bits of the largest ray power of
+// a walked side that emits, [1] bands in use, [2] bad-power flag, [3] lowest
+// face with a bad power, [4] blocks of k_rad_bands done.
+constexpr int kInfoWords = 5;
+
+// Rays of each walk entry's face; entry W (the end) has none. Also resets
+// the status words for the pass.
 __global__ void __launch_bounds__(kRadThreads)
     k_rad_counts(int W, const int* __restrict__ walk_face, const long long* __restrict__ face_off,
-                 long long* __restrict__ cnt) {
+                 long long* __restrict__ cnt, int* __restrict__ info) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k == 0) {
+    info[0] = 0;
+    info[1] = 0;
+    info[2] = 0;
+    info[3] = 0x7fffffff;
+    info[4] = 0;
+  }
   if (k > W) return;
   long long c = 0;
   if (k < W) {
@@ -237,12 +250,12 @@ __global__ void __launch_bounds__(kRadThreads)
 
 // Entries the walk reaches (first position < total; radiance_cpu.cpp:170):
 // the power check (negative or not finite: status bit, lowest such face) and
-// per block the largest power of a side that emits rays (as bits: positive
-// floats order as their bits).
+// the largest power of a side that emits rays, as bits (positive floats order
+// as their bits, so the maximum does not depend on the order of the atomics).
 __global__ void __launch_bounds__(kRadThreads)
     k_rad_entries(int W, long long total, const long long* __restrict__ start, const int* __restrict__ walk_face,
                   const long long* __restrict__ face_off, const int* __restrict__ front, const float* __restrict__ p,
-                  unsigned* __restrict__ part_max, int* __restrict__ info) {
+                  int* __restrict__ info) {
   __shared__ unsigned sh[kRadThreads / 64];
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned vmax = 0u;
@@ -258,31 +271,34 @@ __global__ void __launch_bounds__(kRadThreads)
     if (cr > 0 && pr > 0.f) vmax = max(vmax, __float_as_uint(pr));
   }
   const unsigned r = block_reduce(vmax, [](unsigned a, unsigned b) { return max(a, b); }, sh);
-  if (threadIdx.x == 0) part_max[blockIdx.x] = r;
-}
-
-// One block: the largest power over the blocks' maxima; info[0] = its exponent.
-__global__ void __launch_bounds__(kRadThreads) k_rad_emax(int n_parts, const unsigned* __restrict__ part_max,
-                                                          int* __restrict__ info) {
-  __shared__ unsigned sh[kRadThreads / 64];
-  unsigned v = 0u;
-  for (int i = threadIdx.x; i < n_parts; i += blockDim.x) v = max(v, part_max[i]);
-  v = block_reduce(v, [](unsigned a, unsigned b) { return max(a, b); }, sh);
-  if (threadIdx.x == 0) info[0] = v ? pow_exponent(__uint_as_float(v)) : 0;
+  if (threadIdx.x == 0 && r) atomicMax(reinterpret_cast<unsigned*>(&info[0]), r);
 }
 
 // Band of each emitting side: its binary order below the largest power in
 // steps of kBandOrders (the last band takes everything smaller); per block and
-// band the sum of count * power and the largest power.
+// band the sum of count * power and the largest power (per wavefront by
+// shuffles, then the wavefronts in order). The last block to finish then
+// fixes the scales: per band the largest possible sum (every ray of the band
+// absorbed by one mesh) and the largest single power give a scale that keeps
+// sums below 2^63 and a single ray's value below 2^60, clear of the band
+// bits. A ray of power p in band b is then rounded by at most 0.5 / scale_b,
+// i.e. relatively by at most 2^-55 times the rays in the band, or 2^-52
+// (2^kBandOrders / 2^60). info[1] = bands in use (0: no side emits). The
+// partial sums are added in a fixed order, so the scales are the same every
+// run.
 __global__ void __launch_bounds__(kRadThreads)
     k_rad_bands(int W, long long total, const long long* __restrict__ start, const int* __restrict__ walk_face,
                 const long long* __restrict__ face_off, const int* __restrict__ front, const float* __restrict__ p,
-                const int* __restrict__ info, unsigned char* __restrict__ band, double* __restrict__ part_sum,
-                unsigned* __restrict__ part_max) {
-  __shared__ double shd[kRadThreads / 64];
-  __shared__ unsigned shu[kRadThreads / 64];
+                int* __restrict__ info, unsigned char* __restrict__ band, double* __restrict__ part_sum,
+                unsigned* __restrict__ part_max, double* __restrict__ scale) {
+  constexpr int kWaves = kRadThreads / 64;
+  __shared__ double shd[kWaves][kPowerBands];
+  __shared__ unsigned shu[kWaves][kPowerBands];
+  __shared__ int last;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int e_max = info[0];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const unsigned pmax_bits = (unsigned)info[0];
+  const int e_max = pmax_bits ? pow_exponent(__uint_as_float(pmax_bits)) : 0;
   int b2[2] = {-1, -1};
   double v2[2] = {0.0, 0.0};
   unsigned u2[2] = {0u, 0u};
@@ -301,37 +317,64 @@ __global__ void __launch_bounds__(kRadThreads)
     }
   }
   for (int b = 0; b < kPowerBands; ++b) {
-    const double v = (b2[0] == b ? v2[0] : 0.0) + (b2[1] == b ? v2[1] : 0.0);
-    const unsigned u = max(b2[0] == b ? u2[0] : 0u, b2[1] == b ? u2[1] : 0u);
-    const double rs = block_reduce(v, [](double x, double y) { return x + y; }, shd);
-    const unsigned ru = block_reduce(u, [](unsigned x, unsigned y) { return max(x, y); }, shu);
-    if (threadIdx.x == 0) {
-      part_sum[(size_t)blockIdx.x * kPowerBands + b] = rs;
-      part_max[(size_t)blockIdx.x * kPowerBands + b] = ru;
+    double v = (b2[0] == b ? v2[0] : 0.0) + (b2[1] == b ? v2[1] : 0.0);
+    unsigned u = max(b2[0] == b ? u2[0] : 0u, b2[1] == b ? u2[1] : 0u);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      v += __shfl_down(v, off, 64);
+      u = max(u, __shfl_down(u, off, 64));
+    }
+    if (lane == 0) {
+      shd[w][b] = v;
+      shu[w][b] = u;
     }
   }
-}
-
-// One block: per band the largest possible sum (every ray of the band
-// absorbed by one mesh) and the largest single power fix the scale: sums stay
-// below 2^63 and a single ray's value below 2^60, clear of the band bits. A
-// ray of power p in band b is then rounded by at most 0.5 / scale_b, i.e.
-// relatively by at most 2^-55 times the rays in the band, or 2^-52
-// (2^kBandOrders / 2^60). info[1] = bands in use (0: no side emits).
-__global__ void k_rad_scales(int n_parts, const double* __restrict__ part_sum, const unsigned* __restrict__ part_max,
-                             double* __restrict__ scale, int* __restrict__ info) {
+  __syncthreads();
+  if (threadIdx.x < kPowerBands) {
+    const int b = threadIdx.x;
+    double v = shd[0][b];
+    unsigned u = shu[0][b];
+    for (int i = 1; i < kWaves; ++i) {
+      v += shd[i][b];
+      u = max(u, shu[i][b]);
+    }
+    part_sum[(size_t)blockIdx.x * kPowerBands + b] = v;
+    part_max[(size_t)blockIdx.x * kPowerBands + b] = u;
+  }
+  // the last block to finish reduces the partials
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) last = atomicAdd(&info[4], 1) == (int)gridDim.x - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  // thread t sums band t % 16 over the partials i = t / 16, t / 16 + 16, ...
+  // in order; then the 16 chunk sums of each band in order
+  const int b = threadIdx.x % kPowerBands, c = threadIdx.x / kPowerBands;
+  constexpr int kChunks = kRadThreads / kPowerBands;
+  double v = 0.0;
+  unsigned u = 0u;
+#pragma unroll 8
+  for (int i = c; i < (int)gridDim.x; i += kChunks) {
+    v += __builtin_nontemporal_load(&part_sum[(size_t)i * kPowerBands + b]);
+    u = max(u, __builtin_nontemporal_load(&part_max[(size_t)i * kPowerBands + b]));
+  }
+  __shared__ double chd[kChunks][kPowerBands];
+  __shared__ unsigned chu[kChunks][kPowerBands];
+  chd[c][b] = v;
+  chu[c][b] = u;
+  __syncthreads();
   __shared__ int used[kPowerBands];
-  const int b = threadIdx.x;
-  if (b < kPowerBands) {
-    double up = 0.0;
-    unsigned um = 0u;
-    for (int i = 0; i < n_parts; ++i) {  // in block order: the same sum every run
-      up += part_sum[(size_t)i * kPowerBands + b];
-      um = max(um, part_max[(size_t)i * kPowerBands + b]);
+  if (threadIdx.x < kPowerBands) {
+    double up = chd[0][b];
+    unsigned um = chu[0][b];
+    for (int i = 1; i < kChunks; ++i) {
+      up += chd[i][b];
+      um = max(um, chu[i][b]);
     }
     int e_sum = 0, e_one = 0;
-    (void)frexp(up, &e_sum);                             // up < 2^e_sum
-    (void)frexp((double)__uint_as_float(um), &e_one);   // the largest power < 2^e_one
+    (void)frexp(up, &e_sum);                            // up < 2^e_sum
+    (void)frexp((double)__uint_as_float(um), &e_one);  // the largest power < 2^e_one
     scale[b] = up > 0.0 ? ldexp(1.0, min(63 - e_sum, kBandShift - e_one)) : 1.0;
     used[b] = up > 0.0 ? 1 : 0;
   }
@@ -346,12 +389,17 @@ __global__ void k_rad_scales(int n_parts, const double* __restrict__ part_sum, c
 
 // Fixed-point word of each walked side that emits (band in the top bits);
 // the walk positions capped at the walk's end (the entries past it are never
-// reached: no ray position lies at or past their start).
+// reached: no ray position lies at or past their start). Also zeroes the
+// accumulator rows of the bands in use.
 __global__ void __launch_bounds__(kRadThreads)
     k_rad_q(int W, long long total, long long walk_total, long long* __restrict__ start,
             const int* __restrict__ walk_face, const long long* __restrict__ face_off, const int* __restrict__ front,
             const float* __restrict__ p, const unsigned char* __restrict__ band, const double* __restrict__ scale,
-            unsigned long long* __restrict__ q) {
+            unsigned long long* __restrict__ q, unsigned long long* __restrict__ acc, int nm,
+            const int* __restrict__ info) {
+  const size_t n_acc = (size_t)info[1] * nm;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_acc; i += (size_t)gridDim.x * blockDim.x)
+    acc[i] = 0ull;
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k > W) return;
   const long long s0 = start[k];
@@ -368,13 +416,6 @@ __global__ void __launch_bounds__(kRadThreads)
     }
   }
   start[k] = min(s0, walk_total);
-}
-
-// Zeroes the accumulator rows of the bands in use.
-__global__ void k_rad_zero(unsigned long long* __restrict__ acc, int nm, const int* __restrict__ info) {
-  const size_t n = (size_t)info[1] * nm;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-    acc[i] = 0ull;
 }
 
 // Absorbed power per mesh: the band rows' fixed-point sums over their scales.
@@ -416,8 +457,8 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
                o_acc = take(8 * (size_t)kPowerBands * nm), o_abs = take(8 * (size_t)nm),
                o_scale = take(8 * kPowerBands), o_p = take(8 * (size_t)W + 8), o_pface = take(8 * (size_t)nf),
                o_te = take(12 * (size_t)nm),
-               o_area = take(4 * (size_t)nf), o_pu = take(4 * (size_t)n_parts * (kPowerBands + 1)),
-               o_pd = take(8 * (size_t)n_parts * kPowerBands), o_info = take(16), o_scan = take(scan_bytes);
+               o_area = take(4 * (size_t)nf), o_pu = take(4 * (size_t)n_parts * kPowerBands),
+               o_pd = take(8 * (size_t)n_parts * kPowerBands), o_info = take(4 * kInfoWords), o_scan = take(scan_bytes);
   free_radiance(s);
   int r;
   if ((r = dmalloc(&s->d_rad_block, off)) < 0) return r;
@@ -443,7 +484,7 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
   s->rad_parts = n_parts;
   // pinned staging: T and eps (3 nm floats), ray powers (2 nf floats), then
   // the absorption (nm doubles) and the status words (4 ints)
-  const size_t pin = up(12 * (size_t)nm) + up(8 * (size_t)nf) + up(8 * (size_t)nm) + 16;
+  const size_t pin = up(12 * (size_t)nm) + up(8 * (size_t)nf) + up(8 * (size_t)nm) + 4 * kInfoWords;
   FFK_HIP(hipHostMalloc(&s->h_rad_pin, pin, hipHostMallocDefault));
   // the walk entries' faces and meshes and the face areas, once
   std::vector<int> wf((size_t)W + 1, 0), wm((size_t)W + 1, 0);
@@ -510,25 +551,17 @@ int rad_pass(ffk_scene* s, unsigned long long seed, double* h_absorption, unsign
   const long long total = s->n_rays;
   const long long walk_total = s->h_mesh_off[nm];  // the walk's end: min(rays of the meshes' faces, total)
   int* info = pin_info(s);
-  info[0] = 0;
-  info[1] = 0;
-  info[2] = 0;
-  info[3] = 0x7fffffff;
-  FFK_HIP(hipMemcpyAsync(s->d_rad_info, info, 16, hipMemcpyHostToDevice, st));
-  k_rad_counts<<<nb, kRadThreads, 0, st>>>(W, s->d_walk_face, s->d_face_off, s->d_walk_cnt);
+  k_rad_counts<<<nb, kRadThreads, 0, st>>>(W, s->d_walk_face, s->d_face_off, s->d_walk_cnt, s->d_rad_info);
   size_t scan_bytes = s->rad_scan_bytes;
   FFK_HIP(hipcub::DeviceScan::ExclusiveSum(s->d_rad_scan, scan_bytes, s->d_walk_cnt, s->d_walk_start, W + 1, st));
   k_rad_entries<<<nb, kRadThreads, 0, st>>>(W, total, s->d_walk_start, s->d_walk_face, s->d_face_off, s->d_front,
-                                            s->d_rad_p, s->d_rad_part_u, s->d_rad_info);
-  k_rad_emax<<<1, kRadThreads, 0, st>>>(nb, s->d_rad_part_u, s->d_rad_info);
-  unsigned* part_bmax = s->d_rad_part_u + s->rad_parts;
+                                            s->d_rad_p, s->d_rad_info);
   k_rad_bands<<<nb, kRadThreads, 0, st>>>(W, total, s->d_walk_start, s->d_walk_face, s->d_face_off, s->d_front,
-                                          s->d_rad_p, s->d_rad_info, s->d_walk_band, s->d_rad_part_d, part_bmax);
-  k_rad_scales<<<1, 64, 0, st>>>(nb, s->d_rad_part_d, part_bmax, s->d_rad_scale, s->d_rad_info);
+                                          s->d_rad_p, s->d_rad_info, s->d_walk_band, s->d_rad_part_d,
+                                          s->d_rad_part_u, s->d_rad_scale);
   k_rad_q<<<nb, kRadThreads, 0, st>>>(W, total, walk_total, s->d_walk_start, s->d_walk_face, s->d_face_off,
-                                      s->d_front, s->d_rad_p, s->d_walk_band, s->d_rad_scale, s->d_walk_q);
-  k_rad_zero<<<(unsigned)std::min<long long>(1024, ((long long)kPowerBands * nm + 255) / 256), 256, 0, st>>>(
-      s->d_rad_acc, nm, s->d_rad_info);
+                                      s->d_front, s->d_rad_p, s->d_walk_band, s->d_rad_scale, s->d_walk_q,
+                                      s->d_rad_acc, nm, s->d_rad_info);
   FFK_HIP(hipGetLastError());
   const uint32_t k0 = (uint32_t)seed, k1 = (uint32_t)(seed >> 32);
   if (walk_total > 0 && s->tune_refill > 0 && s->tune_pool > 0) {
@@ -536,9 +569,13 @@ int rad_pass(ffk_scene* s, unsigned long long seed, double* h_absorption, unsign
     // not left part empty), at most 16384 rays and at least a ray per lane
     // per block.
     auto go = [&](auto kernel) {
-      int per_cu = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1)
-        per_cu = 1;
+      static int per_cu_cached = 0;  // per kernel instance: resident blocks per CU (the device's one property)
+      int per_cu = per_cu_cached;
+      if (per_cu < 1) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess || per_cu < 1)
+          per_cu = 1;
+        per_cu_cached = per_cu;
+      }
       const long long resident = (long long)per_cu * device_cu_count();
       const long long rounds = std::max<long long>(1, (walk_total + resident * 16384 - 1) / (resident * 16384));
       const long long chunk = std::max<long long>(256, (walk_total + resident * rounds - 1) / (resident * rounds));
@@ -565,7 +602,7 @@ int rad_pass(ffk_scene* s, unsigned long long seed, double* h_absorption, unsign
   FFK_HIP(hipGetLastError());
   double* ab = pin_abs(s);
   FFK_HIP(hipMemcpyAsync(ab, s->d_rad_abs, 8 * (size_t)nm, hipMemcpyDeviceToHost, st));
-  FFK_HIP(hipMemcpyAsync(info, s->d_rad_info, 16, hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipMemcpyAsync(info, s->d_rad_info, 4 * kInfoWords, hipMemcpyDeviceToHost, st));
   FFK_HIP(hipStreamSynchronize(st));
   FFK_REQUIRE(info[2] == 0, FFK_ERR_ARG, "ffk_radiance: ray power of face %d is negative or not finite", info[3]);
   std::memcpy(h_absorption, ab, 8 * (size_t)nm);

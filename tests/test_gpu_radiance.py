@@ -185,3 +185,27 @@ def test_radiance_step_equals_explicit_calls(gpu):
     assert torch.equal(Fa.view(torch.int32), Fb.view(torch.int32))
     for s in (a, b, c):
         s.close()
+
+
+def test_radiance_large_walk_table(gpu, oracle):
+    """More than 65,536 walk entries (67,600 faces in 33,800 meshes): the walk
+    positions come from the multi-block scan instead of the one-block scan;
+    emission bit for bit and absorption against the double-precision sum, as
+    in test_radiance_matches_oracle."""
+    from form_factors_amd import scenes
+
+    faces, meshes = scenes.two_plates_thermal(cells=130, gap=5.0, size=10.0)
+    assert int(meshes[:, 1].sum()) > 65536
+    rng = np.random.default_rng(21)
+    eps = rng.uniform(0.1, 1, (len(meshes), 2)).astype(np.float32)
+    T = rng.uniform(250, 400, len(meshes)).astype(np.float32)
+    n_rays = 60_000
+    sc = gpu.Scene(faces, meshes)
+    sc.build_grid()
+    em, ab = sc.radiance_step(eps, T, n_rays, 0xB16, frame=1)
+    em_o, ab_o, ab64 = oracle.radiance(faces, meshes, eps, T, n_rays, rng=oracle.RNG_PHILOX, seed=0xB16, frame=1,
+                                       caster=1, n_threads=8)
+    assert np.array_equal(bits(em), bits(em_o))
+    assert ab64.sum() > 0
+    assert np.all(np.abs(ab - ab64) <= rel_tol(n_rays) * ab64)
+    sc.close()
