@@ -551,7 +551,43 @@ def controller_leg():
             "wall_with_import_seconds": round(min(r[1] for r in runs), 4),
             "reference_seconds": ref, "reference_hardware": "GeForce GTX 780 Ti (recorded in the model's frames file)",
             "speedup_vs_recorded": round(ref / best, 1),
-            "max_abs_mean_T_diff_K": round(max(r[2] for r in runs), 4)}
+            "max_abs_mean_T_diff_K": round(max(r[2] for r in runs), 4),
+            "patch_sphere_20k": controller_20k_leg()}
+
+
+def controller_20k_leg():
+    """The reference's largest model, models/patch_sphere_20k.obj (20,000
+    one-face black-body meshes) with patch_sphere_20k_sun.task (a 30 W/m^2
+    distant source and 1e4 sun rays per step), read from the files and run
+    by the controller for 100 Adams steps at 1e6 radiance rays per step.
+    The reference recorded no time for this model."""
+    import gzip
+    import tempfile
+
+    from form_factors_amd import controller
+
+    models = os.path.join(ROOT, "tests", "golden", "models")
+    with tempfile.TemporaryDirectory() as tmp:
+        paths = []
+        for name in ("patch_sphere_20k.obj", "patch_sphere_20k_sun.task"):
+            paths.append(os.path.join(tmp, name))
+            with open(os.path.join(models, name + ".gz"), "rb") as f, open(paths[-1], "wb") as o:
+                o.write(gzip.decompress(f.read()))
+        controller.run(paths[0], paths[1], n_rays=1_000_000, n_steps=5, seed=99)  # warm-up
+        runs = []
+        for seed in range(2):
+            t0 = time.perf_counter()
+            out = controller.run(paths[0], paths[1], n_rays=1_000_000, n_steps=100, seed=seed)
+            runs.append((out["seconds"], time.perf_counter() - t0))
+    T = out["frames"][-1][2]
+    best = min(r[0] for r in runs)
+    log(f"controller patch_sphere_20k sun: {best:.3f} s for 100 steps ({best * 10:.2f} ms/step), "
+        f"T after 1 s: {T.min():.2f}..{T.max():.2f} K")
+    return {"workload": "reference controller run, patch_sphere_20k model (20,000 meshes) with the sun task, "
+                        "100 Adams steps dt=0.01 s, 1e6 radiance rays + 1e4 sun rays per step",
+            "seconds": round(best, 4), "seconds_all": [round(r[0], 4) for r in runs],
+            "wall_with_import_seconds": round(min(r[1] for r in runs), 4),
+            "T_range_after_100_steps_K": [round(float(T.min()), 3), round(float(T.max()), 3)]}
 
 
 ADAMS_U = np.array([[1.0, 0, 0, 0, 0], [3 / 2, -1 / 2, 0, 0, 0], [23 / 12, -4 / 3, 5 / 12, 0, 0],

@@ -97,6 +97,84 @@ def test_sun_task_energy_balance(oracle):
     assert abs(stored - ab64.sum()) <= 2e-3 * ab64.sum()
 
 
+def unpacked(name, tmp_path):
+    """A gzipped model file under tests/golden/models written out into tmp_path."""
+    import gzip
+
+    out = tmp_path / name[:-3]
+    out.write_bytes(gzip.decompress(open(os.path.join(MODELS, name), "rb").read()))
+    return str(out)
+
+
+def test_patch_sphere_20k_sun_end_to_end(tmp_path, oracle):
+    """The reference's models/patch_sphere_20k.obj (20,000 one-face black
+    body meshes, inline material) with patch_sphere_20k_sun.task (three
+    meshes at 400 K, the rest at 10 K, 30 W/m^2 along -x), read from the
+    files and run by the controller for three Adams steps (AB1, AB2, AB3)
+    at the controller's 1e6 radiance rays per step. Each step is recomputed
+    from the frames: the oracle's radiance (the interior emitter's k-th key
+    and frame 0) and sunlight (the exterior emitter's k-th key) at the
+    frame's temperatures give E = (absorbed - emitted) / C, combined by the
+    oracle's Adams update from its own FP64 state. Every mesh agrees to the
+    float rounding of the powers and of the frame, except meshes that an
+    edge-grazing ray can move (a different nearest face within the casters'
+    1e-6 barycentric margin: at most 1e-5 of the rays, each moving one ray's
+    power between two meshes)."""
+    from form_factors_amd import controller, formats
+
+    obj = unpacked("patch_sphere_20k.obj.gz", tmp_path)
+    task = unpacked("patch_sphere_20k_sun.task.gz", tmp_path)
+    seed, n_rays, sun_rays, n_steps = 4, 1_000_000, 10_000, 3
+    print("controller patch_sphere_20k sun task, 3 steps", flush=True)
+    out = controller.run(obj, task, n_rays=n_rays, n_steps=n_steps, seed=seed, distant_rays=sun_rays,
+                         result_path=str(tmp_path / "run.frames"))
+    assert out["n_meshes"] == 20000 and out["n_faces"] == 20000 and out["task_temperatures"] == 20000
+    assert [f[0] for f in out["frames"]] == list(range(n_steps + 1))
+    frames = np.array([f[2] for f in out["frames"]])
+    assert np.isfinite(frames).all()
+    sc = formats.obj_import(obj, inline_materials=True)
+    t = formats.task_import(task, 20000, 1, step=0.1)
+    assert np.array_equal(frames[0], t["frames"][0][2])
+    dt = float(np.float32(t["frames"][0][1]))
+    assert all(float(f[1]) == dt for f in out["frames"])
+    faces, m = sc["faces"], sc["meshes"]
+    mat = sc["materials"][m[:, 2]].astype(np.float64)
+    tri = faces.reshape(-1, 3, 3).astype(np.float64)
+    area = 0.5 * np.linalg.norm(np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0]), axis=1)
+    C = np.array([area[a:a + k].sum() for a, k, _ in m]) * mat[:, 0] * mat[:, 1] * mat[:, 3]
+    eps = np.stack([mat[:, 8], mat[:, 13]], 1).astype(np.float32)
+    absorb = np.stack([mat[:, 6], mat[:, 11]], 1).astype(np.float32)
+    power, dx, dy, dz = t["distant_source"]
+    golden_step = 0x9E3779B97F4A7C15
+    E = np.zeros((5, 20000))
+    T_o = frames[0].astype(np.float64)
+    scale = np.zeros(20000)
+    for k in range(n_steps):
+        em, _, ab = oracle.radiance(faces, m, eps, frames[k], n_rays, rng=oracle.RNG_PHILOX,
+                                    seed=(seed + golden_step * k) % (1 << 64), frame=0, caster=1, n_threads=8)
+        _, sun = oracle.sun(faces, m, absorb, power, np.array([dx, dy, dz], np.float32), sun_rays,
+                            seed=(seed + 1 + golden_step * k) % (1 << 64), n_threads=8)
+        E[k % 5] = (ab + sun - em) / C
+        T_o = oracle.adams_combine_f64(k, dt, T_o, E)
+        scale += 4.0 * (ab + sun + em) / C
+        got = frames[k + 1].astype(np.float64)
+        err = np.abs(got - T_o)
+        tol = np.spacing(np.float32(T_o)).astype(np.float64) + 1e-6 * dt * scale
+        off = np.flatnonzero(err > tol)
+        print(f"step {k + 1}: max |T - oracle| {err.max():.3e} K, {len(off)} meshes beyond rounding; "
+              f"sun {sun.sum():.1f} W, radiance absorbed {ab.sum():.1f} W of {em.sum():.1f} W", flush=True)
+        assert sun.sum() > 0 and ab.sum() > 0
+        assert len(off) <= 2e-5 * n_rays
+        assert (err[off] <= tol[off] + 4.0 * dt * em.max() / C[off]).all()
+    # sunlit 10 K meshes warm; the 400 K meshes cool
+    lit = np.flatnonzero(sun > 0)
+    assert len(lit) > 1000 and (frames[-1][lit[lit >= 3]] > 10.0).all()
+    assert (frames[-1][:3] < 400.0).all()
+    back = formats.task_import(str(tmp_path / "run.frames"), 20000, n_steps + 1)
+    assert np.allclose(np.array([f[2] for f in back["frames"]]), frames, rtol=1e-6)
+    print(f"controller patch_sphere_20k: {out['seconds']:.3f} s for {n_steps} steps", flush=True)
+
+
 @pytest.mark.parametrize("caster", ["cuda", "naive_cpu"])
 def test_command_line_run(tmp_path, caster):
     """python -m form_factors_amd.controller with the controller's positional

@@ -176,12 +176,26 @@ def test_inline_materials_read_the_reference_export(io, g, tmp_path, name):
     assert np.abs(s["faces"] - want).max() <= 5e-7 * max(1.0, np.abs(want).max())
 
 
-@pytest.mark.parametrize("name,n_meshes", [("confetti_2k.obj", 2000), ("grid_10x10.obj", 100),
-                                           ("patch_sphere_2k.obj", 2000)])
-def test_inline_materials_read_the_reference_models(io, name, n_meshes):
-    """The reference's models carry their materials inline: the inline mode
-    reads them; vertices and faces equal a plain parse of the v / f lines."""
+def unpacked(name, tmp_path):
+    """A model file under tests/golden/models; a .gz one is written out
+    uncompressed into tmp_path first (the importers read files)."""
     path = os.path.join(MODELS, name)
+    if not name.endswith(".gz"):
+        return path
+    import gzip
+
+    out = tmp_path / os.path.basename(name[:-3])
+    out.write_bytes(gzip.decompress(open(path, "rb").read()))
+    return str(out)
+
+
+@pytest.mark.parametrize("name,n_meshes", [("confetti_2k.obj", 2000), ("grid_10x10.obj", 100),
+                                           ("patch_sphere_2k.obj", 2000), ("patch_sphere_20k.obj.gz", 20000)])
+def test_inline_materials_read_the_reference_models(io, tmp_path, name, n_meshes):
+    """The reference's models carry their materials inline: the inline mode
+    reads them; vertices and faces equal a plain parse of the v / f lines.
+    patch_sphere_20k.obj (models/, 20,000 one-face meshes) is kept gzipped."""
+    path = unpacked(name, tmp_path)
     s = io.obj_import(path, inline_materials=True)
     verts, tris = [], []
     for line in open(path):
@@ -193,3 +207,18 @@ def test_inline_materials_read_the_reference_models(io, name, n_meshes):
     assert np.array_equal(bits(s["faces"]), bits(faces))
     assert len(s["meshes"]) == n_meshes and s["meshes"][:, 1].sum() == len(faces)
     assert len(s["names"]) >= 1 and (s["meshes"][:, 2] < len(s["names"])).all()
+
+
+def test_patch_sphere_20k_sun_task(io, tmp_path):
+    """models/patch_sphere_20k_sun.task: one frame of 20,000 temperatures
+    (three meshes at 400 K, the rest at 10 K), step 0.01 s and a 30 W/m^2
+    distant source along -x; the scene's one inline material is the black
+    body (emissivity and absorbance 1 on both sides, heat capacity 20)."""
+    s = io.obj_import(unpacked("patch_sphere_20k.obj.gz", tmp_path), inline_materials=True)
+    t = io.task_import(unpacked("patch_sphere_20k_sun.task.gz", tmp_path), 20000, 1, step=0.1)
+    count, dt, T = t["frames"][0]
+    assert count == 20000 and np.float32(dt) == np.float32(0.01)
+    assert np.array_equal(np.flatnonzero(T == 400.0), [0, 1, 2]) and (T[3:] == 10.0).all()
+    assert t["distant_source"] == (30.0, -1.0, 0.0, 0.0) and len(t["heat_sources"]) == 0
+    assert s["names"] == ["black_body"]
+    assert s["materials"][0].tolist() == [1, 20, np.float32(0.1), 1, 0, 0, 1, 0, 1, 0, 0, 1, 0, 1]
