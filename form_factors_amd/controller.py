@@ -4,7 +4,9 @@ mesh temperatures out. Radiance (Malley rays over the grid caster, fused on
 the device), conduction, heat sources and the distant source are summed by
 the FP64 Adams solver, one step per frame, exactly in the controller's
 order; every call goes through the C ABI of include/ff_plugin.h and
-include/ff_io.h.
+include/ff_io.h. Over the grid caster each step is device-resident: the
+equations' emission and absorption stay on the device up to the FP64
+update, and one copy brings the new temperatures back for the frame.
 
 Command line (the controller's positional arguments, controller.cpp:35-96):
 
@@ -36,14 +38,18 @@ RAY_CASTER_NOT_SUPPORTED = 2
 
 def run(scene_path, task_path, n_rays=1_000_000, n_steps=100, result_path=None, binary=False, seed=0,
         inline_materials=True, distant_rays=10_000, scene=None, caster_type=P.RAY_CASTER_GRID_HIP,
-        frame_sink=None):
+        frame_sink=None, device_loop=True):
     """controller.cpp:98-242. Returns {"frames": [(n_step, step, T)],
     "seconds": wall time of set-up + steps (the controller's timer),
     "n_meshes", "n_faces", "task_temperatures"}. result_path: the frames are
     also written there as the controller writes them (obj_export::task /
     task_binary). scene: an already imported scene (formats.obj_import's
     dict) instead of scene_path. frame_sink(n_step, step, T) is called with
-    the initial frame and after every step, as the controller writes them."""
+    the initial frame and after every step, as the controller writes them.
+    device_loop: each step keeps the equations' emission and absorption in
+    device memory up to the FP64 update (the grid caster; the brute-force
+    caster always runs the equations on host arrays); False runs every step
+    on host arrays, with the same temperatures bit for bit."""
     L = P.lib()
     formats.lib()
     caster = L.ffp_ray_caster_create(int(caster_type))
@@ -83,6 +89,8 @@ def run(scene_path, task_path, n_rays=1_000_000, n_steps=100, result_path=None, 
         if not sol:
             raise RuntimeError("failed to create the thermal solution")
         systems.append(("sol", sol))
+        if L.ffp_thermal_solution_set_device_loop(sol, 1 if device_loop else 0) < 0:
+            raise RuntimeError("thermal solution: device loop setting refused")
         frames = [(task.n_step, task.time_delta, np.frombuffer(Tc, np.float32).copy())]
         if frame_sink is not None:
             frame_sink(*frames[-1])

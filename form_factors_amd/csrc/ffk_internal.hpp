@@ -358,8 +358,10 @@ struct ffk_scene {
   size_t rad_scan_bytes = 0;
   void* d_rad_block = nullptr;
   void* h_rad_pin = nullptr;
+  hipEvent_t rad_te_ev = nullptr;  // the last upload from the pinned T / eps staging
   int* d_walk_face = nullptr;               // W: face of each walk entry
   int* d_walk_mesh = nullptr;               // W: mesh of each walk entry
+  int* d_mesh_walk = nullptr;               // n_meshes + 1: first walk entry of each mesh
   long long* d_walk_cnt = nullptr;          // W + 1: rays of each entry's face (then 0)
   long long* d_walk_start = nullptr;        // W + 1: first walk position of each entry
   unsigned long long* d_walk_q = nullptr;   // 2W: fixed-point front / rear ray power

@@ -211,6 +211,38 @@ __global__ void __launch_bounds__(kRadThreads)
   p[2 * k + 1] = pr;
 }
 
+// The emission each mesh adds to the step's emission (radiance_cpu.cpp:
+// 148-178): its faces' front and rear emission, added face by face in float
+// to what the array holds, in the expressions and order of the host
+// powers_impl, so the sums are its bits. te as in k_rad_powers; mesh m's
+// faces are walk entries [mesh_walk[m], mesh_walk[m + 1]).
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_emission(int nm, const int* __restrict__ mesh_walk, const int* __restrict__ walk_face,
+                   const float* __restrict__ te, const float* __restrict__ area, float* __restrict__ emission) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const float T = te[m];
+  const float power_density = kSigma * (T * T * T * T);
+  const float front_density = power_density * te[nm + 2 * m];
+  const float rear_density = power_density * te[nm + 2 * m + 1];
+  float e = emission[m];
+  for (int k = mesh_walk[m]; k < mesh_walk[m + 1]; ++k) {
+    const float a = area[walk_face[k]];
+    const float front_emission = front_density * a;
+    const float rear_emission = rear_density * a;
+    e += front_emission + rear_emission;
+  }
+  emission[m] = e;
+}
+
+// The absorbed power of each mesh added, as a float, to the step's
+// absorption (what the host form adds from its double array).
+__global__ void __launch_bounds__(kRadThreads)
+    k_rad_add_absorption(int nm, const double* __restrict__ abs, float* __restrict__ absorption) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m < nm) absorption[m] += (float)abs[m];
+}
+
 // Caller-given powers per face side, copied to the walk entries.
 __global__ void __launch_bounds__(kRadThreads)
     k_rad_expand(int W, const int* __restrict__ walk_face, const float* __restrict__ pface, float* __restrict__ p) {
@@ -452,7 +484,8 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
     off += up(b);
     return o;
   };
-  const size_t o_face = take(4 * (size_t)W + 4), o_mesh = take(4 * (size_t)W + 4), o_cnt = take(8 * ((size_t)W + 1)),
+  const size_t o_face = take(4 * (size_t)W + 4), o_mesh = take(4 * (size_t)W + 4), o_mw = take(4 * ((size_t)nm + 1)),
+               o_cnt = take(8 * ((size_t)W + 1)),
                o_start = take(8 * ((size_t)W + 1)), o_q = take(16 * (size_t)W + 16), o_band = take(2 * (size_t)W + 2),
                o_acc = take(8 * (size_t)kPowerBands * nm), o_abs = take(8 * (size_t)nm),
                o_scale = take(8 * kPowerBands), o_p = take(8 * (size_t)W + 8), o_pface = take(8 * (size_t)nf),
@@ -465,6 +498,7 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
   char* d = static_cast<char*>(s->d_rad_block);
   s->d_walk_face = reinterpret_cast<int*>(d + o_face);
   s->d_walk_mesh = reinterpret_cast<int*>(d + o_mesh);
+  s->d_mesh_walk = reinterpret_cast<int*>(d + o_mw);
   s->d_walk_cnt = reinterpret_cast<long long*>(d + o_cnt);
   s->d_walk_start = reinterpret_cast<long long*>(d + o_start);
   s->d_walk_q = reinterpret_cast<unsigned long long*>(d + o_q);
@@ -487,15 +521,19 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
   const size_t pin = up(12 * (size_t)nm) + up(8 * (size_t)nf) + up(8 * (size_t)nm) + 4 * kInfoWords;
   FFK_HIP(hipHostMalloc(&s->h_rad_pin, pin, hipHostMallocDefault));
   // the walk entries' faces and meshes and the face areas, once
-  std::vector<int> wf((size_t)W + 1, 0), wm((size_t)W + 1, 0);
+  std::vector<int> wf((size_t)W + 1, 0), wm((size_t)W + 1, 0), mw((size_t)nm + 1, 0);
   int k = 0;
-  for (int m = 0; m < nm; ++m)
+  for (int m = 0; m < nm; ++m) {
+    mw[m] = k;
     for (int f = s->h_meshes[3 * m]; f < s->h_meshes[3 * m] + s->h_meshes[3 * m + 1]; ++f) {
       wf[k] = f;
       wm[k++] = m;
     }
+  }
+  mw[nm] = k;
   FFK_HIP(hipMemcpyAsync(s->d_walk_face, wf.data(), 4 * ((size_t)W + 1), hipMemcpyHostToDevice, st));
   FFK_HIP(hipMemcpyAsync(s->d_walk_mesh, wm.data(), 4 * ((size_t)W + 1), hipMemcpyHostToDevice, st));
+  FFK_HIP(hipMemcpyAsync(s->d_mesh_walk, mw.data(), 4 * ((size_t)nm + 1), hipMemcpyHostToDevice, st));
   FFK_HIP(hipMemcpyAsync(s->d_rad_area, s->h_area.data(), 4 * (size_t)nf, hipMemcpyHostToDevice, st));
   FFK_HIP(hipStreamSynchronize(st));  // the vector above goes out of scope
   s->rad_W = W;
@@ -511,6 +549,7 @@ void free_radiance(ffk_scene* s) {
   s->rad_parts = 0;
   s->d_walk_face = nullptr;
   s->d_walk_mesh = nullptr;
+  s->d_mesh_walk = nullptr;
   s->d_walk_cnt = nullptr;
   s->d_walk_start = nullptr;
   s->d_walk_q = nullptr;
@@ -542,11 +581,10 @@ inline int* pin_info(ffk_scene* s) {
 }
 
 // The device pass over the walk entries' ray powers in d_rad_p: walk table, bands, scales
-// and fixed-point words, the emit-cast-deposit kernel, the per-mesh sums;
-// copies the absorption to h_absorption (left untouched when a walked ray
-// power is negative or not finite: FFK_ERR_ARG).
-int rad_pass(ffk_scene* s, unsigned long long seed, double* h_absorption, unsigned long long* d_stats,
-             hipStream_t st) {
+// and fixed-point words, the emit-cast-deposit kernel, the per-mesh sums in
+// d_rad_abs; the status words are copied to the pinned block (read them once
+// the stream has drained).
+int rad_enqueue(ffk_scene* s, unsigned long long seed, unsigned long long* d_stats, hipStream_t st) {
   const int nm = s->n_meshes, W = s->rad_W, nb = (W + kRadThreads) / kRadThreads;
   const long long total = s->n_rays;
   const long long walk_total = s->h_mesh_off[nm];  // the walk's end: min(rays of the meshes' faces, total)
@@ -600,12 +638,29 @@ int rad_pass(ffk_scene* s, unsigned long long seed, double* h_absorption, unsign
   k_rad_absorb<<<(nm + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(nm, s->d_rad_acc, s->d_rad_scale,
                                                                            s->d_rad_info, s->d_rad_abs);
   FFK_HIP(hipGetLastError());
-  double* ab = pin_abs(s);
-  FFK_HIP(hipMemcpyAsync(ab, s->d_rad_abs, 8 * (size_t)nm, hipMemcpyDeviceToHost, st));
   FFK_HIP(hipMemcpyAsync(info, s->d_rad_info, 4 * kInfoWords, hipMemcpyDeviceToHost, st));
-  FFK_HIP(hipStreamSynchronize(st));
+  return FFK_OK;
+}
+
+// Status of the last pass, once its stream has drained: FFK_ERR_ARG when a
+// walked ray power was negative or not finite.
+int rad_status(ffk_scene* s) {
+  const int* info = pin_info(s);
   FFK_REQUIRE(info[2] == 0, FFK_ERR_ARG, "ffk_radiance: ray power of face %d is negative or not finite", info[3]);
-  std::memcpy(h_absorption, ab, 8 * (size_t)nm);
+  return FFK_OK;
+}
+
+// rad_enqueue, then the absorption copied to h_absorption (left untouched
+// when a walked ray power is negative or not finite: FFK_ERR_ARG).
+int rad_pass(ffk_scene* s, unsigned long long seed, double* h_absorption, unsigned long long* d_stats,
+             hipStream_t st) {
+  int r;
+  if ((r = rad_enqueue(s, seed, d_stats, st)) < 0) return r;
+  double* ab = pin_abs(s);
+  FFK_HIP(hipMemcpyAsync(ab, s->d_rad_abs, 8 * (size_t)s->n_meshes, hipMemcpyDeviceToHost, st));
+  FFK_HIP(hipStreamSynchronize(st));
+  if ((r = rad_status(s)) < 0) return r;
+  std::memcpy(h_absorption, ab, 8 * (size_t)s->n_meshes);
   return FFK_OK;
 }
 
@@ -684,6 +739,42 @@ void powers_impl(const float* h_faces, const float* h_area, int n_faces, const i
   }
 }
 
+// The host part of a radiance step (radiance_cpu.cpp:96-146): weights and
+// their float total in mesh order, the emitted counts and, when they changed,
+// the emission layout; then T and eps to the device (pinned staging, reused
+// only once its last upload has run) and the walk entries' ray powers there.
+int step_prepare(ffk_scene* s, const float* h_eps2, const float* h_T, int n_rays, int frame, hipStream_t st) {
+  const int nf = s->n_faces, nm = s->n_meshes;
+  s->rad_w.resize(2 * (size_t)nf);
+  const float total =
+      weights_impl(s->h_faces.data(), s->h_area.data(), nf, s->h_meshes.data(), nm, h_eps2, h_T, s->rad_w.data());
+  s->rad_front.resize(nf);
+  s->rad_rear.resize(nf);
+  int r;
+  if ((r = ffk_emitted_counts(s->rad_w.data(), nf, n_rays, total, s->rad_front.data(), s->rad_rear.data())) < 0)
+    return r;
+  if (!s->has_emission || frame != s->frame || s->rad_front != s->h_front || s->rad_rear != s->h_rear) {
+    if ((r = ffk_scene_set_emission(s, s->rad_front.data(), s->rad_rear.data(), frame, nullptr, st)) < 0)
+      return r;
+  }
+  if ((r = rad_prepare(s, st)) < 0) return r;
+  // T and eps to the device, where the ray powers are computed (their bits
+  // are powers_impl's)
+  if (!s->rad_te_ev) FFK_HIP(hipEventCreateWithFlags(&s->rad_te_ev, hipEventDisableTiming));
+  FFK_HIP(hipEventSynchronize(s->rad_te_ev));
+  float* te = pin_te(s);
+  std::memcpy(te, h_T, 4 * (size_t)nm);
+  std::memcpy(te + nm, h_eps2, 8 * (size_t)nm);
+  FFK_HIP(hipMemcpyAsync(s->d_rad_te, te, 12 * (size_t)nm, hipMemcpyHostToDevice, st));
+  FFK_HIP(hipEventRecord(s->rad_te_ev, st));
+  if (s->rad_W > 0)
+    k_rad_powers<<<(s->rad_W + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(
+        s->rad_W, s->d_walk_face, s->d_walk_mesh, s->d_rad_te, nm, s->d_rad_area, s->d_face_off, s->d_front,
+        s->d_rad_p);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -732,35 +823,36 @@ int ffk_radiance_step(ffk_scene* s, const float* h_eps2, const float* h_T, int n
               FFK_ERR_ARG, "ffk_radiance_step: bad args");
   FFK_REQUIRE(s->has_grid, FFK_ERR_ARG, "ffk_radiance_step: the grid must be built");
   hipStream_t st = as_stream(stream);
-  const int nf = s->n_faces, nm = s->n_meshes;
-  s->rad_w.resize(2 * (size_t)nf);
-  const float total =
-      weights_impl(s->h_faces.data(), s->h_area.data(), nf, s->h_meshes.data(), nm, h_eps2, h_T, s->rad_w.data());
-  s->rad_front.resize(nf);
-  s->rad_rear.resize(nf);
   int r;
-  if ((r = ffk_emitted_counts(s->rad_w.data(), nf, n_rays, total, s->rad_front.data(), s->rad_rear.data())) < 0)
-    return r;
-  if (!s->has_emission || frame != s->frame || s->rad_front != s->h_front || s->rad_rear != s->h_rear) {
-    if ((r = ffk_scene_set_emission(s, s->rad_front.data(), s->rad_rear.data(), frame, nullptr, stream)) < 0)
-      return r;
-  }
-  if ((r = rad_prepare(s, st)) < 0) return r;
-  // T and eps to the device, where the ray powers are computed (their bits
-  // are powers_impl's); meanwhile the per-mesh emission is added to
-  // h_emission here (radiance_cpu.cpp adds to the task's emission)
-  float* te = pin_te(s);
-  std::memcpy(te, h_T, 4 * (size_t)nm);
-  std::memcpy(te + nm, h_eps2, 8 * (size_t)nm);
-  FFK_HIP(hipMemcpyAsync(s->d_rad_te, te, 12 * (size_t)nm, hipMemcpyHostToDevice, st));
-  if (s->rad_W > 0)
-    k_rad_powers<<<(s->rad_W + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(
-        s->rad_W, s->d_walk_face, s->d_walk_mesh, s->d_rad_te, nm, s->d_rad_area, s->d_face_off, s->d_front,
-        s->d_rad_p);
-  FFK_HIP(hipGetLastError());
-  powers_impl(s->h_faces.data(), s->h_area.data(), nf, s->h_meshes.data(), nm, h_eps2, h_T, nullptr, nullptr,
-              nullptr, h_emission);
+  if ((r = step_prepare(s, h_eps2, h_T, n_rays, frame, st)) < 0) return r;
+  // meanwhile the per-mesh emission is added to h_emission here
+  // (radiance_cpu.cpp adds to the task's emission)
+  powers_impl(s->h_faces.data(), s->h_area.data(), s->n_faces, s->h_meshes.data(), s->n_meshes, h_eps2, h_T,
+              nullptr, nullptr, nullptr, h_emission);
   return rad_pass(s, seed, h_absorption, d_stats, st);
+}
+
+int ffk_radiance_step_dev(ffk_scene* s, const float* h_eps2, const float* h_T, int n_rays, int frame,
+                          unsigned long long seed, float* d_emission, float* d_absorption, void* stream) {
+  FFK_REQUIRE(s && s->has_faces && s->has_meshes && h_eps2 && h_T && d_emission && d_absorption && n_rays >= 0,
+              FFK_ERR_ARG, "ffk_radiance_step_dev: bad args");
+  FFK_REQUIRE(s->has_grid, FFK_ERR_ARG, "ffk_radiance_step_dev: the grid must be built");
+  hipStream_t st = as_stream(stream);
+  int r;
+  if ((r = step_prepare(s, h_eps2, h_T, n_rays, frame, st)) < 0) return r;
+  const int nm = s->n_meshes, nb = (nm + kRadThreads - 1) / kRadThreads;
+  k_rad_emission<<<nb, kRadThreads, 0, st>>>(nm, s->d_mesh_walk, s->d_walk_face, s->d_rad_te, s->d_rad_area,
+                                             d_emission);
+  FFK_HIP(hipGetLastError());
+  if ((r = rad_enqueue(s, seed, nullptr, st)) < 0) return r;
+  k_rad_add_absorption<<<nb, kRadThreads, 0, st>>>(nm, s->d_rad_abs, d_absorption);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
+int ffk_radiance_status(ffk_scene* s) {
+  FFK_REQUIRE(s && s->rad_W >= 0, FFK_ERR_ARG, "ffk_radiance_status: no radiance pass on this scene");
+  return rad_status(s);
 }
 
 }  // extern "C"

@@ -16,9 +16,14 @@ __global__ void k_fill_counts(unsigned long long* __restrict__ p, int n) {
 // Ray i: counter (i, 0, 1, i >> 32) as ffk_emit_parallel. Disk aperture
 // (emission/parallel_rays_cpu.cpp:88-106): rho = radius * sqrt(u), angle
 // 2 pi v. Square aperture: x = radius * (2u - 1), y = radius * (2v - 1).
+// The aperture's rotation (rotate_towards from +z to the direction), passed by value.
+struct rot33 {
+  float m[9];
+};
+
 template <bool STATS>
 __global__ void __launch_bounds__(256) k_sun(scene_desc S, v3 dir, v3 center, float distance, float radius,
-                                             int aperture, const float* __restrict__ rot, long long n, uint32_t k0,
+                                             int aperture, rot33 rot, long long n, uint32_t k0,
                                              uint32_t k1, unsigned long long* __restrict__ counts,
                                              unsigned long long* __restrict__ stats) {
   uint32_t nt = 0, nc = 0;
@@ -36,7 +41,7 @@ __global__ void __launch_bounds__(256) k_sun(scene_desc S, v3 dir, v3 center, fl
       x = rho * c;
       y = rho * s;
     }
-    const v3 o = mat_vec(rot, mk(x, y, -distance)) + center;
+    const v3 o = mat_vec(rot.m, mk(x, y, -distance)) + center;
     const v3 e = o + dir * 0.1f;
     hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
     cast_grid<STATS>(S, o, e - o, h, nt, nc);
@@ -67,6 +72,18 @@ __global__ void __launch_bounds__(256) k_sun(scene_desc S, v3 dir, v3 center, fl
 }  // namespace ffk
 
 using namespace ffk;
+
+// ffk_sun_deposit on the device: the double sum of the counts times the
+// reference's per-ray float values (from zero), rounded to float and added to
+// the step's absorption.
+__global__ void k_sun_deposit_f(int nm, const unsigned long long* __restrict__ counts,
+                                const float* __restrict__ absorb2, float ray_power, float* __restrict__ absorption) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= nm) return;
+  const float vf = absorb2[2 * m] * ray_power, vr = absorb2[2 * m + 1] * ray_power;
+  const double a = 0.0 + ((double)counts[2 * (size_t)m] * (double)vf + (double)counts[2 * (size_t)m + 1] * (double)vr);
+  absorption[m] += (float)a;
+}
 
 extern "C" {
 
@@ -107,32 +124,62 @@ int ffk_sun(ffk_scene* s, const float* h_dir3, const float* h_center3, float dis
   if (n_rays == 0) return FFK_OK;
   hipStream_t st = as_stream(stream);
   const v3 dir = mk(h_dir3[0], h_dir3[1], h_dir3[2]);
-  float rot[9];
-  rotate_towards(mk(0.f, 0.f, 1.f), dir, rot);
-  // [0, 2 nm) counts, then the rotation
+  rot33 rot;
+  rotate_towards(mk(0.f, 0.f, 1.f), dir, rot.m);
   unsigned long long* d_buf = nullptr;
   int r;
-  if ((r = dalloc(&d_buf, 2 * (size_t)nm + 5)) < 0) return r;
-  float* d_rot = reinterpret_cast<float*>(d_buf + 2 * (size_t)nm);
-  hipError_t e = hipMemcpyAsync(d_rot, rot, sizeof(rot), hipMemcpyHostToDevice, st);
-  if (e == hipSuccess) {
-    k_fill_counts<<<(2 * nm + 255) / 256, 256, 0, st>>>(d_buf, 2 * nm);
-    const long long blocks_needed = (n_rays + 255) / 256;
-    const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));
-    const v3 center = mk(h_center3[0], h_center3[1], h_center3[2]);
-    if (d_stats)
-      k_sun<true><<<blocks, 256, 0, st>>>(s->desc(), dir, center, distance, radius, aperture, d_rot, n_rays,
-                                          (uint32_t)seed, (uint32_t)(seed >> 32), d_buf, d_stats);
-    else
-      k_sun<false><<<blocks, 256, 0, st>>>(s->desc(), dir, center, distance, radius, aperture, d_rot, n_rays,
-                                           (uint32_t)seed, (uint32_t)(seed >> 32), d_buf, nullptr);
-    e = hipGetLastError();
-  }
+  if ((r = dalloc(&d_buf, 2 * (size_t)nm)) < 0) return r;
+  k_fill_counts<<<(2 * nm + 255) / 256, 256, 0, st>>>(d_buf, 2 * nm);
+  const long long blocks_needed = (n_rays + 255) / 256;
+  const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));
+  const v3 center = mk(h_center3[0], h_center3[1], h_center3[2]);
+  if (d_stats)
+    k_sun<true><<<blocks, 256, 0, st>>>(s->desc(), dir, center, distance, radius, aperture, rot, n_rays,
+                                        (uint32_t)seed, (uint32_t)(seed >> 32), d_buf, d_stats);
+  else
+    k_sun<false><<<blocks, 256, 0, st>>>(s->desc(), dir, center, distance, radius, aperture, rot, n_rays,
+                                         (uint32_t)seed, (uint32_t)(seed >> 32), d_buf, nullptr);
+  hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(h_counts, d_buf, 16 * (size_t)nm, hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipStreamSynchronize(st);
   dfree(d_buf);
   FFK_HIP(e);
+  return FFK_OK;
+}
+
+int ffk_sun_count_dev(ffk_scene* s, const float* h_dir3, const float* h_center3, float distance, float radius,
+                      int aperture, long long n_rays, unsigned long long seed, unsigned long long* d_counts,
+                      void* stream) {
+  FFK_REQUIRE(s && h_dir3 && h_center3 && d_counts && n_rays >= 0, FFK_ERR_ARG, "ffk_sun_count_dev: bad args");
+  FFK_REQUIRE(s->has_grid && s->has_meshes, FFK_ERR_ARG, "ffk_sun_count_dev: meshes and grid must be set");
+  FFK_REQUIRE(aperture == 0 || aperture == 1, FFK_ERR_ARG, "ffk_sun_count_dev: aperture %d (0 disk, 1 square)",
+              aperture);
+  const int nm = s->n_meshes;
+  if (nm == 0) return FFK_OK;
+  hipStream_t st = as_stream(stream);
+  k_fill_counts<<<(2 * nm + 255) / 256, 256, 0, st>>>(d_counts, 2 * nm);
+  if (n_rays > 0) {
+    const v3 dir = mk(h_dir3[0], h_dir3[1], h_dir3[2]);
+    rot33 rot;
+    rotate_towards(mk(0.f, 0.f, 1.f), dir, rot.m);
+    const long long blocks_needed = (n_rays + 255) / 256;
+    const int blocks = (int)std::max<long long>(1, std::min<long long>(blocks_needed, 64LL * device_cu_count()));
+    const v3 center = mk(h_center3[0], h_center3[1], h_center3[2]);
+    k_sun<false><<<blocks, 256, 0, st>>>(s->desc(), dir, center, distance, radius, aperture, rot, n_rays,
+                                         (uint32_t)seed, (uint32_t)(seed >> 32), d_counts, nullptr);
+  }
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
+int ffk_sun_deposit_dev(int n_meshes, const unsigned long long* d_counts, const float* d_absorb2, float ray_power,
+                        float* d_absorption, void* stream) {
+  FFK_REQUIRE(n_meshes >= 0 && d_counts && d_absorb2 && d_absorption, FFK_ERR_ARG, "ffk_sun_deposit_dev: bad args");
+  if (n_meshes == 0) return FFK_OK;
+  k_sun_deposit_f<<<(n_meshes + 255) / 256, 256, 0, as_stream(stream)>>>(n_meshes, d_counts, d_absorb2, ray_power,
+                                                                         d_absorption);
+  FFK_HIP(hipGetLastError());
   return FFK_OK;
 }
 

@@ -202,14 +202,54 @@ __device__ inline double adams_update(int n, int m, int n_step, double dt, doubl
   return tn + dt * sum;
 }
 
+// Conduction from the step's float temperatures into its float arrays: the
+// double per-pair sums of k_conduction (from zero), rounded to float and added.
+__global__ void k_conduction_f(int n, const int* __restrict__ row_ptr, const int* __restrict__ cols,
+                               const double* __restrict__ G, const float* __restrict__ T,
+                               float* __restrict__ emission, float* __restrict__ absorption) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  const double tm = T[m];
+  double ab = 0.0, em = 0.0;
+  for (int k = row_ptr[m]; k < row_ptr[m + 1]; ++k) {
+    const double f = G[k] * ((double)T[cols[k]] - tm);
+    if (f > 0) ab += f;
+    else em -= f;
+  }
+  absorption[m] += (float)(0.0 + ab);
+  emission[m] += (float)(0.0 + em);
+}
+
+// Heat sources in list order (one thread: a handful of additions, some of
+// which may hit the same mesh).
+__global__ void k_heat_sources(int n_sources, const int* __restrict__ mesh, const float* __restrict__ power,
+                               float* __restrict__ emission, float* __restrict__ absorption) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  for (int i = 0; i < n_sources; ++i) {
+    const float p = power[i];
+    if (p > 0) absorption[mesh[i]] += p;
+    else emission[mesh[i]] += -p;
+  }
+}
+
+__global__ void k_step_begin(int n, const double* __restrict__ T_n, float* __restrict__ T_f,
+                             float* __restrict__ emission, float* __restrict__ absorption) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  T_f[m] = (float)T_n[m];
+  emission[m] = 0.f;
+  absorption[m] = 0.f;
+}
+
+template <class In>
 __global__ void k_adams(int n, int n_step, double dt, const double* __restrict__ C,
-                        const double* __restrict__ emission, const double* __restrict__ absorption,
+                        const In* __restrict__ emission, const In* __restrict__ absorption,
                         double* __restrict__ E_hist, const double* __restrict__ T_n,
                         double* __restrict__ T_next, double* __restrict__ balance) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
   double net = 0.0, stored = 0.0;
   if (m < n) {
-    net = absorption[m] - emission[m];
+    net = (double)absorption[m] - (double)emission[m];
     const double tn = T_n[m];
     const double tx = adams_update(n, m, n_step, dt, net, C[m], E_hist, tn);
     T_next[m] = tx;
@@ -726,8 +766,42 @@ int ffk_adams_step(int n, int n_step, double dt, const double* d_C, const double
                    double* d_balance, void* stream) {
   FFK_REQUIRE(n > 0 && n_step >= 0 && d_C && d_emission && d_absorption && d_E_hist && d_T_n && d_T_next,
               FFK_ERR_ARG, "ffk_adams_step: bad args");
-  k_adams<<<(n + 255) / 256, 256, 0, as_stream(stream)>>>(n, n_step, dt, d_C, d_emission, d_absorption,
-                                                         d_E_hist, d_T_n, d_T_next, d_balance);
+  k_adams<double><<<(n + 255) / 256, 256, 0, as_stream(stream)>>>(n, n_step, dt, d_C, d_emission, d_absorption,
+                                                                 d_E_hist, d_T_n, d_T_next, d_balance);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+int ffk_adams_step_f(int n, int n_step, double dt, const double* d_C, const float* d_emission,
+                     const float* d_absorption, double* d_E_hist, const double* d_T_n, double* d_T_next,
+                     double* d_balance, void* stream) {
+  FFK_REQUIRE(n > 0 && n_step >= 0 && d_C && d_emission && d_absorption && d_E_hist && d_T_n && d_T_next,
+              FFK_ERR_ARG, "ffk_adams_step_f: bad args");
+  k_adams<float><<<(n + 255) / 256, 256, 0, as_stream(stream)>>>(n, n_step, dt, d_C, d_emission, d_absorption,
+                                                                d_E_hist, d_T_n, d_T_next, d_balance);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+int ffk_conduction_dev(int n, const int* d_row_ptr, const int* d_cols, const double* d_G, const float* d_T,
+                       float* d_emission, float* d_absorption, void* stream) {
+  FFK_REQUIRE(n > 0 && d_row_ptr && d_T && d_emission && d_absorption, FFK_ERR_ARG, "ffk_conduction_dev: bad args");
+  k_conduction_f<<<(n + 255) / 256, 256, 0, as_stream(stream)>>>(n, d_row_ptr, d_cols, d_G, d_T, d_emission,
+                                                                d_absorption);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+int ffk_heat_sources_dev(int n_sources, const int* d_mesh, const float* d_power, float* d_emission,
+                         float* d_absorption, void* stream) {
+  FFK_REQUIRE(n_sources >= 0 && (n_sources == 0 || (d_mesh && d_power)) && d_emission && d_absorption, FFK_ERR_ARG,
+              "ffk_heat_sources_dev: bad args");
+  if (n_sources == 0) return FFK_OK;
+  k_heat_sources<<<1, 64, 0, as_stream(stream)>>>(n_sources, d_mesh, d_power, d_emission, d_absorption);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+int ffk_step_begin_dev(int n, const double* d_T_n, float* d_T_f, float* d_emission, float* d_absorption,
+                       void* stream) {
+  FFK_REQUIRE(n > 0 && d_T_n && d_T_f && d_emission && d_absorption, FFK_ERR_ARG, "ffk_step_begin_dev: bad args");
+  k_step_begin<<<(n + 255) / 256, 256, 0, as_stream(stream)>>>(n, d_T_n, d_T_f, d_emission, d_absorption);
   FFK_HIP(hipGetLastError());
   return FFK_OK;
 }

@@ -79,6 +79,18 @@ float area_of(const math::triangle_t& f) {
   return sqrtf(nx * nx + ny * ny + nz * nz) / 2;
 }
 
+// A step's equation arrays in device memory (the device-resident step of the
+// Adams solution): each equation adds to d_em / d_ab what its calculate adds
+// to thermal_equation::task_t's emission / absorption, in the same float
+// arithmetic, on the solution's stream.
+struct dev_task {
+  const float* h_T;  // the step's temperatures on the host
+  const float* d_T;  // the same on the device
+  float* d_em;
+  float* d_ab;
+  hipStream_t st;
+};
+
 }  // namespace
 
 // ============================================================ ray caster ==
@@ -736,6 +748,7 @@ struct state {
   std::vector<int> face2mesh;
   std::vector<float> eps2, weights, power;
   std::vector<double> absorption;
+  bool dev_pending = false;  // a device step whose status is still to be read
 };
 struct system : thermal_equation::system_t {
   state* st;
@@ -851,6 +864,45 @@ int calculate(system* sys, thermal_equation::task_t* task) {
   }
   for (int m = 0; m < nm; ++m) task->absorption[m] += (float)s->absorption[m];
   return THERMAL_EQUATION_OK;
+}
+
+// The device-resident form runs where calculate fuses the step on the device
+// (the HIP Malley emitter over the HIP grid caster), or where there is
+// nothing to add.
+bool can_dev(system* sys) {
+  state* s = sys->st;
+  if (!s || !s->scene) return false;
+  if (s->scene->n_faces == 0 || s->scene->n_meshes == 0) return true;
+  emission::system_t* emitter = s->params.emitter;
+  if (!hip_malley_emission::is_hip(emitter)) return false;
+  const hip_malley_emission::state* em = hip_malley_emission::state_of(emitter);
+  return hip_ray_caster::is_hip(em->caster) && hip_ray_caster::method_of(em->caster) == FFK_CAST_GRID;
+}
+
+// calculate's fused branch with the step's arrays on the device.
+int calculate_dev(system* sys, const dev_task& t) {
+  state* s = sys->st;
+  const subject::scene_t* scene = s->scene;
+  if (scene->n_faces == 0 || scene->n_meshes == 0) return THERMAL_EQUATION_OK;
+  hip_malley_emission::state* em = hip_malley_emission::state_of(s->params.emitter);
+  if (!s->grid) {
+    if (ffk_scene_build_grid(s->s, 0, 0, 0, t.st) < 0) return -THERMAL_EQUATION_ERROR;
+    s->grid = true;
+  }
+  const unsigned long long key = hip_malley_emission::next_key(em);
+  if (ffk_radiance_step_dev(s->s, s->eps2.data(), t.h_T, s->params.n_rays, em->frame, key, t.d_em, t.d_ab, t.st) <
+      0)
+    return -THERMAL_EQUATION_ERROR;
+  s->dev_pending = true;
+  return THERMAL_EQUATION_OK;
+}
+
+// After the solution's stream has drained: the pass's power check.
+int status_dev(system* sys) {
+  state* s = sys->st;
+  if (!s->dev_pending) return THERMAL_EQUATION_OK;
+  s->dev_pending = false;
+  return ffk_radiance_status(s->s) < 0 ? -THERMAL_EQUATION_ERROR : THERMAL_EQUATION_OK;
 }
 
 const thermal_equation::system_methods_t methods = {
@@ -969,6 +1021,18 @@ int calculate(system* sys, thermal_equation::task_t* task) {
   return THERMAL_EQUATION_OK;
 }
 
+bool can_dev(system* sys) { return sys->st && sys->st->scene; }
+
+// calculate with the step's arrays on the device: one kernel from the float
+// temperatures, the same double sums, rounded and added.
+int calculate_dev(system* sys, const dev_task& t) {
+  state* s = sys->st;
+  if (s->n == 0) return THERMAL_EQUATION_OK;
+  return ffk_conduction_dev(s->n, s->row.p, s->col.p, s->G.p, t.d_T, t.d_em, t.d_ab, t.st) < 0
+             ? -THERMAL_EQUATION_ERROR
+             : THERMAL_EQUATION_OK;
+}
+
 const thermal_equation::system_methods_t methods = {
     (int (*)(thermal_equation::system_t*, void*))&init,
     (int (*)(thermal_equation::system_t*))&shutdown,
@@ -991,8 +1055,20 @@ thermal_equation::system_t* system_create() {
 // inside its step kernel).
 namespace hip_heat_source {
 
+// Device copies of the sources for the device-resident step: uploaded each
+// step through pinned staging (the caller may change its sources between
+// steps, as the reference reads them on every calculate).
+struct dev_state {
+  dbuf<int> mesh;
+  dbuf<float> power;
+  int* h_mesh = nullptr;
+  float* h_power = nullptr;
+  int cap = 0;
+  hipEvent_t ev = nullptr;  // the last upload from the staging
+};
 struct system : thermal_equation::system_t {
   heat_source_equation::params_t* params;
+  dev_state* dev;
 };
 
 int init(system* sys, heat_source_equation::params_t* params) {
@@ -1002,6 +1078,18 @@ int init(system* sys, heat_source_equation::params_t* params) {
 int shutdown(system* sys) {
   delete sys->params;
   sys->params = nullptr;
+  if (dev_state* d = sys->dev) {
+    if (d->ev) {
+      (void)hipEventSynchronize(d->ev);
+      (void)hipEventDestroy(d->ev);
+    }
+    if (d->h_mesh) (void)hipHostFree(d->h_mesh);
+    if (d->h_power) (void)hipHostFree(d->h_power);
+    d->mesh.release();
+    d->power.release();
+    delete d;
+    sys->dev = nullptr;
+  }
   return THERMAL_EQUATION_OK;
 }
 int set_scene(system*, subject::scene_t*) { return THERMAL_EQUATION_OK; }
@@ -1015,6 +1103,41 @@ int calculate(system* sys, thermal_equation::task_t* task) {
   return THERMAL_EQUATION_OK;
 }
 
+bool can_dev(system* sys) { return sys->params != nullptr; }
+
+// calculate with the step's arrays on the device (the same additions, in
+// list order, by one device thread).
+int calculate_dev(system* sys, const dev_task& t) {
+  const heat_source_equation::params_t& p = *sys->params;
+  if (p.n_sources <= 0) return THERMAL_EQUATION_OK;
+  if (!sys->dev && !(sys->dev = new (std::nothrow) dev_state())) return -THERMAL_EQUATION_ERROR;
+  dev_state& d = *sys->dev;
+  if (!d.ev && hipEventCreateWithFlags(&d.ev, hipEventDisableTiming) != hipSuccess) return -THERMAL_EQUATION_ERROR;
+  if (hipEventSynchronize(d.ev) != hipSuccess) return -THERMAL_EQUATION_ERROR;
+  if (d.cap < p.n_sources) {
+    if (d.h_mesh) (void)hipHostFree(d.h_mesh);
+    if (d.h_power) (void)hipHostFree(d.h_power);
+    d.h_mesh = nullptr;
+    d.h_power = nullptr;
+    d.cap = 0;
+    if (hipHostMalloc((void**)&d.h_mesh, 4 * (size_t)p.n_sources, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void**)&d.h_power, 4 * (size_t)p.n_sources, hipHostMallocDefault) != hipSuccess ||
+        !d.mesh.reserve(p.n_sources) || !d.power.reserve(p.n_sources))
+      return -THERMAL_EQUATION_ERROR;
+    d.cap = p.n_sources;
+  }
+  for (int i = 0; i != p.n_sources; ++i) {
+    d.h_mesh[i] = p.sources[i].mesh_idx;
+    d.h_power[i] = p.sources[i].power;
+  }
+  if (hipMemcpyAsync(d.mesh.p, d.h_mesh, 4 * (size_t)p.n_sources, hipMemcpyHostToDevice, t.st) != hipSuccess ||
+      hipMemcpyAsync(d.power.p, d.h_power, 4 * (size_t)p.n_sources, hipMemcpyHostToDevice, t.st) != hipSuccess ||
+      hipEventRecord(d.ev, t.st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  return ffk_heat_sources_dev(p.n_sources, d.mesh.p, d.power.p, t.d_em, t.d_ab, t.st) < 0 ? -THERMAL_EQUATION_ERROR
+                                                                                        : THERMAL_EQUATION_OK;
+}
+
 const thermal_equation::system_methods_t methods = {
     (int (*)(thermal_equation::system_t*, void*))&init,
     (int (*)(thermal_equation::system_t*))&shutdown,
@@ -1026,6 +1149,7 @@ thermal_equation::system_t* system_create() {
   if (!s) return nullptr;
   s->methods = &methods;
   s->params = nullptr;
+  s->dev = nullptr;
   return s;
 }
 
@@ -1046,6 +1170,13 @@ struct state {
   std::vector<float> absorb2;
   std::vector<unsigned long long> counts;
   std::vector<double> absorption;
+  dbuf<float> d_absorb2;  // device-resident step: absorbances (uploaded once per scene)
+  dbuf<unsigned long long> d_counts;
+  bool dev_ready = false;
+  // a step's hit counts launched ahead on this system's stream (prelaunch_dev)
+  hipEvent_t ev_start = nullptr, ev_counted = nullptr;
+  bool counting = false;
+  float counted_power = 0.f;
 };
 struct system : thermal_equation::system_t {
   state* st;
@@ -1063,6 +1194,10 @@ int shutdown(system* sys) {
   if (!s) return THERMAL_EQUATION_OK;
   if (s->st) (void)hipStreamSynchronize(s->st);
   ffk_scene_destroy(s->s);
+  s->d_absorb2.release();
+  s->d_counts.release();
+  if (s->ev_start) (void)hipEventDestroy(s->ev_start);
+  if (s->ev_counted) (void)hipEventDestroy(s->ev_counted);
   drop_stream(s->st);
   delete s;
   sys->st = nullptr;
@@ -1092,6 +1227,7 @@ int set_scene(system* sys, subject::scene_t* scene) {
   if (nf > 0 && ffk_bsphere((const float*)scene->faces, nf, s->center, &s->bs_radius) < 0)
     return -THERMAL_EQUATION_ERROR;
   s->grid = false;
+  s->dev_ready = false;
   if (nf == 0 || nm == 0) return THERMAL_EQUATION_OK;
   if (!s->s && ffk_scene_create(&s->s) < 0) return -THERMAL_EQUATION_ERROR;
   if (ffk_scene_set_faces(s->s, (const float*)scene->faces, nf, s->st) < 0) return -THERMAL_EQUATION_ERROR;
@@ -1160,6 +1296,73 @@ int calculate(system* sys, thermal_equation::task_t* task) {
   return THERMAL_EQUATION_OK;
 }
 
+// As hip_radiance::can_dev: where calculate fuses the step on the device, or
+// where it adds nothing.
+bool can_dev(system* sys) {
+  state* s = sys->st;
+  if (!s || !s->scene) return false;
+  if (s->params.source == 0 || s->scene->n_faces == 0 || s->scene->n_meshes == 0) return true;
+  emission::system_t* emitter = s->params.emitter;
+  if (!hip_parallel_rays_emission::is_hip(emitter)) return false;
+  const hip_parallel_rays_emission::state* em = hip_parallel_rays_emission::state_of(emitter);
+  return hip_ray_caster::is_hip(em->caster) && hip_ray_caster::method_of(em->caster) == FFK_CAST_GRID;
+}
+
+// The first half of calculate's fused branch, launched when the step starts:
+// the parallel rays' hit counts depend on the scene and the ray key only, so
+// they are counted on this system's own stream while the solution's stream
+// runs the equations before this one.
+int prelaunch_dev(system* sys, const dev_task& t) {
+  state* s = sys->st;
+  s->counting = false;
+  if (s->params.source == 0) return THERMAL_EQUATION_OK;
+  const subject::scene_t* scene = s->scene;
+  if (scene->n_faces == 0 || scene->n_meshes == 0) return THERMAL_EQUATION_OK;
+  const parallel_rays_cpu::source_t source = s->params.source(s->params.source_param);
+  const float radius = s->bs_radius * 1.1f;
+  const float distance = 1.1f + radius;
+  const int n_rays = s->params.n_rays;
+  const int nm = scene->n_meshes;
+  if (!s->grid) {
+    if (ffk_scene_build_grid(s->s, 0, 0, 0, t.st) < 0) return -THERMAL_EQUATION_ERROR;
+    s->grid = true;
+  }
+  if (!s->dev_ready) {
+    if (!s->d_absorb2.reserve(2 * (size_t)nm) || !s->d_counts.reserve(2 * (size_t)nm) ||
+        hipMemcpy(s->d_absorb2.p, s->absorb2.data(), 8 * (size_t)nm, hipMemcpyHostToDevice) != hipSuccess)
+      return -THERMAL_EQUATION_ERROR;
+    if ((!s->ev_start && hipEventCreateWithFlags(&s->ev_start, hipEventDisableTiming) != hipSuccess) ||
+        (!s->ev_counted && hipEventCreateWithFlags(&s->ev_counted, hipEventDisableTiming) != hipSuccess))
+      return -THERMAL_EQUATION_ERROR;
+    s->dev_ready = true;
+  }
+  hip_parallel_rays_emission::state* em = hip_parallel_rays_emission::state_of(s->params.emitter);
+  const unsigned long long key = hip_parallel_rays_emission::next_key(em);
+  // the counts follow what the solution's stream did before (the grid, the
+  // previous step's reading of d_counts)
+  if (hipEventRecord(s->ev_start, t.st) != hipSuccess || hipStreamWaitEvent(s->st, s->ev_start, 0) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  if (ffk_sun_count_dev(s->s, &source.direction.x, s->center, distance, radius, 0, n_rays, key, s->d_counts.p,
+                        s->st) < 0 ||
+      hipEventRecord(s->ev_counted, s->st) != hipSuccess)
+    return -THERMAL_EQUATION_ERROR;
+  s->counted_power = source.power * (float(M_PI) * radius * radius) / n_rays;  // the reference's ray power
+  s->counting = true;
+  return THERMAL_EQUATION_OK;
+}
+
+// The second half, in the equations' order: the counts' powers added to the
+// step's absorption on the solution's stream.
+int calculate_dev(system* sys, const dev_task& t) {
+  state* s = sys->st;
+  if (!s->counting) return THERMAL_EQUATION_OK;
+  s->counting = false;
+  if (hipStreamWaitEvent(t.st, s->ev_counted, 0) != hipSuccess) return -THERMAL_EQUATION_ERROR;
+  return ffk_sun_deposit_dev(s->scene->n_meshes, s->d_counts.p, s->d_absorb2.p, s->counted_power, t.d_ab, t.st) < 0
+             ? -THERMAL_EQUATION_ERROR
+             : THERMAL_EQUATION_OK;
+}
+
 const thermal_equation::system_methods_t methods = {
     (int (*)(thermal_equation::system_t*, void*))&init,
     (int (*)(thermal_equation::system_t*))&shutdown,
@@ -1176,6 +1379,47 @@ thermal_equation::system_t* system_create() {
 
 }  // namespace hip_distant_source
 
+// ============================================== device-resident equations ==
+// The equations of this library that can add to a step's arrays in device
+// memory (dev_task), found by their method tables.
+namespace dev_equations {
+
+bool can(thermal_equation::system_t* e) {
+  if (!e) return false;
+  if (e->methods == &hip_radiance::methods) return hip_radiance::can_dev((hip_radiance::system*)e);
+  if (e->methods == &hip_conduction::methods) return hip_conduction::can_dev((hip_conduction::system*)e);
+  if (e->methods == &hip_heat_source::methods) return hip_heat_source::can_dev((hip_heat_source::system*)e);
+  if (e->methods == &hip_distant_source::methods)
+    return hip_distant_source::can_dev((hip_distant_source::system*)e);
+  return false;
+}
+
+int run(thermal_equation::system_t* e, const dev_task& t) {
+  if (e->methods == &hip_radiance::methods) return hip_radiance::calculate_dev((hip_radiance::system*)e, t);
+  if (e->methods == &hip_conduction::methods) return hip_conduction::calculate_dev((hip_conduction::system*)e, t);
+  if (e->methods == &hip_heat_source::methods)
+    return hip_heat_source::calculate_dev((hip_heat_source::system*)e, t);
+  if (e->methods == &hip_distant_source::methods)
+    return hip_distant_source::calculate_dev((hip_distant_source::system*)e, t);
+  return -THERMAL_EQUATION_ERROR;
+}
+
+// Work an equation can launch when the step starts, ahead of the equations
+// before it (on its own stream; run() then joins it in order).
+int prelaunch(thermal_equation::system_t* e, const dev_task& t) {
+  if (e->methods == &hip_distant_source::methods)
+    return hip_distant_source::prelaunch_dev((hip_distant_source::system*)e, t);
+  return THERMAL_EQUATION_OK;
+}
+
+// Checks made once the step's stream has drained.
+int status(thermal_equation::system_t* e) {
+  if (e->methods == &hip_radiance::methods) return hip_radiance::status_dev((hip_radiance::system*)e);
+  return THERMAL_EQUATION_OK;
+}
+
+}  // namespace dev_equations
+
 // ==================================================== Adams solver (FP64) ==
 namespace hip_adams {
 
@@ -1183,9 +1427,11 @@ struct state {
   thermal_solution::params_t params = {0, nullptr};
   subject::scene_t* scene = nullptr;
   int n_step = 0;
-  std::vector<double> C, T;
+  std::vector<double> C, T, Tn;
   std::vector<float> Tf, em, ab;
   dbuf<double> d;  // C, em, ab, E ring (5N), T_n, T_next
+  dbuf<float> f;   // device-resident step: T, emission, absorption (float, N each)
+  bool dev_loop = true;
   hipStream_t st = nullptr;
 };
 struct system : thermal_solution::system_t {
@@ -1204,6 +1450,7 @@ int shutdown(system* sys) {
   if (!s) return THERMAL_SOLUTION_OK;
   if (s->st) (void)hipStreamSynchronize(s->st);
   s->d.release();
+  s->f.release();
   drop_stream(s->st);
   delete s;
   sys->st = nullptr;
@@ -1229,6 +1476,9 @@ int set_scene(system* sys, subject::scene_t* scene, float* temperatures) {
     s->C[m] = (double)a * sh.thickness * sh.density * sh.heat_capacity;  // cpu_adams.cpp:127-129
   }
   s->T.assign(temperatures, temperatures + N);
+  // the equations set their scenes up on their own streams; the steps run on
+  // this one
+  if (hipDeviceSynchronize() != hipSuccess) return -THERMAL_SOLUTION_ERROR;
   if (!s->d.reserve(10 * (size_t)N)) return -THERMAL_SOLUTION_ERROR;
   if (hipMemsetAsync(s->d.p, 0, 80 * (size_t)N, s->st) != hipSuccess ||
       hipMemcpyAsync(s->d.p, s->C.data(), 8 * N, hipMemcpyHostToDevice, s->st) != hipSuccess ||
@@ -1238,8 +1488,9 @@ int set_scene(system* sys, subject::scene_t* scene, float* temperatures) {
   return THERMAL_SOLUTION_OK;
 }
 
-// thermal_solution/cpu_adams.cpp:107-182 with the update in FP64.
-int calculate(system* sys, thermal_solution::task_t* task) {
+// thermal_solution/cpu_adams.cpp:107-182 with the update in FP64: the
+// equations add to host arrays, which go to the device for the update.
+int calculate_host(system* sys, thermal_solution::task_t* task) {
   state* s = sys->st;
   const int N = s->scene->n_meshes;
   s->Tf.resize(N);
@@ -1268,6 +1519,55 @@ int calculate(system* sys, thermal_solution::task_t* task) {
   for (int m = 0; m < N; ++m) task->temperatures[m] = (float)s->T[m];
   task->n_step = ++s->n_step;
   return THERMAL_SOLUTION_OK;
+}
+
+// The same step with the equations' arrays in device memory: the step's
+// float temperatures, emission and absorption stay on the device from the
+// equations to the FP64 update, and one copy brings the new temperatures
+// back for the task (radiance keeps its weights and counts on the host,
+// from the host temperatures). The arithmetic is calculate_host's, so the
+// temperatures are its bits. A failed step leaves the solution's state as
+// it was.
+int calculate_dev(system* sys, thermal_solution::task_t* task) {
+  state* s = sys->st;
+  const int N = s->scene->n_meshes;
+  s->Tf.resize(N);
+  for (int m = 0; m < N; ++m) s->Tf[m] = (float)s->T[m];
+  if (!s->f.reserve(3 * (size_t)N)) return -THERMAL_SOLUTION_ERROR;
+  double* d = s->d.p;
+  double *dC = d, *dE = d + 3 * (size_t)N, *dT = d + 8 * (size_t)N, *dTn = d + 9 * (size_t)N;
+  float *dTf = s->f.p, *dem = dTf + N, *dab = dTf + 2 * (size_t)N;
+  if (ffk_step_begin_dev(N, dT, dTf, dem, dab, s->st) < 0) return -THERMAL_SOLUTION_ERROR;
+  const dev_task t = {s->Tf.data(), dTf, dem, dab, s->st};
+  int r = THERMAL_SOLUTION_OK;
+  for (int i = 0; i != s->params.n_equations && r >= 0; ++i) r = dev_equations::prelaunch(s->params.equations[i], t);
+  for (int i = 0; i != s->params.n_equations && r >= 0; ++i) r = dev_equations::run(s->params.equations[i], t);
+  if (r >= 0 && ffk_adams_step_f(N, s->n_step, task->time_delta, dC, dem, dab, dE, dT, dTn, nullptr, s->st) < 0)
+    r = -THERMAL_SOLUTION_ERROR;
+  s->Tn.resize(N);
+  if (r >= 0 && hipMemcpyAsync(s->Tn.data(), dTn, 8 * (size_t)N, hipMemcpyDeviceToHost, s->st) != hipSuccess)
+    r = -THERMAL_SOLUTION_ERROR;
+  if (hipStreamSynchronize(s->st) != hipSuccess && r >= 0) r = -THERMAL_SOLUTION_ERROR;
+  for (int i = 0; i != s->params.n_equations; ++i) {  // every pending check is read
+    const int q = dev_equations::status(s->params.equations[i]);
+    if (r >= 0 && q < 0) r = q;
+  }
+  if (r < 0) return r;
+  if (hipMemcpyAsync(dT, dTn, 8 * (size_t)N, hipMemcpyDeviceToDevice, s->st) != hipSuccess)
+    return -THERMAL_SOLUTION_ERROR;
+  s->T.swap(s->Tn);
+  for (int m = 0; m < N; ++m) task->temperatures[m] = (float)s->T[m];
+  task->n_step = ++s->n_step;
+  return THERMAL_SOLUTION_OK;
+}
+
+// The device-resident step when every equation has a device form (and the
+// solution has not been told otherwise), else the host-array step.
+int calculate(system* sys, thermal_solution::task_t* task) {
+  state* s = sys->st;
+  bool dev = s->dev_loop && s->scene && s->scene->n_meshes > 0;
+  for (int i = 0; dev && i != s->params.n_equations; ++i) dev = dev_equations::can(s->params.equations[i]);
+  return dev ? calculate_dev(sys, task) : calculate_host(sys, task);
 }
 
 const thermal_solution::system_methods_t methods = {
@@ -1509,6 +1809,12 @@ int ffp_thermal_solution_set_scene(void* s, ffp_subject_scene* scene, float* T) 
 int ffp_thermal_solution_calculate(void* s, ffp_ts_task* task) {
   thermal_solution::system_t* t = (thermal_solution::system_t*)s;
   return t->methods->calculate(t, (thermal_solution::task_t*)task);
+}
+int ffp_thermal_solution_set_device_loop(void* s, int on) {
+  thermal_solution::system_t* t = (thermal_solution::system_t*)s;
+  if (!t || t->methods != &hip_adams::methods || !((hip_adams::system*)t)->st) return -THERMAL_SOLUTION_ERROR;
+  ((hip_adams::system*)t)->st->dev_loop = on != 0;
+  return THERMAL_SOLUTION_OK;
 }
 void ffp_thermal_solution_free(void* s) {
   if (!s) return;

@@ -89,6 +89,14 @@ SIGNATURES = {
     "ffk_sbff_noalloc": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "ffk_adams_step": (C.c_int, [C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_adams_step_f": (C.c_int, [C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_radiance_step_dev": (C.c_int, [_vp, _f, _f, C.c_int, C.c_int, _ull, _vp, _vp, _vp]),
+    "ffk_radiance_status": (C.c_int, [_vp]),
+    "ffk_sun_count_dev": (C.c_int, [_vp, _f, _f, C.c_float, C.c_float, C.c_int, _ll, _ull, _vp, _vp]),
+    "ffk_sun_deposit_dev": (C.c_int, [C.c_int, _vp, _vp, C.c_float, _vp, _vp]),
+    "ffk_conduction_dev": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_heat_sources_dev": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_step_begin_dev": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction_graph": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, C.POINTER(C.c_double), C.c_int,
                                        _i]),
     "ffk_thermal_create": (C.c_int, [C.POINTER(_vp), C.c_int, _vp, _vp, _vp, _vp]),

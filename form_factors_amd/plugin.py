@@ -135,6 +135,7 @@ SIGNATURES = {
     "ffp_thermal_solution_create": (_P, [C.c_int, C.c_int, C.POINTER(_P)]),
     "ffp_thermal_solution_set_scene": (C.c_int, [_P, C.POINTER(SubjectScene), C.POINTER(C.c_float)]),
     "ffp_thermal_solution_calculate": (C.c_int, [_P, C.POINTER(TsTask)]),
+    "ffp_thermal_solution_set_device_loop": (C.c_int, [_P, C.c_int]),
     "ffp_thermal_solution_free": (None, [_P]),
 }
 

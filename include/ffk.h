@@ -342,6 +342,57 @@ int ffk_radiance_step(ffk_scene* s, const float* h_eps2, const float* h_T, int n
                       unsigned long long seed, float* h_emission, double* h_absorption,
                       unsigned long long* d_stats, void* stream);
 
+/* ---- device-resident step equations ------------------------------------------
+ * The per-step equations of a thermal run (radiance_cpu.cpp:132-190,
+ * conductive_cpu.cpp:83-141, heat_source_cpu.cpp:48-61,
+ * parallel_rays_cpu.cpp:92-131, cpu_adams.cpp:107-182) for a caller that keeps
+ * the step's float emission and absorption (thermal_equation::task_t's
+ * arrays) in device memory. Each adds its terms to d_emission / d_absorption
+ * (float[n_meshes]) with the float arithmetic and in the order of the
+ * host-array forms above, so the sums are theirs bit for bit; nothing is
+ * copied back and the stream is not synchronised. No counterpart in the
+ * reference (whose equations take host arrays): they let a thermal solution
+ * run a step's equations and its Adams update without a host round trip.
+ *
+ * ffk_radiance_step_dev: ffk_radiance_step with device outputs. The weights,
+ * their float total in mesh order and the emitted counts stay on the host
+ * (h_T is the step's float temperature on the host); the emission, the pass
+ * and the absorption (the exact double sum rounded to float) run on the
+ * device. ffk_radiance_status, called once the stream has drained, reports
+ * FFK_ERR_ARG when a walked ray power was negative or not finite (the outputs
+ * are then not meaningful). */
+int ffk_radiance_step_dev(ffk_scene* s, const float* h_eps2, const float* h_T, int n_rays, int frame,
+                          unsigned long long seed, float* d_emission, float* d_absorption, void* stream);
+int ffk_radiance_status(ffk_scene* s);
+/* ffk_sun and ffk_sun_deposit on device arrays. ffk_sun_count_dev overwrites
+ * d_counts (uint64[2 n_meshes], caller scratch) with the hit counts per mesh
+ * side; the counts depend on the scene, the aperture and the seed only, so a
+ * caller may count on another stream while the step's other equations run.
+ * ffk_sun_deposit_dev then adds (float)(counts * (absorbance * ray_power)),
+ * the double sum from zero, to d_absorption, with d_absorb2 (float[2
+ * n_meshes], front and rear absorbance) on the device. */
+int ffk_sun_count_dev(ffk_scene* s, const float* h_dir3, const float* h_center3, float distance, float radius,
+                      int aperture, long long n_rays, unsigned long long seed, unsigned long long* d_counts,
+                      void* stream);
+int ffk_sun_deposit_dev(int n_meshes, const unsigned long long* d_counts, const float* d_absorb2, float ray_power,
+                        float* d_absorption, void* stream);
+/* ffk_conduction from the step's float temperatures: the per-mesh double sums
+ * (from zero) rounded to float and added to d_emission / d_absorption. */
+int ffk_conduction_dev(int n, const int* d_row_ptr, const int* d_cols, const double* d_G, const float* d_T,
+                       float* d_emission, float* d_absorption, void* stream);
+/* Heat sources in list order: power > 0 is added to d_absorption[mesh], else
+ * -power to d_emission[mesh] (d_mesh, d_power: device arrays of n_sources). */
+int ffk_heat_sources_dev(int n_sources, const int* d_mesh, const float* d_power, float* d_emission,
+                         float* d_absorption, void* stream);
+/* Starts a step: d_T_f = (float)d_T_n, d_emission = d_absorption = 0. */
+int ffk_step_begin_dev(int n, const double* d_T_n, float* d_T_f, float* d_emission, float* d_absorption,
+                       void* stream);
+/* ffk_adams_step on the step's float emission and absorption (each promoted
+ * to double, as the host form's caller does). */
+int ffk_adams_step_f(int n, int n_step, double dt, const double* d_C, const float* d_emission,
+                     const float* d_absorption, double* d_E_hist, const double* d_T_n, double* d_T_next,
+                     double* d_balance, void* stream);
+
 /* ---- distant source (sun) absorption ------------------------------------------
  * parallel_rays_cpu::calculate (thermal_equation/parallel_rays_cpu.cpp:92-131).
  * ffk_bsphere is math::triangles_bsphere (math/triangle.cpp:361-383, whose

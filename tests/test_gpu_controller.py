@@ -175,6 +175,33 @@ def test_patch_sphere_20k_sun_end_to_end(tmp_path, oracle):
     print(f"controller patch_sphere_20k: {out['seconds']:.3f} s for {n_steps} steps", flush=True)
 
 
+@pytest.mark.parametrize("obj,task,n_rays,n_steps", [
+    ("confetti_2k.obj", "confetti_2k.task", 200_000, 6),           # radiance + conduction
+    ("patch_sphere_2k.obj", "patch_sphere_2k_sun.task", 100_000, 6),  # distant source, emissivity 0
+    ("grid_10x10.obj", "grid_10x10_heat.task", 50_000, 8),         # heat source and sink
+    ("tgrid_10x10.obj", "tgrid_10x10_heat.task", 50_000, 8),       # sinks, materials from the .mtl
+])
+def test_device_loop_equals_host_arrays(obj, task, n_rays, n_steps):
+    """The Adams solution's device-resident step (the equations' emission and
+    absorption kept in device memory up to the FP64 update) against the same
+    run with every equation on host arrays, as the reference's solutions call
+    them: the same rays, the same float sums in the same order, so every
+    frame is identical bit for bit."""
+    from form_factors_amd import controller, formats
+
+    sc = formats.obj_import(os.path.join(MODELS, obj), inline_materials=True)
+    path = os.path.join(MODELS, task)
+    print(f"device loop vs host arrays: {obj} {task}", flush=True)
+    runs = [controller.run(None, path, n_rays=n_rays, n_steps=n_steps, seed=11, scene=sc, distant_rays=20_000,
+                           device_loop=dev) for dev in (True, False)]
+    Td, Th = (np.array([f[2] for f in r["frames"]]) for r in runs)
+    assert [f[0] for f in runs[0]["frames"]] == [f[0] for f in runs[1]["frames"]] == list(range(n_steps + 1))
+    assert np.isfinite(Td).all() and not np.array_equal(Td[0], Td[-1])
+    assert np.array_equal(Td.view(np.uint32), Th.view(np.uint32))
+    print(f"  {n_steps} steps: device loop {runs[0]['seconds']:.4f} s, host arrays {runs[1]['seconds']:.4f} s",
+          flush=True)
+
+
 @pytest.mark.parametrize("caster", ["cuda", "naive_cpu"])
 def test_command_line_run(tmp_path, caster):
     """python -m form_factors_amd.controller with the controller's positional
