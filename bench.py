@@ -474,17 +474,35 @@ def radiance_leg(ffk, scenes):
     out = {"workload": f"two 100x100-cell plates, {len(prob['faces'])} triangles, {n} meshes, T in [250, 350] K, "
                        "one radiance_equation step (weights, layout, powers, emit-cast-deposit pass)",
            "unit": "Mrays/s"}
+    stream = torch.cuda.current_stream()
     for n_rays, reps in ((1_000_000, 20), (100_000_000, 3)):
+        sc.set_walk(3)
         sc.radiance_step(eps, T, n_rays, 1)  # warm-up
         torch.cuda.synchronize()
+        # Walk mode (the same absorption bits in every mode, checked): one
+        # untimed pass of each at this ray count, the fastest kept, as the
+        # C2 leg does for the fused kernel.
+        w, total = ffk.radiance_weights(sc.faces, sc.meshes, eps, T)
+        front, rear = ffk.emitted_counts(w, n_rays, total)
+        p, _ = ffk.radiance_powers(sc.faces, sc.meshes, eps, T, front, rear)
+        walk_ms, ref = {}, None
+        for mode in range(8):
+            sc.set_walk(mode)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            got = sc.radiance(5, p)
+            e1.record(stream)
+            torch.cuda.synchronize()
+            walk_ms[mode] = e0.elapsed_time(e1)
+            ref = got if ref is None else ref
+            assert np.array_equal(got.view(np.uint64), ref.view(np.uint64)), f"radiance walk mode {mode} differs"
+        walk = min(walk_ms, key=walk_ms.get)
+        sc.set_walk(walk)
         t0 = time.perf_counter()
         for k in range(reps):
             em, ab = sc.radiance_step(eps, T, n_rays, 2 + k)
         step_s = (time.perf_counter() - t0) / reps
-        w, total = ffk.radiance_weights(sc.faces, sc.meshes, eps, T)
-        front, rear = ffk.emitted_counts(w, n_rays, total)
         cast = int(front.sum() + rear.sum())
-        p, _ = ffk.radiance_powers(sc.faces, sc.meshes, eps, T, front, rear)
         t0 = time.perf_counter()
         for k in range(reps):
             sc.radiance(2 + k, p)
@@ -496,7 +514,6 @@ def radiance_leg(ffk, scenes):
         sc.radiance(2, p, stats=stats)
         torch.cuda.synchronize()
         n_tests, n_cells = (int(x) for x in stats.cpu())
-        stream = torch.cuda.current_stream()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record(stream)
         for k in range(reps):
@@ -510,9 +527,11 @@ def radiance_leg(ffk, scenes):
         out[f"rays_{n_rays}"] = {"rays_cast": cast, "ms_per_step": round(step_s * 1e3, 3),
                                  "value": round(cast / step_s / 1e6, 2),
                                  "deposit_ms": round(dep_s * 1e3, 3), "deposit_value": round(cast / dep_s / 1e6, 2),
-                                 "absorbed_over_emitted": round(frac, 5), "roofline": roof}
+                                 "absorbed_over_emitted": round(frac, 5), "roofline": roof, "walk_mode": walk,
+                                 "walk_probe_ms": {str(k): round(v, 3) for k, v in walk_ms.items()}}
         log(f"radiance {n_rays} rays: {step_s * 1e3:.3f} ms/step ({cast / step_s / 1e6:.1f} Mrays/s), "
-            f"deposit pass {dep_s * 1e3:.3f} ms ({cast / dep_s / 1e6:.1f} Mrays/s), absorbed/emitted {frac:.4f}")
+            f"deposit pass {dep_s * 1e3:.3f} ms ({cast / dep_s / 1e6:.1f} Mrays/s), absorbed/emitted {frac:.4f}, "
+            f"walk mode {walk} (" + " / ".join(f"{walk_ms[k]:.3f}" for k in sorted(walk_ms)) + " ms)")
     sc.close()
     return out
 
