@@ -358,7 +358,7 @@ struct ffk_scene {
   size_t rad_scan_bytes = 0;
   void* d_rad_block = nullptr;
   void* h_rad_pin = nullptr;
-  hipEvent_t rad_te_ev = nullptr;  // the last upload from the pinned T / eps staging
+  hipEvent_t rad_pin_ev = nullptr;  // the last transfer through h_rad_pin (T / eps up, status down)
   int* d_walk_face = nullptr;               // W: face of each walk entry
   int* d_walk_mesh = nullptr;               // W: mesh of each walk entry
   int* d_mesh_walk = nullptr;               // n_meshes + 1: first walk entry of each mesh

@@ -541,6 +541,7 @@ int rad_prepare(ffk_scene* s, hipStream_t st) {
 }
 
 void free_radiance(ffk_scene* s) {
+  if (s->rad_pin_ev) (void)hipEventSynchronize(s->rad_pin_ev);  // no transfer through the pinned block in flight
   dfree(s->d_rad_block);
   if (s->h_rad_pin) (void)hipHostFree(s->h_rad_pin);
   s->d_rad_block = nullptr;
@@ -639,6 +640,8 @@ int rad_enqueue(ffk_scene* s, unsigned long long seed, unsigned long long* d_sta
                                                                            s->d_rad_info, s->d_rad_abs);
   FFK_HIP(hipGetLastError());
   FFK_HIP(hipMemcpyAsync(info, s->d_rad_info, 4 * kInfoWords, hipMemcpyDeviceToHost, st));
+  if (!s->rad_pin_ev) FFK_HIP(hipEventCreateWithFlags(&s->rad_pin_ev, hipEventDisableTiming));
+  FFK_HIP(hipEventRecord(s->rad_pin_ev, st));
   return FFK_OK;
 }
 
@@ -759,14 +762,15 @@ int step_prepare(ffk_scene* s, const float* h_eps2, const float* h_T, int n_rays
   }
   if ((r = rad_prepare(s, st)) < 0) return r;
   // T and eps to the device, where the ray powers are computed (their bits
-  // are powers_impl's)
-  if (!s->rad_te_ev) FFK_HIP(hipEventCreateWithFlags(&s->rad_te_ev, hipEventDisableTiming));
-  FFK_HIP(hipEventSynchronize(s->rad_te_ev));
+  // are powers_impl's); the staging is rewritten only once the last transfer
+  // through it (a device step's status copy, at the latest) has run
+  if (!s->rad_pin_ev) FFK_HIP(hipEventCreateWithFlags(&s->rad_pin_ev, hipEventDisableTiming));
+  FFK_HIP(hipEventSynchronize(s->rad_pin_ev));
   float* te = pin_te(s);
   std::memcpy(te, h_T, 4 * (size_t)nm);
   std::memcpy(te + nm, h_eps2, 8 * (size_t)nm);
   FFK_HIP(hipMemcpyAsync(s->d_rad_te, te, 12 * (size_t)nm, hipMemcpyHostToDevice, st));
-  FFK_HIP(hipEventRecord(s->rad_te_ev, st));
+  FFK_HIP(hipEventRecord(s->rad_pin_ev, st));
   if (s->rad_W > 0)
     k_rad_powers<<<(s->rad_W + kRadThreads - 1) / kRadThreads, kRadThreads, 0, st>>>(
         s->rad_W, s->d_walk_face, s->d_walk_mesh, s->d_rad_te, nm, s->d_rad_area, s->d_face_off, s->d_front,

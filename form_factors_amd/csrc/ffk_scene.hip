@@ -374,7 +374,7 @@ void ffk_scene_destroy(ffk_scene* s) {
   dfree(s->d_misc);
   free_radiance(s);
   if (s->h_layout_pin) (void)hipHostFree(s->h_layout_pin);
-  if (s->rad_te_ev) (void)hipEventDestroy(s->rad_te_ev);
+  if (s->rad_pin_ev) (void)hipEventDestroy(s->rad_pin_ev);
   delete s;
 }
 
