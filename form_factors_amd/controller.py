@@ -49,7 +49,8 @@ def run(scene_path, task_path, n_rays=1_000_000, n_steps=100, result_path=None, 
     device_loop: each step keeps the equations' emission and absorption in
     device memory up to the FP64 update (the grid caster; the brute-force
     caster always runs the equations on host arrays); False runs every step
-    on host arrays, with the same temperatures bit for bit."""
+    on host arrays, with the same temperatures bit for bit; a sequence of
+    booleans sets it step by step."""
     L = P.lib()
     formats.lib()
     caster = L.ffp_ray_caster_create(int(caster_type))
@@ -89,7 +90,8 @@ def run(scene_path, task_path, n_rays=1_000_000, n_steps=100, result_path=None, 
         if not sol:
             raise RuntimeError("failed to create the thermal solution")
         systems.append(("sol", sol))
-        if L.ffp_thermal_solution_set_device_loop(sol, 1 if device_loop else 0) < 0:
+        per_step = not isinstance(device_loop, (bool, int))
+        if not per_step and L.ffp_thermal_solution_set_device_loop(sol, 1 if device_loop else 0) < 0:
             raise RuntimeError("thermal solution: device loop setting refused")
         frames = [(task.n_step, task.time_delta, np.frombuffer(Tc, np.float32).copy())]
         if frame_sink is not None:
@@ -99,6 +101,8 @@ def run(scene_path, task_path, n_rays=1_000_000, n_steps=100, result_path=None, 
         if r < 0:
             raise RuntimeError(f"thermal solution set_scene failed ({r})")
         for step in range(n_steps):
+            if per_step and L.ffp_thermal_solution_set_device_loop(sol, 1 if device_loop[step] else 0) < 0:
+                raise RuntimeError("thermal solution: device loop setting refused")
             r = L.ffp_thermal_solution_calculate(sol, C.byref(task))
             if r < 0:
                 raise RuntimeError(f"thermal solution step {step} failed ({r})")

@@ -1352,10 +1352,14 @@ int prelaunch_dev(system* sys, const dev_task& t) {
 }
 
 // The second half, in the equations' order: the counts' powers added to the
-// step's absorption on the solution's stream.
+// step's absorption on the solution's stream (counted here first when the
+// step did not launch them ahead).
 int calculate_dev(system* sys, const dev_task& t) {
   state* s = sys->st;
-  if (!s->counting) return THERMAL_EQUATION_OK;
+  if (!s->counting) {
+    const int r = prelaunch_dev(sys, t);
+    if (r < 0 || !s->counting) return r;
+  }
   s->counting = false;
   if (hipStreamWaitEvent(t.st, s->ev_counted, 0) != hipSuccess) return -THERMAL_EQUATION_ERROR;
   return ffk_sun_deposit_dev(s->scene->n_meshes, s->d_counts.p, s->d_absorb2.p, s->counted_power, t.d_ab, t.st) < 0

@@ -202,6 +202,25 @@ def test_device_loop_equals_host_arrays(obj, task, n_rays, n_steps):
           flush=True)
 
 
+def test_device_loop_switched_mid_run():
+    """Switching between the device-resident step and host arrays from one
+    step to the next (the solution's FP64 state, the Adams history and the
+    emitters' ray keys carry over either way): the frames are those of a
+    run that never switched. Radiance, conduction and the distant source."""
+    from form_factors_amd import controller, formats
+
+    sc = formats.obj_import(os.path.join(MODELS, "patch_sphere_2k.obj"), inline_materials=True)
+    sc["materials"][:, 8] = 0.9  # front emissivity: radiance emits as well as the sun heating
+    path = os.path.join(MODELS, "patch_sphere_2k_sun.task")
+    plan = [True, False, False, True, False, True, True, False]
+    print("device loop switched step by step: patch_sphere_2k + sun", flush=True)
+    runs = [controller.run(None, path, n_rays=100_000, n_steps=len(plan), seed=3, scene=sc, distant_rays=20_000,
+                           device_loop=dl) for dl in (True, plan)]
+    Ta, Tb = (np.array([f[2] for f in r["frames"]]) for r in runs)
+    assert not np.array_equal(Ta[0], Ta[-1])
+    assert np.array_equal(Ta.view(np.uint32), Tb.view(np.uint32))
+
+
 @pytest.mark.parametrize("caster", ["cuda", "naive_cpu"])
 def test_command_line_run(tmp_path, caster):
     """python -m form_factors_amd.controller with the controller's positional
