@@ -232,6 +232,26 @@ __global__ void k_heat_sources(int n_sources, const int* __restrict__ mesh, cons
   }
 }
 
+// SB_FF from the step's float temperatures: T promoted to double, the double
+// emission and absorption zeroed (what the host form uploads).
+__global__ void k_sbff_begin(int n, const float* __restrict__ T_f, double* __restrict__ T, double* __restrict__ em,
+                             double* __restrict__ ab) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  T[m] = (double)T_f[m];
+  em[m] = 0.0;
+  ab[m] = 0.0;
+}
+
+// Double sums rounded to float and added to the step's float arrays.
+__global__ void k_add_rounded(int n, const double* __restrict__ em, const double* __restrict__ ab,
+                              float* __restrict__ em_f, float* __restrict__ ab_f) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  em_f[m] += (float)em[m];
+  ab_f[m] += (float)ab[m];
+}
+
 __global__ void k_step_begin(int n, const double* __restrict__ T_n, float* __restrict__ T_f,
                              float* __restrict__ emission, float* __restrict__ absorption) {
   int m = blockIdx.x * blockDim.x + threadIdx.x;
@@ -696,6 +716,24 @@ int ffk_sbff(int n, const void* d_S, int dtype, const double* d_row_scale, const
   if (dtype == FFK_BF16) launch_rows<FFK_BF16>(n, d_S, E32, d_absorption, st);
   else if (dtype == FFK_F16) launch_rows<FFK_F16>(n, d_S, E32, d_absorption, st);
   else launch_rows<FFK_F32>(n, d_S, E32, d_absorption, st);
+  FFK_HIP(hipGetLastError());
+  return FFK_OK;
+}
+
+int ffk_sbff_dev(int n, const void* d_S, int dtype, const double* d_row_scale, const float* d_T,
+                 const double* d_area, const double* d_eps_front, const double* d_eps_rear, double* d_T64,
+                 double* d_em64, double* d_ab64, float* d_work, float* d_emission, float* d_absorption,
+                 void* stream) {
+  FFK_REQUIRE(n > 0 && d_T && d_T64 && d_em64 && d_ab64 && d_emission && d_absorption, FFK_ERR_ARG,
+              "ffk_sbff_dev: bad args");
+  hipStream_t st = as_stream(stream);
+  k_sbff_begin<<<(n + 255) / 256, 256, 0, st>>>(n, d_T, d_T64, d_em64, d_ab64);
+  FFK_HIP(hipGetLastError());
+  int r;
+  if ((r = ffk_sbff(n, d_S, dtype, d_row_scale, d_T64, d_area, d_eps_front, d_eps_rear, d_em64, d_ab64, d_work,
+                    stream)) < 0)
+    return r;
+  k_add_rounded<<<(n + 255) / 256, 256, 0, st>>>(n, d_em64, d_ab64, d_emission, d_absorption);
   FFK_HIP(hipGetLastError());
   return FFK_OK;
 }

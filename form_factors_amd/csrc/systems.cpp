@@ -719,6 +719,20 @@ int calculate(system* sys, thermal_equation::task_t* task) {
   return THERMAL_EQUATION_OK;
 }
 
+bool can_dev(system* sys) { return sys->st && sys->st->scene && sys->st->scene->n_meshes > 0; }
+
+// calculate with the step's arrays on the device: the same double exchange
+// from the float temperatures, its sums rounded and added.
+int calculate_dev(system* sys, const dev_task& t) {
+  state* s = sys->st;
+  const int N = s->scene->n_meshes;
+  double* d = s->dvec.p;
+  return ffk_sbff_dev(N, s->dS.p, s->params.f_dtype, d + 6 * N, t.d_T, d + N, d + 2 * N, d + 3 * N, d, d + 4 * N,
+                      d + 5 * N, reinterpret_cast<float*>(d + 7 * N), t.d_em, t.d_ab, t.st) < 0
+             ? -THERMAL_EQUATION_ERROR
+             : THERMAL_EQUATION_OK;
+}
+
 const thermal_equation::system_methods_t methods = {
     (int (*)(thermal_equation::system_t*, void*))&init,
     (int (*)(thermal_equation::system_t*))&shutdown,
@@ -1390,6 +1404,7 @@ namespace dev_equations {
 
 bool can(thermal_equation::system_t* e) {
   if (!e) return false;
+  if (e->methods == &hip_sb_ff::methods) return hip_sb_ff::can_dev((hip_sb_ff::system*)e);
   if (e->methods == &hip_radiance::methods) return hip_radiance::can_dev((hip_radiance::system*)e);
   if (e->methods == &hip_conduction::methods) return hip_conduction::can_dev((hip_conduction::system*)e);
   if (e->methods == &hip_heat_source::methods) return hip_heat_source::can_dev((hip_heat_source::system*)e);
@@ -1399,6 +1414,7 @@ bool can(thermal_equation::system_t* e) {
 }
 
 int run(thermal_equation::system_t* e, const dev_task& t) {
+  if (e->methods == &hip_sb_ff::methods) return hip_sb_ff::calculate_dev((hip_sb_ff::system*)e, t);
   if (e->methods == &hip_radiance::methods) return hip_radiance::calculate_dev((hip_radiance::system*)e, t);
   if (e->methods == &hip_conduction::methods) return hip_conduction::calculate_dev((hip_conduction::system*)e, t);
   if (e->methods == &hip_heat_source::methods)

@@ -95,6 +95,8 @@ SIGNATURES = {
     "ffk_sun_count_dev": (C.c_int, [_vp, _f, _f, C.c_float, C.c_float, C.c_int, _ll, _ull, _vp, _vp]),
     "ffk_sun_deposit_dev": (C.c_int, [C.c_int, _vp, _vp, C.c_float, _vp, _vp]),
     "ffk_conduction_dev": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "ffk_sbff_dev": (C.c_int, [C.c_int, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                               _vp]),
     "ffk_heat_sources_dev": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "ffk_step_begin_dev": (C.c_int, [C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "ffk_conduction_graph": (C.c_int, [_f, C.c_int, _i, C.c_int, _f, _f, _i, _i, C.POINTER(C.c_double), C.c_int,

@@ -120,9 +120,9 @@ void* ffp_thermal_solution_create(int type, int n_equations, void** equations);
 int ffp_thermal_solution_set_scene(void* system, ffp_subject_scene* scene, float* temperatures);
 int ffp_thermal_solution_calculate(void* system, ffp_ts_task* task);
 /* No counterpart in the reference: where every equation of the solution is one
- * of this library's radiance (HIP Malley emitter over the HIP grid caster),
- * conduction, heat-source or distant-source (HIP parallel-rays emitter over
- * the grid caster) systems, a step keeps its emission and absorption in device
+ * of this library's SB_FF, radiance (HIP Malley emitter over the HIP grid
+ * caster), conduction, heat-source or distant-source (HIP parallel-rays emitter
+ * over the grid caster) systems, a step keeps its emission and absorption in device
  * memory from the equations to the FP64 update and copies only the new
  * temperatures back (on = 1, the default); on = 0 runs the equations on host
  * arrays, as the reference's solutions do. Both give the same temperatures

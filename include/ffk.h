@@ -380,6 +380,14 @@ int ffk_sun_deposit_dev(int n_meshes, const unsigned long long* d_counts, const 
  * (from zero) rounded to float and added to d_emission / d_absorption. */
 int ffk_conduction_dev(int n, const int* d_row_ptr, const int* d_cols, const double* d_G, const float* d_T,
                        float* d_emission, float* d_absorption, void* stream);
+/* ffk_sbff from the step's float temperatures d_T: promoted into d_T64 (n
+ * doubles of caller scratch), the exchange into d_em64 / d_ab64 (n doubles
+ * each, zeroed here), and those double sums rounded to float and added to
+ * d_emission / d_absorption. */
+int ffk_sbff_dev(int n, const void* d_S, int dtype, const double* d_row_scale, const float* d_T,
+                 const double* d_area, const double* d_eps_front, const double* d_eps_rear, double* d_T64,
+                 double* d_em64, double* d_ab64, float* d_work, float* d_emission, float* d_absorption,
+                 void* stream);
 /* Heat sources in list order: power > 0 is added to d_absorption[mesh], else
  * -power to d_emission[mesh] (d_mesh, d_power: device arrays of n_sources). */
 int ffk_heat_sources_dev(int n_sources, const int* d_mesh, const float* d_power, float* d_emission,

@@ -188,3 +188,58 @@ def test_radiance_nested_boxes_preserve_energy(P, caster):
     print(f"nested boxes ({caster}): T {frames[-1]}, energy ratio {e / e0:.6f}", flush=True)
     assert frames[-1][1] < 300.0 and frames[-1][0] > 20.0  # heat moved from the inner box to the outer one
     assert abs(e / e0 - 1.0) < 1e-3
+
+
+@pytest.mark.parametrize("dtype", [0, 1, 2])
+def test_sbff_conduction_sources_device_equals_host(P, dtype):
+    """A C5-shaped solution through the plugin ABI: SB_FF exchange (F in
+    FP32, BF16 or FP16 on the device) with conduction and a heat source and
+    sink, on two facing 4 x 4-cell plates (aluminium shell, emissivity 0.8),
+    30 steps of 1 s. The device-resident step equals the host-array step bit
+    for bit; both plates radiate to open space, the hot one faster, and the
+    source and the sink warm and cool their own meshes."""
+    from form_factors_amd import scenes
+
+    L = P.lib()
+    faces, meshes = scenes.two_plates_thermal(cells=4, gap=0.5)
+    n = len(meshes)
+    mat = material_al(P, 0.001)
+    mat.front[4] = mat.rear[4] = 0.8
+    sm = meshes.copy()
+    sm[:, 2] = 0
+    hs = (P.HeatSource * 2)(P.HeatSource(0, 50.0), P.HeatSource(n - 1, -50.0))
+
+    def run(dev):
+        rc = L.ffp_ray_caster_create(P.RAY_CASTER_GRID_HIP)
+        em = L.ffp_emission_create(P.EMISSION_MALLEY_HIP, rc)
+        L.ffp_emission_set_seed(em, 5, 0)
+        ff = L.ffp_form_factors_create(P.FORM_FACTORS_HIP, em)
+        eqs = [L.ffp_thermal_equation_create(P.THERMAL_EQUATION_SB_FF_HIP, ff, 200_000, dtype),
+               L.ffp_thermal_equation_create(P.THERMAL_EQUATION_CONDUCTIVE_HIP, None, 0, 0),
+               L.ffp_heat_source_create(2, C.cast(hs, C.c_void_p))]
+        assert all(eqs)
+        arr = (C.c_void_p * 3)(*eqs)
+        sol = L.ffp_thermal_solution_create(P.THERMAL_SOLUTION_HIP_ADAMS, 3, arr)
+        assert sol and L.ffp_thermal_solution_set_device_loop(sol, int(dev)) == 0
+        sc, keep = subject_scene(P, faces, sm, [mat])
+        T = (C.c_float * n)(*([400.0] * (n // 2) + [300.0] * (n // 2)))
+        assert L.ffp_thermal_solution_set_scene(sol, C.byref(sc), T) == 0
+        task = P.TsTask(0, 1.0, C.cast(T, C.POINTER(C.c_float)))
+        frames = []
+        for _ in range(30):
+            assert L.ffp_thermal_solution_calculate(sol, C.byref(task)) == 0
+            frames.append(np.frombuffer(T, np.float32).copy())
+        L.ffp_thermal_solution_free(sol)
+        for e in eqs:
+            L.ffp_thermal_equation_free(e)
+        L.ffp_form_factors_free(ff)
+        L.ffp_emission_free(em)
+        L.ffp_ray_caster_free(rc)
+        return frames
+
+    frames = both(run)
+    T = frames[-1]
+    print(f"SB_FF + conduction + sources (dtype {dtype}): hot plate {T[:n // 2].mean():.3f} K, "
+          f"cold plate {T[n // 2:].mean():.3f} K, source mesh {T[0]:.3f} K, sink mesh {T[-1]:.3f} K", flush=True)
+    assert 400.0 - T[1:n // 2].mean() > 300.0 - T[n // 2:-1].mean() > 0
+    assert T[0] > T[1:n // 2].mean() and T[-1] < T[n // 2:-1].mean()
