@@ -72,3 +72,34 @@ def test_errors_are_codes_not_aborts(lib):
     assert b"NULL" in L.ffk_last_error()
     # host-only argument checks that never reach the device
     assert L.ffk_emitted_counts(None, 1, 1, 1.0, None, None) == -1
+
+
+def test_device_step_entry_points_check_their_arguments(lib):
+    """The device-resident step's entry points refuse missing arrays and bad
+    sizes with FFK_ERR_ARG before any device call (host checks only)."""
+    from form_factors_amd import ffk
+
+    L = ffk.lib()
+    p = C.c_void_p(0x1000)  # never dereferenced: each call fails on another argument first
+    f3 = (C.c_float * 3)(0, 0, 1)
+    calls = [
+        ("ffk_radiance_step_dev", L.ffk_radiance_step_dev(None, f3, f3, 10, 0, 1, p, p, None)),
+        ("ffk_radiance_status", L.ffk_radiance_status(None)),
+        ("ffk_sun_count_dev", L.ffk_sun_count_dev(None, f3, f3, 1.0, 1.0, 0, 10, 1, p, None)),
+        ("ffk_sun_deposit_dev", L.ffk_sun_deposit_dev(4, None, p, 1.0, p, None)),
+        ("ffk_conduction_dev", L.ffk_conduction_dev(0, p, p, p, p, p, p, None)),
+        ("ffk_sbff_dev", L.ffk_sbff_dev(4, p, 0, p, None, p, p, p, p, p, p, p, p, p, None)),
+        ("ffk_heat_sources_dev", L.ffk_heat_sources_dev(-1, p, p, p, p, None)),
+        ("ffk_step_begin_dev", L.ffk_step_begin_dev(0, p, p, p, p, None)),
+        ("ffk_adams_step_f", L.ffk_adams_step_f(4, 0, 0.1, p, None, p, p, p, p, None, None)),
+    ]
+    for name, r in calls:
+        assert r == -1, name  # FFK_ERR_ARG
+        assert L.ffk_last_error(), name
+
+
+def test_solution_device_loop_setter_checks_its_system(lib):
+    from form_factors_amd import plugin
+
+    L = plugin.lib()
+    assert L.ffp_thermal_solution_set_device_loop(None, 1) < 0
