@@ -380,6 +380,7 @@ struct ffk_scene {
   // layout upload staging (pinned): face offsets, front counts, mesh offsets
   void* h_layout_pin = nullptr;
   size_t layout_pin_bytes = 0;
+  hipEvent_t layout_pin_ev = nullptr;  // the last upload from h_layout_pin
   // host copies
   std::vector<float> h_faces;
   std::vector<int> h_meshes, h_front, h_rear;
@@ -405,6 +406,8 @@ namespace ffk {
 // Releases the radiance work buffers (ffk_radiance.hip); they are laid out
 // again for the current faces and meshes on the next radiance pass.
 void free_radiance(ffk_scene* s);
+int set_emission(ffk_scene* s, const int* h_front, const int* h_rear, int frame, long long* n_rays_out,
+                 hipStream_t st, bool one_stream);
 int build_tiles(ffk_scene* s, int rays_per_tile, hipStream_t st);
 int ensure_tiles(ffk_scene* s, hipStream_t st);
 int device_cu_count();

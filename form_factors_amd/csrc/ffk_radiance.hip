@@ -757,8 +757,7 @@ int step_prepare(ffk_scene* s, const float* h_eps2, const float* h_T, int n_rays
   if ((r = ffk_emitted_counts(s->rad_w.data(), nf, n_rays, total, s->rad_front.data(), s->rad_rear.data())) < 0)
     return r;
   if (!s->has_emission || frame != s->frame || s->rad_front != s->h_front || s->rad_rear != s->h_rear) {
-    if ((r = ffk_scene_set_emission(s, s->rad_front.data(), s->rad_rear.data(), frame, nullptr, st)) < 0)
-      return r;
+    if ((r = set_emission(s, s->rad_front.data(), s->rad_rear.data(), frame, nullptr, st, true)) < 0) return r;
   }
   if ((r = rad_prepare(s, st)) < 0) return r;
   // T and eps to the device, where the ray powers are computed (their bits

@@ -375,6 +375,7 @@ void ffk_scene_destroy(ffk_scene* s) {
   free_radiance(s);
   if (s->h_layout_pin) (void)hipHostFree(s->h_layout_pin);
   if (s->rad_pin_ev) (void)hipEventDestroy(s->rad_pin_ev);
+  if (s->layout_pin_ev) (void)hipEventDestroy(s->layout_pin_ev);
   delete s;
 }
 
@@ -440,16 +441,22 @@ int ffk_scene_set_meshes(ffk_scene* s, const int* h_meshes, int n_meshes, void* 
   return FFK_OK;
 }
 
-int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, int frame,
-                           long long* n_rays_out, void* stream) {
+}  // extern "C"
+
+namespace ffk {
+// The emission layout of ffk_scene_set_emission. one_stream: the caller
+// orders every use of the scene on this stream (the radiance step does), so
+// stream order replaces the two waits; the pinned staging is then reused only
+// once its last upload has run.
+int set_emission(ffk_scene* s, const int* h_front, const int* h_rear, int frame, long long* n_rays_out,
+                 hipStream_t st, bool one_stream) {
   FFK_REQUIRE(s && s->has_faces && s->has_meshes, FFK_ERR_ARG,
               "ffk_scene_set_emission: faces and meshes must be set first");
   FFK_REQUIRE(frame == FFK_FRAME_REFERENCE || frame == FFK_FRAME_ORTHONORMAL, FFK_ERR_ARG,
               "unknown frame %d", frame);
-  hipStream_t st = as_stream(stream);
   // The layout buffers are reused (their sizes follow the faces and meshes,
   // whose setters free them): wait for the work that may still read them.
-  FFK_HIP(hipStreamSynchronize(st));
+  if (!one_stream) FFK_HIP(hipStreamSynchronize(st));
   s->has_emission = false;
   ++s->generation;  // ray counts, offsets and tiles change in place
   const int nf = s->n_faces, nm = s->n_meshes;
@@ -481,9 +488,10 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
   if (!s->d_face_off && (r = dalloc(&s->d_face_off, (size_t)nf + 1)) < 0) return r;
   if (!s->d_front && (r = dalloc(&s->d_front, (size_t)nf)) < 0) return r;
   if (!s->d_mesh_off && (r = dalloc(&s->d_mesh_off, (size_t)nm + 1)) < 0) return r;
-  // Uploads through pinned staging (the stream was drained above, so the
-  // staging is free to be rewritten).
+  // Uploads through pinned staging, rewritten once its last upload has run.
   const size_t b_face = 8 * ((size_t)nf + 1), b_front = 4 * (size_t)nf, b_mesh = 8 * ((size_t)nm + 1);
+  if (!s->layout_pin_ev) FFK_HIP(hipEventCreateWithFlags(&s->layout_pin_ev, hipEventDisableTiming));
+  FFK_HIP(hipEventSynchronize(s->layout_pin_ev));
   if (s->layout_pin_bytes < b_face + b_front + b_mesh) {
     if (s->h_layout_pin) (void)hipHostFree(s->h_layout_pin);
     s->h_layout_pin = nullptr;
@@ -498,8 +506,9 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
   FFK_HIP(hipMemcpyAsync(s->d_face_off, pin, b_face, hipMemcpyHostToDevice, st));
   FFK_HIP(hipMemcpyAsync(s->d_front, pin + b_face, b_front, hipMemcpyHostToDevice, st));
   FFK_HIP(hipMemcpyAsync(s->d_mesh_off, pin + b_face + b_front, b_mesh, hipMemcpyHostToDevice, st));
+  FFK_HIP(hipEventRecord(s->layout_pin_ev, st));
   // one wait for the three copies: callers may go on with another stream
-  FFK_HIP(hipStreamSynchronize(st));
+  if (!one_stream) FFK_HIP(hipStreamSynchronize(st));
   if (s->rot_frame != frame) {  // the rotations depend on the faces and the frame only
     k_face_rotation<<<(nf + 255) / 256, 256, 0, st>>>(s->d_faces, nf, frame, s->d_rot);
     FFK_HIP(hipGetLastError());
@@ -509,6 +518,14 @@ int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, 
   s->tiles_dirty = true;  // the fused kernel's tile list is built when a fused launch needs it
   if (n_rays_out) *n_rays_out = total;
   return FFK_OK;
+}
+}  // namespace ffk
+
+extern "C" {
+
+int ffk_scene_set_emission(ffk_scene* s, const int* h_front, const int* h_rear, int frame,
+                           long long* n_rays_out, void* stream) {
+  return set_emission(s, h_front, h_rear, frame, n_rays_out, as_stream(stream), false);
 }
 
 int ffk_scene_layout(const ffk_scene* s, float* h_faces, int* h_meshes, int* h_front, int* h_rear, int* frame) {
