@@ -289,15 +289,44 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
           add(d);
         }
         float best_g = 3.0e38f;
+        auto time_grid = [&](const int* dims, float* ms) {
+          int e = ffk_scene_build_grid(s, dims[0], dims[1], dims[2], stream);
+          if (e == FFK_OK)
+            e = time_ms(st, 2, [&] { return ffk_ff_fused(s, FFK_CAST_GRID, 1, d_tally, d_F, nullptr, stream); }, ms);
+          return e;
+        };
         for (size_t k = 0; k < cand.size() && r == FFK_OK; ++k) {
-          const int* dims = cand[k].data();
-          if ((r = ffk_scene_build_grid(s, dims[0], dims[1], dims[2], stream)) < 0) break;
           float ms = 0.f;
-          r = time_ms(st, 2, [&] { return ffk_ff_fused(s, FFK_CAST_GRID, 1, d_tally, d_F, nullptr, stream); }, &ms);
+          r = time_grid(cand[k].data(), &ms);
           if (r == FFK_OK && ms < best_g) {
             best_g = ms;
-            std::copy(dims, dims + 3, t.dims);
+            std::copy(cand[k].data(), cand[k].data() + 3, t.dims);
           }
+        }
+        // The fused kernel often gains from finer grids than the sample cast
+        // ranks first (empty cells are cheap to cross, smaller cells mean
+        // fewer tests): while the finest resolution tried is the fastest,
+        // step a factor sqrt(2) finer, keeping a step only when it wins by
+        // 1 %, up to 2^22 cells and at most 8 steps (an axis capped at 512
+        // cells can repeat a resolution).
+        double cpf = trials[0].cpf * std::sqrt(2.0);
+        for (int step = 0; step < 8 && r == FFK_OK; ++step) {
+          const long long cells_best = (long long)t.dims[0] * t.dims[1] * t.dims[2];
+          bool finest = true;
+          for (const auto& c : cand) finest = finest && (long long)c[0] * c[1] * c[2] <= cells_best;
+          if (!finest) break;
+          cpf *= std::sqrt(2.0);
+          int d[3];
+          dims_for(lo, hi, cpf, nf, d);
+          if ((long long)d[0] * d[1] * d[2] > (1LL << 22)) break;
+          std::array<int, 3> a = {d[0], d[1], d[2]};
+          if (std::find(cand.begin(), cand.end(), a) != cand.end()) continue;
+          cand.push_back(a);
+          float ms = 0.f;
+          if ((r = time_grid(d, &ms)) < 0) break;
+          if (!(ms < 0.99f * best_g)) break;
+          best_g = ms;
+          std::copy(d, d + 3, t.dims);
         }
         if (r == FFK_OK) r = ffk_scene_build_grid(s, t.dims[0], t.dims[1], t.dims[2], stream);
         float best_b = 3.0e38f;
