@@ -94,6 +94,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
   // The faces and the walk entries of the block's first and last rays bound
   // their lookups.
   __shared__ int s_faces[2], s_walk[2];
+  // Each wavefront's hits waiting for their deposit: ray index in the block
+  // and face, fewer than 64 before an iteration and at most 64 added by one.
+  __shared__ int s_hit_ray[4][2 * 64], s_hit_face[4][2 * 64];
   const long long r0 = (long long)blockIdx.x * chunk;
   const long long cnt = min(chunk, n_rays - r0);
   if (threadIdx.x < 4) {  // the four searches side by side on lanes 0-3
@@ -106,8 +109,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
   __syncthreads();
   uint32_t nt = 0, nc = 0;
   int hint = s_faces[0], whint = s_walk[0];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  int* const hit_ray = s_hit_ray[wave];
+  int* const hit_face = s_hit_face[wave];
+  int n_queued = 0;              // wave-uniform: changed only where every lane runs
+  int pend_ray = 0, pend_face = -1;  // this lane's hit of this iteration, not yet queued
   pooled_walk<STATS, WALK>(
-      S, best + kPoolWords * (threadIdx.x >> 6), &s_next, cnt, refill, pool,
+      S, best + kPoolWords * wave, &s_next, cnt, refill, pool,
       [&](long long i, v3& o, v3& dir) {
         const long long g = r0 + i;
         const int f = find_face_in(S.face_off, s_faces, g, hint);
@@ -116,20 +124,46 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
         malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
         dir = e - o;
       },
+      // A ray's deposit needs a chain of dependent loads (its mesh, walk
+      // entry, the entry's face and power); queued, 64 of them run side by
+      // side at the end of an iteration instead of holding up the walk. The
+      // sums are integers, so their order does not matter. A lane ends at
+      // most one ray per iteration.
       [&](long long i, v3, v3, unsigned long long key) {
         float d2;
-        int hf;
-        decode_key(key, d2, hf);
-        if (hf < 0) return;
-        const int m = gld(S.face2mesh + hf);
-        if (m < 0) return;
-        const long long g = r0 + i;
-        const int k = find_face_in(walk_start, s_walk, g, whint);
-        whint = k;
-        const int wf = walk_face[k];
-        deposit(acc, S.n_meshes, m, q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)]);
+        decode_key(key, d2, pend_face);
+        pend_ray = (int)i;
       },
-      nt, nc);
+      nt, nc,
+      [&](bool last) {
+        const unsigned long long hits = __ballot(pend_face >= 0);
+        if (pend_face >= 0) {
+          const int at = n_queued + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(hits >> 32),
+                                                                   __builtin_amdgcn_mbcnt_lo((uint32_t)hits, 0u));
+          hit_ray[at] = pend_ray;
+          hit_face[at] = pend_face;
+          pend_face = -1;
+        }
+        n_queued += __popcll(hits);
+        if (n_queued < 64 && !(last && n_queued > 0)) return;
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the queue as every lane wrote it
+        while (n_queued >= 64 || (last && n_queued > 0)) {
+          const int take = min(n_queued, 64);
+          n_queued -= take;
+          if (lane < take) {
+            const int i = hit_ray[n_queued + lane], hf = hit_face[n_queued + lane];
+            const int m = gld(S.face2mesh + hf);
+            if (m >= 0) {
+              const long long g = r0 + i;
+              const int k = find_face_in(walk_start, s_walk, g, whint);
+              whint = k;
+              const int wf = walk_face[k];
+              deposit(acc, S.n_meshes, m, q[2 * k + ((g - walk_start[k]) < gld(S.front + wf) ? 0 : 1)]);
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // read before the next writes
+      });
   if (STATS) {
     unsigned long long t = nt, c = nc;
 #pragma unroll

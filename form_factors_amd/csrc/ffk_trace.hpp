@@ -523,10 +523,18 @@ __device__ inline int wave_incl_max(int x) {
 //          the rest for the next round (their best hit so far stays in LDS)
 //   start(i, o, dir):            sets up ray i
 //   finish(i, o, dir, key):      ray i is done; key = hit_key of its hit or kNoHit
+//                                (called by the lanes whose rays end)
+//   after(last):                 called by every lane of the wavefront together
+//                                at the end of each iteration (last = false)
+//                                and once after the last one (last = true)
 // Every lane of the wavefront must call this together.
-template <bool STATS, int WALK, class Start, class Finish>
+struct walk_no_after {
+  __device__ void operator()(bool) const {}
+};
+template <bool STATS, int WALK, class Start, class Finish, class After = walk_no_after>
 __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best, int* next, long long n, int refill,
-                                   int pool, Start start, Finish finish, uint32_t& n_tests, uint32_t& n_cells) {
+                                   int pool, Start start, Finish finish, uint32_t& n_tests, uint32_t& n_cells,
+                                   After after = After()) {
   const int lane = threadIdx.x & 63;
   const bool whole = (pool & kPoolWhole) != 0, staged = (pool & kPoolStaged) != 0;
   pool &= 0xff;
@@ -649,7 +657,9 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
         have = false;
       }
     }
+    after(false);
   }
+  after(true);
 }
 
 // One ray, walked to its end in a single loop (the same steps as
