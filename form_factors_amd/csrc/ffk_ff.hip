@@ -153,6 +153,8 @@ constexpr int kDefaultRefill = 32;
 // Pooled triangle tests run once waiting lanes x kDefaultPool >= active
 // lanes; 0 = each lane tests its own cells.
 constexpr int kDefaultPool = 2;
+// Hit faces a pooled wavefront queues before counting them (two rounds of 64).
+constexpr int kHitQueue = 128;
 
 struct fused_args {
   const scene_desc* descs;
@@ -235,9 +237,18 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
       // As SCHED 1, with the triangle tests of the wavefront's waiting lanes
       // pooled over its 64 lanes (pooled_walk); SCHED 3 leaves every empty
       // cell by the jump code.
-      unsigned long long* best =
-          reinterpret_cast<unsigned long long*>(hist + ((HIST == 0 ? 0 : (words + 1)) & ~1)) +
-          kPoolWords * (threadIdx.x >> 6);
+      unsigned long long* const best_all =
+          reinterpret_cast<unsigned long long*>(hist + ((HIST == 0 ? 0 : (words + 1)) & ~1));
+      const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+      unsigned long long* best = best_all + kPoolWords * wave;
+      // Each wavefront's hit faces waiting to be counted: fewer than 64
+      // before an iteration and at most 64 added by one. Counted 64 at a
+      // time, the mesh loads of 64 hits are in flight together instead of
+      // holding up the walk once per iteration; the counts are integers, so
+      // their order does not matter.
+      int* const queue = reinterpret_cast<int*>(best_all + kPoolWords * (blockDim.x >> 6)) + kHitQueue * wave;
+      int n_queued = 0;     // wave-uniform: changed only where every lane runs
+      int pend_face = -1;  // the hit of the ray this lane ended in this iteration
       pooled_walk<STATS, SCHED == 3 ? 1 : 2>(
           S, best, &s_next, T.g1 - T.g0, refill, pool,
           [&](long long i, v3& o, v3& dir) {
@@ -249,11 +260,31 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
             dir = e - o;
           },
           [&](long long, v3, v3, unsigned long long key) {
-            hit_rec h = {3.402823466e38f, -1, mk(0.f, 0.f, 0.f)};
-            decode_key(key, h.d2, h.f);
-            count_hit(h);
+            float d2;
+            decode_key(key, d2, pend_face);
           },
-          nt, nc);
+          nt, nc,
+          [&](bool last) {
+            const unsigned long long hits = __ballot(pend_face >= 0);
+            if (pend_face >= 0) {
+              queue[n_queued + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(hits >> 32),
+                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)hits, 0u))] =
+                  pend_face;
+              pend_face = -1;
+            }
+            n_queued += __popcll(hits);
+            if (n_queued < 64 && !(last && n_queued > 0)) return;
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the queue as every lane wrote it
+            while (n_queued >= 64 || (last && n_queued > 0)) {
+              const int take = min(n_queued, 64);
+              n_queued -= take;
+              if (lane < take) {
+                hit_rec h = {0.f, queue[n_queued + lane], mk(0.f, 0.f, 0.f)};
+                count_hit(h);
+              }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // read before the next writes
+          });
     } else if (SCHED == 1) {
       const long long n = T.g1 - T.g0;
       const int lane = threadIdx.x & 63;
@@ -387,10 +418,11 @@ static void set_lds_attr(size_t bytes) {
 }
 
 // Dynamic LDS of a launch: the row histogram, then (pooled tests) the
-// wavefronts' kPoolWords 64-bit words each (best hits and owner slots).
+// wavefronts' kPoolWords 64-bit words each (best hits and owner slots) and
+// their hit queues.
 static size_t fused_lds(int hist, size_t hist_lds, int sched, int block) {
   size_t b = hist ? hist_lds : 0;
-  if (sched >= 2) b = ((b + 7) & ~(size_t)7) + (size_t)(block / 64) * kPoolWords * 8;
+  if (sched >= 2) b = ((b + 7) & ~(size_t)7) + (size_t)(block / 64) * (kPoolWords * 8 + kHitQueue * 4);
   return b;
 }
 

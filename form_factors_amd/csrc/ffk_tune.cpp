@@ -305,16 +305,14 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
         }
         // The fused kernel often gains from finer grids than the sample cast
         // ranks first (empty cells are cheap to cross, smaller cells mean
-        // fewer tests): while the finest resolution tried is the fastest,
-        // step a factor sqrt(2) finer, keeping a step only when it wins by
-        // 1 %, up to 2^22 cells and at most 8 steps (an axis capped at 512
-        // cells can repeat a resolution).
-        double cpf = trials[0].cpf * std::sqrt(2.0);
-        for (int step = 0; step < 8 && r == FFK_OK; ++step) {
-          const long long cells_best = (long long)t.dims[0] * t.dims[1] * t.dims[2];
-          bool finest = true;
-          for (const auto& c : cand) finest = finest && (long long)c[0] * c[1] * c[2] <= cells_best;
-          if (!finest) break;
+        // fewer tests), and its timings differ by a few per cent between
+        // nearby resolutions: step a factor sqrt(2) finer than the finest
+        // candidate, keeping a step only when it wins by 1 %, until a step
+        // is 3 % slower than the best, after 4 steps, or past 2^22 cells (an
+        // axis capped at 512 cells can repeat a resolution).
+        double cpf = trials[0].cpf * std::sqrt(2.0);  // the finest candidate
+        if (trials.size() > 1) cpf = std::max(cpf, trials[1].cpf);
+        for (int step = 0; step < 4 && r == FFK_OK; ++step) {
           cpf *= std::sqrt(2.0);
           int d[3];
           dims_for(lo, hi, cpf, nf, d);
@@ -324,9 +322,12 @@ int ffk_autotune(ffk_scene* s, long long sample_rays, int* out_dims3, int* out_b
           cand.push_back(a);
           float ms = 0.f;
           if ((r = time_grid(d, &ms)) < 0) break;
-          if (!(ms < 0.99f * best_g)) break;
-          best_g = ms;
-          std::copy(d, d + 3, t.dims);
+          if (ms < 0.99f * best_g) {
+            best_g = ms;
+            std::copy(d, d + 3, t.dims);
+          } else if (ms > 1.03f * best_g) {
+            break;
+          }
         }
         if (r == FFK_OK) r = ffk_scene_build_grid(s, t.dims[0], t.dims[1], t.dims[2], stream);
         float best_b = 3.0e38f;
