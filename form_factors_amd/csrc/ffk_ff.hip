@@ -153,8 +153,10 @@ constexpr int kDefaultRefill = 32;
 // Pooled triangle tests run once waiting lanes x kDefaultPool >= active
 // lanes; 0 = each lane tests its own cells.
 constexpr int kDefaultPool = 2;
-// Hit faces a pooled wavefront queues before counting them (two rounds of 64).
-constexpr int kHitQueue = 128;
+// Hit faces a pooled wavefront queues before counting them: two rounds of 64
+// in a one-scene launch; one in a persistent launch, whose LDS classes would
+// lose resident blocks to the larger queue.
+__host__ __device__ constexpr int hit_queue(bool persist) { return persist ? 64 : 128; }
 
 struct fused_args {
   const scene_desc* descs;
@@ -241,12 +243,12 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
           reinterpret_cast<unsigned long long*>(hist + ((HIST == 0 ? 0 : (words + 1)) & ~1));
       const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
       unsigned long long* best = best_all + kPoolWords * wave;
-      // Each wavefront's hit faces waiting to be counted: fewer than 64
-      // before an iteration and at most 64 added by one. Counted 64 at a
-      // time, the mesh loads of 64 hits are in flight together instead of
-      // holding up the walk once per iteration; the counts are integers, so
-      // their order does not matter.
-      int* const queue = reinterpret_cast<int*>(best_all + kPoolWords * (blockDim.x >> 6)) + kHitQueue * wave;
+      // Each wavefront's hit faces waiting to be counted, fewer than 64
+      // between iterations. Counted 64 at a time, the mesh loads of 64 hits
+      // are in flight together instead of holding up the walk once per
+      // iteration; the counts are integers, so their order does not matter.
+      constexpr int kQueue = hit_queue(PERSIST);
+      int* const queue = reinterpret_cast<int*>(best_all + kPoolWords * (blockDim.x >> 6)) + kQueue * wave;
       int n_queued = 0;     // wave-uniform: changed only where every lane runs
       int pend_face = -1;  // the hit of the ray this lane ended in this iteration
       pooled_walk<STATS, SCHED == 3 ? 1 : 2>(
@@ -266,24 +268,56 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
           nt, nc,
           [&](bool last) {
             const unsigned long long hits = __ballot(pend_face >= 0);
-            if (pend_face >= 0) {
-              queue[n_queued + (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(hits >> 32),
-                                                              __builtin_amdgcn_mbcnt_lo((uint32_t)hits, 0u))] =
-                  pend_face;
-              pend_face = -1;
-            }
-            n_queued += __popcll(hits);
-            if (n_queued < 64 && !(last && n_queued > 0)) return;
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the queue as every lane wrote it
-            while (n_queued >= 64 || (last && n_queued > 0)) {
-              const int take = min(n_queued, 64);
-              n_queued -= take;
-              if (lane < take) {
-                hit_rec h = {0.f, queue[n_queued + lane], mk(0.f, 0.f, 0.f)};
-                count_hit(h);
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(hits >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)hits, 0u));
+            const int n_new = __popcll(hits);
+            if constexpr (kQueue >= 128) {
+              // Room for this iteration's hits beside fewer than 64 queued;
+              // then whole rounds of 64 from the top (and the rest at the end).
+              if (pend_face >= 0) queue[n_queued + rank] = pend_face;
+              n_queued += n_new;
+              if (n_queued < 64 && !(last && n_queued > 0)) {
+                pend_face = -1;
+                return;
+              }
+              __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the queue as every lane wrote it
+              while (n_queued >= 64 || (last && n_queued > 0)) {
+                const int take = min(n_queued, 64);
+                n_queued -= take;
+                if (lane < take) {
+                  hit_rec h = {0.f, queue[n_queued + lane], mk(0.f, 0.f, 0.f)};
+                  count_hit(h);
+                }
+              }
+              __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // read before the next writes
+            } else {
+              // This iteration's hits fill the queue up to 64; a full queue
+              // is counted and the remaining hits start the next one.
+              const int room = kQueue - n_queued;
+              if (pend_face >= 0 && rank < room) queue[n_queued + rank] = pend_face;
+              if (n_new >= room || (last && n_queued + n_new > 0)) {
+                const int take = min(n_queued + n_new, kQueue);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the queue as every lane wrote it
+                if (lane < take) {
+                  hit_rec h = {0.f, queue[lane], mk(0.f, 0.f, 0.f)};
+                  count_hit(h);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // read before the next writes
+                if (pend_face >= 0 && rank >= room) queue[rank - room] = pend_face;
+                n_queued = n_queued + n_new - take;
+                if (last && n_queued > 0) {  // the hits that did not fit the last round
+                  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                  if (lane < n_queued) {
+                    hit_rec h = {0.f, queue[lane], mk(0.f, 0.f, 0.f)};
+                    count_hit(h);
+                  }
+                  n_queued = 0;
+                }
+              } else {
+                n_queued += n_new;
               }
             }
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // read before the next writes
+            pend_face = -1;
           });
     } else if (SCHED == 1) {
       const long long n = T.g1 - T.g0;
@@ -420,15 +454,15 @@ static void set_lds_attr(size_t bytes) {
 // Dynamic LDS of a launch: the row histogram, then (pooled tests) the
 // wavefronts' kPoolWords 64-bit words each (best hits and owner slots) and
 // their hit queues.
-static size_t fused_lds(int hist, size_t hist_lds, int sched, int block) {
+static size_t fused_lds(int hist, size_t hist_lds, int sched, int block, bool persist) {
   size_t b = hist ? hist_lds : 0;
-  if (sched >= 2) b = ((b + 7) & ~(size_t)7) + (size_t)(block / 64) * (kPoolWords * 8 + kHitQueue * 4);
+  if (sched >= 2) b = ((b + 7) & ~(size_t)7) + (size_t)(block / 64) * (kPoolWords * 8 + hit_queue(persist) * 4);
   return b;
 }
 
 template <int H, bool P, bool S, int R>
 static void launch_one(dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
-  const size_t bytes = fused_lds(H, lds, R, block);
+  const size_t bytes = fused_lds(H, lds, R, block, P);
   set_lds_attr<H, P, S, R>(bytes);
   k_ff_fused<H, P, S, R><<<grid, block, bytes, st>>>(A.descs, A.tiles, A.n_tiles, A.queue, A.seeds, A.tallies,
                                                      A.Fs, A.stats, A.refill, pool_arg(A.pool, A.walk));
@@ -448,15 +482,15 @@ static hipError_t launch_mode(int hist, dim3 grid, int block, size_t lds, hipStr
 // pooled tests (when the best-hit words fit beside the histogram), 3 = as 2
 // with the jump code for every empty cell (walk bit 0). All cast the same rays
 // and tally the same hits.
-static int sched_of(int refill, int pool, int walk, int hist, size_t hist_lds, int block) {
+static int sched_of(int refill, int pool, int walk, int hist, size_t hist_lds, int block, bool persist) {
   if (refill <= 0) return 0;
-  if (pool <= 0 || fused_lds(hist, hist_lds, 2, block) > (size_t)max_lds_bytes()) return 1;
+  if (pool <= 0 || fused_lds(hist, hist_lds, 2, block, persist) > (size_t)max_lds_bytes()) return 1;
   return (walk & kWalkJump) ? 3 : 2;
 }
 
 template <bool P, bool S>
 static hipError_t launch_refill(int hist, dim3 grid, int block, size_t lds, hipStream_t st, const fused_args& A) {
-  const int sched = sched_of(A.refill, A.pool, A.walk, hist, lds, block);
+  const int sched = sched_of(A.refill, A.pool, A.walk, hist, lds, block, P);
   return sched == 3   ? launch_mode<P, S, 3>(hist, grid, block, lds, st, A)
          : sched == 2 ? launch_mode<P, S, 2>(hist, grid, block, lds, st, A)
          : sched == 1 ? launch_mode<P, S, 1>(hist, grid, block, lds, st, A)
@@ -1087,7 +1121,7 @@ int ffk_batch_get_tuning(const ffk_batch* b, int* block_size, int* rays_per_tile
 
 template <int R>
 static hipError_t persistent_occupancy(int hist, int block, size_t lds, int* per_cu) {
-  const size_t bytes = fused_lds(hist, lds, R, block);
+  const size_t bytes = fused_lds(hist, lds, R, block, true);
   switch (hist) {
     case 1:
       set_lds_attr<1, true, false, R>(bytes);
@@ -1111,7 +1145,7 @@ static int batch_blocks(const ffk_batch* b, int hist, size_t lds, int block, int
   int n_blocks = b->n_blocks;
   if (n_blocks <= 0) {
     int per_cu = 0;
-    const int sched = sched_of(refill, pool, walk, hist, lds, block);
+    const int sched = sched_of(refill, pool, walk, hist, lds, block, true);
     const hipError_t e = sched == 3   ? persistent_occupancy<3>(hist, block, lds, &per_cu)
                          : sched == 2 ? persistent_occupancy<2>(hist, block, lds, &per_cu)
                          : sched == 1 ? persistent_occupancy<1>(hist, block, lds, &per_cu)
