@@ -255,7 +255,9 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
           S, best, &s_next, T.g1 - T.g0, refill, pool,
           [&](long long i, v3& o, v3& dir) {
             const long long g = T.g0 + i;
-            const int f = find_face_in(S.face_off, s_faces, g, hint);
+            // A tile within one face needs no search (one load fewer ahead
+            // of the ray's own loads).
+            const int f = T.f_lo == T.f_hi ? T.f_lo : find_face_in(S.face_off, s_faces, g, hint);
             hint = f;
             v3 e;
             malley_ray(S, f, (int)(g - gld(gidx(S.face_off, (uint32_t)f))), k0, k1, &o, &e);
@@ -343,7 +345,7 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
             const long long off = (long long)base + rank;
             if (off < n) {
               const long long g = T.g0 + off;
-              const int f = find_face_in(S.face_off, s_faces, g, hint);
+              const int f = T.f_lo == T.f_hi ? T.f_lo : find_face_in(S.face_off, s_faces, g, hint);
               hint = f;
               v3 e;
               malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
