@@ -255,12 +255,19 @@ __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6)))
           S, best, &s_next, T.g1 - T.g0, refill, pool,
           [&](long long i, v3& o, v3& dir) {
             const long long g = T.g0 + i;
-            // A tile within one face needs no search (one load fewer ahead
-            // of the ray's own loads).
-            const int f = T.f_lo == T.f_hi ? T.f_lo : find_face_in(S.face_off, s_faces, g, hint);
-            hint = f;
+            // A tile within one face needs neither the search nor the face's
+            // first ray (two loads fewer ahead of the ray's own loads).
+            int f, j;
+            if (T.f_lo == T.f_hi && T.j0 >= 0) {
+              f = T.f_lo;
+              j = (int)i + T.j0;
+            } else {
+              f = find_face_in(S.face_off, s_faces, g, hint);
+              hint = f;
+              j = (int)(g - gld(gidx(S.face_off, (uint32_t)f)));
+            }
             v3 e;
-            malley_ray(S, f, (int)(g - gld(gidx(S.face_off, (uint32_t)f))), k0, k1, &o, &e);
+            malley_ray(S, f, j, k0, k1, &o, &e);
             dir = e - o;
           },
           [&](long long, v3, v3, unsigned long long key) {
@@ -532,7 +539,10 @@ static bool longer_class(const tile& a, const tile& b) { return size_class(a) > 
 static void scene_tiles(const ffk_scene* s, int scene_id, long long rpt, std::vector<tile>& out,
                         std::vector<int>& indirect) {
   auto add = [&](long long g0, long long g1, int m, int direct) {
-    out.push_back(tile{g0, g1, m, scene_id, direct, host_face_of(s, g0), host_face_of(s, g1 - 1), 0});
+    const int f_lo = host_face_of(s, g0);
+    const long long j0 = g0 - s->h_face_off[f_lo];
+    out.push_back(tile{g0, g1, m, scene_id, direct, f_lo, host_face_of(s, g1 - 1),
+                       j0 + (g1 - g0) < (1LL << 31) ? (int)j0 : -1});
   };
   for (int m = 0; m < s->n_meshes; ++m) {
     long long a = s->h_mesh_off[m], b = s->h_mesh_off[m + 1];

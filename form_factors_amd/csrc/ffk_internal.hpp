@@ -311,8 +311,8 @@ struct tile {
   int mesh;
   int scene;
   int direct;
-  int f_lo, f_hi;
-  int pad;
+  int f_lo, f_hi;  // faces of the first and the last ray
+  int j0;          // the first ray's index within face f_lo, or -1 when it is too large for int
 };
 
 }  // namespace ffk
