@@ -108,20 +108,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
   }
   __syncthreads();
   uint32_t nt = 0, nc = 0;
-  int hint = s_faces[0], whint = s_walk[0];
+  int whint = s_walk[0];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   int* const hit_ray = s_hit_ray[wave];
   int* const hit_face = s_hit_face[wave];
   int n_queued = 0;              // wave-uniform: changed only where every lane runs
   int pend_ray = 0, pend_face = -1;  // this lane's hit of this iteration, not yet queued
+  // The face of the ray a refill gives this lane, and the ray's index in it,
+  // found by the whole wavefront before the refill (prep below).
+  int new_face = 0, new_j = 0;
+  int window = s_faces[0];  // wave-uniform: first face of the offsets the wavefront loads
   pooled_walk<STATS, WALK>(
       S, best + kPoolWords * wave, &s_next, cnt, refill, pool,
-      [&](long long i, v3& o, v3& dir) {
-        const long long g = r0 + i;
-        const int f = find_face_in(S.face_off, s_faces, g, hint);
-        hint = f;
+      [&](long long, v3& o, v3& dir) {
         v3 e;
-        malley_ray(S, f, (int)(g - gld(S.face_off + f)), k0, k1, &o, &e);
+        malley_ray(S, new_face, new_j, k0, k1, &o, &e);
         dir = e - o;
       },
       // A ray's deposit needs a chain of dependent loads (its mesh, walk
@@ -163,6 +164,41 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
           }
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // read before the next writes
+      },
+      // A refill's rays are consecutive: the wavefront loads the first ray
+      // offsets of 64 consecutive faces from `window` in one load, and each
+      // new ray finds its face among them by a search over the lanes (the
+      // last face whose first ray is at or before it), instead of a chain of
+      // dependent loads per lane. When the 64 faces do not hold every new
+      // ray, each lane searches the table as before.
+      [&](int base, unsigned long long idle) {
+        const int n_new = __popcll(idle);
+        const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+        // Positions relative to the block's first ray, in 32 bits (clamped:
+        // order is all the search needs, and the window is used only when its
+        // first and last offsets bracket the new rays).
+        const int last = (int)min((long long)base + n_new, cnt) - 1;  // wave-uniform
+        const int gi = min(base + rank, last);
+        const int wf = window + lane;
+        const long long off64 = wf <= S.n_faces ? gld(S.face_off + wf) - r0 : 0x7fffffffLL;
+        const int off = (int)max(min(off64, 0x7fffffffLL), -0x7fffffffLL);
+        const int first_off = __shfl(off, 0, 64), end_off = __shfl(off, 63, 64);
+        if (first_off <= base && last < end_off && first_off > -0x7fffffff) {
+          int k = 0;  // the last lane whose offset is at or before gi
+#pragma unroll
+          for (int step = 32; step >= 1; step >>= 1) {
+            const int o_c = __shfl(off, k + step, 64);
+            if (o_c <= gi) k += step;
+          }
+          new_face = window + k;
+          new_j = gi - __shfl(off, k, 64);
+        } else {
+          const long long g = r0 + gi;
+          new_face = find_face_in(S.face_off, s_faces, g, window);
+          new_j = (int)(g - gld(S.face_off + new_face));
+        }
+        window = __builtin_amdgcn_readlane(new_face, 63 - __builtin_clzll(idle));
       });
   if (STATS) {
     unsigned long long t = nt, c = nc;

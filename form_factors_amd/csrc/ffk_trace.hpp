@@ -527,14 +527,20 @@ __device__ inline int wave_incl_max(int x) {
 //   after(last):                 called by every lane of the wavefront together
 //                                at the end of each iteration (last = false)
 //                                and once after the last one (last = true)
+//   prep(base, idle):            called by every lane together before a refill
+//                                starts rays base + (rank of the lane among the
+//                                idle lanes) in the idle lanes
 // Every lane of the wavefront must call this together.
 struct walk_no_after {
   __device__ void operator()(bool) const {}
 };
-template <bool STATS, int WALK, class Start, class Finish, class After = walk_no_after>
+struct walk_no_prep {
+  __device__ void operator()(int, unsigned long long) const {}
+};
+template <bool STATS, int WALK, class Start, class Finish, class After = walk_no_after, class Prep = walk_no_prep>
 __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best, int* next, long long n, int refill,
                                    int pool, Start start, Finish finish, uint32_t& n_tests, uint32_t& n_cells,
-                                   After after = After()) {
+                                   After after = After(), Prep prep = Prep()) {
   const int lane = threadIdx.x & 63;
   const bool whole = (pool & kPoolWhole) != 0, staged = (pool & kPoolStaged) != 0;
   pool &= 0xff;
@@ -554,6 +560,7 @@ __device__ inline void pooled_walk(const scene_desc& S, unsigned long long* best
       if (lane == 0) base = atomicAdd(next, n_idle);
       base = __shfl(base, 0, 64);
       if ((long long)base + n_idle >= n) more = false;
+      prep(base, idle);
       if (!have) {
         const int rank =
             (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
